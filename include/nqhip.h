@@ -1,0 +1,186 @@
+/*
+ * nqhip.h -- C ABI of libnqhip, the MI355X (gfx950) backend for the
+ * variational Monte Carlo steady-state loop of NeuralQuantum.jl.
+ *
+ * The reference's boundary for this path is Julia method dispatch, not an
+ * FFI; every entry point below names the reference method it stands in for
+ * (file:line in NeuralQuantum.jl).  A Julia wrapper `ccall`s these symbols
+ * (see INTEGRATION.md); in this repository they are exercised from Python
+ * through ctypes.
+ *
+ * Conventions
+ *   - status codes only, nothing is thrown across the ABI; on error the
+ *     context keeps a message, nqh_last_error(ctx), valid until the next call;
+ *   - host buffers are caller-owned, read or written only during the call and
+ *     never retained; every call is synchronous with respect to them;
+ *   - a context is not thread-safe: one context per host task;
+ *   - configurations are packed keys: bit i (0-based) of the row half is site
+ *     i+1 of sigma, bit N+i is site i+1 of eta, a set bit means v = +1; the
+ *     reference's doubled index is toint((sigma, eta)) = key + 1
+ *     (src/Hilbert/SuperHilbert.jl:50-56);
+ *   - parameters are real FP64 in the reference's flat order
+ *     (NDM: b_mu h_mu w_mu u_mu b_la h_la d_la w_la u_la, matrices
+ *     column-major; src/Networks/MixedDensityMatrix/NDM.jl:17-18);
+ *   - complex outputs are interleaved (re, im) doubles;
+ *   - batches of samples are ordered s = chain + slice * chains, the
+ *     reference's (batch, chain-length) layout (BatchedGradSampler.jl:88-110).
+ */
+#ifndef NQHIP_H
+#define NQHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NQH_ABI_VERSION 1
+
+typedef int32_t nqh_status;
+#define NQH_OK          0
+#define NQH_E_ARG      -1  /* invalid argument / unsupported shape */
+#define NQH_E_HIP      -2  /* HIP runtime error */
+#define NQH_E_NOTPD    -3  /* S + shift not positive definite (PosDefException, SRDirect.jl:79) */
+#define NQH_E_NAN      -4  /* non-finite loss, force or update detected */
+#define NQH_E_STATE    -5  /* call order: problem / network / sampler not set */
+#define NQH_E_NOMEM    -6  /* device allocation failed */
+#define NQH_E_CAPACITY -7  /* output capacity too small */
+
+/* precision of the network evaluation / of the S build */
+#define NQH_PREC_FP64 0
+#define NQH_PREC_FP32 1
+#define NQH_PREC_BF16 2    /* S build only: BF16 inputs, FP32 accumulation */
+
+/* SR diagonal shift: additive eps*I (sr_shift, SRDirect.jl:60-65) or eps*diag(S) */
+#define NQH_SHIFT_ADD 0
+#define NQH_SHIFT_DIAG 1
+
+/* sampler flags */
+#define NQH_SAMPLER_COMPAT_STALE_LAST_SLICE 1  /* reproduce the reference's never-sampled last slice (defect D1) */
+#define NQH_SAMPLER_PERSISTENT 2                /* keep chains across iterations instead of restarting */
+
+typedef struct nqh_ctx nqh_ctx;
+
+/*
+ * Flat connection table of a KLocalLiouvillian (or of a KLocalOperator sum
+ * for kind 3), built on the host by the operator algebra in the order the
+ * reference enumerates connections (KLocalLiouvillian.jl:46-52,
+ * KLocalOperatorTensor.jl:129-156):
+ *   term t: kind 0 = row side only (HnH_l), 1 = column side only (HnH_r),
+ *           2 = row (x) column nested loops (LLdag), 3 = ket operator;
+ *           local operators term_op_l[t] / term_op_r[t] (-1 = identity);
+ *   local operator o acts on sites op_sites[op_site_off[o] .. op_site_off[o+1])
+ *           (0-based; local bit m <-> m-th listed site, HomogeneousSpin.jl:171-179)
+ *           and owns rows op_row_off[o] .. op_row_off[o+1]-1 (2^k rows);
+ *   row q owns entries row_ent_off[q] .. row_ent_off[q+1]-1: a complex mel
+ *           and the mask of physical sites it flips (diagonal first, then
+ *           columns ascending, |mel| >= 1e-5; KLocalOperator.jl:67-109).
+ */
+typedef struct nqh_optable {
+    int32_t n_sites;
+    int32_t n_terms;
+    int32_t n_ops;
+    int32_t n_rows;
+    int32_t n_entries;
+    const int32_t* term_kind;
+    const int32_t* term_op_l;
+    const int32_t* term_op_r;
+    const int32_t* op_site_off;
+    const int32_t* op_sites;
+    const int32_t* op_row_off;
+    const int32_t* row_ent_off;
+    const double* ent_mel;
+    const uint64_t* ent_flip;
+} nqh_optable;
+
+int32_t nqh_abi_version(void);
+
+/* context on HIP device `device`; `seed` keys the Philox streams of the sampler
+ * (replaces MetropolisSampler(...; seed), Metropolis.jl:26 / rng.jl:9-11) */
+nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed);
+void nqh_destroy(nqh_ctx* ctx);
+const char* nqh_last_error(const nqh_ctx* ctx);
+
+/* the problem: KLocalLiouvillian connections (kind 0); replaces the Liouvillian
+ * argument of BatchedSampler(net, sampler, prob, algo) (build_Batched.jl:1-14) */
+nqh_status nqh_set_problem(nqh_ctx* ctx, const nqh_optable* table);
+
+/* NDM(Float64, N, alpha_h, alpha_a, af_softplus) with nh = alpha_h N hidden and
+ * na = alpha_a N ancilla units (NDM.jl:50-63); act 0 = softplus (the default) */
+nqh_status nqh_set_ndm(nqh_ctx* ctx, int32_t N, int32_t nh, int32_t na, int32_t act);
+int64_t nqh_num_params(const nqh_ctx* ctx);
+
+/* parameter upload / download (the Julia side keeps the canonical arrays and
+ * re-uploads after Optimisers.update!, or reads back after nqh_update) */
+nqh_status nqh_set_params(nqh_ctx* ctx, const double* flat, int64_t np);
+nqh_status nqh_get_params(const nqh_ctx* ctx, double* flat, int64_t np);
+
+/* precision of the network kernels (FP64 | FP32) and of the S build (FP64 | FP32 | BF16) */
+nqh_status nqh_set_precision(nqh_ctx* ctx, int32_t net_prec, int32_t sbuild_prec);
+
+/* log rho for B host-fed configurations: logpsi!(out, net, cache, sr, sc)
+ * (NDMBatched.jl:94-175); out: 2B doubles */
+nqh_status nqh_logpsi(nqh_ctx* ctx, const uint64_t* keys, int64_t B, double* logpsi);
+
+/* log rho and the Np complex log-derivatives O (reference order):
+ * logpsi_and_gradlogpsi!(grad, out, net, cache, sr, sc) (NDMBatched.jl:177-280);
+ * O: B x Np complex, sample-major */
+nqh_status nqh_logpsi_grad(nqh_ctx* ctx, const uint64_t* keys, int64_t B, double* logpsi, double* O);
+
+/* connections of B configurations as the accumulator sees them (zero mels
+ * skipped): accumulate_connections!(acc, liouv, v) (KLocalLiouvillian.jl:46-52);
+ * counts[b] = number of connections (also when it exceeds cap: then
+ * NQH_E_CAPACITY), keys_out / mel_out: B x cap */
+nqh_status nqh_connections(nqh_ctx* ctx, const uint64_t* keys, int64_t B, int32_t cap, int32_t* counts,
+                           uint64_t* keys_out, double* mel_out);
+
+/* local estimator for B host-fed configurations: init! / accumulate_connections!
+ * / finalize! of AccumulatorObsGrad (AccumulatorObsGrad.jl:39-127):
+ * Lloc: 2B doubles; dLloc (nullable): B x Np complex, reference order */
+nqh_status nqh_local_estimator(nqh_ctx* ctx, const uint64_t* keys, int64_t B, double* Lloc, double* dLloc);
+
+/* sample!(is; sample=false) on host-fed samples (BatchedGradSampler.jl:77-123 +
+ * setup_algorithm!, SRDirect.jl:26-49): chains x chain_length keys ->
+ * stats = {mean, error, variance, tau, R} of |Lloc|^2 (stats.jl:52-77),
+ * S = Re(Oc Oc^H)/Ns (Np x Np column-major, reference order, nullable),
+ * F = Re(grad C) (Np, nullable), Obar (Np complex, nullable) */
+nqh_status nqh_grad_from_samples(nqh_ctx* ctx, const uint64_t* keys, int64_t chains, int64_t chain_length,
+                                 double* stats, double* S, double* F, double* Obar);
+
+/* MetropolisSampler(LocalRule(), chain_length, passes; burn) with `chains`
+ * chains (batch_sz) (Metropolis.jl:26-40); passes is used as given (the
+ * reference bumps even values to odd at construction) */
+nqh_status nqh_set_sampler(nqh_ctx* ctx, int64_t chains, int32_t chain_length, int32_t passes, int32_t burn,
+                           int32_t flags);
+/* iteration index that keys the Philox counters of the next sampling call */
+nqh_status nqh_set_iteration(nqh_ctx* ctx, uint32_t iter);
+
+/* one sampling pass (_sample_state!, BaseIterativeSampler.jl:5-17); keys_out
+ * (nullable): chain_length x chains; accepted (nullable): accepted proposals */
+nqh_status nqh_sample(nqh_ctx* ctx, uint64_t* keys_out, int64_t* accepted);
+
+/* sample!(is) (BatchedGradSampler.jl:77-123): sample, log rho, O, L_loc,
+ * statistics, centring, force and the S build; stats: 5 doubles (nullable) */
+nqh_status nqh_sample_grad(nqh_ctx* ctx, double* stats);
+
+/* precondition!(cache, SR, iter) (SRDirect.jl:51-90): S + shift, Cholesky,
+ * Delta = S^-1 F; `refine` extra iterative-refinement steps in FP64;
+ * delta_out (nullable): Np, reference order */
+nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double* delta_out);
+
+/* Optimisers.update!(Descent(lr), net, Delta) (apply.jl:25-31, rules.jl:15-17) */
+nqh_status nqh_update(nqh_ctx* ctx, double lr);
+
+/* one SR iteration = nqh_sample_grad + nqh_precondition + nqh_update */
+nqh_status nqh_sr_iteration(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr,
+                            double* stats);
+
+/* last samples (chain_length x chains keys) / local values (2 Ns doubles) */
+nqh_status nqh_get_samples(nqh_ctx* ctx, uint64_t* keys_out, int64_t n);
+nqh_status nqh_get_local_values(nqh_ctx* ctx, double* lloc_out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NQHIP_H */

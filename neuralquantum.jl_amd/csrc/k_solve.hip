@@ -1,0 +1,395 @@
+// Loss statistics, force finish, the shifted SR solve and the parameter update.
+//
+//   stats ........ stat_analysis of |L_loc|^2 over chains x slices (stats.jl:52-77)
+//   force ........ F = Re(conj(L_loc) dL_loc)/Ns - C Re(Obar)   (BatchedGradSampler.jl:108-118)
+//   shift ........ S_ii += eps (sr_shift, SRDirect.jl:60-65) or eps * S_ii
+//   Cholesky ..... blocked right-looking, FP64: diagonal block (1 wave, LDS),
+//                  panel triangular solve, trailing update on FP64 MFMA
+//                  (cholesky!(Hermitian(S)) + ldiv!, SRDirect.jl:78-81)
+//   update ....... p -= lr * Delta (Descent, src/Optimisers/rules.jl:15-17)
+#include "nq_kernels.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int NB = 64;
+
+__global__ void k_fill(double* x, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+template <typename T>
+__device__ __forceinline__ double abs2_at(const T* L, int64_t s) {
+    const double re = (double)L[2 * s], im = (double)L[2 * s + 1];
+    return re * re + im * im;
+}
+
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < nw; ++k) t += red[k];
+    return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats(const T* L, int64_t B, int64_t Lc, double* out) {
+    __shared__ double red[16];
+    const double invL = 1.0 / (double)Lc;
+    // pass 1: mean of chain means (= overall mean, chains have equal length)
+    double sm = 0.0;
+    for (int64_t j = threadIdx.x; j < B; j += blockDim.x) {
+        double m = 0.0;
+        for (int64_t i = 0; i < Lc; ++i) m += abs2_at(L, j + i * B);
+        sm += m * invL;
+    }
+    const double mu = block_sum(sm, red) / (double)B;
+    // pass 2: chain variances, variance of chain means, variance of all values
+    double vch = 0.0, vmu = 0.0, vall = 0.0;
+    for (int64_t j = threadIdx.x; j < B; j += blockDim.x) {
+        double m = 0.0;
+        for (int64_t i = 0; i < Lc; ++i) m += abs2_at(L, j + i * B);
+        m *= invL;
+        double v = 0.0;
+        for (int64_t i = 0; i < Lc; ++i) {
+            const double x = abs2_at(L, j + i * B);
+            v += (x - m) * (x - m);
+            vall += (x - mu) * (x - mu);
+        }
+        vch += v / (double)(Lc - 1);
+        vmu += (m - mu) * (m - mu);
+    }
+    vch = block_sum(vch, red);
+    vmu = block_sum(vmu, red);
+    vall = block_sum(vall, red);
+    if (threadIdx.x == 0) {
+        const double var_mu_ch = vmu / (double)(B - 1);
+        const double var_mu = vall / (double)(B * Lc - 1);
+        const double t = var_mu_ch / var_mu;
+        out[0] = mu;
+        out[1] = sqrt(var_mu_ch / (double)B);
+        out[2] = vch / (double)B;
+        out[3] = fmax(0.0, 0.5 * (t * (double)Lc - 1.0));
+        out[4] = sqrt((double)(Lc - 1) / (double)Lc + t);
+    }
+}
+
+__global__ void k_force_finish(const double* Fpart, int nblk, const double* mu, const double* stats,
+                               ndm_dims d, double* F) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.np) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += Fpart[(int64_t)b * d.np + k];
+    const double reObar = (k < d.nlam + d.npi) ? mu[k] : 0.0;
+    F[k] = acc - stats[0] * reObar;
+}
+
+__global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double& s = S[(int64_t)k + (int64_t)k * lds];
+    s += (kind == 0) ? eps : eps * s;
+}
+
+// ---- Cholesky: diagonal block, one 64-thread workgroup, LDS resident
+__global__ __launch_bounds__(64) void k_potrf_diag(double* A, int lda, int k, int kb, int* info) {
+    __shared__ double L[NB][NB + 1];
+    const int t = threadIdx.x;
+    for (int c = 0; c < kb; ++c)
+        if (t < kb) L[t][c] = A[(int64_t)(k + t) + (int64_t)(k + c) * lda];
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (t == 0) {
+            const double v = L[j][j];
+            if (!(v > 0.0) && *info == 0) *info = k + j + 1;
+            L[j][j] = sqrt(v);
+        }
+        __syncthreads();
+        if (t > j && t < kb) L[t][j] /= L[j][j];
+        __syncthreads();
+        if (t > j && t < kb) {
+            const double lij = L[t][j];
+            for (int m = j + 1; m <= t; ++m) L[t][m] -= lij * L[m][j];
+        }
+        __syncthreads();
+    }
+    for (int c = 0; c < kb; ++c)
+        if (t < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = L[t][c];
+}
+
+// ---- panel: rows i >= k + kb: X = A[i, k:k+kb] L_kk^-T (one thread per row)
+__global__ __launch_bounds__(256) void k_trsm_panel(double* A, int n, int lda, int k, int kb) {
+    __shared__ double L[NB][NB + 1];
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int r = e % kb, c = e / kb;
+        L[r][c] = (r >= c) ? A[(int64_t)(k + r) + (int64_t)(k + c) * lda] : 0.0;
+    }
+    __syncthreads();
+    const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) x[j] = (j < kb) ? A[(int64_t)i + (int64_t)(k + j) * lda] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < kb) {
+            double v = x[j];
+#pragma unroll
+            for (int m = 0; m < j; ++m) v -= x[m] * L[j][m];
+            x[j] = v / L[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        if (j < kb) A[(int64_t)i + (int64_t)(k + j) * lda] = x[j];
+}
+
+__device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
+    int j = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+    while ((j + 1) * (j + 2) / 2 <= b) ++j;
+    while (j * (j + 1) / 2 > b) --j;
+    *J = j;
+    *I = b - j * (j + 1) / 2;
+}
+
+// ---- trailing update: A[i, j] -= sum_m X[i, m] X[j, m] for tiles I >= J of the
+// trailing matrix (origin o = k + kb), X = A[:, k:k+kb]; FP64 MFMA 16x16x4
+__global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda, int k, int kb) {
+    __shared__ double Xi[NB][65];   // [m][row]
+    __shared__ double Xj[NB][65];
+    int tI, tJ;
+    tri_decode(blockIdx.x, &tJ, &tI);   // tJ <= tI
+    const int o = k + kb;
+    const int i0 = o + tI * 64, j0 = o + tJ * 64;
+    for (int e = threadIdx.x; e < 64 * NB; e += blockDim.x) {
+        const int r = e & 63, m = e >> 6;
+        double vi = 0.0, vj = 0.0;
+        if (m < kb) {
+            if (i0 + r < n) vi = A[(int64_t)(i0 + r) + (int64_t)(k + m) * lda];
+            if (j0 + r < n) vj = A[(int64_t)(j0 + r) + (int64_t)(k + m) * lda];
+        }
+        Xi[m][r] = vi;
+        Xj[m][r] = vj;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
+    for (int m = 0; m < NB; m += 4) {
+        double av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = Xi[m + lk][wr * 32 + a * 16 + li];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = Xj[m + lk][wc * 32 + b * 16 + li];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    const int lj = lane & 15, li4 = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = i0 + wr * 32 + a * 16 + li4 + v;
+                const int c = j0 + wc * 32 + b * 16 + lj;
+                if (r < n && c < n && r >= c) A[(int64_t)r + (int64_t)c * lda] -= acc[a][b][v];
+            }
+}
+
+// ---- triangular solves with the factor, one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_chol_solve(const double* L, int n, int lda, const double* b,
+                                                      double* x) {
+    extern __shared__ double y[];   // n
+    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    // forward: L y = b
+    for (int k = 0; k < n; k += NB) {
+        const int kb = (n - k < NB) ? n - k : NB;
+        if (threadIdx.x < 64) {
+            double v = (lane < kb) ? y[k + lane] : 0.0;
+            for (int j = 0; j < kb; ++j) {
+                if (lane == j) v = v / L[(int64_t)(k + j) + (int64_t)(k + j) * lda];
+                const double yj = __shfl(v, j, 64);
+                if (lane > j && lane < kb) v -= L[(int64_t)(k + lane) + (int64_t)(k + j) * lda] * yj;
+            }
+            if (lane < kb) y[k + lane] = v;
+        }
+        __syncthreads();
+        for (int i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
+            double acc = 0.0;
+            for (int m = 0; m < kb; ++m) acc += L[(int64_t)i + (int64_t)(k + m) * lda] * y[k + m];
+            y[i] -= acc;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y
+    const int last = ((n - 1) / NB) * NB;
+    for (int k = last; k >= 0; k -= NB) {
+        const int kb = (n - k < NB) ? n - k : NB;
+        if (threadIdx.x < 64) {
+            double v = (lane < kb) ? y[k + lane] : 0.0;
+            for (int j = kb - 1; j >= 0; --j) {
+                if (lane == j) v = v / L[(int64_t)(k + j) + (int64_t)(k + j) * lda];
+                const double xj = __shfl(v, j, 64);
+                if (lane < j) v -= L[(int64_t)(k + j) + (int64_t)(k + lane) * lda] * xj;
+            }
+            if (lane < kb) y[k + lane] = v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += blockDim.x) {
+            double acc = 0.0;
+            for (int m = 0; m < kb; ++m) acc += L[(int64_t)(k + m) + (int64_t)i * lda] * y[k + m];
+            y[i] -= acc;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = y[i];
+}
+
+__global__ void k_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
+                           const double* b, double* r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += S[(int64_t)i + (int64_t)j * lds] * x[j];
+    const double sii = S[(int64_t)i + (int64_t)i * lds];
+    acc += ((kind == 0) ? eps : eps * sii) * x[i];
+    r[i] = b[i] - acc;
+}
+
+__global__ void k_axpy(double* x, const double* y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += y[i];
+}
+
+// P-order index -> reference flat index
+__device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
+    const int N = d.N, nh = d.nh, na = d.na;
+    if (k < N) return d.bla + k;
+    k -= N;
+    if (k < nh) return d.hla + k;
+    k -= nh;
+    if (k < nh * N) return d.wla + k;
+    k -= nh * N;
+    if (k < na) return d.dla + k;
+    k -= na;
+    if (k < na * N) return d.ula + k;
+    k -= na * N;
+    if (k < na * N) return d.umu + k;
+    k -= na * N;
+    if (k < N) return d.bmu + k;
+    k -= N;
+    if (k < nh) return d.hmu + k;
+    k -= nh;
+    return d.wmu + k;
+}
+
+__global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.np) return;
+    const int r = p2ref(k, d);
+    const double v = p64[r] - lr * delta[k];
+    p64[r] = v;
+    if (p32) p32[r] = (float)v;
+}
+
+__global__ void k_to_f32(const double* p64, float* p32, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p32[i] = (float)p64[i];
+}
+
+__global__ void k_check_finite(const double* x, int64_t n, int* flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !isfinite(x[i])) atomicOr(flag, 1);
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t nq_launch_fill(double* x, int64_t n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st) {
+    if (fp64)
+        hipLaunchKernelGGL((k_stats<double>), dim3(1), dim3(256), 0, st, (const double*)Lloc, B, Lc, out);
+    else
+        hipLaunchKernelGGL((k_stats<float>), dim3(1), dim3(256), 0, st, (const float*)Lloc, B, Lc, out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_force_finish(const double* Fpart, int nb, const double* mu, const double* stats,
+                                  ndm_dims d, double* F, hipStream_t st) {
+    hipLaunchKernelGGL(k_force_finish, dim3(nblk(d.np, 256)), dim3(256), 0, st, Fpart, nb, mu, stats, d, F);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st) {
+    hipLaunchKernelGGL(k_shift, dim3(nblk(n, 256)), dim3(256), 0, st, S, n, lds, eps, kind);
+    return hipGetLastError();
+}
+
+hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st) {
+    for (int k = 0; k < n; k += NB) {
+        const int kb = (n - k < NB) ? n - k : NB;
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, S, lds, k, kb, info);
+        const int rest = n - k - kb;
+        if (rest > 0) {
+            hipLaunchKernelGGL(k_trsm_panel, dim3(nblk(rest, 256)), dim3(256), 0, st, S, n, lds, k, kb);
+            const int T = (rest + 63) / 64;
+            hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, n, lds, k, kb);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* b, double* x, hipStream_t st) {
+    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), sizeof(double) * (size_t)n, st, L, n, lds, b, x);
+    return hipGetLastError();
+}
+
+hipError_t nq_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
+                       const double* b, double* r, hipStream_t st) {
+    hipLaunchKernelGGL(k_residual, dim3(nblk(n, 256)), dim3(256), 0, st, S, n, lds, eps, kind, x, b, r);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_axpy(double* x, const double* y, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpy, dim3(nblk(n, 256)), dim3(256), 0, st, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_update, dim3(nblk(d.np, 256)), dim3(256), 0, st, p64, p32, delta, d, lr);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_to_f32(const double* p64, float* p32, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk(n, 256)), dim3(256), 0, st, p64, p32, n);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_check_finite(const double* x, int64_t n, int* flag, hipStream_t st) {
+    hipLaunchKernelGGL(k_check_finite, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, flag);
+    return hipGetLastError();
+}

@@ -1,0 +1,156 @@
+// Device-side building blocks shared by the libnqhip kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NQ_WAVE 64
+
+// ---------------------------------------------------------------- complex
+template <typename T>
+struct cplx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmk(T r, T i) { cplx<T> z; z.re = r; z.im = i; return z; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return cmk(a.re + b.re, a.im + b.im); }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return cmk(a.re - b.re, a.im - b.im); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return cmk(a.re * s, a.im * s); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return cmk(a.re, -a.im); }
+// a += b * s (s real)
+template <typename T>
+__device__ __forceinline__ void cfma_r(cplx<T>& a, cplx<T> b, T s) { a.re += b.re * s; a.im += b.im * s; }
+
+// ---------------------------------------------------------------- math
+// FP64: the accurate device library.  FP32: the hardware transcendentals
+// (v_exp_f32, v_log_f32, v_sin_f32, v_cos_f32); their absolute errors (~1e-7)
+// are below what the FP32 form states against the FP64 oracle.
+__device__ __forceinline__ double nq_exp(double x) { return exp(x); }
+__device__ __forceinline__ float nq_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double nq_log1p(double x) { return log1p(x); }
+__device__ __forceinline__ float nq_log1p(float x) { return __logf(1.0f + x); }
+__device__ __forceinline__ double nq_atan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float nq_atan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ void nq_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void nq_sincos(float x, float* s, float* c) { *s = __sinf(x); *c = __cosf(x); }
+
+// softplus f(x) = log(1 + e^x) and its derivative 1/(1 + e^-x)
+// (src/Networks/activation.jl:14-18), sharing one exponential.
+// e / (1 + e) is accurate for every finite e; e = inf gives the limit 1.
+template <typename T>
+__device__ __forceinline__ T sigmoid_of_exp(T e) {
+    return __builtin_isinf(e) ? T(1) : e / (T(1) + e);
+}
+template <typename T>
+__device__ __forceinline__ void softplus_and_sigmoid(T x, T* f, T* df) {
+    T e = nq_exp(x);
+    *f = nq_log1p(e);
+    *df = sigmoid_of_exp(e);
+}
+template <typename T>
+__device__ __forceinline__ T sigmoid(T x) { return sigmoid_of_exp(nq_exp(x)); }
+template <typename T>
+__device__ __forceinline__ T softplus(T x) { return nq_log1p(nq_exp(x)); }
+
+// complex softplus and its derivative: w = e^z,
+//   log(1 + w) = 1/2 log1p(e^x (2 cos y + e^x)) + i atan2(e^x sin y, 1 + e^x cos y)
+//   f'(z) = w / (1 + w)
+template <typename T>
+__device__ __forceinline__ void csoftplus_and_sigmoid(cplx<T> z, cplx<T>* f, cplx<T>* df) {
+    T ex = nq_exp(z.re), s, c;
+    nq_sincos(z.im, &s, &c);
+    T wr = ex * c, wi = ex * s;
+    T dr = T(1) + wr;
+    f->re = T(0.5) * nq_log1p(ex * (T(2) * c + ex));
+    f->im = nq_atan2(wi, dr);
+    // w / (1 + w) = (wr + i wi)(dr - i wi) / (dr^2 + wi^2)
+    T den = T(1) / (dr * dr + wi * wi);
+    df->re = (wr * dr + wi * wi) * den;
+    df->im = (wi * dr - wr * wi) * den;
+}
+// Re log(1 + e^z) only (what the sampler needs: log p = 2 Re log rho).
+template <typename T>
+__device__ __forceinline__ T csoftplus_re(cplx<T> z) {
+    T ex = nq_exp(z.re), s, c;
+    nq_sincos(z.im, &s, &c);
+    return T(0.5) * nq_log1p(ex * (T(2) * c + ex));
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).
+struct philox_out { uint32_t x[4]; };
+__device__ __forceinline__ philox_out philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                    uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    philox_out o; o.x[0] = c0; o.x[1] = c1; o.x[2] = c2; o.x[3] = c3;
+    return o;
+}
+// 53-bit uniform in [0, 1) from two words (same mapping as the host oracle).
+__device__ __forceinline__ double u01_53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ float u01_24(uint32_t a) { return (float)(a >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------- reductions
+// xor-butterfly sums over aligned groups of `width` lanes (power of two <= 64).
+__device__ __forceinline__ float group_sum(float v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double group_sum(double v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) { return group_sum(v, 64); }
+
+// ---------------------------------------------------------------- NDM layout
+// Offsets of the NDM blocks in the reference's flat order (NDM.jl:17-18):
+// b_mu[N] h_mu[nh] w_mu[nh x N] u_mu[na x N] b_la[N] h_la[nh] d_la[na] w_la[nh x N] u_la[na x N]
+// (matrices column-major, element (h, i) at h + nh*i).
+struct ndm_dims {
+    int N, nh, na;
+    int bmu, hmu, wmu, umu, bla, hla, dla, wla, ula, np;
+    // compact gradient layout (one row per sample, "P order"):
+    //   lambda block (real):  b_la h_la w_la            -> cols [0, nlam)
+    //   Pi block (real part): d_la u_la u_mu            -> cols [nlam, nlam + npi)
+    //   Pi block (imag part): d_la u_la u_mu            -> cols [nlam + npi, nlam + 2 npi)
+    //   mu block (imag):      b_mu h_mu w_mu            -> cols [nlam + 2 npi, R)
+    int nlam, npi, nmu, R;
+};
+
+static inline ndm_dims make_ndm_dims(int N, int nh, int na) {
+    ndm_dims d;
+    d.N = N; d.nh = nh; d.na = na;
+    int p = 0;
+    d.bmu = p; p += N;
+    d.hmu = p; p += nh;
+    d.wmu = p; p += nh * N;
+    d.umu = p; p += na * N;
+    d.bla = p; p += N;
+    d.hla = p; p += nh;
+    d.dla = p; p += na;
+    d.wla = p; p += nh * N;
+    d.ula = p; p += na * N;
+    d.np = p;
+    d.nlam = N + nh + nh * N;
+    d.npi = na + 2 * na * N;
+    d.nmu = N + nh + nh * N;
+    d.R = d.nlam + 2 * d.npi + d.nmu;
+    return d;
+}

@@ -1,0 +1,99 @@
+// Argument blocks and launchers of the libnqhip kernels (host <-> device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nq_device.h"
+#include "nq_table.h"
+
+// ---- fused estimator (k_estimate.hip)
+struct est_args {
+    ndm_dims d;
+    const void* p;            // parameters, reference order, precision T
+    dev_table tab;
+    const uint64_t* keys;     // Ns packed configurations
+    int64_t Ns;
+    double invNs;
+    int do_conn;              // walk connections (L_loc, force)
+    int lds_param_bytes;      // LDS bytes reserved for the staged parameters
+    int max_conn;             // upper bound of connections per configuration (LDS W cache)
+    void* lp;                 // 2 Ns T: log rho (re, im), nullable
+    void* Y;                  // Ns x R T: compact O rows, nullable
+    void* Lloc;               // 2 Ns T, nullable
+    double* Fpart;            // gridDim x np: per-workgroup force partials (P order), nullable
+    void* G;                  // Ns x np complex FP64: per-sample dL_loc (reference order), nullable, pre-zeroed
+};
+size_t est_lds_bytes(const est_args& a, size_t elt);
+hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st);
+
+// ---- Metropolis sampler (k_sampler.hip)
+struct samp_args {
+    ndm_dims d;
+    const void* p;
+    int64_t chains;
+    int passes, burn, stored;  // calls = burn + stored
+    int lpc;                   // lanes per chain (power of two)
+    int restart;               // draw a fresh uniform configuration per chain
+    int compat_stale_last_slice;
+    uint64_t seed;
+    uint32_t iter;
+    uint64_t* chain_state;     // chains, in/out (nullable when restart)
+    uint64_t* samples;         // (stored [+1]) x chains, chain fastest
+    unsigned long long* accepted;
+};
+int nq_sampler_lanes_per_chain(const ndm_dims& d);
+hipError_t nq_launch_metropolis(const samp_args& a, bool fp64, hipStream_t st);
+
+// ---- S build (k_sbuild.hip)
+enum { NQ_GRAM_F64 = 0, NQ_GRAM_F32 = 1, NQ_GRAM_BF16 = 2 };
+// S[o + r, o + c] += scale * sum_s (Y[s, c0 + r] - mu[c0 + r]) (Y[s, c0 + c] - mu[c0 + c])
+// for r, c < n; S is column-major with leading dimension lds.  Y is Ns x ldy of
+// type double (y_fp64) or float.
+struct gram_args {
+    const void* Y;
+    int y_fp64;
+    int64_t Ns;
+    int ldy, c0, n;
+    const double* mu;          // column means (length >= c0 + n), nullable
+    double* S;
+    int lds, o;
+    double scale;
+};
+hipError_t nq_launch_gram(const gram_args& a, int mode, hipStream_t st);
+
+// column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]
+hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double* part, int chunks,
+                            hipStream_t st);
+// mu[c] = scale * sum_chunk part[chunk][c]
+hipError_t nq_launch_colsum_finish(const double* part, int chunks, int R, double scale, double* mu,
+                                   hipStream_t st);
+
+// ---- small kernels and the solve (k_solve.hip)
+hipError_t nq_launch_fill(double* x, int64_t n, double v, hipStream_t st);
+// stats of |L_loc|^2 over B chains x L slices (stats.jl:52-77): out[5]
+hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
+// F[k] = sum_b Fpart[b][k] - C Re(Obar_k), P order; C = stats[0]
+hipError_t nq_launch_force_finish(const double* Fpart, int nblk, const double* mu, const double* stats,
+                                  ndm_dims d, double* F, hipStream_t st);
+// S[k,k] += eps (kind 0) or eps * S[k,k] (kind 1)
+hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st);
+// in-place lower Cholesky of S (column-major, n x n); *info = first failing column + 1 or 0
+hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st);
+// x = (L L^T)^-1 b with the factor from nq_cholesky (b and x may alias)
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* b, double* x, hipStream_t st);
+// r = b - (S + shift) x   (S the unfactored matrix, shift as in nq_launch_shift)
+hipError_t nq_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
+                       const double* b, double* r, hipStream_t st);
+// x += y
+hipError_t nq_launch_axpy(double* x, const double* y, int n, hipStream_t st);
+// params[perm_ref(k)] -= lr * delta[k] (delta in P order); refresh the FP32 copy
+hipError_t nq_launch_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr,
+                            hipStream_t st);
+// FP32 copy of the parameters
+hipError_t nq_launch_to_f32(const double* p64, float* p32, int64_t n, hipStream_t st);
+// NaN / Inf check: *flag |= 1 if any non-finite in x
+hipError_t nq_launch_check_finite(const double* x, int64_t n, int* flag, hipStream_t st);
+
+// ---- connection listing (k_network.hip)
+hipError_t nq_launch_connections(dev_table tab, int N, const uint64_t* keys, int64_t B, int cap,
+                                 int* counts, uint64_t* keys_out, double* mel_out, hipStream_t st);

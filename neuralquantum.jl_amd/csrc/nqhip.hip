@@ -1,0 +1,784 @@
+// libnqhip host runtime: the C ABI of include/nqhip.h on top of the kernels.
+// One context owns one HIP stream and all device buffers of one SR loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nqhip.h"
+#include "nq_kernels.h"
+
+namespace {
+
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct nqh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t seed = 0;
+    std::string err;
+
+    // problem
+    bool has_problem = false;
+    int N_table = 0;
+    dbuf table_mem;
+    dev_table tab{};
+    int max_conn = 0;
+
+    // network
+    bool has_net = false;
+    ndm_dims d{};
+    dbuf p64, p32;
+    int net_prec = NQH_PREC_FP64;
+    int s_prec = NQH_PREC_FP64;
+
+    // sampler
+    bool has_sampler = false;
+    int64_t chains = 0;
+    int chain_len = 0, passes = 1, burn = 0, flags = 0;
+    uint32_t iter = 0;
+    bool chains_initialised = false;
+    dbuf chain_state, samples, acc;
+
+    // per-sample buffers (capacity Ns_cap)
+    int64_t ns_cap = 0;
+    dbuf keys_in, lp, Y, Lloc, G;
+    int est_grid = 0;
+    dbuf Fpart;
+    // iteration buffers
+    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag;
+    int64_t last_ns = 0;
+    bool grad_ready = false;
+};
+
+namespace {
+
+constexpr int kColChunks = 64;
+
+nqh_status fail(nqh_ctx* c, nqh_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+nqh_status hip_fail(nqh_ctx* c, hipError_t e, const char* where) {
+    return fail(c, NQH_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                    \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
+    } while (0)
+
+nqh_status ensure(nqh_ctx* c, dbuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return NQH_OK;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (bytes == 0) return NQH_OK;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(c, NQH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b.bytes = bytes;
+    return NQH_OK;
+}
+
+void release(dbuf& b) {
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+T* P(dbuf& b) { return reinterpret_cast<T*>(b.p); }
+
+bool fp64_net(const nqh_ctx* c) { return c->net_prec == NQH_PREC_FP64; }
+size_t net_elt(const nqh_ctx* c) { return fp64_net(c) ? sizeof(double) : sizeof(float); }
+
+// P-order index -> reference index (same map as k_update's p2ref)
+int p2ref_host(int k, const ndm_dims& d) {
+    const int N = d.N, nh = d.nh, na = d.na;
+    if (k < N) return d.bla + k;
+    k -= N;
+    if (k < nh) return d.hla + k;
+    k -= nh;
+    if (k < nh * N) return d.wla + k;
+    k -= nh * N;
+    if (k < na) return d.dla + k;
+    k -= na;
+    if (k < na * N) return d.ula + k;
+    k -= na * N;
+    if (k < na * N) return d.umu + k;
+    k -= na * N;
+    if (k < N) return d.bmu + k;
+    k -= N;
+    if (k < nh) return d.hmu + k;
+    k -= nh;
+    return d.wmu + k;
+}
+
+nqh_status need_net(nqh_ctx* c) {
+    if (!c->has_net) return fail(c, NQH_E_STATE, "network not set (nqh_set_ndm)");
+    return NQH_OK;
+}
+nqh_status need_problem(nqh_ctx* c) {
+    if (!c->has_problem) return fail(c, NQH_E_STATE, "problem not set (nqh_set_problem)");
+    if (c->has_net && c->N_table != c->d.N)
+        return fail(c, NQH_E_ARG, "problem and network have different numbers of sites");
+    return NQH_OK;
+}
+
+int est_grid_for(int64_t Ns) {
+    int64_t g = (Ns + 3) / 4;
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// per-sample buffers for Ns samples
+nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
+    nqh_status s;
+    const size_t e = net_elt(c);
+    if ((s = ensure(c, c->keys_in, sizeof(uint64_t) * Ns))) return s;
+    if ((s = ensure(c, c->lp, 2 * e * Ns))) return s;
+    if ((s = ensure(c, c->Y, e * (size_t)c->d.R * Ns))) return s;
+    if ((s = ensure(c, c->Lloc, 2 * e * Ns))) return s;
+    c->est_grid = est_grid_for(Ns);
+    if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * c->d.np))) return s;
+    if ((s = ensure(c, c->colpart, sizeof(double) * (size_t)kColChunks * c->d.R))) return s;
+    if ((s = ensure(c, c->mu, sizeof(double) * c->d.R))) return s;
+    if ((s = ensure(c, c->stats, sizeof(double) * 8))) return s;
+    if ((s = ensure(c, c->F, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->S, sizeof(double) * (size_t)c->d.np * c->d.np))) return s;
+    if ((s = ensure(c, c->S0, sizeof(double) * (size_t)c->d.np * c->d.np))) return s;
+    if ((s = ensure(c, c->delta, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->resid, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->info, sizeof(int) * 4))) return s;
+    if ((s = ensure(c, c->flag, sizeof(int) * 4))) return s;
+    return NQH_OK;
+}
+
+est_args make_est(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
+    est_args a{};
+    a.d = c->d;
+    a.p = fp64_net(c) ? c->p64.p : c->p32.p;
+    a.tab = c->tab;
+    a.keys = keys_dev;
+    a.Ns = Ns;
+    a.invNs = 1.0 / (double)Ns;
+    a.do_conn = 0;
+    a.lds_param_bytes = (int)(((size_t)c->d.np * net_elt(c) + 15) / 16 * 16);
+    a.max_conn = std::max(c->max_conn, 1);
+    return a;
+}
+
+// log rho, O rows, L_loc, force partials, then column means, stats, force, S
+nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chains, int64_t Lc) {
+    const int64_t Ns = chains * Lc;
+    const bool f64 = fp64_net(c);
+    est_args a = make_est(c, keys_dev, Ns);
+    a.do_conn = 1;
+    a.lp = c->lp.p;
+    a.Y = c->Y.p;
+    a.Lloc = c->Lloc.p;
+    a.Fpart = P<double>(c->Fpart);
+    HIPCHK(c, nq_launch_estimate(a, f64, c->est_grid, c->stream));
+    HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
+    HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, c->d.R, 1.0 / (double)Ns,
+                                      P<double>(c->mu), c->stream));
+    HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
+    HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->mu), P<double>(c->stats),
+                                     c->d, P<double>(c->F), c->stream));
+    // S in P order: A-gram at [0, nA), B-gram at [nlam, np)
+    const ndm_dims& d = c->d;
+    const int np = d.np;
+    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)np * np, 0.0, c->stream));
+    const int mode = c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : (c->s_prec == NQH_PREC_FP32 ? NQ_GRAM_F32 : NQ_GRAM_BF16);
+    gram_args g{};
+    g.Y = c->Y.p;
+    g.y_fp64 = f64;
+    g.Ns = Ns;
+    g.ldy = d.R;
+    g.mu = P<double>(c->mu);
+    g.S = P<double>(c->S);
+    g.lds = np;
+    g.scale = 1.0 / (double)Ns;
+    g.c0 = 0;
+    g.n = d.nlam + d.npi;
+    g.o = 0;
+    HIPCHK(c, nq_launch_gram(g, mode, c->stream));
+    g.c0 = d.nlam + d.npi;
+    g.n = d.npi + d.nmu;
+    g.o = d.nlam;
+    HIPCHK(c, nq_launch_gram(g, mode, c->stream));
+    c->last_ns = Ns;
+    c->grad_ready = true;
+    return NQH_OK;
+}
+
+nqh_status upload_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
+    nqh_status s = ensure(c, c->keys_in, sizeof(uint64_t) * (size_t)B);
+    if (s) return s;
+    HIPCHK(c, hipMemcpyAsync(c->keys_in.p, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice, c->stream));
+    return NQH_OK;
+}
+
+nqh_status check_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
+    const int N = c->d.N;
+    const uint64_t lim = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1);
+    for (int64_t b = 0; b < B; ++b)
+        if (keys[b] & ~lim) return fail(c, NQH_E_ARG, "configuration key has bits beyond 2N");
+    return NQH_OK;
+}
+
+// compact P-order row (real components) -> reference-order complex vector
+template <typename T>
+void expand_row(const T* y, const ndm_dims& d, double* out /* 2 np */) {
+    const int nA = d.nlam + d.npi;
+    for (int k = 0; k < d.np; ++k) {
+        const int r = p2ref_host(k, d);
+        double re = 0.0, im = 0.0;
+        if (k < d.nlam) re = (double)y[k];
+        else if (k < nA) { re = (double)y[k]; im = (double)y[k + d.npi]; }
+        else im = (double)y[k + d.npi];
+        out[2 * r] = re;
+        out[2 * r + 1] = im;
+    }
+}
+
+template <typename T>
+void copy_cplx(const T* src, double* dst, int64_t n2) {
+    for (int64_t k = 0; k < n2; ++k) dst[k] = (double)src[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nqh_abi_version(void) { return NQH_ABI_VERSION; }
+
+nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
+    if (!out) return NQH_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return NQH_E_HIP;
+    if (device < 0 || device >= n) return NQH_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NQH_E_HIP;
+    nqh_ctx* c = new nqh_ctx();
+    c->device = device;
+    c->seed = seed;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return NQH_E_HIP;
+    }
+    *out = c;
+    return NQH_OK;
+}
+
+void nqh_destroy(nqh_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
+                   &c->Y, &c->Lloc, &c->G, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
+                   &c->delta, &c->resid, &c->info, &c->flag};
+    for (dbuf* b : all) release(*b);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* nqh_last_error(const nqh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
+    if (!c || !t) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    const int N = t->n_sites;
+    if (N < 1 || N > 32) return fail(c, NQH_E_ARG, "n_sites must be in 1..32");
+    if (t->n_terms < 0 || t->n_ops < 0 || t->n_rows < 0 || t->n_entries < 0) return fail(c, NQH_E_ARG, "negative sizes");
+    // validation of the flat table
+    if (t->op_site_off[0] != 0 || t->op_row_off[0] != 0 || t->row_ent_off[0] != 0)
+        return fail(c, NQH_E_ARG, "offsets must start at 0");
+    if (t->op_row_off[t->n_ops] != t->n_rows || t->row_ent_off[t->n_rows] != t->n_entries)
+        return fail(c, NQH_E_ARG, "offset tables inconsistent with n_rows / n_entries");
+    const int nsites_total = t->op_site_off[t->n_ops];
+    for (int o = 0; o < t->n_ops; ++o) {
+        const int k = t->op_site_off[o + 1] - t->op_site_off[o];
+        if (k < 1 || k > 8) return fail(c, NQH_E_ARG, "local operators must act on 1..8 sites");
+        if (t->op_row_off[o + 1] - t->op_row_off[o] != (1 << k)) return fail(c, NQH_E_ARG, "operator must have 2^k rows");
+    }
+    for (int m = 0; m < nsites_total; ++m)
+        if (t->op_sites[m] < 0 || t->op_sites[m] >= N) return fail(c, NQH_E_ARG, "site index out of range");
+    for (int q = 0; q < t->n_rows; ++q)
+        if (t->row_ent_off[q + 1] < t->row_ent_off[q]) return fail(c, NQH_E_ARG, "row offsets not monotone");
+    const uint64_t lim = (N >= 64) ? ~0ull : ((1ull << N) - 1);
+    for (int e = 0; e < t->n_entries; ++e)
+        if (t->ent_flip[e] & ~lim) return fail(c, NQH_E_ARG, "flip mask beyond n_sites");
+    int maxc = 0;
+    for (int q = 0; q < t->n_terms; ++q) {
+        const int k = t->term_kind[q], ol = t->term_op_l[q], orr = t->term_op_r[q];
+        auto ok = [&](int o) { return o >= 0 && o < t->n_ops; };
+        auto maxrow = [&](int o) {
+            int m = 0;
+            for (int r = t->op_row_off[o]; r < t->op_row_off[o + 1]; ++r)
+                m = std::max(m, t->row_ent_off[r + 1] - t->row_ent_off[r]);
+            return m;
+        };
+        if (k == 0 || k == 3) { if (!ok(ol)) return fail(c, NQH_E_ARG, "bad row operator"); maxc += maxrow(ol); }
+        else if (k == 1) { if (!ok(orr)) return fail(c, NQH_E_ARG, "bad column operator"); maxc += maxrow(orr); }
+        else if (k == 2) {
+            if (!ok(ol) || !ok(orr)) return fail(c, NQH_E_ARG, "bad tensor operators");
+            maxc += maxrow(ol) * maxrow(orr);
+        } else return fail(c, NQH_E_ARG, "bad term kind");
+    }
+    // one device allocation: int32 arrays, then doubles, floats, flips
+    const size_t ni = (size_t)3 * t->n_terms + (t->n_ops + 1) + nsites_total + (t->n_ops + 1) + (t->n_rows + 1);
+    const size_t bytes_i = (ni * 4 + 15) / 16 * 16;
+    const size_t bytes_d = (size_t)2 * t->n_entries * sizeof(double);
+    const size_t bytes_f = ((size_t)2 * t->n_entries * sizeof(float) + 15) / 16 * 16;
+    const size_t bytes_m = (size_t)t->n_entries * sizeof(uint32_t);
+    std::vector<unsigned char> host(bytes_i + bytes_d + bytes_f + bytes_m + 16, 0);
+    int32_t* hi = reinterpret_cast<int32_t*>(host.data());
+    size_t off = 0;
+    auto put = [&](const int32_t* src, int n) { std::memcpy(hi + off, src, sizeof(int32_t) * n); size_t o = off; off += n; return o; };
+    const size_t o_kind = put(t->term_kind, t->n_terms);
+    const size_t o_l = put(t->term_op_l, t->n_terms);
+    const size_t o_r = put(t->term_op_r, t->n_terms);
+    const size_t o_so = put(t->op_site_off, t->n_ops + 1);
+    const size_t o_s = put(t->op_sites, nsites_total);
+    const size_t o_ro = put(t->op_row_off, t->n_ops + 1);
+    const size_t o_eo = put(t->row_ent_off, t->n_rows + 1);
+    double* hd = reinterpret_cast<double*>(host.data() + bytes_i);
+    std::memcpy(hd, t->ent_mel, bytes_d);
+    float* hf = reinterpret_cast<float*>(host.data() + bytes_i + bytes_d);
+    for (int e = 0; e < 2 * t->n_entries; ++e) hf[e] = (float)t->ent_mel[e];
+    uint32_t* hm = reinterpret_cast<uint32_t*>(host.data() + bytes_i + bytes_d + bytes_f);
+    for (int e = 0; e < t->n_entries; ++e) hm[e] = (uint32_t)t->ent_flip[e];
+    nqh_status s = ensure(c, c->table_mem, host.size());
+    if (s) return s;
+    HIPCHK(c, hipMemcpy(c->table_mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    unsigned char* base = reinterpret_cast<unsigned char*>(c->table_mem.p);
+    const int32_t* di = reinterpret_cast<const int32_t*>(base);
+    c->tab.n_sites = N;
+    c->tab.n_terms = t->n_terms;
+    c->tab.term_kind = di + o_kind;
+    c->tab.term_op_l = di + o_l;
+    c->tab.term_op_r = di + o_r;
+    c->tab.op_site_off = di + o_so;
+    c->tab.op_sites = di + o_s;
+    c->tab.op_row_off = di + o_ro;
+    c->tab.row_ent_off = di + o_eo;
+    c->tab.ent_mel = reinterpret_cast<const double*>(base + bytes_i);
+    c->tab.ent_mel_f = reinterpret_cast<const float*>(base + bytes_i + bytes_d);
+    c->tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
+    c->N_table = N;
+    c->max_conn = maxc;
+    c->has_problem = true;
+    return NQH_OK;
+}
+
+nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t act) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (N < 1 || N > 32) return fail(c, NQH_E_ARG, "N must be in 1..32");
+    if (nh < 1 || na < 1 || nh > 128 || na > 128) return fail(c, NQH_E_ARG, "nh, na must be in 1..128");
+    if (act != 0) return fail(c, NQH_E_ARG, "only the softplus activation (af_softplus) is implemented");
+    c->d = make_ndm_dims(N, nh, na);
+    nqh_status s;
+    if ((s = ensure(c, c->p64, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->p32, sizeof(float) * c->d.np))) return s;
+    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * c->d.np));
+    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * c->d.np));
+    c->has_net = true;
+    c->ns_cap = 0;
+    c->grad_ready = false;
+    return NQH_OK;
+}
+
+int64_t nqh_num_params(const nqh_ctx* c) { return (c && c->has_net) ? c->d.np : -1; }
+
+nqh_status nqh_set_params(nqh_ctx* c, const double* flat, int64_t np) {
+    if (!c || !flat) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = need_net(c);
+    if (s) return s;
+    if (np != c->d.np) return fail(c, NQH_E_ARG, "parameter count mismatch");
+    for (int64_t k = 0; k < np; ++k)
+        if (!std::isfinite(flat[k])) return fail(c, NQH_E_NAN, "non-finite parameter");
+    HIPCHK(c, hipMemcpyAsync(c->p64.p, flat, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, nq_launch_to_f32(P<double>(c->p64), P<float>(c->p32), np, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NQH_OK;
+}
+
+nqh_status nqh_get_params(const nqh_ctx* cc, double* flat, int64_t np) {
+    nqh_ctx* c = const_cast<nqh_ctx*>(cc);
+    if (!c || !flat) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = need_net(c);
+    if (s) return s;
+    if (np != c->d.np) return fail(c, NQH_E_ARG, "parameter count mismatch");
+    HIPCHK(c, hipMemcpyAsync(flat, c->p64.p, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NQH_OK;
+}
+
+nqh_status nqh_set_precision(nqh_ctx* c, int32_t net_prec, int32_t sbuild_prec) {
+    if (!c) return NQH_E_ARG;
+    if (net_prec != NQH_PREC_FP64 && net_prec != NQH_PREC_FP32) return fail(c, NQH_E_ARG, "net precision FP64 or FP32");
+    if (sbuild_prec < NQH_PREC_FP64 || sbuild_prec > NQH_PREC_BF16) return fail(c, NQH_E_ARG, "bad S-build precision");
+    if (net_prec != c->net_prec) {
+        c->ns_cap = 0;   // element size of the per-sample buffers changes
+        c->grad_ready = false;
+    }
+    c->net_prec = net_prec;
+    c->s_prec = sbuild_prec;
+    return NQH_OK;
+}
+
+static nqh_status eval_host_keys(nqh_ctx* c, const uint64_t* keys, int64_t B, bool want_y, bool conn,
+                                 bool want_g) {
+    nqh_status s;
+    if ((s = check_keys(c, keys, B))) return s;
+    if ((s = ensure_samples(c, B))) return s;
+    if ((s = upload_keys(c, keys, B))) return s;
+    est_args a = make_est(c, P<uint64_t>(c->keys_in), B);
+    a.lp = c->lp.p;
+    a.Y = want_y ? c->Y.p : nullptr;
+    a.do_conn = conn ? 1 : 0;
+    a.Lloc = conn ? c->Lloc.p : nullptr;
+    if (want_g) {
+        const size_t gb = (size_t)2 * sizeof(double) * c->d.np * B;
+        if ((s = ensure(c, c->G, gb))) return s;
+        HIPCHK(c, hipMemsetAsync(c->G.p, 0, gb, c->stream));
+        a.G = c->G.p;
+    }
+    HIPCHK(c, nq_launch_estimate(a, fp64_net(c), est_grid_for(B), c->stream));
+    return NQH_OK;
+}
+
+static nqh_status download_cplx(nqh_ctx* c, dbuf& b, double* dst, int64_t n2) {
+    if (fp64_net(c)) {
+        HIPCHK(c, hipMemcpyAsync(dst, b.p, sizeof(double) * n2, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(n2);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), b.p, sizeof(float) * n2, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        copy_cplx(tmp.data(), dst, n2);
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_logpsi(nqh_ctx* c, const uint64_t* keys, int64_t B, double* logpsi) {
+    if (!c || !keys || !logpsi || B < 0) return NQH_E_ARG;
+    if (B == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s = need_net(c);
+    if (s) return s;
+    if ((s = eval_host_keys(c, keys, B, false, false, false))) return s;
+    return download_cplx(c, c->lp, logpsi, 2 * B);
+}
+
+nqh_status nqh_logpsi_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, double* logpsi, double* O) {
+    if (!c || !keys || B < 0) return NQH_E_ARG;
+    if (B == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s = need_net(c);
+    if (s) return s;
+    if ((s = eval_host_keys(c, keys, B, true, false, false))) return s;
+    if (logpsi && (s = download_cplx(c, c->lp, logpsi, 2 * B))) return s;
+    if (O) {
+        const ndm_dims& d = c->d;
+        if (fp64_net(c)) {
+            std::vector<double> y((size_t)d.R * B);
+            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < B; ++b) expand_row(y.data() + (size_t)b * d.R, d, O + (size_t)2 * b * d.np);
+        } else {
+            std::vector<float> y((size_t)d.R * B);
+            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < B; ++b) expand_row(y.data() + (size_t)b * d.R, d, O + (size_t)2 * b * d.np);
+        }
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_connections(nqh_ctx* c, const uint64_t* keys, int64_t B, int32_t cap, int32_t* counts,
+                           uint64_t* keys_out, double* mel_out) {
+    if (!c || !keys || !counts || B < 0 || cap < 0) return NQH_E_ARG;
+    if (B == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s = need_problem(c);
+    if (s) return s;
+    const int N = c->N_table;
+    const uint64_t lim = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1);
+    for (int64_t b = 0; b < B; ++b)
+        if (keys[b] & ~lim) return fail(c, NQH_E_ARG, "configuration key has bits beyond 2N");
+    const int dcap = std::max(cap, 1);
+    dbuf kin, cnt, ko, mo;
+    auto cleanup = [&]() { release(kin); release(cnt); release(ko); release(mo); };
+    if ((s = ensure(c, kin, sizeof(uint64_t) * B)) || (s = ensure(c, cnt, sizeof(int) * B)) ||
+        (s = ensure(c, ko, sizeof(uint64_t) * B * dcap)) || (s = ensure(c, mo, 2 * sizeof(double) * B * dcap))) {
+        cleanup();
+        return s;
+    }
+    hipError_t e = hipMemcpy(kin.p, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = nq_launch_connections(c->tab, N, P<uint64_t>(kin), B, cap, P<int>(cnt), P<uint64_t>(ko), P<double>(mo),
+                                  c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(counts, cnt.p, sizeof(int) * B, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && keys_out && cap > 0) e = hipMemcpy(keys_out, ko.p, sizeof(uint64_t) * B * cap, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mel_out && cap > 0) e = hipMemcpy(mel_out, mo.p, 2 * sizeof(double) * B * cap, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(c, e, "nqh_connections");
+    for (int64_t b = 0; b < B; ++b)
+        if (counts[b] > cap) return fail(c, NQH_E_CAPACITY, "connection capacity too small");
+    return NQH_OK;
+}
+
+nqh_status nqh_local_estimator(nqh_ctx* c, const uint64_t* keys, int64_t B, double* Lloc, double* dLloc) {
+    if (!c || !keys || !Lloc || B < 0) return NQH_E_ARG;
+    if (B == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    if ((s = eval_host_keys(c, keys, B, false, true, dLloc != nullptr))) return s;
+    if ((s = download_cplx(c, c->Lloc, Lloc, 2 * B))) return s;
+    if (dLloc) {
+        HIPCHK(c, hipMemcpy(dLloc, c->G.p, sizeof(double) * 2 * B * c->d.np, hipMemcpyDeviceToHost));
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chains, int64_t Lc, double* stats,
+                                 double* S, double* F, double* Obar) {
+    if (!c || !keys || chains < 2 || Lc < 2) return fail(c, NQH_E_ARG, "need >= 2 chains and chain length >= 2");
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    const int64_t Ns = chains * Lc;
+    if ((s = check_keys(c, keys, Ns))) return s;
+    if ((s = ensure_samples(c, Ns))) return s;
+    if ((s = upload_keys(c, keys, Ns))) return s;
+    if ((s = run_grad_pipeline(c, P<uint64_t>(c->keys_in), chains, Lc))) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const ndm_dims& d = c->d;
+    if (stats) HIPCHK(c, hipMemcpy(stats, c->stats.p, sizeof(double) * 5, hipMemcpyDeviceToHost));
+    if (F) {
+        std::vector<double> f(d.np);
+        HIPCHK(c, hipMemcpy(f.data(), c->F.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
+        for (int k = 0; k < d.np; ++k) F[p2ref_host(k, d)] = f[k];
+    }
+    if (Obar) {
+        std::vector<double> mu(d.R);
+        HIPCHK(c, hipMemcpy(mu.data(), c->mu.p, sizeof(double) * d.R, hipMemcpyDeviceToHost));
+        expand_row(mu.data(), d, Obar);
+    }
+    if (S) {
+        std::vector<double> sp((size_t)d.np * d.np);
+        HIPCHK(c, hipMemcpy(sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+        std::vector<int> r(d.np);
+        for (int k = 0; k < d.np; ++k) r[k] = p2ref_host(k, d);
+        for (int l = 0; l < d.np; ++l)
+            for (int k = 0; k < d.np; ++k) S[(size_t)r[k] + (size_t)r[l] * d.np] = sp[(size_t)k + (size_t)l * d.np];
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_set_sampler(nqh_ctx* c, int64_t chains, int32_t chain_length, int32_t passes, int32_t burn,
+                           int32_t flags) {
+    if (!c) return NQH_E_ARG;
+    if (chains < 2 || chain_length < 2 || passes < 1 || burn < 0)
+        return fail(c, NQH_E_ARG, "need chains >= 2, chain_length >= 2, passes >= 1, burn >= 0");
+    if ((flags & NQH_SAMPLER_COMPAT_STALE_LAST_SLICE) && (flags & NQH_SAMPLER_PERSISTENT))
+        return fail(c, NQH_E_ARG, "compat and persistent sampler flags are exclusive");
+    (void)hipSetDevice(c->device);
+    c->chains = chains;
+    c->chain_len = chain_length;
+    c->passes = passes;
+    c->burn = burn;
+    c->flags = flags;
+    nqh_status s;
+    if ((s = ensure(c, c->chain_state, sizeof(uint64_t) * chains))) return s;
+    if ((s = ensure(c, c->samples, sizeof(uint64_t) * chains * chain_length))) return s;
+    if ((s = ensure(c, c->acc, sizeof(unsigned long long) * 2))) return s;
+    c->chains_initialised = false;
+    c->has_sampler = true;
+    return NQH_OK;
+}
+
+nqh_status nqh_set_iteration(nqh_ctx* c, uint32_t iter) {
+    if (!c) return NQH_E_ARG;
+    c->iter = iter;
+    return NQH_OK;
+}
+
+static nqh_status run_sampler(nqh_ctx* c) {
+    if (!c->has_sampler) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+    samp_args a{};
+    a.d = c->d;
+    a.p = fp64_net(c) ? c->p64.p : c->p32.p;
+    a.chains = c->chains;
+    a.passes = c->passes;
+    const bool compat = (c->flags & NQH_SAMPLER_COMPAT_STALE_LAST_SLICE) != 0;
+    const bool persistent = (c->flags & NQH_SAMPLER_PERSISTENT) != 0;
+    a.restart = (!persistent || !c->chains_initialised) ? 1 : 0;
+    a.burn = a.restart ? c->burn : 0;
+    a.stored = compat ? c->chain_len - 1 : c->chain_len;
+    a.compat_stale_last_slice = compat ? 1 : 0;
+    a.lpc = nq_sampler_lanes_per_chain(c->d);
+    a.seed = c->seed;
+    a.iter = c->iter;
+    a.chain_state = P<uint64_t>(c->chain_state);
+    a.samples = P<uint64_t>(c->samples);
+    a.accepted = P<unsigned long long>(c->acc);
+    HIPCHK(c, hipMemsetAsync(c->acc.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+    c->chains_initialised = true;
+    c->iter += 1;
+    return NQH_OK;
+}
+
+nqh_status nqh_sample(nqh_ctx* c, uint64_t* keys_out, int64_t* accepted) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c))) return s;
+    if ((s = run_sampler(c))) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (keys_out)
+        HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * c->chains * c->chain_len, hipMemcpyDeviceToHost));
+    if (accepted) {
+        unsigned long long v = 0;
+        HIPCHK(c, hipMemcpy(&v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
+        *accepted = (int64_t)v;
+    }
+    return NQH_OK;
+}
+
+static nqh_status sample_grad_async(nqh_ctx* c) {
+    nqh_status s;
+    if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    if (!c->has_sampler) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+    const int64_t Ns = c->chains * c->chain_len;
+    if ((s = ensure_samples(c, Ns))) return s;
+    if ((s = run_sampler(c))) return s;
+    return run_grad_pipeline(c, P<uint64_t>(c->samples), c->chains, c->chain_len);
+}
+
+nqh_status nqh_sample_grad(nqh_ctx* c, double* stats) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = sample_grad_async(c);
+    if (s) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double st[5];
+    HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    if (stats) std::memcpy(stats, st, sizeof(st));
+    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
+    return NQH_OK;
+}
+
+static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine) {
+    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
+    if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
+    if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
+    const int np = c->d.np;
+    double* S = P<double>(c->S);
+    if (refine > 0)
+        HIPCHK(c, hipMemcpyAsync(c->S0.p, S, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, nq_launch_shift(S, np, np, eps, kind, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->info.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, nq_cholesky(S, np, np, P<int>(c->info), c->stream));
+    HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->F), P<double>(c->delta), c->stream));
+    for (int it = 0; it < refine; ++it) {
+        HIPCHK(c, nq_residual(P<double>(c->S0), np, np, eps, kind, P<double>(c->delta), P<double>(c->F),
+                              P<double>(c->resid), c->stream));
+        HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->resid), P<double>(c->resid), c->stream));
+        HIPCHK(c, nq_launch_axpy(P<double>(c->delta), P<double>(c->resid), np, c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, nq_launch_check_finite(P<double>(c->delta), np, P<int>(c->flag), c->stream));
+    return NQH_OK;
+}
+
+static nqh_status check_solve(nqh_ctx* c) {
+    int info = 0, flag = 0;
+    HIPCHK(c, hipMemcpy(&info, c->info.p, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (info != 0) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "S + shift is not positive definite (column %d)", info);
+        return fail(c, NQH_E_NOTPD, buf);
+    }
+    if (flag) return fail(c, NQH_E_NAN, "non-finite SR update");
+    return NQH_OK;
+}
+
+nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double* delta_out) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = precondition_async(c, eps, shift_kind, refine);
+    if (s) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((s = check_solve(c))) return s;
+    if (delta_out) {
+        const ndm_dims& d = c->d;
+        std::vector<double> dv(d.np);
+        HIPCHK(c, hipMemcpy(dv.data(), c->delta.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
+        for (int k = 0; k < d.np; ++k) delta_out[p2ref_host(k, d)] = dv[k];
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_update(nqh_ctx* c, double lr) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no preconditioned gradient");
+    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NQH_OK;
+}
+
+nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = sample_grad_async(c);
+    if (s) return s;
+    if ((s = precondition_async(c, eps, shift_kind, refine))) return s;
+    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((s = check_solve(c))) return s;
+    double st[5];
+    HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    if (stats) std::memcpy(stats, st, sizeof(st));
+    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
+    return NQH_OK;
+}
+
+nqh_status nqh_get_samples(nqh_ctx* c, uint64_t* keys_out, int64_t n) {
+    if (!c || !keys_out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->has_sampler) return fail(c, NQH_E_STATE, "sampler not set");
+    if (n != c->chains * c->chain_len) return fail(c, NQH_E_ARG, "n must be chains * chain_length");
+    HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    return NQH_OK;
+}
+
+nqh_status nqh_get_local_values(nqh_ctx* c, double* lloc_out, int64_t n) {
+    if (!c || !lloc_out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no local values yet");
+    if (n != c->last_ns) return fail(c, NQH_E_ARG, "n must equal the number of samples of the last gradient");
+    return download_cplx(c, c->Lloc, lloc_out, 2 * n);
+}
+
+}  // extern "C"

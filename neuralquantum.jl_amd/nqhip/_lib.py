@@ -1,0 +1,261 @@
+"""ctypes binding of libnqhip.so (include/nqhip.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+the library is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnqhip.so")
+
+NQH_OK = 0
+NQH_E_ARG = -1
+NQH_E_HIP = -2
+NQH_E_NOTPD = -3
+NQH_E_NAN = -4
+NQH_E_STATE = -5
+NQH_E_NOMEM = -6
+NQH_E_CAPACITY = -7
+
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+SHIFT_ADD, SHIFT_DIAG = 0, 1
+SAMPLER_COMPAT_STALE_LAST_SLICE = 1
+SAMPLER_PERSISTENT = 2
+
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class NQHError(RuntimeError):
+    """A non-zero nqh_status; `.status` holds the code."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"nqhip error {status}: {msg}")
+        self.status = status
+
+
+class OpTable(C.Structure):
+    _fields_ = [("n_sites", C.c_int32), ("n_terms", C.c_int32), ("n_ops", C.c_int32),
+                ("n_rows", C.c_int32), ("n_entries", C.c_int32),
+                ("term_kind", _i32p), ("term_op_l", _i32p), ("term_op_r", _i32p),
+                ("op_site_off", _i32p), ("op_sites", _i32p), ("op_row_off", _i32p),
+                ("row_ent_off", _i32p), ("ent_mel", _dp), ("ent_flip", _u64p)]
+
+
+SIGNATURES = {
+    "nqh_abi_version": (C.c_int32, []),
+    "nqh_create": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.c_uint64]),
+    "nqh_destroy": (None, [C.c_void_p]),
+    "nqh_last_error": (C.c_char_p, [C.c_void_p]),
+    "nqh_set_problem": (C.c_int32, [C.c_void_p, C.POINTER(OpTable)]),
+    "nqh_set_ndm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "nqh_num_params": (C.c_int64, [C.c_void_p]),
+    "nqh_set_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
+    "nqh_get_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
+    "nqh_set_precision": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+    "nqh_logpsi": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, _dp]),
+    "nqh_logpsi_grad": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, _dp, _dp]),
+    "nqh_connections": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, C.c_int32, _i32p, _u64p, _dp]),
+    "nqh_local_estimator": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, _dp, _dp]),
+    "nqh_grad_from_samples": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
+    "nqh_set_sampler": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "nqh_set_iteration": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "nqh_sample": (C.c_int32, [C.c_void_p, _u64p, _i64p]),
+    "nqh_sample_grad": (C.c_int32, [C.c_void_p, _dp]),
+    "nqh_precondition": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, _dp]),
+    "nqh_update": (C.c_int32, [C.c_void_p, C.c_double]),
+    "nqh_sr_iteration": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double, _dp]),
+    "nqh_get_samples": (C.c_int32, [C.c_void_p, _u64p, C.c_int64]),
+    "nqh_get_local_values": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libnqhip.so (raises OSError if it is missing: no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built; run neuralquantum.jl_amd/build.py")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Table:
+    """ctypes view of a FlatOpTable (keeps the arrays alive)."""
+
+    def __init__(self, flat):
+        self.arrays = dict(
+            term_kind=np.ascontiguousarray(flat.term_kind, dtype=np.int32),
+            term_op_l=np.ascontiguousarray(flat.term_op_l, dtype=np.int32),
+            term_op_r=np.ascontiguousarray(flat.term_op_r, dtype=np.int32),
+            op_site_off=np.ascontiguousarray(flat.op_site_off, dtype=np.int32),
+            op_sites=np.ascontiguousarray(flat.op_sites, dtype=np.int32),
+            op_row_off=np.ascontiguousarray(flat.op_row_off, dtype=np.int32),
+            row_ent_off=np.ascontiguousarray(flat.row_ent_off, dtype=np.int32),
+            ent_mel=np.ascontiguousarray(np.asarray(flat.ent_mel, dtype=np.complex128).view(np.float64)),
+            ent_flip=np.ascontiguousarray(flat.ent_flip, dtype=np.uint64),
+        )
+        a = self.arrays
+        n_ops = len(a["op_site_off"]) - 1
+        n_rows = len(a["row_ent_off"]) - 1
+        self.s = OpTable(int(flat.n_sites), len(a["term_kind"]), n_ops, n_rows, len(a["ent_flip"]),
+                         _p(a["term_kind"], _i32p), _p(a["term_op_l"], _i32p), _p(a["term_op_r"], _i32p),
+                         _p(a["op_site_off"], _i32p), _p(a["op_sites"], _i32p), _p(a["op_row_off"], _i32p),
+                         _p(a["row_ent_off"], _i32p), _p(a["ent_mel"], _dp), _p(a["ent_flip"], _u64p))
+
+
+class Context:
+    """Owns one nqh_ctx (one device, one stream, one SR loop)."""
+
+    def __init__(self, device: int = 0, seed: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.nqh_create(C.byref(h), int(device), C.c_uint64(seed))
+        if st != NQH_OK:
+            raise NQHError(st, "nqh_create failed (no HIP device?)")
+        self.h = h
+        self.np_ = None
+        self.N = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nqh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st != NQH_OK:
+            raise NQHError(st, self.lib.nqh_last_error(self.h).decode())
+
+    # -- setup --------------------------------------------------------------
+    def set_problem(self, flat):
+        self._table = Table(flat)
+        self._chk(self.lib.nqh_set_problem(self.h, C.byref(self._table.s)))
+
+    def set_ndm(self, N, nh, na, act=0):
+        self._chk(self.lib.nqh_set_ndm(self.h, N, nh, na, act))
+        self.np_ = int(self.lib.nqh_num_params(self.h))
+        self.N = N
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(self.lib.nqh_set_params(self.h, _p(p, _dp), len(p)))
+
+    def get_params(self):
+        p = np.empty(self.np_)
+        self._chk(self.lib.nqh_get_params(self.h, _p(p, _dp), self.np_))
+        return p
+
+    def set_precision(self, net_prec=PREC_FP64, sbuild_prec=PREC_FP64):
+        self._chk(self.lib.nqh_set_precision(self.h, net_prec, sbuild_prec))
+
+    # -- parity entry points ------------------------------------------------
+    def logpsi(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.empty(2 * len(keys))
+        self._chk(self.lib.nqh_logpsi(self.h, _p(keys, _u64p), len(keys), _p(out, _dp)))
+        return out.view(np.complex128)
+
+    def logpsi_grad(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.empty(2 * len(keys))
+        O = np.empty(2 * len(keys) * self.np_)
+        self._chk(self.lib.nqh_logpsi_grad(self.h, _p(keys, _u64p), len(keys), _p(out, _dp), _p(O, _dp)))
+        return out.view(np.complex128), O.view(np.complex128).reshape(len(keys), self.np_)
+
+    def connections(self, keys, cap):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        B = len(keys)
+        cnt = np.empty(B, dtype=np.int32)
+        ko = np.empty(B * cap, dtype=np.uint64)
+        mo = np.empty(2 * B * cap)
+        self._chk(self.lib.nqh_connections(self.h, _p(keys, _u64p), B, cap, _p(cnt, _i32p),
+                                           _p(ko, _u64p), _p(mo, _dp)))
+        return cnt, ko.reshape(B, cap), mo.view(np.complex128).reshape(B, cap)
+
+    def local_estimator(self, keys, grad=False):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        B = len(keys)
+        L = np.empty(2 * B)
+        G = np.empty(2 * B * self.np_) if grad else None
+        self._chk(self.lib.nqh_local_estimator(self.h, _p(keys, _u64p), B, _p(L, _dp), _p(G, _dp)))
+        L = L.view(np.complex128)
+        if grad:
+            return L, G.view(np.complex128).reshape(B, self.np_)
+        return L
+
+    def grad_from_samples(self, keys, chains, chain_length, want_S=True):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        st = np.empty(5)
+        S = np.empty((self.np_, self.np_), order="F") if want_S else None
+        F = np.empty(self.np_)
+        Ob = np.empty(2 * self.np_)
+        self._chk(self.lib.nqh_grad_from_samples(self.h, _p(keys, _u64p), chains, chain_length,
+                                                 _p(st, _dp), _p(S, _dp), _p(F, _dp), _p(Ob, _dp)))
+        return st, S, F, Ob.view(np.complex128)
+
+    # -- loop -------------------------------------------------------------------
+    def set_sampler(self, chains, chain_length, passes, burn, flags=0):
+        self.chains, self.chain_length = int(chains), int(chain_length)
+        self._chk(self.lib.nqh_set_sampler(self.h, chains, chain_length, passes, burn, flags))
+
+    def set_iteration(self, it):
+        self._chk(self.lib.nqh_set_iteration(self.h, C.c_uint32(it)))
+
+    def sample(self):
+        keys = np.empty(self.chains * self.chain_length, dtype=np.uint64)
+        acc = C.c_int64()
+        self._chk(self.lib.nqh_sample(self.h, _p(keys, _u64p), C.byref(acc)))
+        return keys, int(acc.value)
+
+    def sample_grad(self):
+        st = np.empty(5)
+        self._chk(self.lib.nqh_sample_grad(self.h, _p(st, _dp)))
+        return st
+
+    def precondition(self, eps, shift_kind=SHIFT_ADD, refine=0, want_delta=True):
+        d = np.empty(self.np_) if want_delta else None
+        self._chk(self.lib.nqh_precondition(self.h, eps, shift_kind, refine, _p(d, _dp)))
+        return d
+
+    def update(self, lr):
+        self._chk(self.lib.nqh_update(self.h, lr))
+
+    def sr_iteration(self, eps, lr, shift_kind=SHIFT_ADD, refine=0):
+        st = np.empty(5)
+        self._chk(self.lib.nqh_sr_iteration(self.h, eps, shift_kind, refine, lr, _p(st, _dp)))
+        return st
+
+    def get_samples(self):
+        keys = np.empty(self.chains * self.chain_length, dtype=np.uint64)
+        self._chk(self.lib.nqh_get_samples(self.h, _p(keys, _u64p), len(keys)))
+        return keys
+
+    def get_local_values(self):
+        n = self.chains * self.chain_length
+        L = np.empty(2 * n)
+        self._chk(self.lib.nqh_get_local_values(self.h, _p(L, _dp), n))
+        return L.view(np.complex128)
