@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark of the steady-state VMC loop on one MI355X.
+
+Workload (the metric's configuration): dissipative XYZ model on a 4x4
+periodic lattice (Jx=0.9, Jy=1.0, Jz=1.1, jumps sqrt(gamma) sigma^- with
+gamma=1), NDM alpha_h = alpha_a = 2 (Np = 2176), parameters N(0, 0.01^2) from
+a fixed seed, 4096 chains x 4 samples per SR step (Ns = 16384), LocalRule
+Metropolis with 2N+1 = 33 passes, SR additive shift 0.01, lr 0.005.
+
+One "step" = one SR iteration: sampling, log rho / O / L_loc / force, loss
+statistics, S build, shifted Cholesky solve and the parameter update, all on
+the device.  Two forms are timed:
+  fp64    the plain form: FP64 network and S build, the reference's schedule
+          (chains restarted and burned in (50 calls) every iteration);
+  chosen  FP32 network kernels, BF16 S build on MFMA with FP32 accumulation,
+          FP64 solve, persistent chains (burn-in only once).
+`value` is the chosen form's SR iterations/s.
+
+Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per process,
+no collectives in this build).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neuralquantum.jl_amd"))
+sys.path.insert(0, ROOT)
+
+from nqhip import _lib, operators, problems  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+CFG = dict(lx=4, ly=4, alpha_h=2, alpha_a=2, chains=4096, chain_length=4, passes=33, burn=50,
+           eps=0.01, lr=0.005, sigma=1e-4, param_seed=1234, sampler_seed=4321)
+
+
+def hip_sync():
+    try:
+        lib = ctypes.CDLL("libamdhip64.so")
+        lib.hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def build_ctx(form: str, cfg=CFG):
+    _, _, _, L = problems.dissipative_xyz(cfg["lx"], cfg["ly"], Jx=0.9, Jy=1.0, Jz=1.1, gamma=1.0)
+    N = cfg["lx"] * cfg["ly"]
+    nh, na = cfg["alpha_h"] * N, cfg["alpha_a"] * N
+    ctx = _lib.Context(0, cfg["sampler_seed"])
+    ctx.set_problem(operators.flatten(L))
+    ctx.set_ndm(N, nh, na)
+    rng = np.random.default_rng(cfg["param_seed"])
+    p = rng.standard_normal(ctx.np_) * math.sqrt(cfg["sigma"])   # init_random_pars!(net; sigma=1e-4)
+    ctx.set_params(p)
+    if form == "fp64":
+        ctx.set_precision(_lib.PREC_FP64, _lib.PREC_FP64)
+        flags = 0
+    else:
+        ctx.set_precision(_lib.PREC_FP32, _lib.PREC_BF16)
+        flags = _lib.SAMPLER_PERSISTENT
+    ctx.set_sampler(cfg["chains"], cfg["chain_length"], cfg["passes"], cfg["burn"], flags)
+    return ctx, p
+
+
+def time_form(form, steps, warmup, cfg=CFG):
+    ctx, _ = build_ctx(form, cfg)
+    ctx.set_graph(True)          # the production path: one hipGraph replay per SR iteration
+    losses = []
+    for _ in range(warmup):
+        losses.append(ctx.sr_iteration(cfg["eps"], cfg["lr"])[0])
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        losses.append(ctx.sr_iteration(cfg["eps"], cfg["lr"])[0])
+    hip_sync()
+    dt = time.perf_counter() - t0
+    # a second, profiled pass of the same length for the per-phase breakdown
+    # (eager launches: HIP events around every phase on the library's stream)
+    ctx.set_profiling(True)
+    for _ in range(steps):
+        losses.append(ctx.sr_iteration(cfg["eps"], cfg["lr"])[0])
+    phases = ctx.phase_times()
+    ctx.set_profiling(False)
+    return ctx, dt, phases, losses
+
+
+def tau_int(series):
+    """Integrated autocorrelation time (Sokal automatic window, c = 5) of
+    per-chain series (chains x T), in units of stored samples; tau = 1/2 for
+    uncorrelated samples."""
+    x = series - series.mean(axis=1, keepdims=True)
+    T = x.shape[1]
+    var = np.mean(x * x)
+    if var == 0:
+        return 0.5
+    tau = 0.5
+    for t in range(1, T):
+        rho = np.mean(x[:, :-t] * x[:, t:]) / var
+        tau += rho
+        if t >= 5 * tau:
+            break
+    return max(tau, 0.5)
+
+
+def measure_tau(form, cfg=CFG, slices=48):
+    """tau_int of Re log p and |L_loc|^2 along the chains of the form's sampler
+    at the initial parameters."""
+    c2 = dict(cfg)
+    ctx, _ = build_ctx(form, c2)
+    chains = 1024
+    flags = 0
+    ctx.set_sampler(chains, slices, cfg["passes"], cfg["burn"], flags)
+    keys, _ = ctx.sample()
+    lp = ctx.logpsi(keys).real.reshape(slices, chains).T
+    L = ctx.local_estimator(keys).reshape(slices, chains).T
+    ctx.close()
+    return tau_int(2 * lp), tau_int(np.abs(L) ** 2)
+
+
+def roofline_for(phases, steps, form, d):
+    """Algorithmic figure of the dominant phase / its measured mean duration."""
+    N, nh, na = d["N"], d["nh"], d["na"]
+    np_ = 2 * N + 2 * nh + na + 2 * nh * N + 2 * na * N
+    R = (N + nh + nh * N) * 2 + 2 * (na + 2 * na * N)
+    Ns = d["Ns"]
+    elt = 8 if form == "fp64" else 4
+    per = {k: v[0] / max(v[1], 1) for k, v in phases.items()}   # ms per launch sequence
+    dom = max(per, key=per.get)
+    out = {"phase_ms": {k: round(v, 4) for k, v in per.items()}, "dominant": dom}
+    if dom == "sbuild":
+        flops = 4.0 * np_ * np_ * Ns
+        ach = flops / (per["sbuild"] * 1e-3) / 1e12
+        peak = PEAK_TFLOPS["fp64" if form == "fp64" else "bf16"]
+        out.update(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                   algorithmic=f"4 Np^2 Ns = {flops:.4g} flop per S build")
+    else:
+        # estimator-style HBM figure: keys in, log rho out, compact O rows out
+        byts = Ns * (2 * N / 8 + 2 * elt + R * elt)
+        ph = dom
+        ach = byts / (per[ph] * 1e-3) / 1e9
+        out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                   algorithmic=f"Ns (2N/8 + 2p + R p) = {byts:.4g} B")
+    return out
+
+
+def cpu_baseline(budget_s=20.0):
+    """The oracle in reference-structured mode (full network recompute per
+    proposal, full Np gradient per connection, complex S = Oc Oc^H, LAPACK
+    Cholesky) on a bounded sample of one SR iteration, scaled to the full
+    workload; one thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:   # pragma: no cover
+        threadpool_limits = None
+    from oracle import nqref, ops_ref
+    ref = nqref.Ref()
+    cfg = CFG
+    N = cfg["lx"] * cfg["ly"]
+    nh, na = cfg["alpha_h"] * N, cfg["alpha_a"] * N
+    np_ = ref.ndm_np(N, nh, na)
+    tab = nqref.Table(ops_ref.table_from_rules(ops_ref.xyz_model(cfg["lx"], cfg["ly"])))
+    p = np.random.default_rng(cfg["param_seed"]).standard_normal(np_) * math.sqrt(cfg["sigma"])
+    chains, Lc, Ns = cfg["chains"], cfg["chain_length"], cfg["chains"] * cfg["chain_length"]
+    ctxm = threadpool_limits(limits=1) if threadpool_limits else None
+    try:
+        t0 = time.perf_counter()
+        ch_s = 48
+        keys, _ = ref.metropolis(N, nh, na, p, ch_s, Lc, cfg["passes"], cfg["burn"], cfg["sampler_seed"],
+                                 compat=True)
+        t_samp = (time.perf_counter() - t0) * chains / ch_s
+        nsamp = 192
+        ks = np.random.default_rng(1).integers(0, 2 ** (2 * N), size=nsamp, dtype=np.uint64)
+        t0 = time.perf_counter()
+        ref.logpsi_grad(N, nh, na, p, ks)
+        ref.local_estimator(tab, nh, na, p, ks, grad=True)
+        t_est = (time.perf_counter() - t0) * Ns / nsamp
+        ns_s = 4096
+        X = (np.random.default_rng(2).standard_normal((np_, ns_s)) +
+             1j * np.random.default_rng(3).standard_normal((np_, ns_s)))
+        t0 = time.perf_counter()
+        Sc = X @ X.conj().T
+        t_s = (time.perf_counter() - t0) * Ns / ns_s
+        S = Sc.real / ns_s + 0.01 * np.eye(np_)
+        t0 = time.perf_counter()
+        nqref.sr_solve(S, np.ones(np_), 0.0)
+        t_chol = time.perf_counter() - t0
+    finally:
+        if ctxm is not None:
+            ctxm.__exit__(None, None, None)
+    total = t_samp + t_est + t_s + t_chol
+    return {"value": 1.0 / total, "unit": "SR iters/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle/nqref.c reference-structured, 1 thread: Metropolis on {ch_s} of {chains} "
+                       f"chains, estimator with full per-connection gradients on {nsamp} of {Ns} samples, "
+                       f"complex S = Oc Oc^H (BLAS) on {ns_s} of {Ns} samples, full Cholesky; each part "
+                       f"scaled linearly to the full iteration"),
+            "seconds_per_iteration": {"sampler": t_samp, "estimator": t_est, "sbuild": t_s, "solve": t_chol}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tau", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU per process and has no multi-GPU path; "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    N = CFG["lx"] * CFG["ly"]
+    d = dict(N=N, nh=CFG["alpha_h"] * N, na=CFG["alpha_a"] * N, Ns=CFG["chains"] * CFG["chain_length"])
+    res = {}
+    for form in ("fp64", "chosen"):
+        ctx, dt, phases, losses = time_form(form, args.steps, args.warmup)
+        res[form] = dict(it_s=args.steps / dt, ms=1e3 * dt / args.steps, phases=phases,
+                         loss_last=float(losses[-1]), roof=roofline_for(phases, args.steps, form, d))
+        ctx.close()
+    taus = {}
+    if not args.no_tau:
+        for form in ("fp64", "chosen"):
+            taus[form] = measure_tau(form)
+    ch = res["chosen"]
+    out = {
+        "metric": "SR iters/s + eff. samples/s, 4x4 NDM, mixed vs FP64 GPU at equal <L^dag L>",
+        "value": round(ch["it_s"], 3),
+        "unit": "SR iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ch["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (network), bf16->fp32 (S build), fp64 (solve)",
+        "data": "synthetic: random-init NDM, samples drawn by the sampler itself",
+        "config": {"workload": "dissipative XYZ 4x4 periodic, NDM alpha_h=alpha_a=2 (Np=2176), "
+                               "4096 chains x 4 samples, 33 passes, SR shift 0.01, lr 0.005",
+                   "forms": {"fp64": "FP64 network + FP64 S build, restart + 50 burn-in calls per iteration",
+                             "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"}},
+        "fp64_form": {"it_s": round(res["fp64"]["it_s"], 3), "ms_per_step": round(res["fp64"]["ms"], 4),
+                      "phase_ms": res["fp64"]["roof"]["phase_ms"]},
+        "chosen_form": {"phase_ms": ch["roof"]["phase_ms"]},
+        "speedup_chosen_vs_fp64": round(ch["it_s"] / res["fp64"]["it_s"], 3),
+    }
+    if taus:
+        Ns = d["Ns"]
+        out["tau_int"] = {k: {"logp": round(v[0], 3), "abs2_Lloc": round(v[1], 3)} for k, v in taus.items()}
+        out["eff_samples_per_s"] = {
+            k: round(Ns / (2 * max(taus[k])) * res[k]["it_s"], 1) for k in ("fp64", "chosen")}
+    r = ch["roof"]
+    out["roofline"] = {"kernel": r["dominant"], "bound": r["bound"], "achieved": round(r["achieved"], 3),
+                       "peak": r["peak"], "unit": r["unit"], "frac": round(r["frac"], 4), "traffic": None,
+                       "algorithmic": r["algorithmic"]}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -175,6 +175,22 @@ nqh_status nqh_update(nqh_ctx* ctx, double lr);
 nqh_status nqh_sr_iteration(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr,
                             double* stats);
 
+/* device time per phase, measured with HIP events on the context's stream
+ * while profiling is on (accumulated, in ms; counts = timed occurrences):
+ * [0] sampler  [1] estimator (log rho, O, L_loc, force)  [2] means, statistics,
+ * force finish  [3] S build  [4] shift + Cholesky + solves  [5] update.
+ * No reference counterpart (the reference has no timers). */
+#define NQH_NUM_PHASES 6
+nqh_status nqh_set_profiling(nqh_ctx* ctx, int32_t on);
+nqh_status nqh_get_phase_times(nqh_ctx* ctx, double* ms_out, int64_t* counts_out);
+
+/* capture nqh_sr_iteration into a hipGraph and replay it (on = 1); the graph
+ * is rebuilt when eps, the shift kind, refine or lr change, and dropped by
+ * every setter.  With persistent chains the restart iteration runs eagerly.
+ * No reference counterpart (execution strategy of sample! + precondition! +
+ * update!). */
+nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
+
 /* last samples (chain_length x chains keys) / local values (2 Ns doubles) */
 nqh_status nqh_get_samples(nqh_ctx* ctx, uint64_t* keys_out, int64_t n);
 nqh_status nqh_get_local_values(nqh_ctx* ctx, double* lloc_out, int64_t n);

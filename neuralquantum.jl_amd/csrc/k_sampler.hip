@@ -46,12 +46,13 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     const bool live = chain < a.chains;
     const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t iter = a.iter_dev ? *a.iter_dev : a.iter;
     const T half = T(0.5);
 
     uint64_t key = 0;
     if (live) {
         if (a.restart) {
-            philox_out o = philox4x32_10(0u, a.iter, (uint32_t)chain, 0u, k0, k1);
+            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, 0u, k0, k1);
             const uint64_t m2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1ull);
             key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
         } else {
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
         for (int q = 0; q < a.passes; ++q, ++t) {
             const int slot = (int)(t % (uint32_t)lpc);
             if (slot == 0) {
-                philox_out o = philox4x32_10(t + (uint32_t)g, a.iter, (uint32_t)chain, 1u, k0, k1);
+                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, 1u, k0, k1);
                 rx0 = o.x[0]; rx1 = o.x[1]; rx2 = o.x[2];
             }
             const uint32_t x0 = __shfl(rx0, gbase + slot, 64);

@@ -127,15 +127,17 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
         __syncthreads();
     }
 
-    // D[i][j]: i = 4 (lane / 16) + v, j = lane % 16 within each 16x16 tile
-    const int lj = lane & 15, li4 = (lane >> 4) * 4;
+    // D[i][j] of a 16x16 tile, j = lane % 16:
+    //   f32 / bf16: i = 4 (lane / 16) + v;   f64: i = lane / 16 + 4 v
+    const int lj = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int n = 0; n < 2; ++n)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = r0 + wr * 32 + m * 16 + li4 + v;
+                const int ti = (MODE == NQ_GRAM_F64) ? lg + 4 * v : 4 * lg + v;
+                const int r = r0 + wr * 32 + m * 16 + ti;
                 const int c = q0 + wc * 32 + n * 16 + lj;
                 if (r < a.n && c < a.n) {
                     const double val = a.scale * (double)acc[m][n][v];

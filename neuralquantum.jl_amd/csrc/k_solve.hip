@@ -19,6 +19,15 @@ __global__ void k_fill(double* x, int64_t n, double v) {
     if (i < n) x[i] = v;
 }
 
+__global__ void k_fill_u32(uint32_t* x, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+__global__ void k_bump(uint32_t* x) {
+    if (threadIdx.x == 0) x[0] += 1u;
+}
+
 template <typename T>
 __device__ __forceinline__ double abs2_at(const T* L, int64_t s) {
     const double re = (double)L[2 * s], im = (double)L[2 * s + 1];
@@ -198,14 +207,15 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    const int lj = lane & 15, li4 = (lane >> 4) * 4;
+    // f64 MFMA output: D[lane / 16 + 4 v][lane % 16]
+    const int lj = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = i0 + wr * 32 + a * 16 + li4 + v;
+                const int r = i0 + wr * 32 + a * 16 + lg + 4 * v;
                 const int c = j0 + wc * 32 + b * 16 + lj;
                 if (r < n && c < n && r >= c) A[(int64_t)r + (int64_t)c * lda] -= acc[a][b][v];
             }
@@ -326,6 +336,17 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 hipError_t nq_launch_fill(double* x, int64_t n, double v, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_bump(uint32_t* x, hipStream_t st) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, x);
     return hipGetLastError();
 }
 

@@ -37,6 +37,7 @@ struct samp_args {
     int compat_stale_last_slice;
     uint64_t seed;
     uint32_t iter;
+    const uint32_t* iter_dev;  // when set, the iteration index is read here (graph replays)
     uint64_t* chain_state;     // chains, in/out (nullable when restart)
     uint64_t* samples;         // (stored [+1]) x chains, chain fastest
     unsigned long long* accepted;
@@ -70,6 +71,10 @@ hipError_t nq_launch_colsum_finish(const double* part, int chunks, int R, double
 
 // ---- small kernels and the solve (k_solve.hip)
 hipError_t nq_launch_fill(double* x, int64_t n, double v, hipStream_t st);
+// 32-bit fill (counters and flags; used instead of memset so the sequence can be graph-captured)
+hipError_t nq_launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t st);
+// *x += 1 (the iteration counter of a captured iteration)
+hipError_t nq_launch_bump(uint32_t* x, hipStream_t st);
 // stats of |L_loc|^2 over B chains x L slices (stats.jl:52-77): out[5]
 hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
 // F[k] = sum_b Fpart[b][k] - C Re(Obar_k), P order; C = stats[0]

@@ -58,6 +58,22 @@ struct nqh_ctx {
     dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag;
     int64_t last_ns = 0;
     bool grad_ready = false;
+
+    // graph capture of the SR iteration
+    bool use_graph = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double g_eps = 0.0, g_lr = 0.0;
+    int g_kind = -1, g_refine = -1;
+    dbuf iter_dev;
+
+    // per-phase HIP-event timing
+    bool profiling = false;
+    hipEvent_t ev[NQH_NUM_PHASES][2] = {};
+    bool pending[NQH_NUM_PHASES] = {};
+    double phase_ms[NQH_NUM_PHASES] = {};
+    int64_t phase_n[NQH_NUM_PHASES] = {};
 };
 
 namespace {
@@ -78,6 +94,31 @@ nqh_status hip_fail(nqh_ctx* c, hipError_t e, const char* where) {
         hipError_t _e = (expr);                            \
         if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
     } while (0)
+
+enum { PH_SAMPLER = 0, PH_ESTIMATE, PH_POST, PH_SBUILD, PH_SOLVE, PH_UPDATE };
+
+void phase_begin(nqh_ctx* c, int ph) {
+    if (c->profiling) (void)hipEventRecord(c->ev[ph][0], c->stream);
+}
+void phase_end(nqh_ctx* c, int ph) {
+    if (c->profiling) {
+        (void)hipEventRecord(c->ev[ph][1], c->stream);
+        c->pending[ph] = true;
+    }
+}
+// after a stream synchronisation: fold the recorded phases into the totals
+void phase_collect(nqh_ctx* c) {
+    if (!c->profiling) return;
+    for (int ph = 0; ph < NQH_NUM_PHASES; ++ph) {
+        if (!c->pending[ph]) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[ph][0], c->ev[ph][1]) == hipSuccess) {
+            c->phase_ms[ph] += ms;
+            c->phase_n[ph] += 1;
+        }
+        c->pending[ph] = false;
+    }
+}
 
 nqh_status ensure(nqh_ctx* c, dbuf& b, size_t bytes) {
     if (b.bytes >= bytes && b.p) return NQH_OK;
@@ -190,16 +231,21 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     a.Y = c->Y.p;
     a.Lloc = c->Lloc.p;
     a.Fpart = P<double>(c->Fpart);
+    phase_begin(c, PH_ESTIMATE);
     HIPCHK(c, nq_launch_estimate(a, f64, c->est_grid, c->stream));
+    phase_end(c, PH_ESTIMATE);
+    phase_begin(c, PH_POST);
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
     HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, c->d.R, 1.0 / (double)Ns,
                                       P<double>(c->mu), c->stream));
     HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
     HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->mu), P<double>(c->stats),
                                      c->d, P<double>(c->F), c->stream));
+    phase_end(c, PH_POST);
     // S in P order: A-gram at [0, nA), B-gram at [nlam, np)
     const ndm_dims& d = c->d;
     const int np = d.np;
+    phase_begin(c, PH_SBUILD);
     HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)np * np, 0.0, c->stream));
     const int mode = c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : (c->s_prec == NQH_PREC_FP32 ? NQ_GRAM_F32 : NQ_GRAM_BF16);
     gram_args g{};
@@ -219,6 +265,7 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     g.n = d.npi + d.nmu;
     g.o = d.nlam;
     HIPCHK(c, nq_launch_gram(g, mode, c->stream));
+    phase_end(c, PH_SBUILD);
     c->last_ns = Ns;
     c->grad_ready = true;
     return NQH_OK;
@@ -263,6 +310,8 @@ void copy_cplx(const T* src, double* dst, int64_t n2) {
 
 extern "C" {
 
+static void drop_graph(nqh_ctx* c);
+
 int32_t nqh_abi_version(void) { return NQH_ABI_VERSION; }
 
 nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
@@ -290,8 +339,13 @@ void nqh_destroy(nqh_ctx* c) {
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
                    &c->delta, &c->resid, &c->info, &c->flag};
+    drop_graph(c);
+    release(c->iter_dev);
     for (dbuf* b : all) release(*b);
-    hipStreamDestroy(c->stream);
+    for (int ph = 0; ph < NQH_NUM_PHASES; ++ph)
+        for (int k = 0; k < 2; ++k)
+            if (c->ev[ph][k]) (void)hipEventDestroy(c->ev[ph][k]);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -432,6 +486,7 @@ nqh_status nqh_get_params(const nqh_ctx* cc, double* flat, int64_t np) {
 
 nqh_status nqh_set_precision(nqh_ctx* c, int32_t net_prec, int32_t sbuild_prec) {
     if (!c) return NQH_E_ARG;
+    drop_graph(c);
     if (net_prec != NQH_PREC_FP64 && net_prec != NQH_PREC_FP32) return fail(c, NQH_E_ARG, "net precision FP64 or FP32");
     if (sbuild_prec < NQH_PREC_FP64 || sbuild_prec > NQH_PREC_BF16) return fail(c, NQH_E_ARG, "bad S-build precision");
     if (net_prec != c->net_prec) {
@@ -570,6 +625,7 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
     if ((s = upload_keys(c, keys, Ns))) return s;
     if ((s = run_grad_pipeline(c, P<uint64_t>(c->keys_in), chains, Lc))) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     const ndm_dims& d = c->d;
     if (stats) HIPCHK(c, hipMemcpy(stats, c->stats.p, sizeof(double) * 5, hipMemcpyDeviceToHost));
     if (F) {
@@ -612,11 +668,13 @@ nqh_status nqh_set_sampler(nqh_ctx* c, int64_t chains, int32_t chain_length, int
     if ((s = ensure(c, c->acc, sizeof(unsigned long long) * 2))) return s;
     c->chains_initialised = false;
     c->has_sampler = true;
+    drop_graph(c);
     return NQH_OK;
 }
 
 nqh_status nqh_set_iteration(nqh_ctx* c, uint32_t iter) {
     if (!c) return NQH_E_ARG;
+    drop_graph(c);
     c->iter = iter;
     return NQH_OK;
 }
@@ -640,8 +698,11 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.chain_state = P<uint64_t>(c->chain_state);
     a.samples = P<uint64_t>(c->samples);
     a.accepted = P<unsigned long long>(c->acc);
-    HIPCHK(c, hipMemsetAsync(c->acc.p, 0, sizeof(unsigned long long), c->stream));
+    a.iter_dev = c->use_graph ? P<uint32_t>(c->iter_dev) : nullptr;
+    phase_begin(c, PH_SAMPLER);
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 2, 0u, c->stream));
     HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+    phase_end(c, PH_SAMPLER);
     c->chains_initialised = true;
     c->iter += 1;
     return NQH_OK;
@@ -654,6 +715,7 @@ nqh_status nqh_sample(nqh_ctx* c, uint64_t* keys_out, int64_t* accepted) {
     if ((s = need_net(c))) return s;
     if ((s = run_sampler(c))) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     if (keys_out)
         HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * c->chains * c->chain_len, hipMemcpyDeviceToHost));
     if (accepted) {
@@ -680,6 +742,7 @@ nqh_status nqh_sample_grad(nqh_ctx* c, double* stats) {
     nqh_status s = sample_grad_async(c);
     if (s) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     double st[5];
     HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
     if (stats) std::memcpy(stats, st, sizeof(st));
@@ -693,10 +756,11 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
     const int np = c->d.np;
     double* S = P<double>(c->S);
+    phase_begin(c, PH_SOLVE);
     if (refine > 0)
         HIPCHK(c, hipMemcpyAsync(c->S0.p, S, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, nq_launch_shift(S, np, np, eps, kind, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->info.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, c->stream));
     HIPCHK(c, nq_cholesky(S, np, np, P<int>(c->info), c->stream));
     HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->F), P<double>(c->delta), c->stream));
     for (int it = 0; it < refine; ++it) {
@@ -705,8 +769,9 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
         HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->resid), P<double>(c->resid), c->stream));
         HIPCHK(c, nq_launch_axpy(P<double>(c->delta), P<double>(c->resid), np, c->stream));
     }
-    HIPCHK(c, hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, c->stream));
     HIPCHK(c, nq_launch_check_finite(P<double>(c->delta), np, P<int>(c->flag), c->stream));
+    phase_end(c, PH_SOLVE);
     return NQH_OK;
 }
 
@@ -729,6 +794,7 @@ nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
     nqh_status s = precondition_async(c, eps, shift_kind, refine);
     if (s) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     if ((s = check_solve(c))) return s;
     if (delta_out) {
         const ndm_dims& d = c->d;
@@ -743,19 +809,110 @@ nqh_status nqh_update(nqh_ctx* c, double lr) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no preconditioned gradient");
+    phase_begin(c, PH_UPDATE);
     HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    phase_end(c, PH_UPDATE);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
+    return NQH_OK;
+}
+
+static void drop_graph(nqh_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_valid = false;
+}
+
+// enqueue one whole SR iteration (no host synchronisation inside)
+static nqh_status enqueue_iteration(nqh_ctx* c, double eps, int kind, int refine, double lr) {
+    nqh_status s = sample_grad_async(c);
+    if (s) return s;
+    if ((s = precondition_async(c, eps, kind, refine))) return s;
+    phase_begin(c, PH_UPDATE);
+    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    phase_end(c, PH_UPDATE);
+    return NQH_OK;
+}
+
+static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, double lr) {
+    const bool persistent = (c->flags & NQH_SAMPLER_PERSISTENT) != 0;
+    if (persistent && !c->chains_initialised) {
+        // the restart + burn-in iteration runs eagerly; later ones replay a graph
+        drop_graph(c);
+        return enqueue_iteration(c, eps, kind, refine, lr);
+    }
+    if (!c->graph_valid || c->g_eps != eps || c->g_kind != kind || c->g_refine != refine || c->g_lr != lr) {
+        drop_graph(c);
+        nqh_status s;
+        const int64_t Ns = c->chains * c->chain_len;
+        if ((s = ensure_samples(c, Ns))) return s;
+        if ((s = ensure(c, c->iter_dev, sizeof(uint32_t) * 4))) return s;
+        HIPCHK(c, hipMemcpyAsync(c->iter_dev.p, &c->iter, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const bool prof = c->profiling;
+        c->profiling = false;
+        const uint32_t iter0 = c->iter;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        s = enqueue_iteration(c, eps, kind, refine, lr);
+        hipError_t e1 = nq_launch_bump(P<uint32_t>(c->iter_dev), c->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        c->profiling = prof;
+        c->iter = iter0;          // the capture did not run anything
+        if (s) { if (g) (void)hipGraphDestroy(g); return s; }
+        if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(c, e1, "bump"); }
+        if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->graph_valid = true;
+        c->g_eps = eps; c->g_kind = kind; c->g_refine = refine; c->g_lr = lr;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->iter += 1;
+    c->chains_initialised = true;
+    c->grad_ready = true;
+    c->last_ns = c->chains * c->chain_len;
+    return NQH_OK;
+}
+
+nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    c->use_graph = on != 0;
     return NQH_OK;
 }
 
 nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
+    if (c->use_graph && !c->profiling) {
+        nqh_status s0 = need_net(c);
+        if (!s0) s0 = need_problem(c);
+        if (!s0 && !c->has_sampler) s0 = fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+        if (!s0 && (shift_kind != NQH_SHIFT_ADD && shift_kind != NQH_SHIFT_DIAG)) s0 = fail(c, NQH_E_ARG, "bad shift kind");
+        if (!s0 && (!(eps >= 0.0) || refine < 0 || refine > 8)) s0 = fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
+        if (s0) return s0;
+        nqh_status sg = graph_iteration(c, eps, shift_kind, refine, lr);
+        if (sg) return sg;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((sg = check_solve(c))) return sg;
+        double st[5];
+        HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+        if (stats) std::memcpy(stats, st, sizeof(st));
+        if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
+        return NQH_OK;
+    }
     nqh_status s = sample_grad_async(c);
     if (s) return s;
     if ((s = precondition_async(c, eps, shift_kind, refine))) return s;
+    phase_begin(c, PH_UPDATE);
     HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    phase_end(c, PH_UPDATE);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     if ((s = check_solve(c))) return s;
     double st[5];
     HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
@@ -770,6 +927,31 @@ nqh_status nqh_get_samples(nqh_ctx* c, uint64_t* keys_out, int64_t n) {
     if (!c->has_sampler) return fail(c, NQH_E_STATE, "sampler not set");
     if (n != c->chains * c->chain_len) return fail(c, NQH_E_ARG, "n must be chains * chain_length");
     HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    return NQH_OK;
+}
+
+nqh_status nqh_set_profiling(nqh_ctx* c, int32_t on) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (on && !c->ev[0][0]) {
+        for (int ph = 0; ph < NQH_NUM_PHASES; ++ph)
+            for (int k = 0; k < 2; ++k) HIPCHK(c, hipEventCreate(&c->ev[ph][k]));
+    }
+    c->profiling = on != 0;
+    for (int ph = 0; ph < NQH_NUM_PHASES; ++ph) {
+        c->phase_ms[ph] = 0.0;
+        c->phase_n[ph] = 0;
+        c->pending[ph] = false;
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_get_phase_times(nqh_ctx* c, double* ms_out, int64_t* counts_out) {
+    if (!c || !ms_out) return NQH_E_ARG;
+    for (int ph = 0; ph < NQH_NUM_PHASES; ++ph) {
+        ms_out[ph] = c->phase_ms[ph];
+        if (counts_out) counts_out[ph] = c->phase_n[ph];
+    }
     return NQH_OK;
 }
 

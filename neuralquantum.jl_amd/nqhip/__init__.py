@@ -6,5 +6,9 @@ BatchedSampler driver) on top of libnqhip.so, the C ABI of include/nqhip.h.
 """
 from . import hilbert, operators, problems  # noqa: F401
 from ._lib import Context, NQHError, load  # noqa: F401
+from .vmc import (NDM, SR, BatchedSampler, Descent, HIPBatchedSampler, LocalRule,  # noqa: F401
+                  Measurement, MetropolisSampler, init_random_pars_, precondition_, sample_, update_)
 
-__all__ = ["hilbert", "operators", "problems", "Context", "NQHError", "load"]
+__all__ = ["hilbert", "operators", "problems", "Context", "NQHError", "load", "NDM", "SR",
+           "BatchedSampler", "Descent", "HIPBatchedSampler", "LocalRule", "Measurement",
+           "MetropolisSampler", "init_random_pars_", "precondition_", "sample_", "update_"]

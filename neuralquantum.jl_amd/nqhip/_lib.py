@@ -74,7 +74,13 @@ SIGNATURES = {
     "nqh_sr_iteration": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double, _dp]),
     "nqh_get_samples": (C.c_int32, [C.c_void_p, _u64p, C.c_int64]),
     "nqh_get_local_values": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
+    "nqh_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "nqh_get_phase_times": (C.c_int32, [C.c_void_p, _dp, _i64p]),
+    "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
 }
+
+NUM_PHASES = 6
+PHASES = ("sampler", "estimator", "post", "sbuild", "solve", "update")
 
 _lib = None
 
@@ -253,6 +259,18 @@ class Context:
         keys = np.empty(self.chains * self.chain_length, dtype=np.uint64)
         self._chk(self.lib.nqh_get_samples(self.h, _p(keys, _u64p), len(keys)))
         return keys
+
+    def set_graph(self, on=True):
+        self._chk(self.lib.nqh_set_graph(self.h, 1 if on else 0))
+
+    def set_profiling(self, on=True):
+        self._chk(self.lib.nqh_set_profiling(self.h, 1 if on else 0))
+
+    def phase_times(self):
+        ms = np.zeros(NUM_PHASES)
+        n = np.zeros(NUM_PHASES, dtype=np.int64)
+        self._chk(self.lib.nqh_get_phase_times(self.h, _p(ms, _dp), _p(n, _i64p)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(PHASES)}
 
     def get_local_values(self):
         n = self.chains * self.chain_length

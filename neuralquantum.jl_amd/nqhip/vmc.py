@@ -1,0 +1,246 @@
+"""Host-side mirror of the reference's steady-state VMC API on libnqhip.
+
+Julia (reference)                                   here
+-------------------------------------------------   ---------------------------------------------
+NDM(Float64, N, αh, αa, af_softplus)  NDM.jl:50     NDM(N, alpha_h, alpha_a)
+init_random_pars!(net; sigma)          rng.jl:20    net.init_random_pars(rng, sigma)
+MetropolisSampler(LocalRule(), L, passes; burn,     MetropolisSampler(LocalRule(), L, passes,
+                  seed)                Metropolis.jl:26                  burn=, seed=)
+SR(; ϵ, algorithm=sr_cholesky,                       SR(eps=, precondition_type="shift")
+   precondition_type=sr_shift)         SR.jl:50-57
+Optimisers.Descent(η)                  rules.jl:11   Descent(eta)
+BatchedSampler(net, sampl, liouv, algo;              BatchedSampler(net, sampl, liouv, algo,
+               batch_sz)               build_Batched.jl:1-14        batch_sz=)
+sample!(is) -> (L_stat, prec)          BatchedGradSampler.jl:77     sample_(is)
+precondition!(prec, algo, i) -> grad   SRDirect.jl:51-90            precondition_(prec, algo, i)
+Optimisers.update!(opt, net, grad)     apply.jl:9                   update_(opt, net, grad)
+
+The loss statistics come back as `Measurement` (stats.jl:1-13).  Device state
+(samples, O, S, the factor, the update) stays on the GPU; `net.flat` is the
+canonical host copy and is refreshed after every update.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .operators import KLocalLiouvillian, flatten
+
+__all__ = ["NDM", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
+           "BatchedSampler", "HIPBatchedSampler", "sample_", "precondition_", "update_",
+           "init_random_pars_"]
+
+
+class NDM:
+    """Neural density matrix parameters in the reference's flat order
+    (NDM.jl:17-18): b_mu, h_mu, w_mu, u_mu, b_la, h_la, d_la, w_la, u_la,
+    matrices column-major (nh x N, na x N)."""
+
+    FIELDS = ("b_mu", "h_mu", "w_mu", "u_mu", "b_la", "h_la", "d_la", "w_la", "u_la")
+
+    def __init__(self, N: int, alpha_h: float, alpha_a: float, act: str = "softplus"):
+        if act != "softplus":
+            raise NotImplementedError("only af_softplus (the default) is implemented")
+        self.N = int(N)
+        self.nh = int(alpha_h * N)
+        self.na = int(alpha_a * N)
+        if self.nh != alpha_h * N or self.na != alpha_a * N:
+            raise ValueError("alpha * N must be an integer (convert(Int, αh*in), NDM.jl:55)")
+        N, nh, na = self.N, self.nh, self.na
+        sizes = [N, nh, nh * N, na * N, N, nh, na, nh * N, na * N]
+        shapes = [(N,), (nh,), (nh, N), (na, N), (N,), (nh,), (na,), (nh, N), (na, N)]
+        self.flat = np.zeros(sum(sizes))
+        off = 0
+        for name, n, shp in zip(self.FIELDS, sizes, shapes):
+            setattr(self, name, self.flat[off:off + n].reshape(shp, order="F"))
+            off += n
+
+    @property
+    def n_params(self) -> int:
+        return len(self.flat)
+
+    def init_random_pars(self, rng=None, sigma: float = 0.01):
+        """`init_random_pars!(net; sigma)`: randn * sqrt(sigma) (rng.jl:20-31)."""
+        rng = rng if rng is not None else np.random.default_rng()
+        self.flat[:] = rng.standard_normal(self.n_params) * math.sqrt(sigma)
+        return self
+
+
+def init_random_pars_(net: NDM, rng=None, sigma: float = 0.01):
+    return net.init_random_pars(rng, sigma)
+
+
+class LocalRule:
+    """Single-site flip proposal (LocalRule.jl:19-28)."""
+
+
+class MetropolisSampler:
+    """`MetropolisSampler(rule, chain_length, passes; burn=0, seed)`
+    (Metropolis.jl:26-40).  An even `passes` is bumped to the next odd number,
+    as the reference does."""
+
+    def __init__(self, rule, chain_length: int, passes: int, burn: int = 0, seed: int | None = None,
+                 compat_stale_last_slice: bool = False, persistent: bool = False):
+        if not isinstance(rule, LocalRule):
+            raise NotImplementedError("only LocalRule is implemented")
+        if passes <= 0 or chain_length <= 0:
+            raise ValueError("passes and chain_length must be positive")
+        if passes % 2 == 0:
+            warnings.warn(f"LocalRule sampling with an even number of switches per step is known to be "
+                          f"non-ergodic; setting the number of switches to {passes + 1}.")
+            passes += 1
+        self.rule = rule
+        self.chain_length = int(chain_length)
+        self.passes = int(passes)
+        self.burn = int(burn)
+        self.seed = int(seed) if seed is not None else int(np.random.default_rng().integers(0, 2 ** 63))
+        self.compat_stale_last_slice = compat_stale_last_slice
+        self.persistent = persistent
+
+    @property
+    def flags(self):
+        f = 0
+        if self.compat_stale_last_slice:
+            f |= _lib.SAMPLER_COMPAT_STALE_LAST_SLICE
+        if self.persistent:
+            f |= _lib.SAMPLER_PERSISTENT
+        return f
+
+
+class SR:
+    """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift)`
+    (SR.jl:50-57).  `precondition_type` "shift" adds eps*I (reference);
+    "diag" adds eps*diag(S).  `refine` extra FP64 refinement steps."""
+
+    def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
+                 refine: int = 0):
+        if algorithm != "cholesky":
+            raise NotImplementedError("only sr_cholesky (the default) is implemented")
+        if precondition_type not in ("shift", "diag"):
+            raise ValueError("precondition_type must be 'shift' or 'diag'")
+        self.eps = float(eps)
+        self.algorithm = algorithm
+        self.precondition_type = precondition_type
+        self.refine = int(refine)
+
+    @property
+    def shift_kind(self):
+        return _lib.SHIFT_ADD if self.precondition_type == "shift" else _lib.SHIFT_DIAG
+
+
+class Descent:
+    """`Optimisers.Descent(η)`: p -= η Δ (rules.jl:11-17)."""
+
+    def __init__(self, eta: float):
+        self.eta = float(eta)
+
+
+@dataclass
+class Measurement:
+    """`Measurement` of stats.jl:1-13."""
+    mean: float
+    error: float
+    variance: float
+    tau: float
+    R: float
+
+    @classmethod
+    def from_array(cls, st):
+        return cls(*(float(x) for x in st[:5]))
+
+    def __str__(self):
+        return (f"({self.mean:6.4f}) ± {self.error:6.4f} [var={self.variance:6.4f}, "
+                f"tau={self.tau:6.4f}, R={self.R:6.4f}]")
+
+
+_PREC = {"fp64": (_lib.PREC_FP64, _lib.PREC_FP64),
+         "fp32": (_lib.PREC_FP32, _lib.PREC_FP32),
+         "mixed": (_lib.PREC_FP32, _lib.PREC_BF16)}
+
+
+class HIPBatchedSampler:
+    """`BatchedGradSampler` (BatchedGradSampler.jl:29-75) running on libnqhip."""
+
+    def __init__(self, net: NDM, sampler: MetropolisSampler, prob: KLocalLiouvillian, algo: SR,
+                 batch_sz: int = 16, device: int = 0, precision: str = "fp64"):
+        if not isinstance(prob, KLocalLiouvillian):
+            raise NotImplementedError("only the KLocalLiouvillian (L†L) problem is implemented")
+        if prob.phys.n_sites != net.N:
+            raise ValueError("network and problem have different numbers of sites")
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        self.net, self.sampler, self.prob, self.algo = net, sampler, prob, algo
+        self.batch_sz = int(batch_sz)
+        self.precision = precision
+        self.ctx = _lib.Context(device, sampler.seed)
+        self.ctx.set_problem(flatten(prob))
+        self.ctx.set_ndm(net.N, net.nh, net.na)
+        self.ctx.set_precision(*_PREC[precision])
+        self.ctx.set_params(net.flat)
+        self.ctx.set_sampler(self.batch_sz, sampler.chain_length, sampler.passes, sampler.burn, sampler.flags)
+        self._pending = False
+
+    @property
+    def n_samples(self):
+        return self.batch_sz * self.sampler.chain_length
+
+    def push_params(self):
+        self.ctx.set_params(self.net.flat)
+
+    def sample(self):
+        st = self.ctx.sample_grad()
+        self._pending = True
+        return Measurement.from_array(st), self
+
+    def precondition(self, algo: SR, it: int = 0):
+        if not self._pending:
+            raise RuntimeError("call sample_(is) before precondition_")
+        return self.ctx.precondition(algo.eps, algo.shift_kind, algo.refine)
+
+    def update(self, opt: Descent, grad=None):
+        self.ctx.update(opt.eta)
+        self.net.flat[:] = self.ctx.get_params()
+        self._pending = False
+
+    def step(self, opt: Descent):
+        """One fused SR iteration (sample!, precondition!, update!) without
+        host round trips; returns the loss Measurement."""
+        st = self.ctx.sr_iteration(self.algo.eps, opt.eta, self.algo.shift_kind, self.algo.refine)
+        return Measurement.from_array(st)
+
+    def samples(self):
+        return self.ctx.get_samples()
+
+    def local_values(self):
+        return self.ctx.get_local_values()
+
+    def close(self):
+        self.ctx.close()
+
+
+def BatchedSampler(net, sampler, prob, algo, batch_sz: int = 16, **kw):  # noqa: N802
+    return HIPBatchedSampler(net, sampler, prob, algo, batch_sz=batch_sz, **kw)
+
+
+def sample_(is_: HIPBatchedSampler):
+    """`sample!(is)` -> (L_stat, precond_cache)."""
+    return is_.sample()
+
+
+def precondition_(cache: HIPBatchedSampler, algo: SR, it: int = 0):
+    """`precondition!(cache, algo, iter)` -> the preconditioned gradient
+    (reference order)."""
+    return cache.precondition(algo, it)
+
+
+def update_(opt: Descent, is_or_net, grad=None):
+    """`Optimisers.update!(opt, net, grad)`: applies p -= eta * grad on the
+    device and refreshes the host parameters."""
+    if isinstance(is_or_net, HIPBatchedSampler):
+        is_or_net.update(opt, grad)
+        return is_or_net.net
+    raise TypeError("pass the HIPBatchedSampler that owns the network")

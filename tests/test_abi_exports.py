@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every entry point include/nqhip.h
+declares (CPU: no compute calls), and the host API mirror behaves like the
+reference's constructors."""
+import os
+import re
+import subprocess
+import warnings
+
+import pytest
+
+import nqhip
+from nqhip import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nqhip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nqh_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "nqh_sr_iteration" in names and "nqh_local_estimator" in names
+    assert set(names) == set(_lib.SIGNATURES), "binding and header disagree"
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libnqhip.so is not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (nqh_[a-z_0-9]+)$", out, flags=re.M))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_reports_abi_version():
+    lib = _lib.load()
+    assert lib.nqh_abi_version() == 1
+
+
+def test_metropolis_even_passes_bumped_to_odd():
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        s = nqhip.MetropolisSampler(nqhip.LocalRule(), 10, 4, burn=2, seed=1)
+    assert s.passes == 5 and any("non-ergodic" in str(x.message) for x in w)
+
+
+def test_ndm_flat_layout_and_init():
+    import numpy as np
+    net = nqhip.NDM(4, 2, 1)
+    assert net.n_params == 2 * 4 + 2 * 8 + 4 + 2 * 8 * 4 + 2 * 4 * 4
+    net.flat[:] = np.arange(net.n_params)
+    assert net.b_mu[0] == 0 and net.h_mu[0] == 4 and net.w_mu[1, 0] == 13 and net.w_mu[0, 1] == 20
+    assert net.u_la[-1, -1] == net.n_params - 1
+    net.init_random_pars(np.random.default_rng(0), sigma=1e-4)
+    assert abs(np.std(net.flat) - 1e-2) < 3e-3
