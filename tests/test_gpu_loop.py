@@ -87,7 +87,7 @@ def test_fp32_network_errors_stated(gpu_ok, ref):
     ctx.close()
 
 
-@pytest.mark.parametrize("sprec,tol", [(_lib.PREC_FP32, 1e-5), (_lib.PREC_BF16, 2e-2)])
+@pytest.mark.parametrize("sprec,tol", [(_lib.PREC_FP32, 1e-4), (_lib.PREC_BF16, 2e-2)])
 def test_reduced_precision_s_build_error(gpu_ok, ref, sprec, tol):
     """S from FP32 / BF16 MFMA on centred O against the FP64 oracle
     (max abs error relative to max |S|)."""
