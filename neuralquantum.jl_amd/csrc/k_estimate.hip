@@ -230,6 +230,14 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
         cplx<T> Lloc = cmk(T(0), T(0));
         int nc = 0;
         table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+            if ((ms | me) == 0u) {
+                // unchanged configuration: exp(0) = 1, W = mel exactly (the
+                // reference evaluates the network anyway, AccumulatorObsGrad.jl:72)
+                Lloc = cadd(Lloc, mel);
+                if (nc < a.max_conn) wbuf[nc] = mel;
+                ++nc;
+                return;
+            }
             T dre = T(0), dim = T(0);
 #pragma unroll
             for (int r = 0; r < RS; ++r) {
@@ -325,10 +333,15 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
 #pragma unroll
             for (int r = 0; r < PS; ++r) rho1[r] = rho2[r] = T(0);
             int c = 0;
+            cplx<T> VWdiag = cmk(T(0), T(0));   // sum of V W_c over unchanged configurations
             table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
                 (void)mel;
                 const cplx<T> VW = cmul(V, c < a.max_conn ? wbuf[c] : cmk(T(0), T(0)));
                 ++c;
+                if ((ms | me) == 0u) {   // O(c) = O(s): only the dense parts, added once below
+                    VWdiag = cadd(VWdiag, VW);
+                    return;
+                }
 #pragma unroll
                 for (int r = 0; r < RS; ++r) {
                     const int u = lane + 64 * r;
@@ -431,6 +444,26 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
                 }
             });
 
+            // unchanged configurations contribute O(s) with weight sum VWdiag
+#pragma unroll
+            for (int r = 0; r < RS; ++r) {
+                const int u = lane + 64 * r;
+                if (u < 2 * nh) {
+                    const bool lam = u < nh;
+                    const T gam = lam ? VWdiag.re : -VWdiag.im;
+                    e_s[r] += gam * a_s[r];
+                    e_e[r] += (lam ? gam : -gam) * a_e[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < PS; ++r) {
+                const int q = lane + 64 * r;
+                if (q < na) {
+                    const cplx<T> vp = cmul(VWdiag, pi_p[r]);
+                    rho1[r] += vp.re;
+                    rho2[r] += vp.im;
+                }
+            }
             // dense parts: x_s times the per-unit sums
 #pragma unroll
             for (int r = 0; r < RS; ++r) {

@@ -147,6 +147,106 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
             }
 }
 
+// ---- BF16 path: centred, transposed copy Yt[c][s] = bf16(Y[s][c0 + c] - mu),
+// samples padded with zeros to a multiple of 64, then a 128x128-tile Gram.
+template <typename YT>
+__global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int ldy, int c0, int n,
+                                                  const double* mu, __bf16* Yt, int64_t nsp) {
+    __shared__ float tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int cb = blockIdx.x * 64;
+    const int64_t sb = (int64_t)blockIdx.y * 64;
+    const int c = cb + tx;
+    const double m = (c < n && mu) ? mu[c0 + c] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int64_t si = sb + ty + 4 * r;
+        float v = 0.f;
+        if (c < n && si < Ns) v = (float)((double)Y[si * ldy + c0 + c] - m);
+        tile[ty + 4 * r][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int cc = cb + ty + 4 * r;
+        const int64_t si = sb + tx;
+        if (cc < n && si < nsp) Yt[(int64_t)cc * nsp + si] = (__bf16)tile[tx][ty + 4 * r];
+    }
+}
+
+// S[o + r, o + c] += scale * sum_k Yt[r][k] Yt[c][k] over upper 128x128 tiles;
+// 4 waves in 2x2, each 64x64 = 4x4 tiles of v_mfma_f32_16x16x32_bf16.
+__global__ __launch_bounds__(256) void k_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds,
+                                                    int o, double scale) {
+    constexpr int KS = 32, PITCH = 40;   // bf16 per LDS row (80 B: 16-byte aligned, staggered banks)
+    __shared__ __align__(16) __bf16 As[128 * PITCH];
+    __shared__ __align__(16) __bf16 Bs[128 * PITCH];
+    int I, J;
+    tri_decode(blockIdx.x, &I, &J);
+    const int r0 = I * 128, q0 = J * 128;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.f;
+
+    // staging: 128 rows x 4 chunks of 16 B per operand = 512 chunks, 2 per thread
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const int row0 = tid >> 2, ch = tid & 3;            // rows row0 and row0 + 64
+    auto gload = [&](int base, int row, int64_t k0) -> u4 {
+        const int rr = base + row;
+        if (rr >= n) return u4{0u, 0u, 0u, 0u};
+        return *reinterpret_cast<const u4*>(Yt + (int64_t)rr * nsp + k0 + ch * 8);
+    };
+    u4 pa0 = gload(r0, row0, 0), pa1 = gload(r0, row0 + 64, 0);
+    u4 pb0 = gload(q0, row0, 0), pb1 = gload(q0, row0 + 64, 0);
+    for (int64_t k0 = 0; k0 < nsp; k0 += KS) {
+        __syncthreads();
+        *reinterpret_cast<u4*>(&As[row0 * PITCH + ch * 8]) = pa0;
+        *reinterpret_cast<u4*>(&As[(row0 + 64) * PITCH + ch * 8]) = pa1;
+        *reinterpret_cast<u4*>(&Bs[row0 * PITCH + ch * 8]) = pb0;
+        *reinterpret_cast<u4*>(&Bs[(row0 + 64) * PITCH + ch * 8]) = pb1;
+        __syncthreads();
+        if (k0 + KS < nsp) {   // prefetch the next k-step while the MFMAs run
+            pa0 = gload(r0, row0, k0 + KS); pa1 = gload(r0, row0 + 64, k0 + KS);
+            pb0 = gload(q0, row0, k0 + KS); pb1 = gload(q0, row0 + 64, k0 + KS);
+        }
+        bf8 av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            av[a] = *reinterpret_cast<const bf8*>(&As[(wr * 64 + a * 16 + li) * PITCH + lk * 8]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            bv[b] = *reinterpret_cast<const bf8*>(&Bs[(wc * 64 + b * 16 + li) * PITCH + lk * 8]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    const int lj = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + wr * 64 + a * 16 + 4 * lg + v;
+                const int c = q0 + wc * 64 + b * 16 + lj;
+                if (r < n && c < n) {
+                    const double val = scale * (double)acc[a][b][v];
+                    S[(int64_t)(o + r) + (int64_t)(o + c) * lds] += val;
+                    if (I != J) S[(int64_t)(o + c) + (int64_t)(o + r) * lds] += val;
+                }
+            }
+}
+
 template <typename YT>
 __global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, double* part, int chunks) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,6 +289,25 @@ hipError_t nq_launch_gram(const gram_args& a, int mode, hipStream_t st) {
         case NQ_GRAM_BF16: return launch_mode<NQ_GRAM_BF16>(a, st);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, int c0, int n, const double* mu,
+                              __bf16* Yt, int64_t nsp, hipStream_t st) {
+    dim3 grid((n + 63) / 64, (unsigned)((nsp + 63) / 64));
+    if (y_fp64)
+        hipLaunchKernelGGL((k_center_t<double>), grid, dim3(256), 0, st, (const double*)Y, Ns, ldy, c0, n, mu, Yt, nsp);
+    else
+        hipLaunchKernelGGL((k_center_t<float>), grid, dim3(256), 0, st, (const float*)Y, Ns, ldy, c0, n, mu, Yt, nsp);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds, int o, double scale,
+                                hipStream_t st) {
+    const int T = (n + 127) / 128;
+    const int blocks = T * (T + 1) / 2;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gram_bf16t, dim3(blocks), dim3(256), 0, st, Yt, nsp, n, S, lds, o, scale);
+    return hipGetLastError();
 }
 
 hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double* part, int chunks,

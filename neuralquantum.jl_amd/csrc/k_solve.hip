@@ -87,12 +87,24 @@ __global__ __launch_bounds__(256) void k_stats(const T* L, int64_t B, int64_t Lc
     }
 }
 
-__global__ void k_force_finish(const double* Fpart, int nblk, const double* mu, const double* stats,
+// stage 1: part2[g][k] = sum of Fpart rows g*per .. (g+1)*per - 1 (grid: k chunks x groups)
+__global__ void k_force_part(const double* Fpart, int nblk, int per, int np, double* part2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (k >= np) return;
+    const int b0 = g * per, b1 = (b0 + per < nblk) ? b0 + per : nblk;
+    double acc = 0.0;
+    for (int b = b0; b < b1; ++b) acc += Fpart[(int64_t)b * np + k];
+    part2[(int64_t)g * np + k] = acc;
+}
+
+// stage 2: F[k] = sum_g part2[g][k] - C Re(Obar_k)
+__global__ void k_force_finish(const double* part2, int ngroups, const double* mu, const double* stats,
                                ndm_dims d, double* F) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.np) return;
     double acc = 0.0;
-    for (int b = 0; b < nblk; ++b) acc += Fpart[(int64_t)b * d.np + k];
+    for (int g = 0; g < ngroups; ++g) acc += part2[(int64_t)g * d.np + k];
     const double reObar = (k < d.nlam + d.npi) ? mu[k] : 0.0;
     F[k] = acc - stats[0] * reObar;
 }
@@ -104,41 +116,42 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
     s += (kind == 0) ? eps : eps * s;
 }
 
-// ---- Cholesky: diagonal block, one 64-thread workgroup, LDS resident
-__global__ __launch_bounds__(64) void k_potrf_diag(double* A, int lda, int k, int kb, int* info) {
+// ---- Cholesky step k: factor the diagonal block and solve the panel below it.
+// Every workgroup (one wave) factors the kb x kb diagonal block in LDS
+// (right-looking, lane t owns row t); workgroup 0 writes it back, workgroup
+// b >= 1 then solves rows k + kb + 64 (b - 1) + t of the panel,
+// X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution (the dependent
+// chain per row is kb long, the updates of later columns are independent).
+__global__ __launch_bounds__(64) void k_chol_panel(double* A, int n, int lda, int k, int kb, int* info) {
     __shared__ double L[NB][NB + 1];
+    __shared__ double rdiag[NB];
     const int t = threadIdx.x;
     for (int c = 0; c < kb; ++c)
         if (t < kb) L[t][c] = A[(int64_t)(k + t) + (int64_t)(k + c) * lda];
     __syncthreads();
     for (int j = 0; j < kb; ++j) {
-        if (t == 0) {
-            const double v = L[j][j];
-            if (!(v > 0.0) && *info == 0) *info = k + j + 1;
-            L[j][j] = sqrt(v);
-        }
+        const double v = L[j][j];
+        const double piv = sqrt(v);
+        if (t == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
+        const double rp = 1.0 / piv;
         __syncthreads();
-        if (t > j && t < kb) L[t][j] /= L[j][j];
-        __syncthreads();
+        if (t == j) { L[j][j] = piv; rdiag[j] = rp; }
         if (t > j && t < kb) {
-            const double lij = L[t][j];
-            for (int m = j + 1; m <= t; ++m) L[t][m] -= lij * L[m][j];
+            // L[m][j] (m > j) is still unscaled here: scale it on use
+            const double lij = L[t][j] * rp;
+            const double lr = lij * rp;
+            for (int m = j + 1; m <= t; ++m) L[t][m] -= lr * L[m][j];
         }
         __syncthreads();
+        if (t > j && t < kb) L[t][j] *= rp;
+        __syncthreads();
     }
-    for (int c = 0; c < kb; ++c)
-        if (t < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = L[t][c];
-}
-
-// ---- panel: rows i >= k + kb: X = A[i, k:k+kb] L_kk^-T (one thread per row)
-__global__ __launch_bounds__(256) void k_trsm_panel(double* A, int n, int lda, int k, int kb) {
-    __shared__ double L[NB][NB + 1];
-    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-        const int r = e % kb, c = e / kb;
-        L[r][c] = (r >= c) ? A[(int64_t)(k + r) + (int64_t)(k + c) * lda] : 0.0;
+    if (blockIdx.x == 0) {
+        for (int c = 0; c < kb; ++c)
+            if (t < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = L[t][c];
+        return;
     }
-    __syncthreads();
-    const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = k + kb + (blockIdx.x - 1) * 64 + t;
     if (i >= n) return;
     double x[NB];
 #pragma unroll
@@ -146,10 +159,10 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* A, int n, int lda, i
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (j < kb) {
-            double v = x[j];
+            x[j] *= rdiag[j];
 #pragma unroll
-            for (int m = 0; m < j; ++m) v -= x[m] * L[j][m];
-            x[j] = v / L[j][j];
+            for (int m = j + 1; m < NB; ++m)
+                if (m < kb) x[m] -= x[j] * L[m][j];
         }
     }
 #pragma unroll
@@ -221,55 +234,83 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda
             }
 }
 
-// ---- triangular solves with the factor, one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void k_chol_solve(const double* L, int n, int lda, const double* b,
-                                                      double* x) {
-    extern __shared__ double y[];   // n
-    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
+// ---- forward substitution L y = b, one block of 64 per step
+__global__ __launch_bounds__(64) void k_fwd_diag(const double* L, int lda, int k, int kb, double* y) {
+    const int lane = threadIdx.x;
+    double row[NB];   // row[j] = L[k + lane][k + j]
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        row[j] = (lane < kb && j < kb) ? L[(int64_t)(k + lane) + (int64_t)(k + j) * lda] : 1.0;
+    double v = (lane < kb) ? y[k + lane] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < kb) {
+            if (lane == j) v = v / row[j];
+            const double yj = __shfl(v, j, 64);
+            if (lane > j && lane < kb) v -= row[j] * yj;
+        }
+    }
+    if (lane < kb) y[k + lane] = v;
+}
+
+// y[i] -= sum_{m < kb} L[i][k + m] y[k + m] for k + kb <= i < n (coalesced over i)
+__global__ __launch_bounds__(256) void k_fwd_update(const double* L, int n, int lda, int k, int kb, double* y) {
+    __shared__ double ys[NB];
+    if (threadIdx.x < kb) ys[threadIdx.x] = y[k + threadIdx.x];
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    // forward: L y = b
-    for (int k = 0; k < n; k += NB) {
-        const int kb = (n - k < NB) ? n - k : NB;
-        if (threadIdx.x < 64) {
-            double v = (lane < kb) ? y[k + lane] : 0.0;
-            for (int j = 0; j < kb; ++j) {
-                if (lane == j) v = v / L[(int64_t)(k + j) + (int64_t)(k + j) * lda];
-                const double yj = __shfl(v, j, 64);
-                if (lane > j && lane < kb) v -= L[(int64_t)(k + lane) + (int64_t)(k + j) * lda] * yj;
-            }
-            if (lane < kb) y[k + lane] = v;
+    const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int m = 0; m < kb; ++m) acc += L[(int64_t)i + (int64_t)(k + m) * lda] * ys[m];
+    y[i] -= acc;
+}
+
+// A[n, j] = b[j] (the right-hand side as the extra row of the factorization)
+__global__ void k_set_row(double* A, int n, int lda, const double* b) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) A[(int64_t)n + (int64_t)j * lda] = b[j];
+}
+
+// ---- back substitution L^T x = y, one block of 64 per step (last block first)
+// diag step: x[k:k+kb] = L_kk^-T y[k:k+kb] (one wave, the block's columns in registers)
+__global__ __launch_bounds__(64) void k_back_diag(const double* L, int lda, int k, int kb, double* y) {
+    const int lane = threadIdx.x;
+    double col[NB];   // col[j] = L[k + j][k + lane]
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        col[j] = (lane < kb && j < kb) ? L[(int64_t)(k + j) + (int64_t)(k + lane) * lda] : 1.0;
+    double v = (lane < kb) ? y[k + lane] : 0.0;
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        const int j = NB - 1 - jj;
+        if (j < kb) {
+            if (lane == j) v = v / col[j];
+            const double xj = __shfl(v, j, 64);
+            if (lane < j) v -= col[j] * xj;
         }
-        __syncthreads();
-        for (int i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
-            double acc = 0.0;
-            for (int m = 0; m < kb; ++m) acc += L[(int64_t)i + (int64_t)(k + m) * lda] * y[k + m];
-            y[i] -= acc;
-        }
-        __syncthreads();
     }
-    // backward: L^T x = y
-    const int last = ((n - 1) / NB) * NB;
-    for (int k = last; k >= 0; k -= NB) {
-        const int kb = (n - k < NB) ? n - k : NB;
-        if (threadIdx.x < 64) {
-            double v = (lane < kb) ? y[k + lane] : 0.0;
-            for (int j = kb - 1; j >= 0; --j) {
-                if (lane == j) v = v / L[(int64_t)(k + j) + (int64_t)(k + j) * lda];
-                const double xj = __shfl(v, j, 64);
-                if (lane < j) v -= L[(int64_t)(k + j) + (int64_t)(k + lane) * lda] * xj;
-            }
-            if (lane < kb) y[k + lane] = v;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < k; i += blockDim.x) {
-            double acc = 0.0;
-            for (int m = 0; m < kb; ++m) acc += L[(int64_t)(k + m) + (int64_t)i * lda] * y[k + m];
-            y[i] -= acc;
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = y[i];
+    if (lane < kb) y[k + lane] = v;
+}
+
+// update step: y[i] -= sum_{m < kb} L[k + m][i] x[k + m] for i < k; one row i
+// per thread, its 64 factor entries are contiguous (column i, rows k..k+kb-1)
+__global__ __launch_bounds__(256) void k_back_update(const double* L, int lda, int k, int kb, double* y) {
+    __shared__ double xs[NB];
+    if (threadIdx.x < kb) xs[threadIdx.x] = y[k + threadIdx.x];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const double* c = L + (int64_t)i * lda + k;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int m = 0; m < kb; ++m) acc += c[m] * xs[m];
+    y[i] -= acc;
+}
+
+// the forward half of the solve is the factorization's extra row n: copy it out
+__global__ void k_copy_row(const double* A, int n, int lda, double* y) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) y[j] = A[(int64_t)n + (int64_t)j * lda];
 }
 
 __global__ void k_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
@@ -358,9 +399,12 @@ hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, do
     return hipGetLastError();
 }
 
-hipError_t nq_launch_force_finish(const double* Fpart, int nb, const double* mu, const double* stats,
-                                  ndm_dims d, double* F, hipStream_t st) {
-    hipLaunchKernelGGL(k_force_finish, dim3(nblk(d.np, 256)), dim3(256), 0, st, Fpart, nb, mu, stats, d, F);
+hipError_t nq_launch_force_finish(const double* Fpart, int nb, double* part2, const double* mu,
+                                  const double* stats, ndm_dims d, double* F, hipStream_t st) {
+    const int groups = NQ_FORCE_GROUPS;
+    const int per = (nb + groups - 1) / groups;
+    hipLaunchKernelGGL(k_force_part, dim3(nblk(d.np, 256), groups), dim3(256), 0, st, Fpart, nb, per, d.np, part2);
+    hipLaunchKernelGGL(k_force_finish, dim3(nblk(d.np, 256)), dim3(256), 0, st, part2, groups, mu, stats, d, F);
     return hipGetLastError();
 }
 
@@ -370,21 +414,48 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 }
 
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st) {
+    // rows >= n (at most one extra row, the right-hand side) are carried along:
+    // the panel solves and trailing updates turn row n into (L^-1 b)^T
+    const int nr = (lds > n) ? n + 1 : n;
     for (int k = 0; k < n; k += NB) {
         const int kb = (n - k < NB) ? n - k : NB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, S, lds, k, kb, info);
-        const int rest = n - k - kb;
+        const int rest = nr - k - kb;
+        hipLaunchKernelGGL(k_chol_panel, dim3(1 + nblk(rest, 64)), dim3(64), 0, st, S, nr, lds, k, kb, info);
         if (rest > 0) {
-            hipLaunchKernelGGL(k_trsm_panel, dim3(nblk(rest, 256)), dim3(256), 0, st, S, n, lds, k, kb);
             const int T = (rest + 63) / 64;
-            hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, n, lds, k, kb);
+            hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, nr, lds, k, kb);
         }
     }
     return hipGetLastError();
 }
 
-hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* b, double* x, hipStream_t st) {
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), sizeof(double) * (size_t)n, st, L, n, lds, b, x);
+hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, double* y, hipStream_t st) {
+    const int last = ((n - 1) / NB) * NB;
+    for (int k = last; k >= 0; k -= NB) {
+        const int kb = (n - k < NB) ? n - k : NB;
+        hipLaunchKernelGGL(k_back_diag, dim3(1), dim3(64), 0, st, L, lds, k, kb, y);
+        if (k > 0) hipLaunchKernelGGL(k_back_update, dim3(nblk(k, 256)), dim3(256), 0, st, L, lds, k, kb, y);
+    }
+    return hipGetLastError();
+}
+
+hipError_t nq_copy_aug_row(const double* A, int n, int lds, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_row, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, lds, y);
+    return hipGetLastError();
+}
+
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, double* y, hipStream_t st) {
+    for (int k = 0; k < n; k += NB) {
+        const int kb = (n - k < NB) ? n - k : NB;
+        hipLaunchKernelGGL(k_fwd_diag, dim3(1), dim3(64), 0, st, L, lds, k, kb, y);
+        const int rest = n - k - kb;
+        if (rest > 0) hipLaunchKernelGGL(k_fwd_update, dim3(nblk(rest, 256)), dim3(256), 0, st, L, n, lds, k, kb, y);
+    }
+    return nq_cholesky_backsolve(L, n, lds, y, st);
+}
+
+hipError_t nq_set_aug_row(double* A, int n, int lds, const double* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_row, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, lds, b);
     return hipGetLastError();
 }
 

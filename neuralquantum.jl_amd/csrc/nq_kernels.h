@@ -62,6 +62,13 @@ struct gram_args {
 };
 hipError_t nq_launch_gram(const gram_args& a, int mode, hipStream_t st);
 
+// BF16 S build: Yt[c][s] = bf16(Y[s, c0 + c] - mu[c0 + c]) for c < n, s < nsp
+// (zero beyond Ns; nsp a multiple of 64), then S[o + r, o + c] += scale Yt Yt^T
+hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, int c0, int n, const double* mu,
+                              __bf16* Yt, int64_t nsp, hipStream_t st);
+hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds, int o, double scale,
+                                hipStream_t st);
+
 // column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]
 hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double* part, int chunks,
                             hipStream_t st);
@@ -77,15 +84,24 @@ hipError_t nq_launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t st
 hipError_t nq_launch_bump(uint32_t* x, hipStream_t st);
 // stats of |L_loc|^2 over B chains x L slices (stats.jl:52-77): out[5]
 hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
-// F[k] = sum_b Fpart[b][k] - C Re(Obar_k), P order; C = stats[0]
-hipError_t nq_launch_force_finish(const double* Fpart, int nblk, const double* mu, const double* stats,
-                                  ndm_dims d, double* F, hipStream_t st);
+// F[k] = sum_b Fpart[b][k] - C Re(Obar_k), P order; C = stats[0]; part2: NQ_FORCE_GROUPS x np scratch
+#define NQ_FORCE_GROUPS 32
+hipError_t nq_launch_force_finish(const double* Fpart, int nblk, double* part2, const double* mu,
+                                  const double* stats, ndm_dims d, double* F, hipStream_t st);
 // S[k,k] += eps (kind 0) or eps * S[k,k] (kind 1)
 hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st);
-// in-place lower Cholesky of S (column-major, n x n); *info = first failing column + 1 or 0
+// in-place lower Cholesky of S (column-major, n x n, leading dimension lds);
+// *info = first failing column + 1 or 0.  With lds > n, row n holds a right-hand
+// side b^T on entry and (L^-1 b)^T on exit (the forward substitution).
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st);
-// x = (L L^T)^-1 b with the factor from nq_cholesky (b and x may alias)
-hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* b, double* x, hipStream_t st);
+// y <- L^-T y (back substitution with the factor)
+hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, double* y, hipStream_t st);
+// y[j] = A[n, j] (read the forward-substituted right-hand side out of row n)
+hipError_t nq_copy_aug_row(const double* A, int n, int lds, double* y, hipStream_t st);
+// y <- (L L^T)^-1 y with the factor from nq_cholesky (blocked forward + back substitution)
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, double* y, hipStream_t st);
+// A[n, j] = b[j]: the right-hand side as the extra row nq_cholesky carries along
+hipError_t nq_set_aug_row(double* A, int n, int lds, const double* b, hipStream_t st);
 // r = b - (S + shift) x   (S the unfactored matrix, shift as in nq_launch_shift)
 hipError_t nq_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
                        const double* b, double* r, hipStream_t st);

@@ -51,11 +51,11 @@ struct nqh_ctx {
 
     // per-sample buffers (capacity Ns_cap)
     int64_t ns_cap = 0;
-    dbuf keys_in, lp, Y, Lloc, G;
+    dbuf keys_in, lp, Y, Lloc, G, Yt;
     int est_grid = 0;
     dbuf Fpart;
     // iteration buffers
-    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag;
+    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2;
     int64_t last_ns = 0;
     bool grad_ready = false;
 
@@ -198,11 +198,19 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->mu, sizeof(double) * c->d.R))) return s;
     if ((s = ensure(c, c->stats, sizeof(double) * 8))) return s;
     if ((s = ensure(c, c->F, sizeof(double) * c->d.np))) return s;
-    if ((s = ensure(c, c->S, sizeof(double) * (size_t)c->d.np * c->d.np))) return s;
-    if ((s = ensure(c, c->S0, sizeof(double) * (size_t)c->d.np * c->d.np))) return s;
+    // S has one extra row and column: row np carries the right-hand side
+    // through the factorization (forward substitution for free)
+    const size_t ldS = (size_t)c->d.np + 1;
+    if ((s = ensure(c, c->S, sizeof(double) * ldS * ldS))) return s;
+    if ((s = ensure(c, c->S0, sizeof(double) * ldS * ldS))) return s;
+    if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * c->d.np))) return s;
     if ((s = ensure(c, c->delta, sizeof(double) * c->d.np))) return s;
     if ((s = ensure(c, c->resid, sizeof(double) * c->d.np))) return s;
     if ((s = ensure(c, c->info, sizeof(int) * 4))) return s;
+    if (c->s_prec == NQH_PREC_BF16) {
+        const int64_t nsp = (Ns + 63) / 64 * 64;
+        if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)c->d.R * nsp))) return s;
+    }
     if ((s = ensure(c, c->flag, sizeof(int) * 4))) return s;
     return NQH_OK;
 }
@@ -239,15 +247,30 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, c->d.R, 1.0 / (double)Ns,
                                       P<double>(c->mu), c->stream));
     HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
-    HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->mu), P<double>(c->stats),
-                                     c->d, P<double>(c->F), c->stream));
+    HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
+                                     P<double>(c->stats), c->d, P<double>(c->F), c->stream));
     phase_end(c, PH_POST);
     // S in P order: A-gram at [0, nA), B-gram at [nlam, np)
     const ndm_dims& d = c->d;
     const int np = d.np;
     phase_begin(c, PH_SBUILD);
-    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)np * np, 0.0, c->stream));
-    const int mode = c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : (c->s_prec == NQH_PREC_FP32 ? NQ_GRAM_F32 : NQ_GRAM_BF16);
+    const int ldS = np + 1;
+    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
+    if (c->s_prec == NQH_PREC_BF16) {
+        // centred BF16 copy with samples contiguous, then the two Gram blocks
+        const int64_t nsp = (Ns + 63) / 64 * 64;
+        __bf16* Yt = reinterpret_cast<__bf16*>(c->Yt.p);
+        HIPCHK(c, nq_launch_center_t(c->Y.p, f64, Ns, d.R, 0, d.R, P<double>(c->mu), Yt, nsp, c->stream));
+        const int nA = d.nlam + d.npi, nB = d.npi + d.nmu;
+        HIPCHK(c, nq_launch_gram_bf16t(Yt, nsp, nA, P<double>(c->S), ldS, 0, 1.0 / (double)Ns, c->stream));
+        HIPCHK(c, nq_launch_gram_bf16t(Yt + (size_t)nA * nsp, nsp, nB, P<double>(c->S), ldS, d.nlam,
+                                       1.0 / (double)Ns, c->stream));
+        phase_end(c, PH_SBUILD);
+        c->last_ns = Ns;
+        c->grad_ready = true;
+        return NQH_OK;
+    }
+    const int mode = c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : NQ_GRAM_F32;
     gram_args g{};
     g.Y = c->Y.p;
     g.y_fp64 = f64;
@@ -255,7 +278,7 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     g.ldy = d.R;
     g.mu = P<double>(c->mu);
     g.S = P<double>(c->S);
-    g.lds = np;
+    g.lds = ldS;
     g.scale = 1.0 / (double)Ns;
     g.c0 = 0;
     g.n = d.nlam + d.npi;
@@ -337,8 +360,8 @@ void nqh_destroy(nqh_ctx* c) {
     (void)hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
-                   &c->Y, &c->Lloc, &c->G, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag};
+                   &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2};
     drop_graph(c);
     release(c->iter_dev);
     for (dbuf* b : all) release(*b);
@@ -639,12 +662,13 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
         expand_row(mu.data(), d, Obar);
     }
     if (S) {
-        std::vector<double> sp((size_t)d.np * d.np);
+        const size_t ldS = (size_t)d.np + 1;
+        std::vector<double> sp(ldS * ldS);
         HIPCHK(c, hipMemcpy(sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
         std::vector<int> r(d.np);
         for (int k = 0; k < d.np; ++k) r[k] = p2ref_host(k, d);
         for (int l = 0; l < d.np; ++l)
-            for (int k = 0; k < d.np; ++k) S[(size_t)r[k] + (size_t)r[l] * d.np] = sp[(size_t)k + (size_t)l * d.np];
+            for (int k = 0; k < d.np; ++k) S[(size_t)r[k] + (size_t)r[l] * d.np] = sp[(size_t)k + (size_t)l * ldS];
     }
     return NQH_OK;
 }
@@ -754,19 +778,21 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
-    const int np = c->d.np;
+    const int np = c->d.np, ldS = np + 1;
     double* S = P<double>(c->S);
     phase_begin(c, PH_SOLVE);
     if (refine > 0)
-        HIPCHK(c, hipMemcpyAsync(c->S0.p, S, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, nq_launch_shift(S, np, np, eps, kind, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->S0.p, S, sizeof(double) * (size_t)ldS * ldS, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, nq_launch_shift(S, np, ldS, eps, kind, c->stream));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, c->stream));
-    HIPCHK(c, nq_cholesky(S, np, np, P<int>(c->info), c->stream));
-    HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->F), P<double>(c->delta), c->stream));
+    HIPCHK(c, nq_set_aug_row(S, np, ldS, P<double>(c->F), c->stream));
+    HIPCHK(c, nq_cholesky(S, np, ldS, P<int>(c->info), c->stream));       // also row np <- (L^-1 F)^T
+    HIPCHK(c, nq_copy_aug_row(S, np, ldS, P<double>(c->delta), c->stream));
+    HIPCHK(c, nq_cholesky_backsolve(S, np, ldS, P<double>(c->delta), c->stream));
     for (int it = 0; it < refine; ++it) {
-        HIPCHK(c, nq_residual(P<double>(c->S0), np, np, eps, kind, P<double>(c->delta), P<double>(c->F),
+        HIPCHK(c, nq_residual(P<double>(c->S0), np, ldS, eps, kind, P<double>(c->delta), P<double>(c->F),
                               P<double>(c->resid), c->stream));
-        HIPCHK(c, nq_cholesky_solve(S, np, np, P<double>(c->resid), P<double>(c->resid), c->stream));
+        HIPCHK(c, nq_cholesky_solve(S, np, ldS, P<double>(c->resid), c->stream));
         HIPCHK(c, nq_launch_axpy(P<double>(c->delta), P<double>(c->resid), np, c->stream));
     }
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, c->stream));
