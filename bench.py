@@ -127,30 +127,100 @@ def measure_tau(form, cfg=CFG, slices=48):
     return tau_int(2 * lp), tau_int(np.abs(L) ** 2)
 
 
-def roofline_for(phases, steps, form, d):
-    """Algorithmic figure of the dominant phase / its measured mean duration."""
+VALU_LANE_SLOTS_PER_S = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes x 2.4 GHz
+
+
+def ndm_sizes(d):
     N, nh, na = d["N"], d["nh"], d["na"]
     np_ = 2 * N + 2 * nh + na + 2 * nh * N + 2 * na * N
-    R = (N + nh + nh * N) * 2 + 2 * (na + 2 * na * N)
-    Ns = d["Ns"]
+    R = 2 * N + 2 * nh + 2 * nh * N + 2 * na + 4 * na * N
+    return np_, R
+
+
+def kernel_rooflines(phases, form, d, pairs):
+    """Per-kernel roofline figures from the per-phase HIP-event times.
+
+    Each timed phase is one kernel (estimator: k_estimate; sampler:
+    k_metropolis) or a short fixed sequence (S build: centring + two Gram
+    launches; solve: shift, blocked Cholesky, solves).  Algorithmic work per
+    launch, per unit x units:
+      estimator  force contraction (8 nh N + 4 na N) flop per (sample,
+                 connection) pair x M pairs; evaluation lane-slots
+                 (2 nh k + 20 nh) per changed side + 2 na k + 28 na + 20 per
+                 pair (k = flipped sites on that side, k ~ 1.6 here)
+      sbuild     4 Np^2 Ns flop
+      solve      Np^3 / 3 + 2 Np^2 flop
+      sampler    (24 nh + 32 na + 20) lane-slots per chain-step
+      post       2 Ns R p bytes (O read twice)"""
+    N, nh, na, Ns = d["N"], d["nh"], d["na"], d["Ns"]
+    np_, R = ndm_sizes(d)
     elt = 8 if form == "fp64" else 4
-    per = {k: v[0] / max(v[1], 1) for k, v in phases.items()}   # ms per launch sequence
-    dom = max(per, key=per.get)
-    out = {"phase_ms": {k: round(v, 4) for k, v in per.items()}, "dominant": dom}
-    if dom == "sbuild":
-        flops = 4.0 * np_ * np_ * Ns
-        ach = flops / (per["sbuild"] * 1e-3) / 1e12
-        peak = PEAK_TFLOPS["fp64" if form == "fp64" else "bf16"]
-        out.update(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                   algorithmic=f"4 Np^2 Ns = {flops:.4g} flop per S build")
-    else:
-        # estimator-style HBM figure: keys in, log rho out, compact O rows out
-        byts = Ns * (2 * N / 8 + 2 * elt + R * elt)
-        ph = dom
-        ach = byts / (per[ph] * 1e-3) / 1e9
-        out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                   algorithmic=f"Ns (2N/8 + 2p + R p) = {byts:.4g} B")
-    return out
+    per = {k: v[0] / max(v[1], 1) for k, v in phases.items()}   # ms per occurrence
+    fp = "fp64" if form == "fp64" else "fp32"
+    out = {}
+    t = per["estimator"] * 1e-3
+    fl = (8 * nh * N + 4 * na * N) * pairs
+    slots = pairs * (2 * nh * 1.6 + 20 * nh + 2 * na * 1.6 + 28 * na + 20)
+    out["estimator"] = dict(kernel="k_estimate", bound="mfma", achieved=fl / t / 1e12, peak=PEAK_TFLOPS[fp],
+                            unit="TFLOP/s", algorithmic=f"(8 nh N + 4 na N) x {pairs} pairs = {fl:.4g} flop",
+                            valu_frac=slots / t / VALU_LANE_SLOTS_PER_S,
+                            hbm_gbs=Ns * (2 * N / 8 + 4 * elt + R * elt) / t / 1e9, ms=per["estimator"])
+    t = per["sbuild"] * 1e-3
+    fl = 4.0 * np_ * np_ * Ns
+    sp = "fp64" if form == "fp64" else "bf16"
+    out["sbuild"] = dict(kernel="k_gram" if form == "fp64" else "k_gram_bf16t", bound="mfma",
+                         achieved=fl / t / 1e12, peak=PEAK_TFLOPS[sp], unit="TFLOP/s",
+                         algorithmic=f"4 Np^2 Ns = {fl:.4g} flop", ms=per["sbuild"])
+    t = per["solve"] * 1e-3
+    fl = np_ ** 3 / 3 + 2 * np_ ** 2
+    out["solve"] = dict(kernel="k_chol_panel + k_syrk_trailing + solves", bound="mfma", achieved=fl / t / 1e12,
+                        peak=PEAK_TFLOPS["fp64"], unit="TFLOP/s", algorithmic=f"Np^3/3 + 2 Np^2 = {fl:.4g} flop",
+                        ms=per["solve"])
+    t = per["post"] * 1e-3
+    by = 2.0 * Ns * R * elt
+    out["post"] = dict(kernel="k_colsum + k_stats + k_force", bound="hbm", achieved=by / t / 1e9,
+                       peak=HBM_PEAK_GBS, unit="GB/s", algorithmic=f"2 Ns R p = {by:.4g} B", ms=per["post"])
+    out["sampler"] = dict(kernel="k_metropolis", ms=per["sampler"])
+    for v in out.values():
+        if "peak" in v:
+            v["frac"] = v["achieved"] / v["peak"]
+    return out, per
+
+
+def count_pairs(ctx, cap=512, chunk=2048):
+    """Number M of (sample, connection) pairs with a non-zero matrix element
+    over the last iteration's samples (the units of the estimator's work)."""
+    keys = ctx.get_samples()
+    total = 0
+    for i in range(0, len(keys), chunk):
+        cnt, _, _ = ctx.connections(keys[i:i + chunk], cap)
+        total += int(cnt.astype(np.int64).sum())
+    return total
+
+
+def load_traffic(form):
+    """HBM bytes per k_estimate launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), if a
+    measurement for this workload is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_estimate_xyz44.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(form, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes=256 << 20):
+    """One device-to-device copy kernel of known size (reads and writes
+    `nbytes` each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def cpu_baseline(budget_s=20.0):
@@ -213,6 +283,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tau", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device copy (the known-size kernel the --pmc byte counters are "
+                         "scaled by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU per process and has no multi-GPU path; "
@@ -224,9 +297,13 @@ def main():
     res = {}
     for form in ("fp64", "chosen"):
         ctx, dt, phases, losses = time_form(form, args.steps, args.warmup)
-        res[form] = dict(it_s=args.steps / dt, ms=1e3 * dt / args.steps, phases=phases,
-                         loss_last=float(losses[-1]), roof=roofline_for(phases, args.steps, form, d))
+        pairs = count_pairs(ctx)
+        roofs, per = kernel_rooflines(phases, form, d, pairs)
+        res[form] = dict(it_s=args.steps / dt, ms=1e3 * dt / args.steps, per=per, roofs=roofs, pairs=pairs,
+                         loss_last=float(losses[-1]))
         ctx.close()
+    if args.calibrate_copy:
+        copy_calibration()
     taus = {}
     if not args.no_tau:
         for form in ("fp64", "chosen"):
@@ -250,8 +327,9 @@ def main():
                    "forms": {"fp64": "FP64 network + FP64 S build, restart + 50 burn-in calls per iteration",
                              "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"}},
         "fp64_form": {"it_s": round(res["fp64"]["it_s"], 3), "ms_per_step": round(res["fp64"]["ms"], 4),
-                      "phase_ms": res["fp64"]["roof"]["phase_ms"]},
-        "chosen_form": {"phase_ms": ch["roof"]["phase_ms"]},
+                      "phase_ms": {k: round(v, 4) for k, v in res["fp64"]["per"].items()}},
+        "chosen_form": {"phase_ms": {k: round(v, 4) for k, v in ch["per"].items()},
+                        "pairs_per_iteration": ch["pairs"]},
         "speedup_chosen_vs_fp64": round(ch["it_s"] / res["fp64"]["it_s"], 3),
     }
     if taus:
@@ -259,10 +337,19 @@ def main():
         out["tau_int"] = {k: {"logp": round(v[0], 3), "abs2_Lloc": round(v[1], 3)} for k, v in taus.items()}
         out["eff_samples_per_s"] = {
             k: round(Ns / (2 * max(taus[k])) * res[k]["it_s"], 1) for k in ("fp64", "chosen")}
-    r = ch["roof"]
-    out["roofline"] = {"kernel": r["dominant"], "bound": r["bound"], "achieved": round(r["achieved"], 3),
-                       "peak": r["peak"], "unit": r["unit"], "frac": round(r["frac"], 4), "traffic": None,
-                       "algorithmic": r["algorithmic"]}
+    # the dominant kernel of the chosen form (by measured time per iteration)
+    roofs = ch["roofs"]
+    dom = max((k for k in roofs if "frac" in roofs[k]), key=lambda k: roofs[k]["ms"])
+    r = roofs[dom]
+    out["roofline"] = {"kernel": r["kernel"], "bound": r["bound"], "achieved": round(r["achieved"], 4),
+                       "peak": r["peak"], "unit": r["unit"], "frac": round(r["frac"], 5),
+                       "traffic": load_traffic("chosen") if dom == "estimator" else None,
+                       "algorithmic": r["algorithmic"], "ms_per_launch": round(r["ms"], 4)}
+    if dom == "estimator":
+        out["roofline"]["valu_lane_slot_frac"] = round(r["valu_frac"], 4)
+        out["roofline"]["hbm_algorithmic_gbs"] = round(r["hbm_gbs"], 2)
+    out["kernels"] = {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                      for k, v in roofs.items()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -195,6 +195,40 @@ nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
 nqh_status nqh_get_samples(nqh_ctx* ctx, uint64_t* keys_out, int64_t n);
 nqh_status nqh_get_local_values(nqh_ctx* ctx, double* lloc_out, int64_t n);
 
+/* ---- learned proposal ---------------------------------------------------
+ * A MADE over the D = 2N doubled bits (D -> 128 -> 128 -> D, ReLU, sigmoid
+ * Bernoulli conditionals; FP32 parameters, BF16 MFMA products with FP32
+ * accumulation), used as an independence proposal inside the Metropolis
+ * chains with the exact Hastings ratio.  The reference has no learned
+ * proposal; this plugs into the sampler where its `log_prob_bias` hook sits
+ * (Metropolis.jl:54-67,149; precedent OperatorRule.jl:37,57): a global move
+ * y ~ q is accepted iff u - exp(log p(y) - log p(x) + log q(x) - log q(y)) >= 0
+ * is false.  Parameter layout (row-major): W1[128][D] b1[128] W2[128][128]
+ * b2[128] W3[D][128] b3[D]. */
+int64_t nqh_made_num_params(int32_t D);
+/* enable with: `mix` = fraction of stored slices preceded by one global move
+ * (slice i iff floor((i+1) mix) > floor(i mix)); every `retrain_every` sampler
+ * calls, `train_steps` Adam steps (lr, batch) on a ring buffer of the last
+ * `buffer_cap` samples (0: no buffer, no automatic training).  Parameters are
+ * initialised (masked N(0, 1/fan_in), seeded by the context seed).  Needs
+ * nqh_set_ndm; mix = 0 keeps the parameters but turns the global moves off. */
+nqh_status nqh_made_setup(nqh_ctx* ctx, double mix, int32_t retrain_every, int32_t train_steps, int32_t batch,
+                          double lr, int64_t buffer_cap);
+nqh_status nqh_made_set_params(nqh_ctx* ctx, const float* theta, int64_t n);   /* resets Adam */
+nqh_status nqh_made_get_params(nqh_ctx* ctx, float* theta, int64_t n);
+/* gradient of the mean negative log-likelihood over keys (masked), and the NLL */
+nqh_status nqh_made_grad(nqh_ctx* ctx, const uint64_t* keys, int64_t B, float* grad, double* nll);
+/* `steps` Adam steps (Kingma & Ba 2015, beta 0.9/0.999, eps 1e-8); step t uses
+ * the contiguous chunk (t mod (n / batch)) of keys, or of the sample buffer
+ * when keys is NULL; nll_last: mean NLL of the last step's batch */
+nqh_status nqh_made_train(nqh_ctx* ctx, const uint64_t* keys, int64_t n, int32_t steps, int32_t batch, double lr,
+                          double* nll_last);
+nqh_status nqh_made_logq(nqh_ctx* ctx, const uint64_t* keys, int64_t n, float* logq);
+/* n ancestral samples; Philox counter (sample, bit, iter, 3 + 16 tag); logq nullable */
+nqh_status nqh_made_sample(nqh_ctx* ctx, int64_t n, uint32_t iter, uint32_t tag, uint64_t* keys, float* logq);
+/* global moves proposed / accepted by the last sampler call; configurations in the buffer */
+nqh_status nqh_made_stats(nqh_ctx* ctx, int64_t* proposed, int64_t* accepted, int64_t* buffer_fill);
+
 #ifdef __cplusplus
 }
 #endif

@@ -227,17 +227,18 @@ __global__ __launch_bounds__(256) void k_made_logq(const float* th, int D, const
 }
 
 // ---- ancestral sampling of n configurations with their log q;
-// uniforms from Philox, counter (sample, d, iteration, 3)
+// uniforms from Philox, counter (sample, d, iteration, 3 + 16 tag)
 __global__ __launch_bounds__(256) void k_made_sample(const float* th, int D, int64_t n, uint64_t seed, uint32_t iter,
-                                                     uint64_t* keys, float* lq) {
+                                                     const uint32_t* iter_dev, uint32_t tag, uint64_t* keys,
+                                                     float* lq) {
     extern __shared__ __align__(16) unsigned char smem[];
     made_lds& L = *reinterpret_cast<made_lds*>(smem);
     load_weights(L, th, D);
     const int64_t s0 = (int64_t)blockIdx.x * MB;
     for (int e = threadIdx.x; e < MB * 32; e += blockDim.x) L.X[e] = (__bf16)0.f;
-    if (threadIdx.x < MB) L.red[threadIdx.x] = 0.f;
     __syncthreads();
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t it = iter_dev ? *iter_dev : iter;
     // logit d depends on x_<d only (exactly: masked weights are zero), so the
     // logits computed at step d equal those of the finished configuration
     for (int d = 0; d < D; ++d) {
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void k_made_sample(const float* th, int D, int
             const int row = threadIdx.x;
             const float l = L.Lg[row * 32 + d];
             const int64_t sid = s0 + row;
-            philox_out o = philox4x32_10((uint32_t)sid, (uint32_t)d, iter, 3u ^ (uint32_t)(sid >> 32), k0, k1);
+            philox_out o = philox4x32_10((uint32_t)sid, (uint32_t)d, it, 3u + 16u * tag, k0, k1);
             const float u = u01_24(o.x[0]);
             const float p = 1.f / (1.f + expf(-l));
             L.X[row * 32 + d] = (__bf16)((u < p) ? 1.f : 0.f);
@@ -266,7 +267,71 @@ __global__ __launch_bounds__(256) void k_made_sample(const float* th, int D, int
             if (x > 0.5f) k |= 1ull << d;
         }
         keys[s0 + row] = k;
-        lq[s0 + row] = acc;
+        if (lq) lq[s0 + row] = acc;
+    }
+}
+
+// ---- sum of the per-workgroup partial gradients (and log-likelihoods)
+__global__ void k_made_sum_parts(const float* gpart, int nparts, int P, float* g, const float* lpart, float* nll,
+                                 float inv_b) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < P) {
+        float acc = 0.f;
+        for (int w = 0; w < nparts; ++w) acc += gpart[(size_t)w * P + k];
+        g[k] = acc;
+    }
+    if (k == 0 && nll) {
+        float acc = 0.f;
+        for (int w = 0; w < nparts; ++w) acc += lpart[w];
+        *nll = -acc * inv_b;
+    }
+}
+
+// ---- independence move: chain c proposes y_c (drawn from q) and accepts
+// iff u - exp(log p(y) - log p(x) + log q(x) - log q(y)) >= 0 is false,
+// log p = 2 Re log rho; keys[0, n) are the chains' states, keys[n, 2n) the
+// proposals, lp and lq hold their log rho (complex, T) and log q.
+template <typename T>
+__global__ void k_made_accept(const uint64_t* keys, const T* lp, const float* lq, int64_t n, uint64_t seed,
+                              uint32_t iter, const uint32_t* iter_dev, uint32_t slice, uint64_t* chain_state,
+                              unsigned long long* accepted) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (c < n) {
+        const uint32_t it = iter_dev ? *iter_dev : iter;
+        const double dlp = 2.0 * ((double)lp[2 * (n + c)] - (double)lp[2 * c]);
+        const double dlq = (double)lq[c] - (double)lq[n + c];
+        philox_out o = philox4x32_10((uint32_t)c, slice, it, 4u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u = u01_53(o.x[1], o.x[2]);
+        acc = !(u - exp(dlp + dlq) >= 0.0);
+        if (acc) chain_state[c] = keys[n + c];
+    }
+    const unsigned long long m = __ballot(acc);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(accepted, (unsigned long long)__popcll(m));
+}
+
+// ---- dst[i] = src[i], i < n
+__global__ void k_copy_keys(const uint64_t* src, int64_t n, uint64_t* dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ---- append n keys to a ring of `cap` entries; pos[0] = write position,
+// pos[1] = number of valid entries (both kept on the device so the push can
+// sit inside a captured graph)
+__global__ void k_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, int64_t cap, int64_t* pos) {
+    const int64_t p0 = pos[0];
+    const int64_t m = n < cap ? n : cap;
+    const int64_t off = n - m;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        ring[(p0 + i) % cap] = src[off + i];
+}
+
+__global__ void k_ring_advance(int64_t n, int64_t cap, int64_t* pos) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t m = n < cap ? n : cap;
+        pos[0] = (pos[0] + m) % cap;
+        pos[1] = (pos[1] + m) < cap ? (pos[1] + m) : cap;
     }
 }
 
@@ -274,12 +339,33 @@ inline unsigned nwg(int64_t n) { return (unsigned)((n + MB - 1) / MB); }
 
 }  // namespace
 
+namespace {
+template <class K>
+hipError_t allow_big_lds(K kern) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sizeof(made_lds));
+}
+hipError_t made_lds_setup() {
+    static hipError_t once = [] {
+        hipError_t e = allow_big_lds(k_made_grad);
+        if (e == hipSuccess) e = allow_big_lds(k_made_logq);
+        if (e == hipSuccess) e = allow_big_lds(k_made_sample);
+        return e;
+    }();
+    return once;
+}
+}  // namespace
+
+size_t nq_made_lds_bytes() { return sizeof(made_lds); }
+hipError_t nq_made_prepare() { return made_lds_setup(); }
+
 int nq_made_param_count(int D) { return MH * D + MH + MH * MH + MH + D * MH + D; }
 size_t nq_made_grad_parts(int64_t B) { return nwg(B); }
 
 hipError_t nq_launch_made_grad(const float* th, int D, const uint64_t* keys, int B, float* gpart, float* lpart,
                                hipStream_t st) {
     if (D < 2 || D > 32 || B < 1) return hipErrorInvalidValue;
+    if (hipError_t e = made_lds_setup()) return e;
     hipLaunchKernelGGL(k_made_grad, dim3(nwg(B)), dim3(256), sizeof(made_lds), st, th, D, keys, B,
                        1.0f / (float)B, gpart, lpart);
     return hipGetLastError();
@@ -295,13 +381,52 @@ hipError_t nq_launch_made_adam(float* th, float* m, float* v, const float* gpart
 
 hipError_t nq_launch_made_logq(const float* th, int D, const uint64_t* keys, int64_t n, float* lq, hipStream_t st) {
     if (D < 2 || D > 32 || n < 1) return hipErrorInvalidValue;
+    if (hipError_t e = made_lds_setup()) return e;
     hipLaunchKernelGGL(k_made_logq, dim3(nwg(n)), dim3(256), sizeof(made_lds), st, th, D, keys, n, lq);
     return hipGetLastError();
 }
 
-hipError_t nq_launch_made_sample(const float* th, int D, int64_t n, uint64_t seed, uint32_t iter, uint64_t* keys,
-                                 float* lq, hipStream_t st) {
+hipError_t nq_launch_made_sample(const float* th, int D, int64_t n, uint64_t seed, uint32_t iter,
+                                 const uint32_t* iter_dev, uint32_t tag, uint64_t* keys, float* lq, hipStream_t st) {
     if (D < 2 || D > 32 || n < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_made_sample, dim3(nwg(n)), dim3(256), sizeof(made_lds), st, th, D, n, seed, iter, keys, lq);
+    if (hipError_t e = made_lds_setup()) return e;
+    hipLaunchKernelGGL(k_made_sample, dim3(nwg(n)), dim3(256), sizeof(made_lds), st, th, D, n, seed, iter, iter_dev,
+                       tag, keys, lq);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_made_sum_parts(const float* gpart, int nparts, int D, float* g, const float* lpart, float* nll,
+                                    int64_t B, hipStream_t st) {
+    const int P = nq_made_param_count(D);
+    hipLaunchKernelGGL(k_made_sum_parts, dim3((P + 255) / 256), dim3(256), 0, st, gpart, nparts, P, g, lpart, nll,
+                       1.0f / (float)B);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_made_accept(const uint64_t* keys, const void* lp, bool lp_fp64, const float* lq, int64_t n,
+                                 uint64_t seed, uint32_t iter, const uint32_t* iter_dev, uint32_t slice,
+                                 uint64_t* chain_state, unsigned long long* accepted, hipStream_t st) {
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (lp_fp64)
+        hipLaunchKernelGGL(k_made_accept<double>, dim3(g), dim3(256), 0, st, keys, (const double*)lp, lq, n, seed,
+                           iter, iter_dev, slice, chain_state, accepted);
+    else
+        hipLaunchKernelGGL(k_made_accept<float>, dim3(g), dim3(256), 0, st, keys, (const float*)lp, lq, n, seed, iter,
+                           iter_dev, slice, chain_state, accepted);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_copy_keys(const uint64_t* src, int64_t n, uint64_t* dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, n, dst);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, int64_t cap, int64_t* pos,
+                               hipStream_t st) {
+    const int64_t m = n < cap ? n : cap;
+    int64_t g = (m + 255) / 256;
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_ring_push, dim3((unsigned)g), dim3(256), 0, st, src, n, ring, cap, pos);
+    hipLaunchKernelGGL(k_ring_advance, dim3(1), dim3(64), 0, st, n, cap, pos);
     return hipGetLastError();
 }

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     T th_s[US], th_e[US], f_s[US], f_e[US], fpi[US];
     cplx<T> z[US];
     uint64_t acc = 0;
-    uint32_t t = 0;
+    uint32_t t = a.t0;
     uint32_t rx0 = 0, rx1 = 0, rx2 = 0;   // this lane's pre-drawn proposal
     const int calls = a.burn + a.stored;
 
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
             }
         }
         for (int q = 0; q < a.passes; ++q, ++t) {
-            const int slot = (int)(t % (uint32_t)lpc);
+            const int slot = (int)((t - a.t0) % (uint32_t)lpc);
             if (slot == 0) {
                 philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, 1u, k0, k1);
                 rx0 = o.x[0]; rx1 = o.x[1]; rx2 = o.x[2];
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
             }
         }
         if (call >= a.burn && live && g == 0 && a.samples) {
-            const int slice = call - a.burn;
+            const int slice = a.slice0 + call - a.burn;
             a.samples[chain + (int64_t)slice * a.chains] = key;
         }
     }

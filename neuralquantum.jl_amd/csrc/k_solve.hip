@@ -117,41 +117,80 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
 }
 
 // ---- Cholesky step k: factor the diagonal block and solve the panel below it.
-// Every workgroup (one wave) factors the kb x kb diagonal block in LDS
-// (right-looking, lane t owns row t); workgroup 0 writes it back, workgroup
-// b >= 1 then solves rows k + kb + 64 (b - 1) + t of the panel,
-// X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution (the dependent
-// chain per row is kb long, the updates of later columns are independent).
-__global__ __launch_bounds__(64) void k_chol_panel(double* A, int n, int lda, int k, int kb, int* info) {
+// Workgroup of 256 threads (4 waves).  Every workgroup factors the kb x kb
+// diagonal block (padded with the identity to 64 x 64): thread (tr, tc) of a
+// 16 x 16 grid keeps the 16 entries (tr + 16a, tc + 16b) in registers, and
+// step j is one barrier: the owners of column j publish it to LDS
+// (double-buffered), every thread applies the rank-1 update
+// A[r][c] -= A[r][j] A[c][j] / A[j][j] to its own entries and the owners
+// scale column j by 1 / sqrt(A[j][j]).  Workgroup 0 writes the factor back;
+// workgroup b >= 1 solves rows k + kb + 256 (b - 1) + threadIdx.x of the panel,
+// X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution against the
+// factor held in LDS.
+__global__ __launch_bounds__(256) void k_chol_panel(double* A, int n, int lda, int k, int kb, int* info) {
     __shared__ double L[NB][NB + 1];
     __shared__ double rdiag[NB];
-    const int t = threadIdx.x;
-    for (int c = 0; c < kb; ++c)
-        if (t < kb) L[t][c] = A[(int64_t)(k + t) + (int64_t)(k + c) * lda];
-    __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        const double v = L[j][j];
-        const double piv = sqrt(v);
-        if (t == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
-        const double rp = 1.0 / piv;
-        __syncthreads();
-        if (t == j) { L[j][j] = piv; rdiag[j] = rp; }
-        if (t > j && t < kb) {
-            // L[m][j] (m > j) is still unscaled here: scale it on use
-            const double lij = L[t][j] * rp;
-            const double lr = lij * rp;
-            for (int m = j + 1; m <= t; ++m) L[t][m] -= lr * L[m][j];
+    __shared__ double col[2][NB];
+    const int tid = threadIdx.x;
+    const int tr = tid & 15, tc = tid >> 4;
+    double e[4][4];                                // e[a][b] = A[tr + 16a][tc + 16b]
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int r = tr + 16 * a, c = tc + 16 * b;
+            e[a][b] = (r < kb && c < kb && r >= c) ? A[(int64_t)(k + r) + (int64_t)(k + c) * lda]
+                                                    : (r == c ? 1.0 : 0.0);
+        }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int jb = j >> 4, jc = j & 15;
+        double* cb = col[j & 1];
+        if (tc == jc) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) cb[tr + 16 * a] = e[a][jb];
         }
         __syncthreads();
-        if (t > j && t < kb) L[t][j] *= rp;
-        __syncthreads();
+        const double v = cb[j];
+        if (j < kb && tid == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
+        const double rv = 1.0 / v;
+        double cr[4], cc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) { cr[a] = cb[tr + 16 * a] * rv; cc[a] = cb[tc + 16 * a]; }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = tr + 16 * a, c = tc + 16 * b;
+                if (b >= jb && c > j && r >= c) e[a][b] -= cr[a] * cc[b];
+            }
+        if (tc == jc) {
+            const double rp = 1.0 / sqrt(v);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int r = tr + 16 * a;
+                if (r >= j) e[a][jb] *= rp;
+            }
+        }
     }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) L[tr + 16 * a][tc + 16 * b] = e[a][b];
+    __syncthreads();
+    if (tid < NB) rdiag[tid] = 1.0 / L[tid][tid];
     if (blockIdx.x == 0) {
-        for (int c = 0; c < kb; ++c)
-            if (t < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = L[t][c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = tr + 16 * a, c = tc + 16 * b;
+                if (r < kb && c < kb && r >= c) A[(int64_t)(k + r) + (int64_t)(k + c) * lda] = e[a][b];
+            }
         return;
     }
-    const int i = k + kb + (blockIdx.x - 1) * 64 + t;
+    __syncthreads();
+    const int i = k + kb + (blockIdx.x - 1) * 256 + tid;
     if (i >= n) return;
     double x[NB];
 #pragma unroll
@@ -420,7 +459,7 @@ hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st) {
     for (int k = 0; k < n; k += NB) {
         const int kb = (n - k < NB) ? n - k : NB;
         const int rest = nr - k - kb;
-        hipLaunchKernelGGL(k_chol_panel, dim3(1 + nblk(rest, 64)), dim3(64), 0, st, S, nr, lds, k, kb, info);
+        hipLaunchKernelGGL(k_chol_panel, dim3(1 + nblk(rest, 256)), dim3(256), 0, st, S, nr, lds, k, kb, info);
         if (rest > 0) {
             const int T = (rest + 63) / 64;
             hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, nr, lds, k, kb);

@@ -32,6 +32,8 @@ struct samp_args {
     const void* p;
     int64_t chains;
     int passes, burn, stored;  // calls = burn + stored
+    uint32_t t0;               // index of this launch's first proposal within the iteration
+    int slice0;                // slice index of this launch's first stored sample
     int lpc;                   // lanes per chain (power of two)
     int restart;               // draw a fresh uniform configuration per chain
     int compat_stale_last_slice;
@@ -118,3 +120,33 @@ hipError_t nq_launch_check_finite(const double* x, int64_t n, int* flag, hipStre
 // ---- connection listing (k_network.hip)
 hipError_t nq_launch_connections(dev_table tab, int N, const uint64_t* keys, int64_t B, int cap,
                                  int* counts, uint64_t* keys_out, double* mel_out, hipStream_t st);
+
+// ---- learned proposal: MADE over the D = 2N doubled bits (k_made.hip)
+// parameters (FP32, row-major): W1[128][D] b1[128] W2[128][128] b2[128] W3[D][128] b3[D]
+int nq_made_param_count(int D);
+size_t nq_made_lds_bytes();
+// raise the dynamic-LDS limit of the MADE kernels (once per process, before any capture)
+hipError_t nq_made_prepare();
+// number of per-workgroup partials a gradient over B configurations produces
+size_t nq_made_grad_parts(int64_t B);
+// gpart[w][P]: partial gradients of the mean NLL; lpart[w]: partial sums of log q
+hipError_t nq_launch_made_grad(const float* th, int D, const uint64_t* keys, int B, float* gpart, float* lpart,
+                               hipStream_t st);
+// g = sum_w gpart[w]; *nll = -sum_w lpart[w] / B (nll nullable)
+hipError_t nq_launch_made_sum_parts(const float* gpart, int nparts, int D, float* g, const float* lpart, float* nll,
+                                    int64_t B, hipStream_t st);
+// one Adam step on the summed partials, t counted from 1
+hipError_t nq_launch_made_adam(float* th, float* m, float* v, const float* gpart, int nparts, int D, float lr, int t,
+                               hipStream_t st);
+hipError_t nq_launch_made_logq(const float* th, int D, const uint64_t* keys, int64_t n, float* lq, hipStream_t st);
+// n ancestral samples and their log q (lq nullable); iteration from iter_dev when set
+hipError_t nq_launch_made_sample(const float* th, int D, int64_t n, uint64_t seed, uint32_t iter,
+                                 const uint32_t* iter_dev, uint32_t tag, uint64_t* keys, float* lq, hipStream_t st);
+// independence Metropolis-Hastings step of n chains (keys[0,n) states, keys[n,2n) proposals)
+hipError_t nq_launch_made_accept(const uint64_t* keys, const void* lp, bool lp_fp64, const float* lq, int64_t n,
+                                 uint64_t seed, uint32_t iter, const uint32_t* iter_dev, uint32_t slice,
+                                 uint64_t* chain_state, unsigned long long* accepted, hipStream_t st);
+hipError_t nq_launch_copy_keys(const uint64_t* src, int64_t n, uint64_t* dst, hipStream_t st);
+// append the last min(n, cap) keys of src to the ring; pos = {write position, fill}
+hipError_t nq_launch_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, int64_t cap, int64_t* pos,
+                               hipStream_t st);

@@ -68,6 +68,18 @@ struct nqh_ctx {
     int g_kind = -1, g_refine = -1;
     dbuf iter_dev;
 
+    // learned proposal (MADE over the doubled bits)
+    bool made_ready = false;           // parameters allocated and initialised
+    double made_mix = 0.0;             // fraction of stored slices preceded by a global move
+    int made_every = 0, made_steps = 0, made_batch = 1024;
+    float made_lr = 1e-3f;
+    int made_D = 0, made_P = 0, made_t = 0;
+    int64_t ring_cap = 0, ring_fill = 0, ring_pos = 0;   // host mirror of the device ring state
+    int64_t made_calls = 0;            // sampler invocations through the public entry points
+    int64_t made_proposed = 0;         // global moves proposed by the last sampler run
+    dbuf made_th, made_m, made_v, made_gpart, made_lpart, made_g, made_nll, made_keys, made_lp, made_lq, ring,
+        ring_posd, made_tmp;
+
     // per-phase HIP-event timing
     bool profiling = false;
     hipEvent_t ev[NQH_NUM_PHASES][2] = {};
@@ -334,6 +346,20 @@ void copy_cplx(const T* src, double* dst, int64_t n2) {
 extern "C" {
 
 static void drop_graph(nqh_ctx* c);
+static nqh_status made_chain_bufs(nqh_ctx* c);
+static nqh_status made_train_async(nqh_ctx* c, const uint64_t* src, int64_t n, int steps, int batch, float lr);
+
+// retrain the learned proposal on the sample buffer every `made_every`
+// sampler invocations (eagerly, before the invocation is enqueued)
+static nqh_status made_maybe_retrain(nqh_ctx* c) {
+    if (!c->made_ready) return NQH_OK;
+    const bool due = c->made_every > 0 && c->made_steps > 0 && c->made_calls % c->made_every == 0 &&
+                     c->ring_cap > 0 && c->ring_fill >= c->made_batch;
+    c->made_calls += 1;
+    if (!due) return NQH_OK;
+    return made_train_async(c, reinterpret_cast<const uint64_t*>(c->ring.p), c->ring_fill, c->made_steps,
+                            c->made_batch, c->made_lr);
+}
 
 int32_t nqh_abi_version(void) { return NQH_ABI_VERSION; }
 
@@ -361,7 +387,9 @@ void nqh_destroy(nqh_ctx* c) {
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2};
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->made_th, &c->made_m, &c->made_v,
+                   &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
+                   &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp};
     drop_graph(c);
     release(c->iter_dev);
     for (dbuf* b : all) release(*b);
@@ -693,7 +721,7 @@ nqh_status nqh_set_sampler(nqh_ctx* c, int64_t chains, int32_t chain_length, int
     c->chains_initialised = false;
     c->has_sampler = true;
     drop_graph(c);
-    return NQH_OK;
+    return made_chain_bufs(c);
 }
 
 nqh_status nqh_set_iteration(nqh_ctx* c, uint32_t iter) {
@@ -701,6 +729,35 @@ nqh_status nqh_set_iteration(nqh_ctx* c, uint32_t iter) {
     drop_graph(c);
     c->iter = iter;
     return NQH_OK;
+}
+
+// one global (independence) move of every chain, proposals drawn from the MADE
+static nqh_status made_global_move(nqh_ctx* c, uint32_t slice, const uint32_t* iter_dev) {
+    const int64_t n = c->chains;
+    uint64_t* kk = P<uint64_t>(c->made_keys);
+    HIPCHK(c, nq_launch_copy_keys(P<uint64_t>(c->chain_state), n, kk, c->stream));
+    HIPCHK(c, nq_launch_made_sample(P<float>(c->made_th), c->made_D, n, c->seed, c->iter, iter_dev, slice, kk + n,
+                                    nullptr, c->stream));
+    HIPCHK(c, nq_launch_made_logq(P<float>(c->made_th), c->made_D, kk, 2 * n, P<float>(c->made_lq), c->stream));
+    est_args e = make_est(c, kk, 2 * n);
+    e.lp = c->made_lp.p;
+    HIPCHK(c, nq_launch_estimate(e, fp64_net(c), est_grid_for(2 * n), c->stream));
+    HIPCHK(c, nq_launch_made_accept(kk, c->made_lp.p, fp64_net(c), P<float>(c->made_lq), n, c->seed, c->iter,
+                                    iter_dev, slice, P<uint64_t>(c->chain_state),
+                                    P<unsigned long long>(c->acc) + 1, c->stream));
+    return NQH_OK;
+}
+
+// host mirror of the device ring state after one push of a sample set
+static void ring_mirror_advance(nqh_ctx* c) {
+    const int64_t m = std::min(c->chains * c->chain_len, c->ring_cap);
+    c->ring_pos = (c->ring_pos + m) % c->ring_cap;
+    c->ring_fill = std::min(c->ring_cap, c->ring_fill + m);
+}
+
+// slice i is preceded by a global move iff floor((i + 1) mix) > floor(i mix)
+static bool made_scheduled(double mix, int i) {
+    return std::floor((i + 1) * mix + 1e-12) > std::floor(i * mix + 1e-12);
 }
 
 static nqh_status run_sampler(nqh_ctx* c) {
@@ -712,6 +769,8 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.passes = c->passes;
     const bool compat = (c->flags & NQH_SAMPLER_COMPAT_STALE_LAST_SLICE) != 0;
     const bool persistent = (c->flags & NQH_SAMPLER_PERSISTENT) != 0;
+    const bool mixed = c->made_ready && c->made_mix > 0.0;
+    if (mixed && compat) return fail(c, NQH_E_STATE, "the learned proposal and the compat sampler flag are exclusive");
     a.restart = (!persistent || !c->chains_initialised) ? 1 : 0;
     a.burn = a.restart ? c->burn : 0;
     a.stored = compat ? c->chain_len - 1 : c->chain_len;
@@ -724,8 +783,38 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.accepted = P<unsigned long long>(c->acc);
     a.iter_dev = c->use_graph ? P<uint32_t>(c->iter_dev) : nullptr;
     phase_begin(c, PH_SAMPLER);
-    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 2, 0u, c->stream));
-    HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 4, 0u, c->stream));
+    if (!mixed) {
+        HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+    } else {
+        // burn-in (and restart) first, then per stored slice: [global move],
+        // `passes` local proposals, store.  Proposal and slice indices continue
+        // across the launches, so the local moves use the same random streams
+        // as one uninterrupted launch.
+        const int stored = a.stored, burn = a.burn;
+        a.stored = 0;
+        HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+        a.restart = 0;
+        a.burn = 0;
+        a.stored = 1;
+        c->made_proposed = 0;
+        for (int i = 0; i < stored; ++i) {
+            if (made_scheduled(c->made_mix, i)) {
+                nqh_status s = made_global_move(c, (uint32_t)i, a.iter_dev);
+                if (s) return s;
+                c->made_proposed += c->chains;
+            }
+            a.t0 = (uint32_t)((burn + i) * c->passes);
+            a.slice0 = i;
+            HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+        }
+    }
+    if (c->made_ready && c->ring_cap > 0) {
+        const int64_t Ns = c->chains * c->chain_len;
+        HIPCHK(c, nq_launch_ring_push(P<uint64_t>(c->samples), Ns, P<uint64_t>(c->ring), c->ring_cap,
+                                      P<int64_t>(c->ring_posd), c->stream));
+        ring_mirror_advance(c);
+    }
     phase_end(c, PH_SAMPLER);
     c->chains_initialised = true;
     c->iter += 1;
@@ -737,6 +826,7 @@ nqh_status nqh_sample(nqh_ctx* c, uint64_t* keys_out, int64_t* accepted) {
     (void)hipSetDevice(c->device);
     nqh_status s;
     if ((s = need_net(c))) return s;
+    if ((s = made_maybe_retrain(c))) return s;
     if ((s = run_sampler(c))) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
@@ -763,7 +853,9 @@ static nqh_status sample_grad_async(nqh_ctx* c) {
 nqh_status nqh_sample_grad(nqh_ctx* c, double* stats) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
-    nqh_status s = sample_grad_async(c);
+    nqh_status s = made_maybe_retrain(c);
+    if (s) return s;
+    s = sample_grad_async(c);
     if (s) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
@@ -880,6 +972,7 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
         const bool prof = c->profiling;
         c->profiling = false;
         const uint32_t iter0 = c->iter;
+        const int64_t ring_pos0 = c->ring_pos, ring_fill0 = c->ring_fill;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         s = enqueue_iteration(c, eps, kind, refine, lr);
         hipError_t e1 = nq_launch_bump(P<uint32_t>(c->iter_dev), c->stream);
@@ -887,6 +980,8 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
         hipError_t e2 = hipStreamEndCapture(c->stream, &g);
         c->profiling = prof;
         c->iter = iter0;          // the capture did not run anything
+        c->ring_pos = ring_pos0;
+        c->ring_fill = ring_fill0;
         if (s) { if (g) (void)hipGraphDestroy(g); return s; }
         if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(c, e1, "bump"); }
         if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
@@ -896,6 +991,7 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
         c->g_eps = eps; c->g_kind = kind; c->g_refine = refine; c->g_lr = lr;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    if (c->made_ready && c->ring_cap > 0) ring_mirror_advance(c);
     c->iter += 1;
     c->chains_initialised = true;
     c->grad_ready = true;
@@ -914,6 +1010,7 @@ nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
 nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
+    if (nqh_status sm = made_maybe_retrain(c)) return sm;
     if (c->use_graph && !c->profiling) {
         nqh_status s0 = need_net(c);
         if (!s0) s0 = need_problem(c);
@@ -987,6 +1084,264 @@ nqh_status nqh_get_local_values(nqh_ctx* c, double* lloc_out, int64_t n) {
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no local values yet");
     if (n != c->last_ns) return fail(c, NQH_E_ARG, "n must equal the number of samples of the last gradient");
     return download_cplx(c, c->Lloc, lloc_out, 2 * n);
+}
+
+// ---------------------------------------------------------------------------
+// learned proposal
+
+int64_t nqh_made_num_params(int32_t D) { return (D >= 2 && D <= 32) ? nq_made_param_count(D) : -1; }
+
+static nqh_status made_chain_bufs(nqh_ctx* c) {
+    if (!c->made_ready || !c->has_sampler) return NQH_OK;
+    nqh_status s;
+    const int64_t n2 = 2 * c->chains;
+    if ((s = ensure(c, c->made_keys, sizeof(uint64_t) * n2))) return s;
+    if ((s = ensure(c, c->made_lq, sizeof(float) * n2))) return s;
+    if ((s = ensure(c, c->made_lp, 2 * sizeof(double) * n2))) return s;
+    return NQH_OK;
+}
+
+static nqh_status need_made(nqh_ctx* c) {
+    if (!c->made_ready) return fail(c, NQH_E_STATE, "learned proposal not set up (nqh_made_setup)");
+    return NQH_OK;
+}
+
+// masked Gaussian initialisation, weights N(0, 1/fan_in), biases zero
+static void made_init_host(int D, uint64_t seed, std::vector<float>& th) {
+    const int H = 128;
+    th.assign(nq_made_param_count(D), 0.f);
+    uint64_t st = seed ^ 0x9e3779b97f4a7c15ull;
+    auto next = [&st]() {   // splitmix64
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    auto normal = [&]() {   // Box-Muller
+        const double u1 = ((next() >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+        const double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    auto deg = [D](int k) { return (k % (D - 1)) + 1; };
+    float* W1 = th.data();
+    float* W2 = W1 + H * D + H;
+    float* W3 = W2 + H * H + H;
+    for (int k = 0; k < H; ++k)
+        for (int d = 0; d < D; ++d) W1[k * D + d] = deg(k) >= d + 1 ? (float)(normal() / std::sqrt((double)D)) : 0.f;
+    for (int k2 = 0; k2 < H; ++k2)
+        for (int k = 0; k < H; ++k) W2[k2 * H + k] = deg(k2) >= deg(k) ? (float)(normal() / std::sqrt((double)H)) : 0.f;
+    for (int d = 0; d < D; ++d)
+        for (int k = 0; k < H; ++k) W3[d * H + k] = d + 1 > deg(k) ? (float)(normal() / std::sqrt((double)H)) : 0.f;
+}
+
+nqh_status nqh_made_setup(nqh_ctx* c, double mix, int32_t retrain_every, int32_t train_steps, int32_t batch,
+                          double lr, int64_t buffer_cap) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c))) return s;
+    if (!(mix >= 0.0 && mix <= 1.0) || retrain_every < 0 || train_steps < 0 || batch < 1 || !(lr > 0.0) ||
+        buffer_cap < 0)
+        return fail(c, NQH_E_ARG, "need mix in [0,1], retrain_every >= 0, train_steps >= 0, batch >= 1, lr > 0, "
+                                  "buffer_cap >= 0");
+    const int D = 2 * c->d.N;
+    if (D < 2 || D > 32) return fail(c, NQH_E_ARG, "the learned proposal needs 2 <= 2N <= 32");
+    if (buffer_cap > 0 && buffer_cap < batch) return fail(c, NQH_E_ARG, "buffer_cap must be 0 or >= batch");
+    drop_graph(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int P = nq_made_param_count(D);
+    const size_t parts = nq_made_grad_parts(batch);
+    if ((s = ensure(c, c->made_th, sizeof(float) * P))) return s;
+    if ((s = ensure(c, c->made_m, sizeof(float) * P))) return s;
+    if ((s = ensure(c, c->made_v, sizeof(float) * P))) return s;
+    if ((s = ensure(c, c->made_g, sizeof(float) * P))) return s;
+    if ((s = ensure(c, c->made_gpart, sizeof(float) * P * parts))) return s;
+    if ((s = ensure(c, c->made_lpart, sizeof(float) * parts))) return s;
+    if ((s = ensure(c, c->made_nll, sizeof(float) * 4))) return s;
+    if ((s = ensure(c, c->ring_posd, sizeof(int64_t) * 2))) return s;
+    if (buffer_cap > 0 && (s = ensure(c, c->ring, sizeof(uint64_t) * buffer_cap))) return s;
+    std::vector<float> th;
+    made_init_host(D, c->seed, th);
+    HIPCHK(c, hipMemcpy(c->made_th.p, th.data(), sizeof(float) * P, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->made_m.p, 0, sizeof(float) * P));
+    HIPCHK(c, hipMemset(c->made_v.p, 0, sizeof(float) * P));
+    HIPCHK(c, hipMemset(c->ring_posd.p, 0, sizeof(int64_t) * 2));
+    HIPCHK(c, nq_made_prepare());
+    c->made_D = D;
+    c->made_P = P;
+    c->made_t = 0;
+    c->made_mix = mix;
+    c->made_every = retrain_every;
+    c->made_steps = train_steps;
+    c->made_batch = batch;
+    c->made_lr = (float)lr;
+    c->ring_cap = buffer_cap;
+    c->ring_fill = c->ring_pos = 0;
+    c->made_calls = 0;
+    c->made_ready = true;
+    return made_chain_bufs(c);
+}
+
+nqh_status nqh_made_set_params(nqh_ctx* c, const float* theta, int64_t n) {
+    if (!c || !theta) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c))) return s;
+    if (n != c->made_P) return fail(c, NQH_E_ARG, "parameter count mismatch (nqh_made_num_params)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->made_th.p, theta, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->made_m.p, 0, sizeof(float) * n));
+    HIPCHK(c, hipMemset(c->made_v.p, 0, sizeof(float) * n));
+    c->made_t = 0;
+    return NQH_OK;
+}
+
+nqh_status nqh_made_get_params(nqh_ctx* c, float* theta, int64_t n) {
+    if (!c || !theta) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c))) return s;
+    if (n != c->made_P) return fail(c, NQH_E_ARG, "parameter count mismatch (nqh_made_num_params)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(theta, c->made_th.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return NQH_OK;
+}
+
+static nqh_status made_keys_to_device(nqh_ctx* c, const uint64_t* keys, int64_t n) {
+    nqh_status s;
+    if ((s = ensure(c, c->made_tmp, sizeof(uint64_t) * n))) return s;
+    HIPCHK(c, hipMemcpyAsync(c->made_tmp.p, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream));
+    return NQH_OK;
+}
+
+static nqh_status made_check_keys(nqh_ctx* c, const uint64_t* keys, int64_t n) {
+    const uint64_t m = (c->made_D >= 64) ? ~0ull : ((1ull << c->made_D) - 1ull);
+    for (int64_t i = 0; i < n; ++i)
+        if (keys[i] & ~m) return fail(c, NQH_E_ARG, "configuration key has bits above 2N");
+    return NQH_OK;
+}
+
+nqh_status nqh_made_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, float* grad, double* nll) {
+    if (!c || !keys || !grad || B < 1 || B > (int64_t)1 << 30) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c)) || (s = made_check_keys(c, keys, B))) return s;
+    const size_t parts = nq_made_grad_parts(B);
+    if ((s = ensure(c, c->made_gpart, sizeof(float) * c->made_P * parts))) return s;
+    if ((s = ensure(c, c->made_lpart, sizeof(float) * parts))) return s;
+    if ((s = made_keys_to_device(c, keys, B))) return s;
+    HIPCHK(c, nq_launch_made_grad(P<float>(c->made_th), c->made_D, P<uint64_t>(c->made_tmp), (int)B,
+                                  P<float>(c->made_gpart), P<float>(c->made_lpart), c->stream));
+    HIPCHK(c, nq_launch_made_sum_parts(P<float>(c->made_gpart), (int)parts, c->made_D, P<float>(c->made_g),
+                                       P<float>(c->made_lpart), P<float>(c->made_nll), B, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(grad, c->made_g.p, sizeof(float) * c->made_P, hipMemcpyDeviceToHost));
+    if (nll) {
+        float v = 0.f;
+        HIPCHK(c, hipMemcpy(&v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
+        *nll = v;
+    }
+    return NQH_OK;
+}
+
+// `steps` Adam steps; step t uses the contiguous chunk (t mod (n / batch)) of src
+static nqh_status made_train_async(nqh_ctx* c, const uint64_t* src, int64_t n, int steps, int batch, float lr) {
+    const int64_t chunks = n / batch;
+    const size_t parts = nq_made_grad_parts(batch);
+    nqh_status s;
+    if ((s = ensure(c, c->made_gpart, sizeof(float) * c->made_P * parts))) return s;
+    if ((s = ensure(c, c->made_lpart, sizeof(float) * parts))) return s;
+    for (int t = 0; t < steps; ++t) {
+        const uint64_t* kb = src + (int64_t)(c->made_t % chunks) * batch;
+        HIPCHK(c, nq_launch_made_grad(P<float>(c->made_th), c->made_D, kb, batch, P<float>(c->made_gpart),
+                                      P<float>(c->made_lpart), c->stream));
+        if (t == steps - 1)
+            HIPCHK(c, nq_launch_made_sum_parts(P<float>(c->made_gpart), (int)parts, c->made_D, P<float>(c->made_g),
+                                               P<float>(c->made_lpart), P<float>(c->made_nll), batch, c->stream));
+        c->made_t += 1;
+        HIPCHK(c, nq_launch_made_adam(P<float>(c->made_th), P<float>(c->made_m), P<float>(c->made_v),
+                                      P<float>(c->made_gpart), (int)parts, c->made_D, lr, c->made_t, c->stream));
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_made_train(nqh_ctx* c, const uint64_t* keys, int64_t n, int32_t steps, int32_t batch, double lr,
+                          double* nll_last) {
+    if (!c || steps < 0 || batch < 1 || !(lr > 0.0)) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c))) return s;
+    if (steps == 0) return NQH_OK;
+    const uint64_t* src;
+    if (keys) {
+        if (n < batch) return fail(c, NQH_E_ARG, "need n >= batch");
+        if ((s = made_check_keys(c, keys, n)) || (s = made_keys_to_device(c, keys, n))) return s;
+        src = P<uint64_t>(c->made_tmp);
+    } else {
+        n = c->ring_fill;
+        if (n < batch) return fail(c, NQH_E_STATE, "the sample buffer holds fewer than batch configurations");
+        src = P<uint64_t>(c->ring);
+    }
+    if ((s = made_train_async(c, src, n, steps, batch, (float)lr))) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nll_last) {
+        float v = 0.f;
+        HIPCHK(c, hipMemcpy(&v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
+        *nll_last = v;
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_made_logq(nqh_ctx* c, const uint64_t* keys, int64_t n, float* logq) {
+    if (!c || !keys || !logq || n < 0) return NQH_E_ARG;
+    if (n == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c)) || (s = made_check_keys(c, keys, n))) return s;
+    if ((s = made_keys_to_device(c, keys, n))) return s;
+    dbuf out;
+    if ((s = ensure(c, out, sizeof(float) * n))) return s;
+    hipError_t e = nq_launch_made_logq(P<float>(c->made_th), c->made_D, P<uint64_t>(c->made_tmp), n, P<float>(out),
+                                       c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(logq, out.p, sizeof(float) * n, hipMemcpyDeviceToHost);
+    release(out);
+    if (e != hipSuccess) return hip_fail(c, e, "nqh_made_logq");
+    return NQH_OK;
+}
+
+nqh_status nqh_made_sample(nqh_ctx* c, int64_t n, uint32_t iter, uint32_t tag, uint64_t* keys, float* logq) {
+    if (!c || !keys || n < 0) return NQH_E_ARG;
+    if (n == 0) return NQH_OK;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_made(c))) return s;
+    dbuf kd, lq;
+    if ((s = ensure(c, kd, sizeof(uint64_t) * n))) return s;
+    if ((s = ensure(c, lq, sizeof(float) * n))) { release(kd); return s; }
+    hipError_t e = nq_launch_made_sample(P<float>(c->made_th), c->made_D, n, c->seed, iter, nullptr, tag,
+                                         P<uint64_t>(kd), P<float>(lq), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(keys, kd.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && logq) e = hipMemcpy(logq, lq.p, sizeof(float) * n, hipMemcpyDeviceToHost);
+    release(kd);
+    release(lq);
+    if (e != hipSuccess) return hip_fail(c, e, "nqh_made_sample");
+    return NQH_OK;
+}
+
+nqh_status nqh_made_stats(nqh_ctx* c, int64_t* proposed, int64_t* accepted, int64_t* buffer_fill) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (proposed) *proposed = c->made_proposed;
+    if (accepted) {
+        unsigned long long v[2] = {0, 0};
+        if (c->acc.p) HIPCHK(c, hipMemcpy(v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
+        *accepted = (int64_t)v[1];
+    }
+    if (buffer_fill) *buffer_fill = c->ring_fill;
+    return NQH_OK;
 }
 
 }  // extern "C"

@@ -31,6 +31,7 @@ _dp = C.POINTER(C.c_double)
 _u64p = C.POINTER(C.c_uint64)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 
 class NQHError(RuntimeError):
@@ -77,6 +78,16 @@ SIGNATURES = {
     "nqh_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "nqh_get_phase_times": (C.c_int32, [C.c_void_p, _dp, _i64p]),
     "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "nqh_made_num_params": (C.c_int64, [C.c_int32]),
+    "nqh_made_setup": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                   C.c_int64]),
+    "nqh_made_set_params": (C.c_int32, [C.c_void_p, _fp, C.c_int64]),
+    "nqh_made_get_params": (C.c_int32, [C.c_void_p, _fp, C.c_int64]),
+    "nqh_made_grad": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, _fp, _dp]),
+    "nqh_made_train": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, C.c_int32, C.c_int32, C.c_double, _dp]),
+    "nqh_made_logq": (C.c_int32, [C.c_void_p, _u64p, C.c_int64, _fp]),
+    "nqh_made_sample": (C.c_int32, [C.c_void_p, C.c_int64, C.c_uint32, C.c_uint32, _u64p, _fp]),
+    "nqh_made_stats": (C.c_int32, [C.c_void_p, _i64p, _i64p, _i64p]),
 }
 
 NUM_PHASES = 6
@@ -277,3 +288,49 @@ class Context:
         L = np.empty(2 * n)
         self._chk(self.lib.nqh_get_local_values(self.h, _p(L, _dp), n))
         return L.view(np.complex128)
+
+    # -- learned proposal (MADE over the 2N doubled bits) --------------------
+    def made_setup(self, mix=0.5, retrain_every=20, train_steps=50, batch=1024, lr=1e-3, buffer_cap=65536):
+        self._chk(self.lib.nqh_made_setup(self.h, mix, retrain_every, train_steps, batch, lr, buffer_cap))
+        self.made_np = int(self.lib.nqh_made_num_params(2 * self.N))
+
+    def made_set_params(self, th):
+        th = np.ascontiguousarray(th, dtype=np.float32)
+        self._chk(self.lib.nqh_made_set_params(self.h, _p(th, _fp), len(th)))
+
+    def made_get_params(self):
+        th = np.empty(self.made_np, dtype=np.float32)
+        self._chk(self.lib.nqh_made_get_params(self.h, _p(th, _fp), len(th)))
+        return th
+
+    def made_grad(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        g = np.empty(self.made_np, dtype=np.float32)
+        nll = C.c_double(0.0)
+        self._chk(self.lib.nqh_made_grad(self.h, _p(keys, _u64p), len(keys), _p(g, _fp), C.byref(nll)))
+        return float(nll.value), g
+
+    def made_train(self, keys, steps, batch, lr):
+        """Adam steps on `keys` (None: on the context's sample buffer)."""
+        k = None if keys is None else np.ascontiguousarray(keys, dtype=np.uint64)
+        nll = C.c_double(0.0)
+        self._chk(self.lib.nqh_made_train(self.h, _p(k, _u64p), 0 if k is None else len(k), steps, batch, lr,
+                                          C.byref(nll)))
+        return float(nll.value)
+
+    def made_logq(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        lq = np.empty(len(keys), dtype=np.float32)
+        self._chk(self.lib.nqh_made_logq(self.h, _p(keys, _u64p), len(keys), _p(lq, _fp)))
+        return lq
+
+    def made_sample(self, n, it=0, tag=0):
+        keys = np.empty(n, dtype=np.uint64)
+        lq = np.empty(n, dtype=np.float32)
+        self._chk(self.lib.nqh_made_sample(self.h, n, it, tag, _p(keys, _u64p), _p(lq, _fp)))
+        return keys, lq
+
+    def made_stats(self):
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        self._chk(self.lib.nqh_made_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
