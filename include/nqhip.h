@@ -108,7 +108,20 @@ nqh_status nqh_set_problem(nqh_ctx* ctx, const nqh_optable* table);
 /* NDM(Float64, N, alpha_h, alpha_a, af_softplus) with nh = alpha_h N hidden and
  * na = alpha_a N ancilla units (NDM.jl:50-63); act 0 = softplus (the default) */
 nqh_status nqh_set_ndm(nqh_ctx* ctx, int32_t N, int32_t nh, int32_t na, int32_t act);
+
+/* RBMSplit(T, N, alpha) with nh = alpha N hidden units (RBMSplit.jl:33-40):
+ * complex parameters ar[N] ac[N] b[nh] Wr[nh x N] Wc[nh x N] (functor order,
+ * column-major), passed to nqh_set_params interleaved (re, im).  The SR system
+ * is complex Hermitian (SRDirect.jl:35-45 with a complex parameter type):
+ * nqh_precondition returns Delta interleaved; nqh_grad_from_samples returns F
+ * and Obar interleaved and S as its real embedding [[Re S, -Im S], [Im S, Re S]]
+ * (2 npc x 2 npc). */
+nqh_status nqh_set_rbmsplit(nqh_ctx* ctx, int32_t N, int32_t nh);
+
+/* doubles in the flat parameter vector (NDM: Np; RBMSplit: 2 x complex count) */
 int64_t nqh_num_params(const nqh_ctx* ctx);
+/* complex entries of one O row (NDM: Np; RBMSplit: complex parameter count) */
+int64_t nqh_num_derivs(const nqh_ctx* ctx);
 
 /* parameter upload / download (the Julia side keeps the canonical arrays and
  * re-uploads after Optimisers.update!, or reads back after nqh_update) */

@@ -1,0 +1,510 @@
+// RBMSplit density-matrix network (src/Networks/MixedDensityMatrix/RBMSplit.jl,
+// RBMSplitBatched.jl) on the steady-state path:
+//
+//   log rho(sr, sc) = ar . sr + ac . sc + sum_h logL(theta_h),
+//   theta = b + Wr sr + Wc sc,                            (RBMSplit.jl:45-50,84-102)
+//
+// complex parameters in the reference's order ar[N] ac[N] b[nh] Wr[nh x N]
+// Wc[nh x N] (column-major), stored interleaved (re, im) in precision T.  The
+// network is analytic, so O is the holomorphic derivative (RBMSplit.jl:104-123)
+// and the SR system is complex Hermitian (SRDirect.jl:26-46 with a complex
+// parameter type): S_jk = sum_s conj(Oc_js) Oc_ks / Ns, F = conj(L)^T dL / Ns
+// - C Obar.  It is solved as the real symmetric system
+//   [[A, -B], [B, A]] [x; y] = [Re F; Im F],   S = A + i B,   Delta = x + i y
+// whose matrix is assembled from the Gram matrix of the rows [Re Oc | Im Oc].
+//
+// Kernels
+//   k_rbm_estimate   one wavefront per sample, lane h owns hidden unit h: log
+//                    rho, the O row [Re O | Im O], L_loc, and the force
+//                    sum_s conj(L_loc)/Ns sum_c W_c O(c) accumulated per unit
+//                    (dense part x_s e_h plus sparse flipped-site terms) in FP64
+//                    LDS; or per-sample dL_loc (parity mode, one wave per block)
+//   k_rbm_metropolis LocalRule Metropolis on |rho|^2, the same random streams
+//                    and schedule as the NDM sampler; a group of lanes per chain
+//                    keeps theta in registers
+//   k_embed_complex  in-place [[G_rr, G_ri], [G_ir, G_ii]] -> [[A, -B], [B, A]]
+//   k_force_all      F[k] = sum of partials - C mean[k] for every column
+//   k_update_complex p -= lr (x + i y)
+#include "nq_device.h"
+#include "nq_kernels.h"
+#include "nq_table.h"
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T xv(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(1) : T(-1); }
+template <typename T>
+__device__ __forceinline__ T cflip(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(-2) : T(2); }
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cexp_(cplx<T> z) {
+    T s, c;
+    nq_sincos(z.im, &s, &c);
+    const T e = nq_exp(z.re);
+    return cmk(e * c, e * s);
+}
+
+template <typename T, int NMAX, int HS>
+__global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, nh = a.nh, npc = a.npc;
+    cplx<T>* sp = reinterpret_cast<cplx<T>*>(smem);
+    double* sF = reinterpret_cast<double*>(smem + a.lds_param_bytes);   // [Re (npc) | Im (npc)]
+    const bool useF = a.Fpart != nullptr || a.G != nullptr;
+    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + (useF ? 2 * npc * (int)sizeof(double) : 0));
+    const cplx<T>* gp = reinterpret_cast<const cplx<T>*>(a.p);
+    for (int k = threadIdx.x; k < npc; k += blockDim.x) sp[k] = gp[k];
+    if (useF)
+        for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) sF[k] = 0.0;
+    __syncthreads();
+    auto fadd = [&](int k, cplx<T> v) {
+        atomicAdd(&sF[k], (double)v.re);
+        atomicAdd(&sF[npc + k], (double)v.im);
+    };
+    const int oar = 0, oac = N, ob = 2 * N, owr = 2 * N + nh, owc = 2 * N + nh + nh * N;
+
+    const int lane0 = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int64_t gw = (int64_t)blockIdx.x * wpb + wave;
+    const int64_t nw = (int64_t)gridDim.x * wpb;
+    const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
+    cplx<T>* wbuf = sW + wave * a.max_conn;
+    const int R = 2 * npc;
+
+    for (int64_t s = gw; s < a.Ns; s += nw) {
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const uint64_t key = a.keys[s];
+        const uint32_t ks = (uint32_t)key & pmask;
+        const uint32_t ke = (uint32_t)(key >> N) & pmask;
+
+        cplx<T> th[HS], f[HS], fp[HS];
+#pragma unroll
+        for (int r = 0; r < HS; ++r) {
+            const int h = lane + 64 * r;
+            th[r] = f[r] = fp[r] = cmk(T(0), T(0));
+            if (h < nh) {
+                cplx<T> t = sp[ob + h];
+#pragma unroll
+                for (int i = 0; i < NMAX; ++i)
+                    if (i < N) {
+                        cfma_r(t, sp[owr + h + nh * i], xv<T>(ks, i));
+                        cfma_r(t, sp[owc + h + nh * i], xv<T>(ke, i));
+                    }
+                th[r] = t;
+                csoftplus_and_sigmoid(t, &f[r], &fp[r]);
+            }
+        }
+        // log rho
+        {
+            cplx<T> acc = cmk(T(0), T(0));
+#pragma unroll
+            for (int r = 0; r < HS; ++r) acc = cadd(acc, f[r]);
+            if (lane < N) {
+                cfma_r(acc, sp[oar + lane], xv<T>(ks, lane));
+                cfma_r(acc, sp[oac + lane], xv<T>(ke, lane));
+            }
+            acc.re = wave_sum(acc.re);
+            acc.im = wave_sum(acc.im);
+            if (lane == 0 && a.lp) {
+                T* lp = reinterpret_cast<T*>(a.lp);
+                lp[2 * s] = acc.re;
+                lp[2 * s + 1] = acc.im;
+            }
+        }
+        // O row: [Re O | Im O], reference order
+        if (a.Y) {
+            T* y = reinterpret_cast<T*>(a.Y) + (int64_t)s * R;
+#pragma unroll
+            for (int r = 0; r < HS; ++r) {
+                const int h = lane + 64 * r;
+                if (h < nh) {
+                    y[ob + h] = fp[r].re;
+                    y[npc + ob + h] = fp[r].im;
+#pragma unroll
+                    for (int i = 0; i < NMAX; ++i)
+                        if (i < N) {
+                            const T xs = xv<T>(ks, i), xe = xv<T>(ke, i);
+                            y[owr + h + nh * i] = fp[r].re * xs;
+                            y[npc + owr + h + nh * i] = fp[r].im * xs;
+                            y[owc + h + nh * i] = fp[r].re * xe;
+                            y[npc + owc + h + nh * i] = fp[r].im * xe;
+                        }
+                }
+            }
+            if (lane < N) {
+                y[oar + lane] = xv<T>(ks, lane);
+                y[npc + oar + lane] = T(0);
+                y[oac + lane] = xv<T>(ke, lane);
+                y[npc + oac + lane] = T(0);
+            }
+        }
+        if (!a.do_conn) continue;
+
+        // ---- pass 1: W_c = mel exp(log rho_c - log rho_s), L_loc
+        cplx<T> Lloc = cmk(T(0), T(0));
+        int nc = 0;
+        table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+            if ((ms | me) == 0u) {
+                Lloc = cadd(Lloc, mel);
+                if (nc < a.max_conn) wbuf[nc] = mel;
+                ++nc;
+                return;
+            }
+            cplx<T> dl = cmk(T(0), T(0));
+#pragma unroll
+            for (int r = 0; r < HS; ++r) {
+                const int h = lane + 64 * r;
+                if (h < nh) {
+                    cplx<T> t = th[r];
+                    for (uint32_t m = ms; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        cfma_r(t, sp[owr + h + nh * i], cflip<T>(ks, i));
+                    }
+                    for (uint32_t m = me; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        cfma_r(t, sp[owc + h + nh * i], cflip<T>(ke, i));
+                    }
+                    cplx<T> f2, p2;
+                    csoftplus_and_sigmoid(t, &f2, &p2);
+                    dl = cadd(dl, csub(f2, f[r]));
+                }
+            }
+            dl.re = wave_sum(dl.re);
+            dl.im = wave_sum(dl.im);
+            for (uint32_t m = ms; m; m &= m - 1) {
+                const int i = __builtin_ctz(m);
+                cfma_r(dl, sp[oar + i], cflip<T>(ks, i));
+            }
+            for (uint32_t m = me; m; m &= m - 1) {
+                const int i = __builtin_ctz(m);
+                cfma_r(dl, sp[oac + i], cflip<T>(ke, i));
+            }
+            const cplx<T> W = cmul(mel, cexp_(dl));
+            Lloc = cadd(Lloc, W);
+            if (nc < a.max_conn) wbuf[nc] = W;
+            ++nc;
+        });
+        if (lane == 0 && a.Lloc) {
+            T* lo = reinterpret_cast<T*>(a.Lloc);
+            lo[2 * s] = Lloc.re;
+            lo[2 * s + 1] = Lloc.im;
+        }
+        if (!useF) continue;
+
+        // ---- pass 2: sum_c V W_c O(c), V = conj(L_loc) / Ns (or 1 for dL_loc)
+        const cplx<T> V = a.G ? cmk(T(1), T(0)) : cmk((T)((double)Lloc.re * a.invNs), (T)(-(double)Lloc.im * a.invNs));
+        cplx<T> e[HS];
+#pragma unroll
+        for (int r = 0; r < HS; ++r) e[r] = cmk(T(0), T(0));
+        cplx<T> sumVW = cmk(T(0), T(0));
+        int c = 0;
+        table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+            (void)mel;
+            const cplx<T> VW = cmul(V, c < a.max_conn ? wbuf[c] : cmk(T(0), T(0)));
+            ++c;
+            sumVW = cadd(sumVW, VW);
+            if ((ms | me) == 0u) {
+#pragma unroll
+                for (int r = 0; r < HS; ++r) e[r] = cadd(e[r], cmul(VW, fp[r]));
+                return;
+            }
+#pragma unroll
+            for (int r = 0; r < HS; ++r) {
+                const int h = lane + 64 * r;
+                if (h < nh) {
+                    cplx<T> t = th[r];
+                    for (uint32_t m = ms; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        cfma_r(t, sp[owr + h + nh * i], cflip<T>(ks, i));
+                    }
+                    for (uint32_t m = me; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        cfma_r(t, sp[owc + h + nh * i], cflip<T>(ke, i));
+                    }
+                    cplx<T> f2, p2;
+                    csoftplus_and_sigmoid(t, &f2, &p2);
+                    const cplx<T> g = cmul(VW, p2);
+                    e[r] = cadd(e[r], g);
+                    // x_c = x_s + Delta on the flipped sites
+                    for (uint32_t m = ms; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        fadd(owr + h + nh * i, cscale(g, cflip<T>(ks, i)));
+                    }
+                    for (uint32_t m = me; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        fadd(owc + h + nh * i, cscale(g, cflip<T>(ke, i)));
+                    }
+                }
+            }
+            if (lane < N) {
+                if ((ms >> lane) & 1u) fadd(oar + lane, cscale(VW, cflip<T>(ks, lane)));
+                if ((me >> lane) & 1u) fadd(oac + lane, cscale(VW, cflip<T>(ke, lane)));
+            }
+        });
+        // dense parts
+#pragma unroll
+        for (int r = 0; r < HS; ++r) {
+            const int h = lane + 64 * r;
+            if (h < nh) {
+                fadd(ob + h, e[r]);
+#pragma unroll
+                for (int i = 0; i < NMAX; ++i)
+                    if (i < N) {
+                        fadd(owr + h + nh * i, cscale(e[r], xv<T>(ks, i)));
+                        fadd(owc + h + nh * i, cscale(e[r], xv<T>(ke, i)));
+                    }
+            }
+        }
+        if (lane < N) {
+            fadd(oar + lane, cscale(sumVW, xv<T>(ks, lane)));
+            fadd(oac + lane, cscale(sumVW, xv<T>(ke, lane)));
+        }
+        if (a.G) {   // one wave per block: sF belongs to this sample
+            __syncthreads();
+            double* g = a.G + (int64_t)s * 2 * npc;
+            for (int k = lane; k < npc; k += 64) {
+                g[2 * k] = sF[k];
+                g[2 * k + 1] = sF[npc + k];
+                sF[k] = 0.0;
+                sF[npc + k] = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+    if (a.Fpart && !a.G) {
+        __syncthreads();
+        double* out = a.Fpart + (int64_t)blockIdx.x * 2 * npc;
+        for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) out[k] = sF[k];
+    }
+}
+
+// ---- Metropolis (LocalRule) on |rho|^2; log p = 2 Re log rho.  Lane g of a
+// group of lpc lanes owns hidden units h = g + lpc r.
+template <typename T, int US>
+__global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, nh = a.nh, npc = a.npc;
+    cplx<T>* sp = reinterpret_cast<cplx<T>*>(smem);
+    const cplx<T>* gp = reinterpret_cast<const cplx<T>*>(a.p);
+    for (int k = threadIdx.x; k < npc; k += blockDim.x) sp[k] = gp[k];
+    __syncthreads();
+    const int oar = 0, oac = N, ob = 2 * N, owr = 2 * N + nh, owc = 2 * N + nh + nh * N;
+
+    const int lpc = a.lpc;
+    const int lane = threadIdx.x & 63;
+    const int g = lane & (lpc - 1);
+    const int gbase = lane - g;
+    const int cpw = 64 / lpc;
+    const int64_t wave_glob = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t chain = wave_glob * cpw + lane / lpc;
+    const bool live = chain < a.chains;
+    const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t iter = a.iter_dev ? *a.iter_dev : a.iter;
+
+    uint64_t key = 0;
+    if (live) {
+        if (a.restart) {
+            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, 0u, k0, k1);
+            const uint64_t m2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1ull);
+            key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
+        } else {
+            key = a.chain_state[chain];
+        }
+    }
+    cplx<T> th[US];
+    T fr[US];
+    uint64_t acc = 0;
+    uint32_t t = a.t0;
+    uint32_t rx0 = 0, rx1 = 0, rx2 = 0;
+    const int calls = a.burn + a.stored;
+    for (int call = 0; call < calls; ++call) {
+        const uint32_t ks = (uint32_t)key & pmask, ke = (uint32_t)(key >> N) & pmask;
+#pragma unroll
+        for (int r = 0; r < US; ++r) {
+            const int h = g + lpc * r;
+            th[r] = cmk(T(0), T(0));
+            fr[r] = T(0);
+            if (h < nh) {
+                cplx<T> z = sp[ob + h];
+                for (int i = 0; i < N; ++i) {
+                    cfma_r(z, sp[owr + h + nh * i], xv<T>(ks, i));
+                    cfma_r(z, sp[owc + h + nh * i], xv<T>(ke, i));
+                }
+                th[r] = z;
+                fr[r] = csoftplus_re(z);
+            }
+        }
+        for (int q = 0; q < a.passes; ++q, ++t) {
+            const int slot = (int)((t - a.t0) % (uint32_t)lpc);
+            if (slot == 0) {
+                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, 1u, k0, k1);
+                rx0 = o.x[0]; rx1 = o.x[1]; rx2 = o.x[2];
+            }
+            const uint32_t x0 = __shfl(rx0, gbase + slot, 64);
+            const uint32_t x1 = __shfl(rx1, gbase + slot, 64);
+            const uint32_t x2 = __shfl(rx2, gbase + slot, 64);
+            const int site = (int)(((uint64_t)x0 * (uint64_t)(2 * N)) >> 32);
+            const bool col = site >= N;
+            const int j = col ? site - N : site;
+            const uint32_t bits = col ? (uint32_t)(key >> N) : (uint32_t)key;
+            const T c = ((bits >> j) & 1u) ? T(-2) : T(2);
+            const int ow = col ? owc : owr;
+            T delta = T(0);
+            cplx<T> nth[US];
+            T nfr[US];
+#pragma unroll
+            for (int r = 0; r < US; ++r) {
+                const int h = g + lpc * r;
+                nth[r] = th[r];
+                nfr[r] = fr[r];
+                if (h < nh) {
+                    cfma_r(nth[r], sp[ow + h + nh * j], c);
+                    nfr[r] = csoftplus_re(nth[r]);
+                    delta += nfr[r] - fr[r];
+                }
+            }
+            delta = group_sum(delta, lpc);
+            delta += c * sp[(col ? oac : oar) + j].re;
+            delta *= T(2);
+            bool accept;
+            if constexpr (sizeof(T) == 8) {
+                const double u = u01_53(x1, x2);
+                accept = !(u - exp((double)delta) >= 0.0);
+            } else {
+                const float u = u01_24(x1);
+                accept = !(u - expf((float)delta) >= 0.0f);
+            }
+            if (accept && live) {
+                key ^= (1ull << site);
+                ++acc;
+#pragma unroll
+                for (int r = 0; r < US; ++r) { th[r] = nth[r]; fr[r] = nfr[r]; }
+            }
+        }
+        if (call >= a.burn && live && g == 0 && a.samples) {
+            const int slice = a.slice0 + call - a.burn;
+            a.samples[chain + (int64_t)slice * a.chains] = key;
+        }
+    }
+    if (live && g == 0) {
+        if (a.chain_state) a.chain_state[chain] = key;
+        if (a.compat_stale_last_slice && a.samples)
+            a.samples[chain + (int64_t)(a.slice0 + a.stored) * a.chains] = 0ull;
+        if (a.accepted) atomicAdd(a.accepted, (unsigned long long)acc);
+    }
+}
+
+// [[G_rr, G_ri], [G_ir, G_ii]] (n = npc blocks of a 2n x 2n symmetric G, column-major,
+// leading dimension ld) -> [[A, -B], [B, A]], A = G_rr + G_ii, B = G_ri - G_ir.
+// Thread (j, k) owns the four positions (j,k), (n+j,n+k), (n+j,k), (j,n+k).
+__global__ void k_embed_complex(double* S, int n, int ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= n || k >= n) return;
+    const int64_t L = ld;
+    const double grr = S[j + k * L], gii = S[(n + j) + (n + k) * L];
+    const double gri = S[j + (n + k) * L], gir = S[(n + j) + k * L];
+    const double A = grr + gii, B = gri - gir;
+    S[j + k * L] = A;
+    S[(n + j) + (n + k) * L] = A;
+    S[(n + j) + k * L] = B;
+    S[j + (n + k) * L] = -B;
+}
+
+// F[k] = sum_g part2[g][k] - C mean[k] for every column k < n
+__global__ void k_force_all(const double* part2, int ngroups, const double* mu, const double* stats, int n,
+                            double* F) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double acc = 0.0;
+    for (int g = 0; g < ngroups; ++g) acc += part2[(int64_t)g * n + k];
+    F[k] = acc - stats[0] * mu[k];
+}
+
+// p[k] -= lr (delta[k] + i delta[npc + k]) on the interleaved complex parameters
+__global__ void k_update_complex(double* p64, float* p32, const double* delta, int npc, double lr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npc) return;
+    const double re = p64[2 * k] - lr * delta[k];
+    const double im = p64[2 * k + 1] - lr * delta[npc + k];
+    p64[2 * k] = re;
+    p64[2 * k + 1] = im;
+    p32[2 * k] = (float)re;
+    p32[2 * k + 1] = (float)im;
+}
+
+template <typename T, int NMAX, int HS>
+hipError_t est_t(const rbm_args& a, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_rbm_estimate<T, NMAX, HS>), dim3(grid), dim3(a.G ? 64 : 256), rbm_est_lds_bytes(a, sizeof(T)),
+                       st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int NMAX>
+hipError_t est_n(const rbm_args& a, int grid, hipStream_t st) {
+    const int hs = (a.nh + 63) / 64;
+    if (hs <= 1) return est_t<T, NMAX, 1>(a, grid, st);
+    if (hs <= 2) return est_t<T, NMAX, 2>(a, grid, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int US>
+hipError_t samp_t(const rbm_samp_args& a, hipStream_t st) {
+    const int cpw = 64 / a.lpc;
+    const int64_t waves = (a.chains + cpw - 1) / cpw;
+    const int64_t grid = (waves + 3) / 4;
+    hipLaunchKernelGGL((k_rbm_metropolis<T, US>), dim3((unsigned)grid), dim3(256), 2 * sizeof(T) * (size_t)a.npc, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t rbm_est_lds_bytes(const rbm_args& a, size_t elt) {
+    return (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? 2 * sizeof(double) * (size_t)a.npc : 0) +
+           4 * (size_t)a.max_conn * 2 * elt;
+}
+
+hipError_t nq_launch_rbm_estimate(const rbm_args& a, bool fp64, int grid, hipStream_t st) {
+    if (a.N > 16 || a.nh < 1) return hipErrorInvalidValue;
+    if (rbm_est_lds_bytes(a, fp64 ? 8 : 4) > 160 * 1024) return hipErrorInvalidValue;
+    if (fp64) return a.N <= 8 ? est_n<double, 8>(a, grid, st) : est_n<double, 16>(a, grid, st);
+    return a.N <= 8 ? est_n<float, 8>(a, grid, st) : est_n<float, 16>(a, grid, st);
+}
+
+hipError_t nq_launch_rbm_metropolis(const rbm_samp_args& a, bool fp64, hipStream_t st) {
+    const int us = (a.nh + a.lpc - 1) / a.lpc;
+    if (a.N > 32 || a.lpc < 1 || a.lpc > 64 || (a.lpc & (a.lpc - 1))) return hipErrorInvalidValue;
+    if (fp64) {
+        if (us <= 1) return samp_t<double, 1>(a, st);
+        if (us <= 2) return samp_t<double, 2>(a, st);
+        if (us <= 4) return samp_t<double, 4>(a, st);
+    } else {
+        if (us <= 1) return samp_t<float, 1>(a, st);
+        if (us <= 2) return samp_t<float, 2>(a, st);
+        if (us <= 4) return samp_t<float, 4>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t nq_launch_embed_complex(double* S, int npc, int ld, hipStream_t st) {
+    hipLaunchKernelGGL(k_embed_complex, dim3((npc + 255) / 256, npc), dim3(256), 0, st, S, npc, ld);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_force_all(const double* Fpart, int nblk, double* part2, const double* mu, const double* stats,
+                               int n, double* F, hipStream_t st) {
+    hipError_t e = nq_launch_force_part(Fpart, nblk, n, part2, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_force_all, dim3((n + 255) / 256), dim3(256), 0, st, part2, NQ_FORCE_GROUPS, mu, stats, n,
+                       F);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL(k_update_complex, dim3((npc + 255) / 256), dim3(256), 0, st, p64, p32, delta, npc, lr);
+    return hipGetLastError();
+}

@@ -447,6 +447,13 @@ hipError_t nq_launch_force_finish(const double* Fpart, int nb, double* part2, co
     return hipGetLastError();
 }
 
+hipError_t nq_launch_force_part(const double* Fpart, int nb, int n, double* part2, hipStream_t st) {
+    const int groups = NQ_FORCE_GROUPS;
+    const int per = (nb + groups - 1) / groups;
+    hipLaunchKernelGGL(k_force_part, dim3(nblk(n, 256), groups), dim3(256), 0, st, Fpart, nb, per, n, part2);
+    return hipGetLastError();
+}
+
 hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st) {
     hipLaunchKernelGGL(k_shift, dim3(nblk(n, 256)), dim3(256), 0, st, S, n, lds, eps, kind);
     return hipGetLastError();

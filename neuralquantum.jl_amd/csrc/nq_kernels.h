@@ -90,6 +90,8 @@ hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, do
 #define NQ_FORCE_GROUPS 32
 hipError_t nq_launch_force_finish(const double* Fpart, int nblk, double* part2, const double* mu,
                                   const double* stats, ndm_dims d, double* F, hipStream_t st);
+// part2[g][k] = sum of the Fpart rows of group g (NQ_FORCE_GROUPS groups), k < n
+hipError_t nq_launch_force_part(const double* Fpart, int nb, int n, double* part2, hipStream_t st);
 // S[k,k] += eps (kind 0) or eps * S[k,k] (kind 1)
 hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st);
 // in-place lower Cholesky of S (column-major, n x n, leading dimension lds);
@@ -150,3 +152,50 @@ hipError_t nq_launch_copy_keys(const uint64_t* src, int64_t n, uint64_t* dst, hi
 // append the last min(n, cap) keys of src to the ring; pos = {write position, fill}
 hipError_t nq_launch_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, int64_t cap, int64_t* pos,
                                hipStream_t st);
+
+// ---- RBMSplit network (k_rbm.hip): complex parameters ar[N] ac[N] b[nh] Wr[nh x N]
+// Wc[nh x N], interleaved (re, im) in precision T, reference order
+struct rbm_args {
+    int N, nh, npc;           // npc = 2N + nh + 2 nh N complex parameters
+    const void* p;
+    dev_table tab;
+    const uint64_t* keys;
+    int64_t Ns;
+    double invNs;
+    int do_conn;
+    int lds_param_bytes;      // >= 2 npc sizeof(T), multiple of 16
+    int max_conn;
+    void* lp;                 // 2 Ns T, nullable
+    void* Y;                  // Ns x 2 npc T: [Re O | Im O], nullable
+    void* Lloc;               // 2 Ns T, nullable
+    double* Fpart;            // gridDim x 2 npc: [Re | Im] force partials, nullable
+    double* G;                // Ns x npc complex FP64 dL_loc (parity mode), nullable, pre-zeroed
+};
+size_t rbm_est_lds_bytes(const rbm_args& a, size_t elt);
+hipError_t nq_launch_rbm_estimate(const rbm_args& a, bool fp64, int grid, hipStream_t st);
+struct rbm_samp_args {
+    int N, nh, npc;
+    const void* p;
+    int64_t chains;
+    int passes, burn, stored;
+    uint32_t t0;
+    int slice0;
+    int lpc;
+    int restart;
+    int compat_stale_last_slice;
+    uint64_t seed;
+    uint32_t iter;
+    const uint32_t* iter_dev;
+    uint64_t* chain_state;
+    uint64_t* samples;
+    unsigned long long* accepted;
+};
+hipError_t nq_launch_rbm_metropolis(const rbm_samp_args& a, bool fp64, hipStream_t st);
+// in place: [[G_rr, G_ri], [G_ir, G_ii]] -> [[A, -B], [B, A]] (A = G_rr + G_ii, B = G_ri - G_ir)
+hipError_t nq_launch_embed_complex(double* S, int npc, int ld, hipStream_t st);
+// F[k] = sum_b Fpart[b][k] - C mu[k], k < n
+hipError_t nq_launch_force_all(const double* Fpart, int nblk, double* part2, const double* mu, const double* stats,
+                               int n, double* F, hipStream_t st);
+// p -= lr (delta[0:npc] + i delta[npc:2 npc]); refresh the FP32 copy
+hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,
+                                    hipStream_t st);

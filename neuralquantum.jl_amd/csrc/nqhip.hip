@@ -34,9 +34,11 @@ struct nqh_ctx {
     dev_table tab{};
     int max_conn = 0;
 
-    // network
+    // network: 0 = NDM (d), 1 = RBMSplit (rbm_*; complex parameters interleaved)
     bool has_net = false;
+    int net_kind = 0;
     ndm_dims d{};
+    int rbm_N = 0, rbm_nh = 0, rbm_npc = 0;
     dbuf p64, p32;
     int net_prec = NQH_PREC_FP64;
     int s_prec = NQH_PREC_FP64;
@@ -155,6 +157,14 @@ T* P(dbuf& b) { return reinterpret_cast<T*>(b.p); }
 
 bool fp64_net(const nqh_ctx* c) { return c->net_prec == NQH_PREC_FP64; }
 size_t net_elt(const nqh_ctx* c) { return fp64_net(c) ? sizeof(double) : sizeof(float); }
+bool is_rbm(const nqh_ctx* c) { return c->net_kind == 1; }
+int net_N(const nqh_ctx* c) { return is_rbm(c) ? c->rbm_N : c->d.N; }
+// columns of a stored O row: NDM compact P order, RBMSplit [Re O | Im O]
+int ycols(const nqh_ctx* c) { return is_rbm(c) ? 2 * c->rbm_npc : c->d.R; }
+// size of the real SR system = doubles in the flat parameter vector
+int nsys(const nqh_ctx* c) { return is_rbm(c) ? 2 * c->rbm_npc : c->d.np; }
+// complex entries of a reference-order O row
+int nderiv(const nqh_ctx* c) { return is_rbm(c) ? c->rbm_npc : c->d.np; }
 
 // P-order index -> reference index (same map as k_update's p2ref)
 int p2ref_host(int k, const ndm_dims& d) {
@@ -184,7 +194,7 @@ nqh_status need_net(nqh_ctx* c) {
 }
 nqh_status need_problem(nqh_ctx* c) {
     if (!c->has_problem) return fail(c, NQH_E_STATE, "problem not set (nqh_set_problem)");
-    if (c->has_net && c->N_table != c->d.N)
+    if (c->has_net && c->N_table != net_N(c))
         return fail(c, NQH_E_ARG, "problem and network have different numbers of sites");
     return NQH_OK;
 }
@@ -202,26 +212,26 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     const size_t e = net_elt(c);
     if ((s = ensure(c, c->keys_in, sizeof(uint64_t) * Ns))) return s;
     if ((s = ensure(c, c->lp, 2 * e * Ns))) return s;
-    if ((s = ensure(c, c->Y, e * (size_t)c->d.R * Ns))) return s;
+    if ((s = ensure(c, c->Y, e * (size_t)ycols(c) * Ns))) return s;
     if ((s = ensure(c, c->Lloc, 2 * e * Ns))) return s;
     c->est_grid = est_grid_for(Ns);
-    if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * c->d.np))) return s;
-    if ((s = ensure(c, c->colpart, sizeof(double) * (size_t)kColChunks * c->d.R))) return s;
-    if ((s = ensure(c, c->mu, sizeof(double) * c->d.R))) return s;
+    if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * nsys(c)))) return s;
+    if ((s = ensure(c, c->colpart, sizeof(double) * (size_t)kColChunks * ycols(c)))) return s;
+    if ((s = ensure(c, c->mu, sizeof(double) * ycols(c)))) return s;
     if ((s = ensure(c, c->stats, sizeof(double) * 8))) return s;
-    if ((s = ensure(c, c->F, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->F, sizeof(double) * nsys(c)))) return s;
     // S has one extra row and column: row np carries the right-hand side
     // through the factorization (forward substitution for free)
-    const size_t ldS = (size_t)c->d.np + 1;
+    const size_t ldS = (size_t)nsys(c) + 1;
     if ((s = ensure(c, c->S, sizeof(double) * ldS * ldS))) return s;
     if ((s = ensure(c, c->S0, sizeof(double) * ldS * ldS))) return s;
-    if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * c->d.np))) return s;
-    if ((s = ensure(c, c->delta, sizeof(double) * c->d.np))) return s;
-    if ((s = ensure(c, c->resid, sizeof(double) * c->d.np))) return s;
+    if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * nsys(c)))) return s;
+    if ((s = ensure(c, c->delta, sizeof(double) * nsys(c)))) return s;
+    if ((s = ensure(c, c->resid, sizeof(double) * nsys(c)))) return s;
     if ((s = ensure(c, c->info, sizeof(int) * 4))) return s;
     if (c->s_prec == NQH_PREC_BF16) {
         const int64_t nsp = (Ns + 63) / 64 * 64;
-        if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)c->d.R * nsp))) return s;
+        if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)ycols(c) * nsp))) return s;
     }
     if ((s = ensure(c, c->flag, sizeof(int) * 4))) return s;
     return NQH_OK;
@@ -241,10 +251,128 @@ est_args make_est(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
     return a;
 }
 
-// log rho, O rows, L_loc, force partials, then column means, stats, force, S
-nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chains, int64_t Lc) {
+rbm_args make_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
+    rbm_args a{};
+    a.N = c->rbm_N;
+    a.nh = c->rbm_nh;
+    a.npc = c->rbm_npc;
+    a.p = fp64_net(c) ? c->p64.p : c->p32.p;
+    a.tab = c->tab;
+    a.keys = keys_dev;
+    a.Ns = Ns;
+    a.invNs = 1.0 / (double)Ns;
+    a.lds_param_bytes = (int)((2 * (size_t)c->rbm_npc * net_elt(c) + 15) / 16 * 16);
+    a.max_conn = std::max(c->max_conn, 1);
+    return a;
+}
+
+// log rho (and optionally O rows) of Ns device keys
+hipError_t launch_logpsi(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns, void* lp, void* Y) {
+    if (is_rbm(c)) {
+        rbm_args a = make_rbm(c, keys_dev, Ns);
+        a.lp = lp;
+        a.Y = Y;
+        return nq_launch_rbm_estimate(a, fp64_net(c), est_grid_for(Ns), c->stream);
+    }
+    est_args a = make_est(c, keys_dev, Ns);
+    a.lp = lp;
+    a.Y = Y;
+    return nq_launch_estimate(a, fp64_net(c), est_grid_for(Ns), c->stream);
+}
+
+hipError_t launch_update(nqh_ctx* c, double lr) {
+    if (is_rbm(c))
+        return nq_launch_update_complex(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->rbm_npc, lr,
+                                        c->stream);
+    return nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream);
+}
+
+hipError_t launch_metropolis(nqh_ctx* c, const samp_args& a) {
+    if (!is_rbm(c)) return nq_launch_metropolis(a, fp64_net(c), c->stream);
+    rbm_samp_args r{};
+    r.N = c->rbm_N;
+    r.nh = c->rbm_nh;
+    r.npc = c->rbm_npc;
+    r.p = a.p;
+    r.chains = a.chains;
+    r.passes = a.passes;
+    r.burn = a.burn;
+    r.stored = a.stored;
+    r.t0 = a.t0;
+    r.slice0 = a.slice0;
+    int l = 1;
+    while (l < c->rbm_nh && l < 64) l <<= 1;
+    r.lpc = l;
+    r.restart = a.restart;
+    r.compat_stale_last_slice = a.compat_stale_last_slice;
+    r.seed = a.seed;
+    r.iter = a.iter;
+    r.iter_dev = a.iter_dev;
+    r.chain_state = a.chain_state;
+    r.samples = a.samples;
+    r.accepted = a.accepted;
+    return nq_launch_rbm_metropolis(r, fp64_net(c), c->stream);
+}
+
+// RBMSplit: estimator, means, statistics, complex force, and the real
+// embedding [[A, -B], [B, A]] of the Hermitian S = A + i B
+nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t chains, int64_t Lc) {
     const int64_t Ns = chains * Lc;
     const bool f64 = fp64_net(c);
+    const int npc = c->rbm_npc, n = 2 * npc, R = n;
+    rbm_args a = make_rbm(c, keys_dev, Ns);
+    a.do_conn = 1;
+    a.lp = c->lp.p;
+    a.Y = c->Y.p;
+    a.Lloc = c->Lloc.p;
+    a.Fpart = P<double>(c->Fpart);
+    phase_begin(c, PH_ESTIMATE);
+    HIPCHK(c, nq_launch_rbm_estimate(a, f64, c->est_grid, c->stream));
+    phase_end(c, PH_ESTIMATE);
+    phase_begin(c, PH_POST);
+    HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, R, P<double>(c->colpart), kColChunks, c->stream));
+    HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, R, 1.0 / (double)Ns, P<double>(c->mu),
+                                      c->stream));
+    HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
+    HIPCHK(c, nq_launch_force_all(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
+                                  P<double>(c->stats), n, P<double>(c->F), c->stream));
+    phase_end(c, PH_POST);
+    phase_begin(c, PH_SBUILD);
+    const int ldS = n + 1;
+    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
+    if (c->s_prec == NQH_PREC_BF16) {
+        const int64_t nsp = (Ns + 63) / 64 * 64;
+        __bf16* Yt = reinterpret_cast<__bf16*>(c->Yt.p);
+        HIPCHK(c, nq_launch_center_t(c->Y.p, f64, Ns, R, 0, R, P<double>(c->mu), Yt, nsp, c->stream));
+        HIPCHK(c, nq_launch_gram_bf16t(Yt, nsp, R, P<double>(c->S), ldS, 0, 1.0 / (double)Ns, c->stream));
+    } else {
+        gram_args g{};
+        g.Y = c->Y.p;
+        g.y_fp64 = f64;
+        g.Ns = Ns;
+        g.ldy = R;
+        g.mu = P<double>(c->mu);
+        g.S = P<double>(c->S);
+        g.lds = ldS;
+        g.scale = 1.0 / (double)Ns;
+        g.c0 = 0;
+        g.n = R;
+        g.o = 0;
+        HIPCHK(c, nq_launch_gram(g, c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : NQ_GRAM_F32, c->stream));
+    }
+    HIPCHK(c, nq_launch_embed_complex(P<double>(c->S), npc, ldS, c->stream));
+    phase_end(c, PH_SBUILD);
+    c->last_ns = Ns;
+    c->grad_ready = true;
+    return NQH_OK;
+}
+
+// log rho, O rows, L_loc, force partials, then column means, stats, force, S
+nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chains, int64_t Lc) {
+    if (is_rbm(c)) return run_grad_pipeline_rbm(c, keys_dev, chains, Lc);
+    const int64_t Ns = chains * Lc;
+    const bool f64 = fp64_net(c);
+
     est_args a = make_est(c, keys_dev, Ns);
     a.do_conn = 1;
     a.lp = c->lp.p;
@@ -314,7 +442,7 @@ nqh_status upload_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
 }
 
 nqh_status check_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
-    const int N = c->d.N;
+    const int N = net_N(c);
     const uint64_t lim = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1);
     for (int64_t b = 0; b < B; ++b)
         if (keys[b] & ~lim) return fail(c, NQH_E_ARG, "configuration key has bits beyond 2N");
@@ -496,6 +624,8 @@ nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t ac
     if (nh < 1 || na < 1 || nh > 128 || na > 128) return fail(c, NQH_E_ARG, "nh, na must be in 1..128");
     if (act != 0) return fail(c, NQH_E_ARG, "only the softplus activation (af_softplus) is implemented");
     c->d = make_ndm_dims(N, nh, na);
+    c->net_kind = 0;
+    drop_graph(c);
     nqh_status s;
     if ((s = ensure(c, c->p64, sizeof(double) * c->d.np))) return s;
     if ((s = ensure(c, c->p32, sizeof(float) * c->d.np))) return s;
@@ -507,14 +637,42 @@ nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t ac
     return NQH_OK;
 }
 
-int64_t nqh_num_params(const nqh_ctx* c) { return (c && c->has_net) ? c->d.np : -1; }
+nqh_status nqh_set_rbmsplit(nqh_ctx* c, int32_t N, int32_t nh) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (N < 1 || N > 16) return fail(c, NQH_E_ARG, "N must be in 1..16");
+    if (nh < 1 || nh > 128) return fail(c, NQH_E_ARG, "nh must be in 1..128");
+    drop_graph(c);
+    c->net_kind = 1;
+    c->rbm_N = N;
+    c->rbm_nh = nh;
+    c->rbm_npc = 2 * N + nh + 2 * nh * N;
+    c->d = ndm_dims{};
+    c->d.N = N;
+    const int n = 2 * c->rbm_npc;
+    nqh_status s;
+    if ((s = ensure(c, c->p64, sizeof(double) * n))) return s;
+    if ((s = ensure(c, c->p32, sizeof(float) * n))) return s;
+    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * n));
+    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * n));
+    c->has_net = true;
+    c->ns_cap = 0;
+    c->grad_ready = false;
+    c->made_ready = false;
+    return NQH_OK;
+}
+
+int64_t nqh_num_params(const nqh_ctx* c) { return (c && c->has_net) ? nsys(c) : -1; }
+
+int64_t nqh_num_derivs(const nqh_ctx* c) { return (c && c->has_net) ? nderiv(c) : -1; }
+
 
 nqh_status nqh_set_params(nqh_ctx* c, const double* flat, int64_t np) {
     if (!c || !flat) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
     nqh_status s = need_net(c);
     if (s) return s;
-    if (np != c->d.np) return fail(c, NQH_E_ARG, "parameter count mismatch");
+    if (np != nsys(c)) return fail(c, NQH_E_ARG, "parameter count mismatch");
     for (int64_t k = 0; k < np; ++k)
         if (!std::isfinite(flat[k])) return fail(c, NQH_E_NAN, "non-finite parameter");
     HIPCHK(c, hipMemcpyAsync(c->p64.p, flat, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
@@ -529,7 +687,7 @@ nqh_status nqh_get_params(const nqh_ctx* cc, double* flat, int64_t np) {
     (void)hipSetDevice(c->device);
     nqh_status s = need_net(c);
     if (s) return s;
-    if (np != c->d.np) return fail(c, NQH_E_ARG, "parameter count mismatch");
+    if (np != nsys(c)) return fail(c, NQH_E_ARG, "parameter count mismatch");
     HIPCHK(c, hipMemcpyAsync(flat, c->p64.p, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return NQH_OK;
@@ -555,17 +713,29 @@ static nqh_status eval_host_keys(nqh_ctx* c, const uint64_t* keys, int64_t B, bo
     if ((s = check_keys(c, keys, B))) return s;
     if ((s = ensure_samples(c, B))) return s;
     if ((s = upload_keys(c, keys, B))) return s;
+    double* G = nullptr;
+    if (want_g) {
+        const size_t gb = (size_t)2 * sizeof(double) * nderiv(c) * B;
+        if ((s = ensure(c, c->G, gb))) return s;
+        HIPCHK(c, hipMemsetAsync(c->G.p, 0, gb, c->stream));
+        G = P<double>(c->G);
+    }
+    if (is_rbm(c)) {
+        rbm_args a = make_rbm(c, P<uint64_t>(c->keys_in), B);
+        a.lp = c->lp.p;
+        a.Y = want_y ? c->Y.p : nullptr;
+        a.do_conn = conn ? 1 : 0;
+        a.Lloc = conn ? c->Lloc.p : nullptr;
+        a.G = G;
+        HIPCHK(c, nq_launch_rbm_estimate(a, fp64_net(c), est_grid_for(B), c->stream));
+        return NQH_OK;
+    }
     est_args a = make_est(c, P<uint64_t>(c->keys_in), B);
     a.lp = c->lp.p;
     a.Y = want_y ? c->Y.p : nullptr;
     a.do_conn = conn ? 1 : 0;
     a.Lloc = conn ? c->Lloc.p : nullptr;
-    if (want_g) {
-        const size_t gb = (size_t)2 * sizeof(double) * c->d.np * B;
-        if ((s = ensure(c, c->G, gb))) return s;
-        HIPCHK(c, hipMemsetAsync(c->G.p, 0, gb, c->stream));
-        a.G = c->G.p;
-    }
+    a.G = G;
     HIPCHK(c, nq_launch_estimate(a, fp64_net(c), est_grid_for(B), c->stream));
     return NQH_OK;
 }
@@ -601,7 +771,22 @@ nqh_status nqh_logpsi_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, double* 
     if (s) return s;
     if ((s = eval_host_keys(c, keys, B, true, false, false))) return s;
     if (logpsi && (s = download_cplx(c, c->lp, logpsi, 2 * B))) return s;
-    if (O) {
+    if (O && is_rbm(c)) {
+        const int npc = c->rbm_npc;
+        std::vector<double> y((size_t)2 * npc * B);
+        if (fp64_net(c)) {
+            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> yf(y.size());
+            HIPCHK(c, hipMemcpy(yf.data(), c->Y.p, sizeof(float) * yf.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < y.size(); ++k) y[k] = yf[k];
+        }
+        for (int64_t b = 0; b < B; ++b)
+            for (int k = 0; k < npc; ++k) {
+                O[(size_t)2 * (b * npc + k)] = y[(size_t)b * 2 * npc + k];
+                O[(size_t)2 * (b * npc + k) + 1] = y[(size_t)b * 2 * npc + npc + k];
+            }
+    } else if (O) {
         const ndm_dims& d = c->d;
         if (fp64_net(c)) {
             std::vector<double> y((size_t)d.R * B);
@@ -659,7 +844,7 @@ nqh_status nqh_local_estimator(nqh_ctx* c, const uint64_t* keys, int64_t B, doub
     if ((s = eval_host_keys(c, keys, B, false, true, dLloc != nullptr))) return s;
     if ((s = download_cplx(c, c->Lloc, Lloc, 2 * B))) return s;
     if (dLloc) {
-        HIPCHK(c, hipMemcpy(dLloc, c->G.p, sizeof(double) * 2 * B * c->d.np, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(dLloc, c->G.p, sizeof(double) * 2 * B * nderiv(c), hipMemcpyDeviceToHost));
     }
     return NQH_OK;
 }
@@ -679,7 +864,31 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
     phase_collect(c);
     const ndm_dims& d = c->d;
     if (stats) HIPCHK(c, hipMemcpy(stats, c->stats.p, sizeof(double) * 5, hipMemcpyDeviceToHost));
+    if (is_rbm(c)) {
+        // F and Obar interleaved complex (reference order); S as the real
+        // embedding [[Re S, -Im S], [Im S, Re S]] (2 npc x 2 npc, column-major)
+        const int npc = c->rbm_npc, n = 2 * npc;
+        if (F) {
+            std::vector<double> f(n);
+            HIPCHK(c, hipMemcpy(f.data(), c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int k = 0; k < npc; ++k) { F[2 * k] = f[k]; F[2 * k + 1] = f[npc + k]; }
+        }
+        if (Obar) {
+            std::vector<double> mu(n);
+            HIPCHK(c, hipMemcpy(mu.data(), c->mu.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int k = 0; k < npc; ++k) { Obar[2 * k] = mu[k]; Obar[2 * k + 1] = mu[npc + k]; }
+        }
+        if (S) {
+            const size_t ldS = (size_t)n + 1;
+            std::vector<double> sp(ldS * ldS);
+            HIPCHK(c, hipMemcpy(sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+            for (int l = 0; l < n; ++l)
+                for (int k = 0; k < n; ++k) S[(size_t)k + (size_t)l * n] = sp[(size_t)k + (size_t)l * ldS];
+        }
+        return NQH_OK;
+    }
     if (F) {
+
         std::vector<double> f(d.np);
         HIPCHK(c, hipMemcpy(f.data(), c->F.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
         for (int k = 0; k < d.np; ++k) F[p2ref_host(k, d)] = f[k];
@@ -739,9 +948,7 @@ static nqh_status made_global_move(nqh_ctx* c, uint32_t slice, const uint32_t* i
     HIPCHK(c, nq_launch_made_sample(P<float>(c->made_th), c->made_D, n, c->seed, c->iter, iter_dev, slice, kk + n,
                                     nullptr, c->stream));
     HIPCHK(c, nq_launch_made_logq(P<float>(c->made_th), c->made_D, kk, 2 * n, P<float>(c->made_lq), c->stream));
-    est_args e = make_est(c, kk, 2 * n);
-    e.lp = c->made_lp.p;
-    HIPCHK(c, nq_launch_estimate(e, fp64_net(c), est_grid_for(2 * n), c->stream));
+    HIPCHK(c, launch_logpsi(c, kk, 2 * n, c->made_lp.p, nullptr));
     HIPCHK(c, nq_launch_made_accept(kk, c->made_lp.p, fp64_net(c), P<float>(c->made_lq), n, c->seed, c->iter,
                                     iter_dev, slice, P<uint64_t>(c->chain_state),
                                     P<unsigned long long>(c->acc) + 1, c->stream));
@@ -775,7 +982,7 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.burn = a.restart ? c->burn : 0;
     a.stored = compat ? c->chain_len - 1 : c->chain_len;
     a.compat_stale_last_slice = compat ? 1 : 0;
-    a.lpc = nq_sampler_lanes_per_chain(c->d);
+    a.lpc = is_rbm(c) ? 1 : nq_sampler_lanes_per_chain(c->d);
     a.seed = c->seed;
     a.iter = c->iter;
     a.chain_state = P<uint64_t>(c->chain_state);
@@ -785,7 +992,7 @@ static nqh_status run_sampler(nqh_ctx* c) {
     phase_begin(c, PH_SAMPLER);
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 4, 0u, c->stream));
     if (!mixed) {
-        HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+        HIPCHK(c, launch_metropolis(c, a));
     } else {
         // burn-in (and restart) first, then per stored slice: [global move],
         // `passes` local proposals, store.  Proposal and slice indices continue
@@ -793,7 +1000,7 @@ static nqh_status run_sampler(nqh_ctx* c) {
         // as one uninterrupted launch.
         const int stored = a.stored, burn = a.burn;
         a.stored = 0;
-        HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+        HIPCHK(c, launch_metropolis(c, a));
         a.restart = 0;
         a.burn = 0;
         a.stored = 1;
@@ -806,7 +1013,7 @@ static nqh_status run_sampler(nqh_ctx* c) {
             }
             a.t0 = (uint32_t)((burn + i) * c->passes);
             a.slice0 = i;
-            HIPCHK(c, nq_launch_metropolis(a, fp64_net(c), c->stream));
+            HIPCHK(c, launch_metropolis(c, a));
         }
     }
     if (c->made_ready && c->ring_cap > 0) {
@@ -870,7 +1077,7 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
-    const int np = c->d.np, ldS = np + 1;
+    const int np = nsys(c), ldS = np + 1;
     double* S = P<double>(c->S);
     phase_begin(c, PH_SOLVE);
     if (refine > 0)
@@ -914,7 +1121,12 @@ nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
     if ((s = check_solve(c))) return s;
-    if (delta_out) {
+    if (delta_out && is_rbm(c)) {
+        const int npc = c->rbm_npc;
+        std::vector<double> dv(2 * npc);
+        HIPCHK(c, hipMemcpy(dv.data(), c->delta.p, sizeof(double) * 2 * npc, hipMemcpyDeviceToHost));
+        for (int k = 0; k < npc; ++k) { delta_out[2 * k] = dv[k]; delta_out[2 * k + 1] = dv[npc + k]; }
+    } else if (delta_out) {
         const ndm_dims& d = c->d;
         std::vector<double> dv(d.np);
         HIPCHK(c, hipMemcpy(dv.data(), c->delta.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
@@ -928,7 +1140,7 @@ nqh_status nqh_update(nqh_ctx* c, double lr) {
     (void)hipSetDevice(c->device);
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no preconditioned gradient");
     phase_begin(c, PH_UPDATE);
-    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
@@ -949,7 +1161,7 @@ static nqh_status enqueue_iteration(nqh_ctx* c, double eps, int kind, int refine
     if (s) return s;
     if ((s = precondition_async(c, eps, kind, refine))) return s;
     phase_begin(c, PH_UPDATE);
-    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
     return NQH_OK;
 }
@@ -1032,7 +1244,7 @@ nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
     if (s) return s;
     if ((s = precondition_async(c, eps, shift_kind, refine))) return s;
     phase_begin(c, PH_UPDATE);
-    HIPCHK(c, nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream));
+    HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
@@ -1144,7 +1356,7 @@ nqh_status nqh_made_setup(nqh_ctx* c, double mix, int32_t retrain_every, int32_t
         buffer_cap < 0)
         return fail(c, NQH_E_ARG, "need mix in [0,1], retrain_every >= 0, train_steps >= 0, batch >= 1, lr > 0, "
                                   "buffer_cap >= 0");
-    const int D = 2 * c->d.N;
+    const int D = 2 * net_N(c);
     if (D < 2 || D > 32) return fail(c, NQH_E_ARG, "the learned proposal needs 2 <= 2N <= 32");
     if (buffer_cap > 0 && buffer_cap < batch) return fail(c, NQH_E_ARG, "buffer_cap must be 0 or >= batch");
     drop_graph(c);

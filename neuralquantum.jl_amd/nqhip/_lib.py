@@ -58,6 +58,8 @@ SIGNATURES = {
     "nqh_set_problem": (C.c_int32, [C.c_void_p, C.POINTER(OpTable)]),
     "nqh_set_ndm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "nqh_num_params": (C.c_int64, [C.c_void_p]),
+    "nqh_num_derivs": (C.c_int64, [C.c_void_p]),
+    "nqh_set_rbmsplit": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "nqh_set_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_get_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_set_precision": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -175,7 +177,18 @@ class Context:
     def set_ndm(self, N, nh, na, act=0):
         self._chk(self.lib.nqh_set_ndm(self.h, N, nh, na, act))
         self.np_ = int(self.lib.nqh_num_params(self.h))
+        self.nd = int(self.lib.nqh_num_derivs(self.h))
         self.N = N
+        self.complex_params = False
+
+    def set_rbmsplit(self, N, nh):
+        """RBMSplit network; parameters are complex, passed interleaved
+        (re, im) in the reference order ar, ac, b, Wr, Wc."""
+        self._chk(self.lib.nqh_set_rbmsplit(self.h, N, nh))
+        self.np_ = int(self.lib.nqh_num_params(self.h))
+        self.nd = int(self.lib.nqh_num_derivs(self.h))
+        self.N = N
+        self.complex_params = True
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
@@ -199,9 +212,9 @@ class Context:
     def logpsi_grad(self, keys):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.empty(2 * len(keys))
-        O = np.empty(2 * len(keys) * self.np_)
+        O = np.empty(2 * len(keys) * self.nd)
         self._chk(self.lib.nqh_logpsi_grad(self.h, _p(keys, _u64p), len(keys), _p(out, _dp), _p(O, _dp)))
-        return out.view(np.complex128), O.view(np.complex128).reshape(len(keys), self.np_)
+        return out.view(np.complex128), O.view(np.complex128).reshape(len(keys), self.nd)
 
     def connections(self, keys, cap):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
@@ -217,11 +230,11 @@ class Context:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         B = len(keys)
         L = np.empty(2 * B)
-        G = np.empty(2 * B * self.np_) if grad else None
+        G = np.empty(2 * B * self.nd) if grad else None
         self._chk(self.lib.nqh_local_estimator(self.h, _p(keys, _u64p), B, _p(L, _dp), _p(G, _dp)))
         L = L.view(np.complex128)
         if grad:
-            return L, G.view(np.complex128).reshape(B, self.np_)
+            return L, G.view(np.complex128).reshape(B, self.nd)
         return L
 
     def grad_from_samples(self, keys, chains, chain_length, want_S=True):
@@ -229,9 +242,11 @@ class Context:
         st = np.empty(5)
         S = np.empty((self.np_, self.np_), order="F") if want_S else None
         F = np.empty(self.np_)
-        Ob = np.empty(2 * self.np_)
+        Ob = np.empty(2 * self.nd)
         self._chk(self.lib.nqh_grad_from_samples(self.h, _p(keys, _u64p), chains, chain_length,
                                                  _p(st, _dp), _p(S, _dp), _p(F, _dp), _p(Ob, _dp)))
+        if self.complex_params:   # F complex; S returned as its real embedding [[A, -B], [B, A]]
+            return st, S, F.view(np.complex128), Ob.view(np.complex128)
         return st, S, F, Ob.view(np.complex128)
 
     # -- loop -------------------------------------------------------------------
@@ -256,6 +271,8 @@ class Context:
     def precondition(self, eps, shift_kind=SHIFT_ADD, refine=0, want_delta=True):
         d = np.empty(self.np_) if want_delta else None
         self._chk(self.lib.nqh_precondition(self.h, eps, shift_kind, refine, _p(d, _dp)))
+        if d is not None and self.complex_params:
+            return d.view(np.complex128)
         return d
 
     def update(self, lr):
