@@ -118,7 +118,16 @@ nqh_status nqh_set_ndm(nqh_ctx* ctx, int32_t N, int32_t nh, int32_t na, int32_t 
  * (2 npc x 2 npc). */
 nqh_status nqh_set_rbmsplit(nqh_ctx* ctx, int32_t N, int32_t nh);
 
-/* doubles in the flat parameter vector (NDM: Np; RBMSplit: 2 x complex count) */
+/* RBM(T, N, alpha) on kets (src/Networks/ClosedSystems/RBM.jl:28-61,
+ * af_softplus): complex a[N] b[nh] W[nh x N], interleaved, for a Hamiltonian
+ * problem (a table whose terms are all kets, kind 3).  One iteration is then
+ * the energy path of BatchedValSampler.sample! (BatchedValSampler.jl:117-139):
+ * E_loc = sum <s|H|s'> psi(s')/psi(s) (AccumulatorObsScalar), statistics of
+ * E_loc, grad C = sum_s E_loc(s) conj(Oc(s)) / Ns, and the complex SR system
+ * as for RBMSplit; configurations are keys of N bits. */
+nqh_status nqh_set_rbm(nqh_ctx* ctx, int32_t N, int32_t nh);
+
+/* doubles in the flat parameter vector (NDM: Np; RBMSplit, RBM: 2 x complex count) */
 int64_t nqh_num_params(const nqh_ctx* ctx);
 /* complex entries of one O row (NDM: Np; RBMSplit: complex parameter count) */
 int64_t nqh_num_derivs(const nqh_ctx* ctx);
@@ -203,6 +212,11 @@ nqh_status nqh_get_phase_times(nqh_ctx* ctx, double* ms_out, int64_t* counts_out
  * No reference counterpart (execution strategy of sample! + precondition! +
  * update!). */
 nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
+
+/* statistics of the last gradient: [mean, error, variance, tau, R] of |L_loc|^2
+ * (Liouvillian) or of E_loc (Hamiltonian: mean real part; out[5] = imaginary
+ * part of the mean); n <= 8 values */
+nqh_status nqh_get_stats(nqh_ctx* ctx, double* out, int32_t n);
 
 /* last samples (chain_length x chains keys) / local values (2 Ns doubles) */
 nqh_status nqh_get_samples(nqh_ctx* ctx, uint64_t* keys_out, int64_t n);

@@ -61,7 +61,9 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
         atomicAdd(&sF[k], (double)v.re);
         atomicAdd(&sF[npc + k], (double)v.im);
     };
-    const int oar = 0, oac = N, ob = 2 * N, owr = 2 * N + nh, owc = 2 * N + nh + nh * N;
+    // ket (RBM.jl): a[N] b[nh] W[nh x N] on the row configuration only
+    const bool ket = a.ket != 0;
+    const int oar = 0, oac = N, ob = ket ? N : 2 * N, owr = ob + nh, owc = owr + nh * N;
 
     const int lane0 = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
                 for (int i = 0; i < NMAX; ++i)
                     if (i < N) {
                         cfma_r(t, sp[owr + h + nh * i], xv<T>(ks, i));
-                        cfma_r(t, sp[owc + h + nh * i], xv<T>(ke, i));
+                        if (!ket) cfma_r(t, sp[owc + h + nh * i], xv<T>(ke, i));
                     }
                 th[r] = t;
                 csoftplus_and_sigmoid(t, &f[r], &fp[r]);
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
             for (int r = 0; r < HS; ++r) acc = cadd(acc, f[r]);
             if (lane < N) {
                 cfma_r(acc, sp[oar + lane], xv<T>(ks, lane));
-                cfma_r(acc, sp[oac + lane], xv<T>(ke, lane));
+                if (!ket) cfma_r(acc, sp[oac + lane], xv<T>(ke, lane));
             }
             acc.re = wave_sum(acc.re);
             acc.im = wave_sum(acc.im);
@@ -128,16 +130,20 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
                             const T xs = xv<T>(ks, i), xe = xv<T>(ke, i);
                             y[owr + h + nh * i] = fp[r].re * xs;
                             y[npc + owr + h + nh * i] = fp[r].im * xs;
-                            y[owc + h + nh * i] = fp[r].re * xe;
-                            y[npc + owc + h + nh * i] = fp[r].im * xe;
+                            if (!ket) {
+                                y[owc + h + nh * i] = fp[r].re * xe;
+                                y[npc + owc + h + nh * i] = fp[r].im * xe;
+                            }
                         }
                 }
             }
             if (lane < N) {
                 y[oar + lane] = xv<T>(ks, lane);
                 y[npc + oar + lane] = T(0);
-                y[oac + lane] = xv<T>(ke, lane);
-                y[npc + oac + lane] = T(0);
+                if (!ket) {
+                    y[oac + lane] = xv<T>(ke, lane);
+                    y[npc + oac + lane] = T(0);
+                }
             }
         }
         if (!a.do_conn) continue;
@@ -290,7 +296,9 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
     const cplx<T>* gp = reinterpret_cast<const cplx<T>*>(a.p);
     for (int k = threadIdx.x; k < npc; k += blockDim.x) sp[k] = gp[k];
     __syncthreads();
-    const int oar = 0, oac = N, ob = 2 * N, owr = 2 * N + nh, owc = 2 * N + nh + nh * N;
+    const bool ket = a.ket != 0;
+    const int oar = 0, oac = N, ob = ket ? N : 2 * N, owr = ob + nh, owc = owr + nh * N;
+    const int nsite = ket ? N : 2 * N;   // sites a proposal picks from
 
     const int lpc = a.lpc;
     const int lane = threadIdx.x & 63;
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
     if (live) {
         if (a.restart) {
             philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, 0u, k0, k1);
-            const uint64_t m2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1ull);
+            const uint64_t m2 = (nsite >= 64) ? ~0ull : ((1ull << nsite) - 1ull);
             key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
         } else {
             key = a.chain_state[chain];
@@ -331,7 +339,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
                 cplx<T> z = sp[ob + h];
                 for (int i = 0; i < N; ++i) {
                     cfma_r(z, sp[owr + h + nh * i], xv<T>(ks, i));
-                    cfma_r(z, sp[owc + h + nh * i], xv<T>(ke, i));
+                    if (!ket) cfma_r(z, sp[owc + h + nh * i], xv<T>(ke, i));
                 }
                 th[r] = z;
                 fr[r] = csoftplus_re(z);
@@ -346,7 +354,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
             const uint32_t x0 = __shfl(rx0, gbase + slot, 64);
             const uint32_t x1 = __shfl(rx1, gbase + slot, 64);
             const uint32_t x2 = __shfl(rx2, gbase + slot, 64);
-            const int site = (int)(((uint64_t)x0 * (uint64_t)(2 * N)) >> 32);
+            const int site = (int)(((uint64_t)x0 * (uint64_t)nsite) >> 32);
             const bool col = site >= N;
             const int j = col ? site - N : site;
             const uint32_t bits = col ? (uint32_t)(key >> N) : (uint32_t)key;
@@ -412,6 +420,90 @@ __global__ void k_embed_complex(double* S, int n, int ld) {
     S[(n + j) + (n + k) * L] = A;
     S[(n + j) + k * L] = B;
     S[j + (n + k) * L] = -B;
+}
+
+// ---- energy path (BatchedValSampler.jl:70-139): statistics of the complex
+// local energies, and grad C = sum_s E_s conj(Oc_s) / Ns.
+
+__device__ double blk_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < nw; ++k) t += red[k];
+    return t;
+}
+
+// stat_analysis (stats.jl:52-77) of complex values E[s], s = chain + slice B:
+// out = [Re mean, error, mean chain variance, tau, R, Im mean], variances
+// of complex values as mean |x - m|^2 with the n - 1 normalisation
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats_cplx(const T* E, int64_t B, int64_t Lc, double* out) {
+    __shared__ double red[16];
+    const double invL = 1.0 / (double)Lc;
+    double sr = 0.0, si = 0.0;
+    for (int64_t j = threadIdx.x; j < B; j += blockDim.x)
+        for (int64_t i = 0; i < Lc; ++i) {
+            sr += (double)E[2 * (j + i * B)] * invL;
+            si += (double)E[2 * (j + i * B) + 1] * invL;
+        }
+    const double mr = blk_sum(sr, red) / (double)B, mi = blk_sum(si, red) / (double)B;
+    double vch = 0.0, vmu = 0.0, vall = 0.0;
+    for (int64_t j = threadIdx.x; j < B; j += blockDim.x) {
+        double cr = 0.0, ci = 0.0;
+        for (int64_t i = 0; i < Lc; ++i) {
+            cr += (double)E[2 * (j + i * B)];
+            ci += (double)E[2 * (j + i * B) + 1];
+        }
+        cr *= invL;
+        ci *= invL;
+        double v = 0.0;
+        for (int64_t i = 0; i < Lc; ++i) {
+            const double xr = (double)E[2 * (j + i * B)], xi = (double)E[2 * (j + i * B) + 1];
+            v += (xr - cr) * (xr - cr) + (xi - ci) * (xi - ci);
+            vall += (xr - mr) * (xr - mr) + (xi - mi) * (xi - mi);
+        }
+        vch += v / (double)(Lc - 1);
+        vmu += (cr - mr) * (cr - mr) + (ci - mi) * (ci - mi);
+    }
+    vch = blk_sum(vch, red);
+    vmu = blk_sum(vmu, red);
+    vall = blk_sum(vall, red);
+    if (threadIdx.x == 0) {
+        const double var_mu_ch = vmu / (double)(B - 1);
+        const double var_mu = vall / (double)(B * Lc - 1);
+        const double t = var_mu_ch / var_mu;
+        out[0] = mr;
+        out[1] = sqrt(var_mu_ch / (double)B);
+        out[2] = vch / (double)B;
+        out[3] = fmax(0.0, 0.5 * (t * (double)Lc - 1.0));
+        out[4] = sqrt((double)(Lc - 1) / (double)Lc + t);
+        out[5] = mi;
+    }
+}
+
+// part[chunk][k] (+ [npc + k]) = Re (Im) sum over the chunk's samples of
+// E_s conj(Y_s,k + i Y_s,npc+k - mean); Y rows [Re O | Im O]
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy_force_part(const T* Y, const T* E, const double* mu, int64_t Ns,
+                                                           int npc, int chunks, double* part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ch = blockIdx.y;
+    if (k >= npc) return;
+    const int64_t per = (Ns + chunks - 1) / chunks;
+    const int64_t s0 = (int64_t)ch * per, s1 = (s0 + per < Ns) ? s0 + per : Ns;
+    const double mr = mu[k], mi = mu[npc + k];
+    double ar = 0.0, ai = 0.0;
+    for (int64_t s = s0; s < s1; ++s) {
+        const double er = (double)E[2 * s], ei = (double)E[2 * s + 1];
+        const double orr = (double)Y[s * 2 * npc + k] - mr, oi = (double)Y[s * 2 * npc + npc + k] - mi;
+        ar += er * orr + ei * oi;
+        ai += ei * orr - er * oi;
+    }
+    part[(int64_t)ch * 2 * npc + k] = ar;
+    part[(int64_t)ch * 2 * npc + npc + k] = ai;
 }
 
 // F[k] = sum_g part2[g][k] - C mean[k] for every column k < n
@@ -501,6 +593,28 @@ hipError_t nq_launch_force_all(const double* Fpart, int nblk, double* part2, con
     hipLaunchKernelGGL(k_force_all, dim3((n + 255) / 256), dim3(256), 0, st, part2, NQ_FORCE_GROUPS, mu, stats, n,
                        F);
     return hipGetLastError();
+}
+
+hipError_t nq_launch_stats_cplx(const void* E, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st) {
+    if (fp64)
+        hipLaunchKernelGGL(k_stats_cplx<double>, dim3(1), dim3(256), 0, st, (const double*)E, B, Lc, out);
+    else
+        hipLaunchKernelGGL(k_stats_cplx<float>, dim3(1), dim3(256), 0, st, (const float*)E, B, Lc, out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_energy_force(const void* Y, const void* E, int fp64, const double* mu, int64_t Ns, int npc,
+                                  double* part, int chunks, double* F, hipStream_t st) {
+    dim3 grid((npc + 255) / 256, chunks);
+    if (fp64)
+        hipLaunchKernelGGL(k_energy_force_part<double>, grid, dim3(256), 0, st, (const double*)Y, (const double*)E, mu,
+                           Ns, npc, chunks, part);
+    else
+        hipLaunchKernelGGL(k_energy_force_part<float>, grid, dim3(256), 0, st, (const float*)Y, (const float*)E, mu,
+                           Ns, npc, chunks, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return nq_launch_colsum_finish(part, chunks, 2 * npc, 1.0 / (double)Ns, F, st);
 }
 
 hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,

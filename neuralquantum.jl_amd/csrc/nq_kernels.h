@@ -156,7 +156,8 @@ hipError_t nq_launch_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, i
 // ---- RBMSplit network (k_rbm.hip): complex parameters ar[N] ac[N] b[nh] Wr[nh x N]
 // Wc[nh x N], interleaved (re, im) in precision T, reference order
 struct rbm_args {
-    int N, nh, npc;           // npc = 2N + nh + 2 nh N complex parameters
+    int N, nh, npc;           // npc = 2N + nh + 2 nh N complex parameters (ket: N + nh + nh N)
+    int ket;                  // RBM on kets (a[N] b[nh] W[nh x N]), keys of N bits
     const void* p;
     dev_table tab;
     const uint64_t* keys;
@@ -175,6 +176,7 @@ size_t rbm_est_lds_bytes(const rbm_args& a, size_t elt);
 hipError_t nq_launch_rbm_estimate(const rbm_args& a, bool fp64, int grid, hipStream_t st);
 struct rbm_samp_args {
     int N, nh, npc;
+    int ket;
     const void* p;
     int64_t chains;
     int passes, burn, stored;
@@ -199,3 +201,8 @@ hipError_t nq_launch_force_all(const double* Fpart, int nblk, double* part2, con
 // p -= lr (delta[0:npc] + i delta[npc:2 npc]); refresh the FP32 copy
 hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,
                                     hipStream_t st);
+// energy path: statistics of complex values, out = [Re mean, err, var, tau, R, Im mean]
+hipError_t nq_launch_stats_cplx(const void* E, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
+// F = [Re | Im] of sum_s E_s conj(Y_s - mu) / Ns (Y rows [Re O | Im O]); part: chunks x 2 npc scratch
+hipError_t nq_launch_energy_force(const void* Y, const void* E, int fp64, const double* mu, int64_t Ns, int npc,
+                                  double* part, int chunks, double* F, hipStream_t st);

@@ -34,7 +34,8 @@ struct nqh_ctx {
     dev_table tab{};
     int max_conn = 0;
 
-    // network: 0 = NDM (d), 1 = RBMSplit (rbm_*; complex parameters interleaved)
+    bool problem_ket = false;          // every term acts on kets (a Hamiltonian), not (row, column) pairs
+    // network: 0 = NDM (d), 1 = RBMSplit, 2 = RBM on kets (rbm_*; complex parameters interleaved)
     bool has_net = false;
     int net_kind = 0;
     ndm_dims d{};
@@ -157,7 +158,8 @@ T* P(dbuf& b) { return reinterpret_cast<T*>(b.p); }
 
 bool fp64_net(const nqh_ctx* c) { return c->net_prec == NQH_PREC_FP64; }
 size_t net_elt(const nqh_ctx* c) { return fp64_net(c) ? sizeof(double) : sizeof(float); }
-bool is_rbm(const nqh_ctx* c) { return c->net_kind == 1; }
+bool is_rbm(const nqh_ctx* c) { return c->net_kind == 1 || c->net_kind == 2; }
+bool is_ket(const nqh_ctx* c) { return c->net_kind == 2; }
 int net_N(const nqh_ctx* c) { return is_rbm(c) ? c->rbm_N : c->d.N; }
 // columns of a stored O row: NDM compact P order, RBMSplit [Re O | Im O]
 int ycols(const nqh_ctx* c) { return is_rbm(c) ? 2 * c->rbm_npc : c->d.R; }
@@ -196,6 +198,9 @@ nqh_status need_problem(nqh_ctx* c) {
     if (!c->has_problem) return fail(c, NQH_E_STATE, "problem not set (nqh_set_problem)");
     if (c->has_net && c->N_table != net_N(c))
         return fail(c, NQH_E_ARG, "problem and network have different numbers of sites");
+    if (c->has_net && c->problem_ket != is_ket(c))
+        return fail(c, NQH_E_ARG, c->problem_ket ? "a Hamiltonian (ket operator) needs the ket RBM (nqh_set_rbm)"
+                                                 : "a Liouvillian needs a density-matrix network (NDM or RBMSplit)");
     return NQH_OK;
 }
 
@@ -256,6 +261,7 @@ rbm_args make_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
     a.N = c->rbm_N;
     a.nh = c->rbm_nh;
     a.npc = c->rbm_npc;
+    a.ket = is_ket(c) ? 1 : 0;
     a.p = fp64_net(c) ? c->p64.p : c->p32.p;
     a.tab = c->tab;
     a.keys = keys_dev;
@@ -293,6 +299,7 @@ hipError_t launch_metropolis(nqh_ctx* c, const samp_args& a) {
     r.N = c->rbm_N;
     r.nh = c->rbm_nh;
     r.npc = c->rbm_npc;
+    r.ket = is_ket(c) ? 1 : 0;
     r.p = a.p;
     r.chains = a.chains;
     r.passes = a.passes;
@@ -320,12 +327,13 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
     const int64_t Ns = chains * Lc;
     const bool f64 = fp64_net(c);
     const int npc = c->rbm_npc, n = 2 * npc, R = n;
+    const bool ket = is_ket(c);
     rbm_args a = make_rbm(c, keys_dev, Ns);
     a.do_conn = 1;
     a.lp = c->lp.p;
     a.Y = c->Y.p;
-    a.Lloc = c->Lloc.p;
-    a.Fpart = P<double>(c->Fpart);
+    a.Lloc = c->Lloc.p;                       // L_loc, or the local energy on kets
+    a.Fpart = ket ? nullptr : P<double>(c->Fpart);
     phase_begin(c, PH_ESTIMATE);
     HIPCHK(c, nq_launch_rbm_estimate(a, f64, c->est_grid, c->stream));
     phase_end(c, PH_ESTIMATE);
@@ -333,9 +341,16 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, R, P<double>(c->colpart), kColChunks, c->stream));
     HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, R, 1.0 / (double)Ns, P<double>(c->mu),
                                       c->stream));
-    HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
-    HIPCHK(c, nq_launch_force_all(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
-                                  P<double>(c->stats), n, P<double>(c->F), c->stream));
+    if (ket) {
+        // energy: statistics of E_loc, grad C = sum E conj(Oc) / Ns (BatchedValSampler.jl:100-116)
+        HIPCHK(c, nq_launch_stats_cplx(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
+        HIPCHK(c, nq_launch_energy_force(c->Y.p, c->Lloc.p, f64, P<double>(c->mu), Ns, npc, P<double>(c->colpart),
+                                         kColChunks, P<double>(c->F), c->stream));
+    } else {
+        HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
+        HIPCHK(c, nq_launch_force_all(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
+                                      P<double>(c->stats), n, P<double>(c->F), c->stream));
+    }
     phase_end(c, PH_POST);
     phase_begin(c, PH_SBUILD);
     const int ldS = n + 1;
@@ -442,8 +457,8 @@ nqh_status upload_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
 }
 
 nqh_status check_keys(nqh_ctx* c, const uint64_t* keys, int64_t B) {
-    const int N = net_N(c);
-    const uint64_t lim = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1);
+    const int nb = is_ket(c) ? net_N(c) : 2 * net_N(c);
+    const uint64_t lim = (nb >= 64) ? ~0ull : ((1ull << nb) - 1);
     for (int64_t b = 0; b < B; ++b)
         if (keys[b] & ~lim) return fail(c, NQH_E_ARG, "configuration key has bits beyond 2N");
     return NQH_OK;
@@ -554,7 +569,10 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     const uint64_t lim = (N >= 64) ? ~0ull : ((1ull << N) - 1);
     for (int e = 0; e < t->n_entries; ++e)
         if (t->ent_flip[e] & ~lim) return fail(c, NQH_E_ARG, "flip mask beyond n_sites");
-    int maxc = 0;
+    int maxc = 0, n_ket = 0;
+    for (int q = 0; q < t->n_terms; ++q) n_ket += t->term_kind[q] == 3;
+    if (n_ket != 0 && n_ket != t->n_terms)
+        return fail(c, NQH_E_ARG, "ket terms (kind 3) cannot be mixed with row/column terms");
     for (int q = 0; q < t->n_terms; ++q) {
         const int k = t->term_kind[q], ol = t->term_op_l[q], orr = t->term_op_r[q];
         auto ok = [&](int o) { return o >= 0 && o < t->n_ops; };
@@ -613,7 +631,9 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     c->tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
     c->N_table = N;
     c->max_conn = maxc;
+    c->problem_ket = n_ket > 0;
     c->has_problem = true;
+    drop_graph(c);
     return NQH_OK;
 }
 
@@ -647,6 +667,31 @@ nqh_status nqh_set_rbmsplit(nqh_ctx* c, int32_t N, int32_t nh) {
     c->rbm_N = N;
     c->rbm_nh = nh;
     c->rbm_npc = 2 * N + nh + 2 * nh * N;
+    c->d = ndm_dims{};
+    c->d.N = N;
+    const int n = 2 * c->rbm_npc;
+    nqh_status s;
+    if ((s = ensure(c, c->p64, sizeof(double) * n))) return s;
+    if ((s = ensure(c, c->p32, sizeof(float) * n))) return s;
+    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * n));
+    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * n));
+    c->has_net = true;
+    c->ns_cap = 0;
+    c->grad_ready = false;
+    c->made_ready = false;
+    return NQH_OK;
+}
+
+nqh_status nqh_set_rbm(nqh_ctx* c, int32_t N, int32_t nh) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (N < 1 || N > 16) return fail(c, NQH_E_ARG, "N must be in 1..16");
+    if (nh < 1 || nh > 128) return fail(c, NQH_E_ARG, "nh must be in 1..128");
+    drop_graph(c);
+    c->net_kind = 2;
+    c->rbm_N = N;
+    c->rbm_nh = nh;
+    c->rbm_npc = N + nh + nh * N;
     c->d = ndm_dims{};
     c->d.N = N;
     const int n = 2 * c->rbm_npc;
@@ -841,6 +886,7 @@ nqh_status nqh_local_estimator(nqh_ctx* c, const uint64_t* keys, int64_t B, doub
     (void)hipSetDevice(c->device);
     nqh_status s;
     if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    if (dLloc && is_ket(c)) return fail(c, NQH_E_ARG, "the energy estimator has no per-sample gradient");
     if ((s = eval_host_keys(c, keys, B, false, true, dLloc != nullptr))) return s;
     if ((s = download_cplx(c, c->Lloc, Lloc, 2 * B))) return s;
     if (dLloc) {
@@ -1290,6 +1336,15 @@ nqh_status nqh_get_phase_times(nqh_ctx* c, double* ms_out, int64_t* counts_out) 
     return NQH_OK;
 }
 
+nqh_status nqh_get_stats(nqh_ctx* c, double* out, int32_t n) {
+    if (!c || !out || n < 0 || n > 8) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no statistics yet");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->stats.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return NQH_OK;
+}
+
 nqh_status nqh_get_local_values(nqh_ctx* c, double* lloc_out, int64_t n) {
     if (!c || !lloc_out) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
@@ -1356,7 +1411,7 @@ nqh_status nqh_made_setup(nqh_ctx* c, double mix, int32_t retrain_every, int32_t
         buffer_cap < 0)
         return fail(c, NQH_E_ARG, "need mix in [0,1], retrain_every >= 0, train_steps >= 0, batch >= 1, lr > 0, "
                                   "buffer_cap >= 0");
-    const int D = 2 * net_N(c);
+    const int D = is_ket(c) ? net_N(c) : 2 * net_N(c);
     if (D < 2 || D > 32) return fail(c, NQH_E_ARG, "the learned proposal needs 2 <= 2N <= 32");
     if (buffer_cap > 0 && buffer_cap < batch) return fail(c, NQH_E_ARG, "buffer_cap must be 0 or >= batch");
     drop_graph(c);

@@ -60,6 +60,8 @@ SIGNATURES = {
     "nqh_num_params": (C.c_int64, [C.c_void_p]),
     "nqh_num_derivs": (C.c_int64, [C.c_void_p]),
     "nqh_set_rbmsplit": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+    "nqh_set_rbm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+    "nqh_get_stats": (C.c_int32, [C.c_void_p, _dp, C.c_int32]),
     "nqh_set_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_get_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_set_precision": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -190,6 +192,21 @@ class Context:
         self.N = N
         self.complex_params = True
 
+    def set_rbm(self, N, nh):
+        """Ket RBM for a Hamiltonian problem; complex parameters interleaved
+        (re, im) in the reference order a, b, W."""
+        self._chk(self.lib.nqh_set_rbm(self.h, N, nh))
+        self.np_ = int(self.lib.nqh_num_params(self.h))
+        self.nd = int(self.lib.nqh_num_derivs(self.h))
+        self.N = N
+        self.complex_params = True
+        self.ket = True
+
+    def get_stats(self, n=6):
+        out = np.empty(n)
+        self._chk(self.lib.nqh_get_stats(self.h, _p(out, _dp), n))
+        return out
+
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(self.lib.nqh_set_params(self.h, _p(p, _dp), len(p)))
@@ -309,7 +326,8 @@ class Context:
     # -- learned proposal (MADE over the 2N doubled bits) --------------------
     def made_setup(self, mix=0.5, retrain_every=20, train_steps=50, batch=1024, lr=1e-3, buffer_cap=65536):
         self._chk(self.lib.nqh_made_setup(self.h, mix, retrain_every, train_steps, batch, lr, buffer_cap))
-        self.made_np = int(self.lib.nqh_made_num_params(2 * self.N))
+        D = self.N if getattr(self, "ket", False) else 2 * self.N
+        self.made_np = int(self.lib.nqh_made_num_params(D))
 
     def made_set_params(self, th):
         th = np.ascontiguousarray(th, dtype=np.float32)

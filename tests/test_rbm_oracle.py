@@ -136,3 +136,96 @@ def test_metropolis_oracle_is_stationary():
     pex /= pex.sum()
     hist = np.bincount(keys.astype(np.int64), minlength=16) / len(keys)
     assert 0.5 * np.abs(hist - pex).sum() < 0.05
+
+
+# ---- ket RBM and the energy path -------------------------------------------
+
+def kron_ising_hamiltonian(N, g, V, periodic):
+    """H = sum g/2 sx_i + sum V/4 sz_i sz_j by explicit Kronecker products in
+    the packed-key basis (site 1 = least significant bit, bit 0 <-> v = -1 <->
+    the reference's local index 1, on which sigmaz = diag(1, -1))."""
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+
+    def op(site_mats):
+        out = np.eye(1, dtype=complex)
+        for i in range(N, 0, -1):          # most significant bit first
+            out = np.kron(out, site_mats.get(i, np.eye(2)))
+        return out
+    H = np.zeros((2 ** N, 2 ** N), dtype=complex)
+    for i in range(1, N + 1):
+        H += (g / 2) * op({i: sx})
+    bonds = [(i, i + 1) for i in range(1, N)] + ([(1, N)] if periodic and N > 2 else [])
+    for i, j in bonds:
+        H += (V / 4) * op({i: sz, j: sz})
+    return H
+
+
+def test_ket_hamiltonian_matches_kronecker_form():
+    for N, periodic in ((3, True), (4, False)):
+        model = ops_ref.ising_chain_model(N, V=2.0, g=1.0, periodic=periodic)
+        H = rbm_ref.dense_hamiltonian(model)
+        assert np.allclose(H, kron_ising_hamiltonian(N, 1.0, 2.0, periodic), atol=1e-14)
+
+
+def test_ket_rbm_O_and_local_energy():
+    N, nh = 3, 4
+    rng = np.random.default_rng(8)
+    n = rbm_ref.ket_n_params(N, nh)
+    p = 0.2 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    keys = np.arange(2 ** N, dtype=np.uint64)
+    lp, O = rbm_ref.ket_logpsi_grad(p, keys, N, nh)
+    h = 1e-6
+    for k in range(n):
+        e = np.zeros(n, dtype=complex)
+        e[k] = h
+        fd = (rbm_ref.ket_logpsi(p + e, keys, N, nh) - rbm_ref.ket_logpsi(p - e, keys, N, nh)) / (2 * h)
+        assert np.max(np.abs(fd - O[:, k])) < 1e-7
+    model = ops_ref.ising_chain_model(N)
+    psi = np.exp(lp)
+    E = rbm_ref.local_energy(model, p, keys, N, nh)
+    Hk = kron_ising_hamiltonian(N, 1.0, 2.0, True)
+    assert np.max(np.abs(E - (Hk @ psi) / psi)) < 1e-12
+
+
+def test_energy_gradient_is_dE_dp():
+    """With exact |psi|^2 weights, sum p E_loc conj(Oc) = dE/d conj(p) =
+    (dE/dRe p + i dE/dIm p) / 2 for E = <psi|H|psi>/<psi|psi>."""
+    N, nh = 3, 2
+    rng = np.random.default_rng(9)
+    n = rbm_ref.ket_n_params(N, nh)
+    p = 0.2 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    keys = np.arange(2 ** N, dtype=np.uint64)
+    model = ops_ref.ising_chain_model(N)
+    H = kron_ising_hamiltonian(N, 1.0, 2.0, True)
+
+    def energy(q):
+        psi = np.exp(rbm_ref.ket_logpsi(q, keys, N, nh))
+        return (np.conj(psi) @ H @ psi).real / np.sum(np.abs(psi) ** 2)
+    lp, O = rbm_ref.ket_logpsi_grad(p, keys, N, nh)
+    w = np.abs(np.exp(lp)) ** 2
+    w /= w.sum()
+    E = rbm_ref.local_energy(model, p, keys, N, nh)
+    Oc = O - w @ O
+    F = (w * E) @ np.conj(Oc)
+    h = 1e-6
+    for k in range(n):
+        e = np.zeros(n, dtype=complex)
+        e[k] = h
+        dre = (energy(p + e) - energy(p - e)) / (2 * h)
+        dim = (energy(p + 1j * e) - energy(p - 1j * e)) / (2 * h)
+        assert abs(F[k] - 0.5 * (dre + 1j * dim)) < 1e-6
+
+
+def test_energy_stats_known_answer():
+    """Complex statistics: the mean, and the variances as mean |x - m|^2."""
+    B, Lc = 3, 4
+    rng = np.random.default_rng(10)
+    E = rng.standard_normal(B * Lc) + 1j * rng.standard_normal(B * Lc)
+    O = np.zeros((B * Lc, 1), dtype=complex)
+    _, _, _, st = rbm_ref.energy_grad_step(B, Lc, O, E)
+    v = E.reshape(Lc, B).T
+    assert abs(st[0] + 1j * st[1] - E.mean()) < 1e-14
+    chain_var = np.array([np.var(v[j], ddof=1) for j in range(B)])   # numpy var of complex = mean |x-m|^2
+    assert abs(st[3] - chain_var.mean()) < 1e-14
+    assert abs(st[2] - np.sqrt(np.var(v.mean(1), ddof=1) / B)) < 1e-14
