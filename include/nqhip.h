@@ -213,6 +213,15 @@ nqh_status nqh_get_phase_times(nqh_ctx* ctx, double* ms_out, int64_t* counts_out
  * update!). */
 nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
 
+/* exact sums over all 2^(2N) doubled configurations (2^N for kets), the
+ * batched ExactSampler (Exact.jl:110-181) with the accumulator on top:
+ * out[0] = sum p |L_loc|^2 (= ||L rho||^2 / ||rho||^2 for a Liouvillian),
+ * out[1], out[2] = Re, Im of sum p L_loc (the energy <H> for kets),
+ * out[3] = log sum |psi|^2; p = |psi|^2 / sum |psi|^2.  logp_out (nullable,
+ * 2^(2N) or 2^N doubles): log p of every configuration, indexed by key.
+ * At most 2^30 configurations. */
+nqh_status nqh_exact(nqh_ctx* ctx, double* out, double* logp_out);
+
 /* statistics of the last gradient: [mean, error, variance, tau, R] of |L_loc|^2
  * (Liouvillian) or of E_loc (Hamiltonian: mean real part; out[5] = imaginary
  * part of the mean); n <= 8 values */

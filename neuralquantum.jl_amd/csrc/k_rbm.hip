@@ -622,3 +622,68 @@ hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta
     hipLaunchKernelGGL(k_update_complex, dim3((npc + 255) / 256), dim3(256), 0, st, p64, p32, delta, npc, lr);
     return hipGetLastError();
 }
+
+// ---- exact full sums over every configuration (ExactSampler, Exact.jl:110-181)
+namespace {
+__global__ void k_iota_keys(uint64_t* keys, int64_t n, uint64_t start) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = start + (uint64_t)i;
+}
+
+// one block: out = {max of log p, sum w, sum w |L|^2, sum w Re L, sum w Im L},
+// w = exp(log p - max), log p = 2 Re log psi
+template <typename T>
+__global__ __launch_bounds__(1024) void k_exact_partial(const T* lp, const T* L, int64_t n, double* out) {
+    __shared__ double red[16];
+    double m = -INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, 2.0 * (double)lp[2 * i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) red[w] = m;
+    __syncthreads();
+    m = red[0];
+    for (int k = 1; k < nw; ++k) m = fmax(m, red[k]);
+    __syncthreads();
+    double z = 0.0, zc = 0.0, zr = 0.0, zi = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double wt = exp(2.0 * (double)lp[2 * i] - m);
+        const double lr = (double)L[2 * i], li = (double)L[2 * i + 1];
+        z += wt;
+        zc += wt * (lr * lr + li * li);
+        zr += wt * lr;
+        zi += wt * li;
+    }
+    double v[4] = {z, zc, zr, zi};
+    for (int q = 0; q < 4; ++q) {
+        double x = wave_sum(v[q]);
+        __syncthreads();
+        if (lane == 0) red[w] = x;
+        __syncthreads();
+        double t = 0.0;
+        for (int k = 0; k < nw; ++k) t += red[k];
+        v[q] = t;
+    }
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = v[0];
+        out[2] = v[1];
+        out[3] = v[2];
+        out[4] = v[3];
+    }
+}
+}  // namespace
+
+hipError_t nq_launch_iota_keys(uint64_t* keys, int64_t n, uint64_t start, hipStream_t st) {
+    hipLaunchKernelGGL(k_iota_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, start);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_exact_partial(const void* lp, const void* L, int fp64, int64_t n, double* out, hipStream_t st) {
+    if (fp64)
+        hipLaunchKernelGGL(k_exact_partial<double>, dim3(1), dim3(1024), 0, st, (const double*)lp, (const double*)L, n,
+                           out);
+    else
+        hipLaunchKernelGGL(k_exact_partial<float>, dim3(1), dim3(1024), 0, st, (const float*)lp, (const float*)L, n,
+                           out);
+    return hipGetLastError();
+}

@@ -206,3 +206,7 @@ hipError_t nq_launch_stats_cplx(const void* E, int fp64, int64_t B, int64_t Lc, 
 // F = [Re | Im] of sum_s E_s conj(Y_s - mu) / Ns (Y rows [Re O | Im O]); part: chunks x 2 npc scratch
 hipError_t nq_launch_energy_force(const void* Y, const void* E, int fp64, const double* mu, int64_t Ns, int npc,
                                   double* part, int chunks, double* F, hipStream_t st);
+// ---- exact full sums (k_rbm.hip)
+hipError_t nq_launch_iota_keys(uint64_t* keys, int64_t n, uint64_t start, hipStream_t st);
+// out[0..4] = {max log p, sum w, sum w |L|^2, sum w Re L, sum w Im L}, w = exp(2 Re lp - max)
+hipError_t nq_launch_exact_partial(const void* lp, const void* L, int fp64, int64_t n, double* out, hipStream_t st);

@@ -1336,6 +1336,86 @@ nqh_status nqh_get_phase_times(nqh_ctx* c, double* ms_out, int64_t* counts_out) 
     return NQH_OK;
 }
 
+// exact sums over every configuration, in chunks of at most 2^20 keys
+nqh_status nqh_exact(nqh_ctx* c, double* out, double* logp_out) {
+    if (!c || !out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    const int bits = is_ket(c) ? net_N(c) : 2 * net_N(c);
+    if (bits > 30) return fail(c, NQH_E_ARG, "the exact sum is limited to 2^30 configurations");
+    const int64_t total = (int64_t)1 << bits;
+    const int64_t chunk = std::min<int64_t>(total, (int64_t)1 << 20);
+    const int64_t nch = (total + chunk - 1) / chunk;
+    const size_t e = net_elt(c);
+    dbuf kb, lpb, Lb, pb;
+    auto cleanup = [&]() { release(kb); release(lpb); release(Lb); release(pb); };
+    if ((s = ensure(c, kb, sizeof(uint64_t) * chunk)) || (s = ensure(c, lpb, 2 * e * chunk)) ||
+        (s = ensure(c, Lb, 2 * e * chunk)) || (s = ensure(c, pb, sizeof(double) * 5 * nch))) {
+        cleanup();
+        return s;
+    }
+    std::vector<float> lpf;
+    std::vector<double> lpd;
+    hipError_t err = hipSuccess;
+    for (int64_t ci = 0; ci < nch && err == hipSuccess; ++ci) {
+        const int64_t start = ci * chunk, n = std::min(chunk, total - start);
+        err = nq_launch_iota_keys(P<uint64_t>(kb), n, (uint64_t)start, c->stream);
+        if (err != hipSuccess) break;
+        if (is_rbm(c)) {
+            rbm_args a = make_rbm(c, P<uint64_t>(kb), n);
+            a.do_conn = 1;
+            a.lp = lpb.p;
+            a.Lloc = Lb.p;
+            err = nq_launch_rbm_estimate(a, fp64_net(c), est_grid_for(n), c->stream);
+        } else {
+            est_args a = make_est(c, P<uint64_t>(kb), n);
+            a.do_conn = 1;
+            a.lp = lpb.p;
+            a.Lloc = Lb.p;
+            err = nq_launch_estimate(a, fp64_net(c), est_grid_for(n), c->stream);
+        }
+        if (err == hipSuccess)
+            err = nq_launch_exact_partial(lpb.p, Lb.p, fp64_net(c), n, P<double>(pb) + 5 * ci, c->stream);
+        if (err == hipSuccess && logp_out) {
+            if (fp64_net(c)) {
+                lpd.resize(2 * n);
+                err = hipMemcpyAsync(lpd.data(), lpb.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+                if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+                for (int64_t i = 0; i < n; ++i) logp_out[start + i] = 2.0 * lpd[2 * i];
+            } else {
+                lpf.resize(2 * n);
+                err = hipMemcpyAsync(lpf.data(), lpb.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+                if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+                for (int64_t i = 0; i < n; ++i) logp_out[start + i] = 2.0 * (double)lpf[2 * i];
+            }
+        }
+    }
+    std::vector<double> part(5 * nch);
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    if (err == hipSuccess) err = hipMemcpy(part.data(), pb.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost);
+    cleanup();
+    if (err != hipSuccess) return hip_fail(c, err, "nqh_exact");
+    double M = -INFINITY;
+    for (int64_t ci = 0; ci < nch; ++ci) M = std::max(M, part[5 * ci]);
+    double Z = 0.0, ZC = 0.0, ZR = 0.0, ZI = 0.0;
+    for (int64_t ci = 0; ci < nch; ++ci) {
+        const double f = std::exp(part[5 * ci] - M);
+        Z += f * part[5 * ci + 1];
+        ZC += f * part[5 * ci + 2];
+        ZR += f * part[5 * ci + 3];
+        ZI += f * part[5 * ci + 4];
+    }
+    out[0] = ZC / Z;
+    out[1] = ZR / Z;
+    out[2] = ZI / Z;
+    out[3] = M + std::log(Z);
+    if (logp_out)
+        for (int64_t i = 0; i < total; ++i) logp_out[i] -= out[3];
+    if (!std::isfinite(out[0])) return fail(c, NQH_E_NAN, "non-finite exact sum");
+    return NQH_OK;
+}
+
 nqh_status nqh_get_stats(nqh_ctx* c, double* out, int32_t n) {
     if (!c || !out || n < 0 || n > 8) return NQH_E_ARG;
     (void)hipSetDevice(c->device);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "nqh_set_rbmsplit": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "nqh_set_rbm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "nqh_get_stats": (C.c_int32, [C.c_void_p, _dp, C.c_int32]),
+    "nqh_exact": (C.c_int32, [C.c_void_p, _dp, _dp]),
     "nqh_set_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_get_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_set_precision": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -182,6 +183,7 @@ class Context:
         self.nd = int(self.lib.nqh_num_derivs(self.h))
         self.N = N
         self.complex_params = False
+        self.ket = False
 
     def set_rbmsplit(self, N, nh):
         """RBMSplit network; parameters are complex, passed interleaved
@@ -191,6 +193,7 @@ class Context:
         self.nd = int(self.lib.nqh_num_derivs(self.h))
         self.N = N
         self.complex_params = True
+        self.ket = False
 
     def set_rbm(self, N, nh):
         """Ket RBM for a Hamiltonian problem; complex parameters interleaved
@@ -201,6 +204,15 @@ class Context:
         self.N = N
         self.complex_params = True
         self.ket = True
+
+    def exact(self, want_logp=False):
+        """Exact sums over every configuration: (sum p |L|^2, sum p L, log Z[, log p])."""
+        out = np.empty(4)
+        bits = self.N if getattr(self, "ket", False) else 2 * self.N
+        lp = np.empty(1 << bits) if want_logp else None
+        self._chk(self.lib.nqh_exact(self.h, _p(out, _dp), _p(lp, _dp)))
+        res = (float(out[0]), complex(out[1], out[2]), float(out[3]))
+        return res + (lp,) if want_logp else res
 
     def get_stats(self, n=6):
         out = np.empty(n)
