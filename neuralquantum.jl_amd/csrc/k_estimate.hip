@@ -67,11 +67,15 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
     double* sF = reinterpret_cast<double*>(smem + a.lds_param_bytes);
     const int fbytes = (a.Fpart || a.G) ? d.np * (int)sizeof(double) : 0;
     cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + fbytes);
+    const int wpb0 = blockDim.x >> 6;
+    uint2* sM = reinterpret_cast<uint2*>(sW + wpb0 * a.max_conn);          // flip masks of the cached connections
+    unsigned char* sT = smem + ((reinterpret_cast<unsigned char*>(sM + wpb0 * a.max_conn) - smem + 15) & ~15);
     const T* gp = reinterpret_cast<const T*>(a.p);
     const bool useF = a.Fpart != nullptr || a.G != nullptr;
     for (int k = threadIdx.x; k < d.np; k += blockDim.x) sp[k] = gp[k];
     if (useF)
         for (int k = threadIdx.x; k < d.np; k += blockDim.x) sF[k] = 0.0;
+    const dev_table tab = a.tab_lds ? table_to_lds(a.tab, sT) : a.tab;
     __syncthreads();
     auto fadd = [&](int pidx, double v) { atomicAdd(&sF[pidx], v); };
 
@@ -82,6 +86,7 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
     const int64_t nw = (int64_t)gridDim.x * wpb;
     const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
     cplx<T>* wbuf = sW + wave * a.max_conn;
+    uint2* mbuf = sM + wave * a.max_conn;
 
     // P-order offsets of the force vector (compact layout, see ndm_dims)
     const int PB_bla = 0, PB_hla = N, PB_wla = N + nh;
@@ -163,8 +168,8 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
                 re += half * sp[d.bla + lane] * (xs + xe);
                 im += half * sp[d.bmu + lane] * (xs - xe);
             }
-            re = wave_sum(re);
-            im = wave_sum(im);
+            re = wave_total(re);
+            im = wave_total(im);
             if (lane == 0 && a.lp) {
                 T* lp = reinterpret_cast<T*>(a.lp);
                 lp[2 * s] = re;
@@ -226,16 +231,17 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
 
         if (!a.do_conn) continue;
 
-        // ---- pass 1: W_c and L_loc
+        // ---- pass 1: W_c and L_loc; the changed connections (W, masks) are
+        // cached for pass 2, the unchanged ones only summed (Wdiag)
         cplx<T> Lloc = cmk(T(0), T(0));
+        cplx<T> Wdiag = cmk(T(0), T(0));
         int nc = 0;
-        table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+        table_walk<T>(tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
             if ((ms | me) == 0u) {
                 // unchanged configuration: exp(0) = 1, W = mel exactly (the
                 // reference evaluates the network anyway, AccumulatorObsGrad.jl:72)
                 Lloc = cadd(Lloc, mel);
-                if (nc < a.max_conn) wbuf[nc] = mel;
-                ++nc;
+                Wdiag = cadd(Wdiag, mel);
                 return;
             }
             T dre = T(0), dim = T(0);
@@ -289,8 +295,8 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
                     dim += f2.im - pi_f[r].im;
                 }
             }
-            dre = wave_sum(dre);
-            dim = wave_sum(dim);
+            dre = wave_total(dre);
+            dim = wave_total(dim);
             for (uint32_t m = ms; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
                 const T c = cflip<T>(ks, i) * half;
@@ -308,7 +314,10 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
             const T er = nq_exp(dre);
             const cplx<T> W = cmul(mel, cmk(er * cs, er * sn));
             Lloc = cadd(Lloc, W);
-            if (nc < a.max_conn) wbuf[nc] = W;   // every lane writes the same value
+            if (nc < a.max_conn) {   // every lane writes the same values
+                wbuf[nc] = W;
+                mbuf[nc] = make_uint2(ms, me);
+            }
             ++nc;
         });
 
@@ -332,16 +341,14 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
             for (int r = 0; r < RS; ++r) e_s[r] = e_e[r] = T(0);
 #pragma unroll
             for (int r = 0; r < PS; ++r) rho1[r] = rho2[r] = T(0);
-            int c = 0;
-            cplx<T> VWdiag = cmk(T(0), T(0));   // sum of V W_c over unchanged configurations
-            table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
-                (void)mel;
-                const cplx<T> VW = cmul(V, c < a.max_conn ? wbuf[c] : cmk(T(0), T(0)));
-                ++c;
-                if ((ms | me) == 0u) {   // O(c) = O(s): only the dense parts, added once below
-                    VWdiag = cadd(VWdiag, VW);
-                    return;
-                }
+            // unchanged configurations contribute V W = V mel with O(c) = O(s):
+            // only the dense parts, added once below
+            const cplx<T> VWdiag = cmul(V, Wdiag);
+            const int ncc = nc < a.max_conn ? nc : a.max_conn;
+            for (int c = 0; c < ncc; ++c) {
+                const cplx<T> VW = cmul(V, wbuf[c]);
+                const uint2 mm = mbuf[c];
+                const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
 #pragma unroll
                 for (int r = 0; r < RS; ++r) {
                     const int u = lane + 64 * r;
@@ -442,7 +449,7 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
                         fadd(PB_bmu + lane, (double)(cc * VW.im));
                     }
                 }
-            });
+            }
 
             // unchanged configurations contribute O(s) with weight sum VWdiag
 #pragma unroll
@@ -537,8 +544,10 @@ hipError_t launch_n(const est_args& a, int grid, hipStream_t st) {
 }  // namespace
 
 size_t est_lds_bytes(const est_args& a, size_t elt) {
-    return (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? sizeof(double) * (size_t)a.d.np : 0) +
-           4 * (size_t)a.max_conn * 2 * elt;
+    // params | FP64 force | 4 waves x (W cache + flip masks) | table (16-aligned)
+    const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? sizeof(double) * (size_t)a.d.np : 0) +
+                     4 * (size_t)a.max_conn * (2 * elt + 8);
+    return ((b + 15) & ~(size_t)15) + (a.tab_lds ? (size_t)a.tab.bytes : 0);
 }
 
 hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st) {

@@ -52,10 +52,12 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
     double* sF = reinterpret_cast<double*>(smem + a.lds_param_bytes);   // [Re (npc) | Im (npc)]
     const bool useF = a.Fpart != nullptr || a.G != nullptr;
     cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + (useF ? 2 * npc * (int)sizeof(double) : 0));
+    unsigned char* sT = smem + ((reinterpret_cast<unsigned char*>(sW + (blockDim.x >> 6) * a.max_conn) - smem + 15) & ~15);
     const cplx<T>* gp = reinterpret_cast<const cplx<T>*>(a.p);
     for (int k = threadIdx.x; k < npc; k += blockDim.x) sp[k] = gp[k];
     if (useF)
         for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) sF[k] = 0.0;
+    const dev_table tab = a.tab_lds ? table_to_lds(a.tab, sT) : a.tab;
     __syncthreads();
     auto fadd = [&](int k, cplx<T> v) {
         atomicAdd(&sF[k], (double)v.re);
@@ -107,8 +109,8 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
                 cfma_r(acc, sp[oar + lane], xv<T>(ks, lane));
                 if (!ket) cfma_r(acc, sp[oac + lane], xv<T>(ke, lane));
             }
-            acc.re = wave_sum(acc.re);
-            acc.im = wave_sum(acc.im);
+            acc.re = wave_total(acc.re);
+            acc.im = wave_total(acc.im);
             if (lane == 0 && a.lp) {
                 T* lp = reinterpret_cast<T*>(a.lp);
                 lp[2 * s] = acc.re;
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
         // ---- pass 1: W_c = mel exp(log rho_c - log rho_s), L_loc
         cplx<T> Lloc = cmk(T(0), T(0));
         int nc = 0;
-        table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+        table_walk<T>(tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
             if ((ms | me) == 0u) {
                 Lloc = cadd(Lloc, mel);
                 if (nc < a.max_conn) wbuf[nc] = mel;
@@ -177,8 +179,8 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
                     dl = cadd(dl, csub(f2, f[r]));
                 }
             }
-            dl.re = wave_sum(dl.re);
-            dl.im = wave_sum(dl.im);
+            dl.re = wave_total(dl.re);
+            dl.im = wave_total(dl.im);
             for (uint32_t m = ms; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
                 cfma_r(dl, sp[oar + i], cflip<T>(ks, i));
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
         for (int r = 0; r < HS; ++r) e[r] = cmk(T(0), T(0));
         cplx<T> sumVW = cmk(T(0), T(0));
         int c = 0;
-        table_walk<T>(a.tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
+        table_walk<T>(tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
             (void)mel;
             const cplx<T> VW = cmul(V, c < a.max_conn ? wbuf[c] : cmk(T(0), T(0)));
             ++c;
@@ -555,8 +557,9 @@ hipError_t samp_t(const rbm_samp_args& a, hipStream_t st) {
 }  // namespace
 
 size_t rbm_est_lds_bytes(const rbm_args& a, size_t elt) {
-    return (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? 2 * sizeof(double) * (size_t)a.npc : 0) +
-           4 * (size_t)a.max_conn * 2 * elt;
+    const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? 2 * sizeof(double) * (size_t)a.npc : 0) +
+                     4 * (size_t)a.max_conn * 2 * elt;
+    return ((b + 15) & ~(size_t)15) + (a.tab_lds ? (size_t)a.tab.bytes : 0);
 }
 
 hipError_t nq_launch_rbm_estimate(const rbm_args& a, bool fp64, int grid, hipStream_t st) {

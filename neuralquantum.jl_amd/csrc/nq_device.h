@@ -119,6 +119,40 @@ __device__ __forceinline__ double group_sum(double v, int width) {
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) { return group_sum(v, 64); }
 
+// Sum over the 64 lanes, the same value returned to every lane: DPP steps
+// within each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], half-row
+// mirror, row mirror), then the four row sums read out with v_readlane.
+// All 64 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float rl_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double rl_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ float wave_total(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+}
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+}
+
 // ---------------------------------------------------------------- NDM layout
 // Offsets of the NDM blocks in the reference's flat order (NDM.jl:17-18):
 // b_mu[N] h_mu[nh] w_mu[nh x N] u_mu[na x N] b_la[N] h_la[nh] d_la[na] w_la[nh x N] u_la[na x N]

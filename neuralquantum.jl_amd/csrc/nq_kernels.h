@@ -17,6 +17,7 @@ struct est_args {
     int do_conn;              // walk connections (L_loc, force)
     int lds_param_bytes;      // LDS bytes reserved for the staged parameters
     int max_conn;             // upper bound of connections per configuration (LDS W cache)
+    int tab_lds;              // stage the connection table in LDS (tab.bytes small enough)
     void* lp;                 // 2 Ns T: log rho (re, im), nullable
     void* Y;                  // Ns x R T: compact O rows, nullable
     void* Lloc;               // 2 Ns T, nullable
@@ -166,6 +167,7 @@ struct rbm_args {
     int do_conn;
     int lds_param_bytes;      // >= 2 npc sizeof(T), multiple of 16
     int max_conn;
+    int tab_lds;              // stage the connection table in LDS
     void* lp;                 // 2 Ns T, nullable
     void* Y;                  // Ns x 2 npc T: [Re O | Im O], nullable
     void* Lloc;               // 2 Ns T, nullable

@@ -15,7 +15,31 @@ struct dev_table {
     const double* ent_mel;       // interleaved re, im
     const float* ent_mel_f;      // same values in FP32
     const uint32_t* ent_flip;    // physical flip mask (N <= 32)
+    const void* base;            // the one allocation all arrays above live in
+    int bytes;                   // its size (multiple of 16)
 };
+
+// Copy the table into LDS (all threads of the block take part; the caller
+// synchronises) and return a view whose arrays point into the copy.
+__device__ __forceinline__ dev_table table_to_lds(const dev_table& g, unsigned char* lds) {
+    const uint4* src = reinterpret_cast<const uint4*>(g.base);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int i = threadIdx.x; i < (g.bytes >> 4); i += blockDim.x) dst[i] = src[i];
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(g.base);
+    auto rb = [&](const void* q) { return lds + (reinterpret_cast<const unsigned char*>(q) - b); };
+    dev_table t = g;
+    t.term_kind = reinterpret_cast<const int32_t*>(rb(g.term_kind));
+    t.term_op_l = reinterpret_cast<const int32_t*>(rb(g.term_op_l));
+    t.term_op_r = reinterpret_cast<const int32_t*>(rb(g.term_op_r));
+    t.op_site_off = reinterpret_cast<const int32_t*>(rb(g.op_site_off));
+    t.op_sites = reinterpret_cast<const int32_t*>(rb(g.op_sites));
+    t.op_row_off = reinterpret_cast<const int32_t*>(rb(g.op_row_off));
+    t.row_ent_off = reinterpret_cast<const int32_t*>(rb(g.row_ent_off));
+    t.ent_mel = reinterpret_cast<const double*>(rb(g.ent_mel));
+    t.ent_mel_f = reinterpret_cast<const float*>(rb(g.ent_mel_f));
+    t.ent_flip = reinterpret_cast<const uint32_t*>(rb(g.ent_flip));
+    return t;
+}
 
 __device__ __forceinline__ int table_local_row(const dev_table& t, int o, uint32_t bits) {
     int r = 0;

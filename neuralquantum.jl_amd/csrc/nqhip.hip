@@ -94,6 +94,7 @@ struct nqh_ctx {
 namespace {
 
 constexpr int kColChunks = 64;
+constexpr int kTableLdsMax = 32 * 1024;   // connection tables up to this size are staged in LDS
 
 nqh_status fail(nqh_ctx* c, nqh_status s, const std::string& msg) {
     if (c) c->err = msg;
@@ -253,6 +254,7 @@ est_args make_est(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
     a.do_conn = 0;
     a.lds_param_bytes = (int)(((size_t)c->d.np * net_elt(c) + 15) / 16 * 16);
     a.max_conn = std::max(c->max_conn, 1);
+    a.tab_lds = (c->has_problem && c->tab.bytes <= kTableLdsMax) ? 1 : 0;
     return a;
 }
 
@@ -269,6 +271,7 @@ rbm_args make_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
     a.invNs = 1.0 / (double)Ns;
     a.lds_param_bytes = (int)((2 * (size_t)c->rbm_npc * net_elt(c) + 15) / 16 * 16);
     a.max_conn = std::max(c->max_conn, 1);
+    a.tab_lds = (c->has_problem && c->tab.bytes <= kTableLdsMax) ? 1 : 0;
     return a;
 }
 
@@ -595,7 +598,7 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     const size_t bytes_d = (size_t)2 * t->n_entries * sizeof(double);
     const size_t bytes_f = ((size_t)2 * t->n_entries * sizeof(float) + 15) / 16 * 16;
     const size_t bytes_m = (size_t)t->n_entries * sizeof(uint32_t);
-    std::vector<unsigned char> host(bytes_i + bytes_d + bytes_f + bytes_m + 16, 0);
+    std::vector<unsigned char> host((bytes_i + bytes_d + bytes_f + bytes_m + 16 + 15) / 16 * 16, 0);
     int32_t* hi = reinterpret_cast<int32_t*>(host.data());
     size_t off = 0;
     auto put = [&](const int32_t* src, int n) { std::memcpy(hi + off, src, sizeof(int32_t) * n); size_t o = off; off += n; return o; };
@@ -629,6 +632,8 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     c->tab.ent_mel = reinterpret_cast<const double*>(base + bytes_i);
     c->tab.ent_mel_f = reinterpret_cast<const float*>(base + bytes_i + bytes_d);
     c->tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
+    c->tab.base = base;
+    c->tab.bytes = (int)host.size();
     c->N_table = N;
     c->max_conn = maxc;
     c->problem_ket = n_ket > 0;
