@@ -212,13 +212,13 @@ def load_traffic(form):
 
 
 def copy_calibration(nbytes=256 << 20):
-    """One device-to-device copy kernel of known size (reads and writes
-    `nbytes` each), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    """One elementwise kernel of known traffic (reads `nbytes`, writes
+    `nbytes`: dst = 2 src), for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
     import torch
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
-    dst.copy_(src)
+    torch.mul(src, 2.0, out=dst)
     torch.cuda.synchronize()
     del src, dst
 

@@ -3,9 +3,10 @@
 //   stats ........ stat_analysis of |L_loc|^2 over chains x slices (stats.jl:52-77)
 //   force ........ F = Re(conj(L_loc) dL_loc)/Ns - C Re(Obar)   (BatchedGradSampler.jl:108-118)
 //   shift ........ S_ii += eps (sr_shift, SRDirect.jl:60-65) or eps * S_ii
-//   Cholesky ..... blocked right-looking, FP64: diagonal block (1 wave, LDS),
-//                  panel triangular solve, trailing update on FP64 MFMA
-//                  (cholesky!(Hermitian(S)) + ldiv!, SRDirect.jl:78-81)
+//   Cholesky ..... blocked right-looking, FP64, panels of 32 columns: diagonal
+//                  block factored in registers of one wavefront (v_readlane
+//                  broadcasts), panel triangular solve, trailing update on
+//                  FP64 MFMA (cholesky!(Hermitian(S)) + ldiv!, SRDirect.jl:78-81)
 //   update ....... p -= lr * Delta (Descent, src/Optimisers/rules.jl:15-17)
 #include "nq_kernels.h"
 
@@ -116,96 +117,68 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
     s += (kind == 0) ? eps : eps * s;
 }
 
-// ---- Cholesky step k: factor the diagonal block and solve the panel below it.
-// Workgroup of 256 threads (4 waves).  Every workgroup factors the kb x kb
-// diagonal block (padded with the identity to 64 x 64): thread (tr, tc) of a
-// 16 x 16 grid keeps the 16 entries (tr + 16a, tc + 16b) in registers, and
-// step j is one barrier: the owners of column j publish it to LDS
-// (double-buffered), every thread applies the rank-1 update
-// A[r][c] -= A[r][j] A[c][j] / A[j][j] to its own entries and the owners
-// scale column j by 1 / sqrt(A[j][j]).  Workgroup 0 writes the factor back;
-// workgroup b >= 1 solves rows k + kb + 256 (b - 1) + threadIdx.x of the panel,
-// X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution against the
-// factor held in LDS.
-__global__ __launch_bounds__(256) void k_chol_panel(double* A, int n, int lda, int k, int kb, int* info) {
-    __shared__ double L[NB][NB + 1];
-    __shared__ double rdiag[NB];
-    __shared__ double col[2][NB];
-    const int tid = threadIdx.x;
-    const int tr = tid & 15, tc = tid >> 4;
-    double e[4][4];                                // e[a][b] = A[tr + 16a][tc + 16b]
+// ---- Cholesky step k with a panel of CB = 32 columns, one wavefront per block.
+// Every block factors the kb x kb diagonal block (padded with the identity)
+// with lane t < 32 holding row t in registers; column j is broadcast with
+// v_readlane (no LDS and no barrier on the dependent chain).  Block 0 writes
+// the factor back; block b >= 1 solves rows k + kb + 64 (b - 1) + lane of the
+// panel, X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution against
+// the factor in LDS.
+constexpr int CB = 32;
+
+__global__ __launch_bounds__(64, 1) void k_chol_panel32(double* A, int n, int lda, int k, int kb, int* info) {
+    __shared__ double L[CB][CB + 1];
+    __shared__ double rdiag[CB];
+    const int t = threadIdx.x, tr = t & (CB - 1);
+    double r[CB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int c = 0; c < CB; ++c)
+        r[c] = (tr < kb && c < kb && tr >= c) ? A[(int64_t)(k + tr) + (int64_t)(k + c) * lda] : (tr == c ? 1.0 : 0.0);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = tr + 16 * a, c = tc + 16 * b;
-            e[a][b] = (r < kb && c < kb && r >= c) ? A[(int64_t)(k + r) + (int64_t)(k + c) * lda]
-                                                    : (r == c ? 1.0 : 0.0);
-        }
+    for (int j = 0; j < CB; ++j) {
+        const double v = rl_d(r[j], j);                      // current A[j][j]
+        if (j < kb && t == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
+        const double piv = sqrt(v);
+        const double rp = 1.0 / piv;
+        const double lj = (tr == j) ? piv : r[j] * rp;      // L[tr][j] (zero above the diagonal)
+        r[j] = lj;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int jb = j >> 4, jc = j & 15;
-        double* cb = col[j & 1];
-        if (tc == jc) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) cb[tr + 16 * a] = e[a][jb];
-        }
-        __syncthreads();
-        const double v = cb[j];
-        if (j < kb && tid == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
-        const double rv = 1.0 / v;
-        double cr[4], cc[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) { cr[a] = cb[tr + 16 * a] * rv; cc[a] = cb[tc + 16 * a]; }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = tr + 16 * a, c = tc + 16 * b;
-                if (b >= jb && c > j && r >= c) e[a][b] -= cr[a] * cc[b];
-            }
-        if (tc == jc) {
-            const double rp = 1.0 / sqrt(v);
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int r = tr + 16 * a;
-                if (r >= j) e[a][jb] *= rp;
-            }
+        for (int m = j + 1; m < CB; ++m) {
+            const double lmj = rl_d(lj, m);                  // L[m][j]
+            if (tr >= m) r[m] -= lj * lmj;
         }
     }
+    if (t < CB) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) L[tr + 16 * a][tc + 16 * b] = e[a][b];
+        for (int c = 0; c < CB; ++c) L[t][c] = r[c];
+    }
     __syncthreads();
-    if (tid < NB) rdiag[tid] = 1.0 / L[tid][tid];
+    if (t < CB) rdiag[t] = 1.0 / L[t][t];
+    __syncthreads();
     if (blockIdx.x == 0) {
+        if (t < kb) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = tr + 16 * a, c = tc + 16 * b;
-                if (r < kb && c < kb && r >= c) A[(int64_t)(k + r) + (int64_t)(k + c) * lda] = e[a][b];
-            }
+            for (int c = 0; c < CB; ++c)
+                if (c < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = r[c];
+        }
         return;
     }
-    __syncthreads();
-    const int i = k + kb + (blockIdx.x - 1) * 256 + tid;
+    const int i = k + kb + (blockIdx.x - 1) * 64 + t;
     if (i >= n) return;
-    double x[NB];
+    double x[CB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) x[j] = (j < kb) ? A[(int64_t)i + (int64_t)(k + j) * lda] : 0.0;
+    for (int j = 0; j < CB; ++j) x[j] = (j < kb) ? A[(int64_t)i + (int64_t)(k + j) * lda] : 0.0;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
+    for (int j = 0; j < CB; ++j) {
         if (j < kb) {
             x[j] *= rdiag[j];
 #pragma unroll
-            for (int m = j + 1; m < NB; ++m)
+            for (int m = j + 1; m < CB; ++m)
                 if (m < kb) x[m] -= x[j] * L[m][j];
         }
     }
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
+    for (int j = 0; j < CB; ++j)
         if (j < kb) A[(int64_t)i + (int64_t)(k + j) * lda] = x[j];
 }
 
@@ -219,14 +192,15 @@ __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
 
 // ---- trailing update: A[i, j] -= sum_m X[i, m] X[j, m] for tiles I >= J of the
 // trailing matrix (origin o = k + kb), X = A[:, k:k+kb]; FP64 MFMA 16x16x4
+template <int KW>
 __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda, int k, int kb) {
-    __shared__ double Xi[NB][65];   // [m][row]
-    __shared__ double Xj[NB][65];
+    __shared__ double Xi[KW][65];   // [m][row]
+    __shared__ double Xj[KW][65];
     int tI, tJ;
     tri_decode(blockIdx.x, &tJ, &tI);   // tJ <= tI
     const int o = k + kb;
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
-    for (int e = threadIdx.x; e < 64 * NB; e += blockDim.x) {
+    for (int e = threadIdx.x; e < 64 * KW; e += blockDim.x) {
         const int r = e & 63, m = e >> 6;
         double vi = 0.0, vj = 0.0;
         if (m < kb) {
@@ -247,7 +221,7 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
-    for (int m = 0; m < NB; m += 4) {
+    for (int m = 0; m < KW; m += 4) {
         double av[2], bv[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) av[a] = Xi[m + lk][wr * 32 + a * 16 + li];
@@ -463,13 +437,13 @@ hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st) {
     // rows >= n (at most one extra row, the right-hand side) are carried along:
     // the panel solves and trailing updates turn row n into (L^-1 b)^T
     const int nr = (lds > n) ? n + 1 : n;
-    for (int k = 0; k < n; k += NB) {
-        const int kb = (n - k < NB) ? n - k : NB;
+    for (int k = 0; k < n; k += CB) {
+        const int kb = (n - k < CB) ? n - k : CB;
         const int rest = nr - k - kb;
-        hipLaunchKernelGGL(k_chol_panel, dim3(1 + nblk(rest, 256)), dim3(256), 0, st, S, nr, lds, k, kb, info);
+        hipLaunchKernelGGL(k_chol_panel32, dim3(1 + nblk(rest, 64)), dim3(64), 0, st, S, nr, lds, k, kb, info);
         if (rest > 0) {
             const int T = (rest + 63) / 64;
-            hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, nr, lds, k, kb);
+            hipLaunchKernelGGL(k_syrk_trailing<CB>, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, nr, lds, k, kb);
         }
     }
     return hipGetLastError();
