@@ -222,6 +222,24 @@ nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
  * At most 2^30 configurations. */
 nqh_status nqh_exact(nqh_ctx* ctx, double* out, double* logp_out);
 
+/* compute_observables(is) for one observable (BatchedObsDMSampler.jl:59-105):
+ * a Metropolis chain on the diagonal (sigma, sigma) of the density matrix,
+ * restarted at every call and drawn from its own random stream (counter word
+ * 3 = 16 + {0, 1}), `chains` x `chain_length` samples after `burn` calls of
+ * `passes` proposals, each flipping one site of sigma and eta together; then
+ * O_loc(sigma) = sum <sigma|O|sigma'> rho(sigma', sigma) / rho(sigma, sigma) for
+ * the ket operator `op` (kind 3 terms) and the statistics of O_loc:
+ * stats_out = [Re mean, error, variance, tau, R, Im mean]; the mean estimates
+ * Tr(O rho) / Tr(rho).  Two reference behaviours are corrected: the chain uses
+ * log p = Re log rho(sigma, sigma), not abs(log rho) (base_batched_networks.jl:
+ * 249-253), and the network is evaluated at (sigma', sigma), not at the
+ * diagonal (sigma', sigma') the accumulator feeds it (AccumulatorLogPsi.jl:85-91). */
+nqh_status nqh_observable(nqh_ctx* ctx, const nqh_optable* op, int64_t chains, int32_t chain_length,
+                          int32_t passes, int32_t burn, double* stats_out);
+/* the last observable's doubled keys (nullable) and local values (nullable,
+ * 2 n doubles), n <= chains x chain_length */
+nqh_status nqh_get_observable_samples(nqh_ctx* ctx, uint64_t* keys_out, double* local_out, int64_t n);
+
 /* statistics of the last gradient: [mean, error, variance, tau, R] of |L_loc|^2
  * (Liouvillian) or of E_loc (Hamiltonian: mean real part; out[5] = imaginary
  * part of the mean); n <= 8 values */

@@ -300,7 +300,9 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
     __syncthreads();
     const bool ket = a.ket != 0;
     const int oar = 0, oac = N, ob = ket ? N : 2 * N, owr = ob + nh, owc = owr + nh * N;
-    const int nsite = ket ? N : 2 * N;   // sites a proposal picks from
+    const bool diag = a.diag != 0;      // density matrix on its diagonal: log p = Re log rho(sigma, sigma)
+    const int nsite = (ket || diag) ? N : 2 * N;   // sites a proposal picks from
+    const uint32_t ctr3 = 16u * a.tag;
 
     const int lpc = a.lpc;
     const int lane = threadIdx.x & 63;
@@ -317,9 +319,10 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
     uint64_t key = 0;
     if (live) {
         if (a.restart) {
-            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, 0u, k0, k1);
+            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, ctr3, k0, k1);
             const uint64_t m2 = (nsite >= 64) ? ~0ull : ((1ull << nsite) - 1ull);
             key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
+            if (diag) key |= key << N;
         } else {
             key = a.chain_state[chain];
         }
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
         for (int q = 0; q < a.passes; ++q, ++t) {
             const int slot = (int)((t - a.t0) % (uint32_t)lpc);
             if (slot == 0) {
-                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, 1u, k0, k1);
+                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, ctr3 + 1u, k0, k1);
                 rx0 = o.x[0]; rx1 = o.x[1]; rx2 = o.x[2];
             }
             const uint32_t x0 = __shfl(rx0, gbase + slot, 64);
@@ -372,13 +375,15 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
                 nfr[r] = fr[r];
                 if (h < nh) {
                     cfma_r(nth[r], sp[ow + h + nh * j], c);
+                    if (diag) cfma_r(nth[r], sp[owc + h + nh * j], c);
                     nfr[r] = csoftplus_re(nth[r]);
                     delta += nfr[r] - fr[r];
                 }
             }
             delta = group_sum(delta, lpc);
             delta += c * sp[(col ? oac : oar) + j].re;
-            delta *= T(2);
+            if (diag) delta += c * sp[oac + j].re;
+            else delta *= T(2);
             bool accept;
             if constexpr (sizeof(T) == 8) {
                 const double u = u01_53(x1, x2);
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
                 accept = !(u - expf((float)delta) >= 0.0f);
             }
             if (accept && live) {
-                key ^= (1ull << site);
+                key ^= diag ? ((1ull << site) | (1ull << (site + N))) : (1ull << site);
                 ++acc;
 #pragma unroll
                 for (int r = 0; r < US; ++r) { th[r] = nth[r]; fr[r] = nfr[r]; }

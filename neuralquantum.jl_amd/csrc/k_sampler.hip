@@ -49,12 +49,19 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     const uint32_t iter = a.iter_dev ? *a.iter_dev : a.iter;
     const T half = T(0.5);
 
+    const bool diag = a.diag != 0;
+    const uint32_t ctr3 = 16u * a.tag;
     uint64_t key = 0;
     if (live) {
         if (a.restart) {
-            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, 0u, k0, k1);
-            const uint64_t m2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1ull);
-            key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
+            philox_out o = philox4x32_10(0u, iter, (uint32_t)chain, ctr3, k0, k1);
+            if (diag) {
+                const uint64_t s = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & (uint64_t)pmask;
+                key = s | (s << N);
+            } else {
+                const uint64_t m2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1ull);
+                key = ((uint64_t)o.x[0] | ((uint64_t)o.x[1] << 32)) & m2;
+            }
         } else {
             key = a.chain_state[chain];
         }
@@ -99,13 +106,14 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
         for (int q = 0; q < a.passes; ++q, ++t) {
             const int slot = (int)((t - a.t0) % (uint32_t)lpc);
             if (slot == 0) {
-                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, 1u, k0, k1);
+                philox_out o = philox4x32_10(t + (uint32_t)g, iter, (uint32_t)chain, ctr3 + 1u, k0, k1);
                 rx0 = o.x[0]; rx1 = o.x[1]; rx2 = o.x[2];
             }
             const uint32_t x0 = __shfl(rx0, gbase + slot, 64);
             const uint32_t x1 = __shfl(rx1, gbase + slot, 64);
             const uint32_t x2 = __shfl(rx2, gbase + slot, 64);
-            const int site = (int)(((uint64_t)x0 * (uint64_t)(2 * N)) >> 32);
+            // diagonal chain: site j flipped in sigma and eta, log p = Re log rho(sigma, sigma)
+            const int site = (int)(((uint64_t)x0 * (uint64_t)(diag ? N : 2 * N)) >> 32);
             const bool col = site >= N;
             const int j = col ? site - N : site;
             const uint32_t bits = col ? (uint32_t)(key >> N) : (uint32_t)key;
@@ -126,11 +134,17 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                     delta += nf[r] - (col ? f_e[r] : f_s[r]);
                 }
                 if (h < na) {
-                    const T cu = c * half;
-                    nz[r].re += cu * sp[d.ula + h + na * j];
-                    nz[r].im += (col ? -cu : cu) * sp[d.umu + h + na * j];
-                    nfpi[r] = csoftplus_re(nz[r]);
-                    delta += T(2) * (nfpi[r] - fpi[r]);
+                    if (diag) {
+                        nz[r].re += c * sp[d.ula + h + na * j];
+                        nfpi[r] = csoftplus_re(nz[r]);
+                        delta += nfpi[r] - fpi[r];
+                    } else {
+                        const T cu = c * half;
+                        nz[r].re += cu * sp[d.ula + h + na * j];
+                        nz[r].im += (col ? -cu : cu) * sp[d.umu + h + na * j];
+                        nfpi[r] = csoftplus_re(nz[r]);
+                        delta += T(2) * (nfpi[r] - fpi[r]);
+                    }
                 }
             }
             delta = group_sum(delta, lpc);
@@ -145,13 +159,14 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                 accept = !(u - expf((float)delta) >= 0.0f);
             }
             if (accept && live) {
-                key ^= (1ull << site);
+                key ^= diag ? ((1ull << site) | (1ull << (site + N))) : (1ull << site);
                 ++acc;
 #pragma unroll
                 for (int r = 0; r < US; ++r) {
                     const int h = g + lpc * r;
                     if (h < nh) {
-                        if (col) { th_e[r] = nth[r]; f_e[r] = nf[r]; }
+                        if (diag) { th_s[r] = th_e[r] = nth[r]; f_s[r] = f_e[r] = nf[r]; }
+                        else if (col) { th_e[r] = nth[r]; f_e[r] = nf[r]; }
                         else { th_s[r] = nth[r]; f_s[r] = nf[r]; }
                     }
                     if (h < na) { z[r] = nz[r]; fpi[r] = nfpi[r]; }

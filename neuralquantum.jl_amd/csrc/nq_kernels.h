@@ -35,6 +35,8 @@ struct samp_args {
     int passes, burn, stored;  // calls = burn + stored
     uint32_t t0;               // index of this launch's first proposal within the iteration
     int slice0;                // slice index of this launch's first stored sample
+    int diag;                  // chain on the diagonal (sigma, sigma): p ~ rho(sigma, sigma), both sides flipped
+    uint32_t tag;              // random stream: counter word 3 = {0 start, 1 proposal} + 16 tag
     int lpc;                   // lanes per chain (power of two)
     int restart;               // draw a fresh uniform configuration per chain
     int compat_stale_last_slice;
@@ -179,6 +181,8 @@ hipError_t nq_launch_rbm_estimate(const rbm_args& a, bool fp64, int grid, hipStr
 struct rbm_samp_args {
     int N, nh, npc;
     int ket;
+    int diag;                 // diagonal chain (density matrices): p ~ |rho(sigma, sigma)|
+    uint32_t tag;
     const void* p;
     int64_t chains;
     int passes, burn, stored;

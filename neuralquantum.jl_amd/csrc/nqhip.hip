@@ -83,6 +83,10 @@ struct nqh_ctx {
     dbuf made_th, made_m, made_v, made_gpart, made_lpart, made_g, made_nll, made_keys, made_lp, made_lq, ring,
         ring_posd, made_tmp;
 
+    // observables (diagonal chain + ket operator on the row index)
+    dbuf obs_table_mem, obs_keys, obs_state, obs_lp, obs_L, obs_stats;
+    dev_table obs_tab{};
+
     // per-phase HIP-event timing
     bool profiling = false;
     hipEvent_t ev[NQH_NUM_PHASES][2] = {};
@@ -535,7 +539,8 @@ void nqh_destroy(nqh_ctx* c) {
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
                    &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
-                   &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp};
+                   &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
+                   &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats};
     drop_graph(c);
     release(c->iter_dev);
     for (dbuf* b : all) release(*b);
@@ -548,9 +553,10 @@ void nqh_destroy(nqh_ctx* c) {
 
 const char* nqh_last_error(const nqh_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
-    if (!c || !t) return NQH_E_ARG;
-    (void)hipSetDevice(c->device);
+// validate a flat table and upload it as one device allocation; with
+// `ket_as_row` the ket terms (kind 3) are stored as row-side terms (kind 0)
+static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row, dbuf& mem, dev_table& tab,
+                               int* maxc_out, int* nket_out) {
     const int N = t->n_sites;
     if (N < 1 || N > 32) return fail(c, NQH_E_ARG, "n_sites must be in 1..32");
     if (t->n_terms < 0 || t->n_ops < 0 || t->n_rows < 0 || t->n_entries < 0) return fail(c, NQH_E_ARG, "negative sizes");
@@ -603,6 +609,9 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     size_t off = 0;
     auto put = [&](const int32_t* src, int n) { std::memcpy(hi + off, src, sizeof(int32_t) * n); size_t o = off; off += n; return o; };
     const size_t o_kind = put(t->term_kind, t->n_terms);
+    if (ket_as_row)
+        for (int q = 0; q < t->n_terms; ++q)
+            if (hi[o_kind + q] == 3) hi[o_kind + q] = 0;
     const size_t o_l = put(t->term_op_l, t->n_terms);
     const size_t o_r = put(t->term_op_r, t->n_terms);
     const size_t o_so = put(t->op_site_off, t->n_ops + 1);
@@ -615,26 +624,37 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
     for (int e = 0; e < 2 * t->n_entries; ++e) hf[e] = (float)t->ent_mel[e];
     uint32_t* hm = reinterpret_cast<uint32_t*>(host.data() + bytes_i + bytes_d + bytes_f);
     for (int e = 0; e < t->n_entries; ++e) hm[e] = (uint32_t)t->ent_flip[e];
-    nqh_status s = ensure(c, c->table_mem, host.size());
+    nqh_status s = ensure(c, mem, host.size());
     if (s) return s;
-    HIPCHK(c, hipMemcpy(c->table_mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
-    unsigned char* base = reinterpret_cast<unsigned char*>(c->table_mem.p);
+    HIPCHK(c, hipMemcpy(mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    unsigned char* base = reinterpret_cast<unsigned char*>(mem.p);
     const int32_t* di = reinterpret_cast<const int32_t*>(base);
-    c->tab.n_sites = N;
-    c->tab.n_terms = t->n_terms;
-    c->tab.term_kind = di + o_kind;
-    c->tab.term_op_l = di + o_l;
-    c->tab.term_op_r = di + o_r;
-    c->tab.op_site_off = di + o_so;
-    c->tab.op_sites = di + o_s;
-    c->tab.op_row_off = di + o_ro;
-    c->tab.row_ent_off = di + o_eo;
-    c->tab.ent_mel = reinterpret_cast<const double*>(base + bytes_i);
-    c->tab.ent_mel_f = reinterpret_cast<const float*>(base + bytes_i + bytes_d);
-    c->tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
-    c->tab.base = base;
-    c->tab.bytes = (int)host.size();
-    c->N_table = N;
+    tab.n_sites = N;
+    tab.n_terms = t->n_terms;
+    tab.term_kind = di + o_kind;
+    tab.term_op_l = di + o_l;
+    tab.term_op_r = di + o_r;
+    tab.op_site_off = di + o_so;
+    tab.op_sites = di + o_s;
+    tab.op_row_off = di + o_ro;
+    tab.row_ent_off = di + o_eo;
+    tab.ent_mel = reinterpret_cast<const double*>(base + bytes_i);
+    tab.ent_mel_f = reinterpret_cast<const float*>(base + bytes_i + bytes_d);
+    tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
+    tab.base = base;
+    tab.bytes = (int)host.size();
+    *maxc_out = maxc;
+    *nket_out = n_ket;
+    return NQH_OK;
+}
+
+nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
+    if (!c || !t) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    int maxc = 0, n_ket = 0;
+    nqh_status s = upload_table(c, t, false, c->table_mem, c->tab, &maxc, &n_ket);
+    if (s) return s;
+    c->N_table = t->n_sites;
     c->max_conn = maxc;
     c->problem_ket = n_ket > 0;
     c->has_problem = true;
@@ -1418,6 +1438,81 @@ nqh_status nqh_exact(nqh_ctx* c, double* out, double* logp_out) {
     if (logp_out)
         for (int64_t i = 0; i < total; ++i) logp_out[i] -= out[3];
     if (!std::isfinite(out[0])) return fail(c, NQH_E_NAN, "non-finite exact sum");
+    return NQH_OK;
+}
+
+// compute_observables (BatchedObsDMSampler.jl:59-105) for one ket operator
+nqh_status nqh_observable(nqh_ctx* c, const nqh_optable* op, int64_t chains, int32_t chain_length, int32_t passes,
+                          int32_t burn, double* stats_out) {
+    if (!c || !op || !stats_out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c))) return s;
+    if (is_ket(c)) return fail(c, NQH_E_ARG, "observables need a density-matrix network (NDM or RBMSplit)");
+    if (chains < 2 || chain_length < 2 || passes < 1 || burn < 0)
+        return fail(c, NQH_E_ARG, "need chains >= 2, chain_length >= 2, passes >= 1, burn >= 0");
+    if (op->n_sites != net_N(c)) return fail(c, NQH_E_ARG, "operator and network have different numbers of sites");
+    for (int q = 0; q < op->n_terms; ++q)
+        if (op->term_kind[q] != 3) return fail(c, NQH_E_ARG, "the observable must be a ket operator (kind 3 terms)");
+    int maxc = 0, nk = 0;
+    if ((s = upload_table(c, op, true, c->obs_table_mem, c->obs_tab, &maxc, &nk))) return s;
+    const int64_t Ns = chains * chain_length;
+    const size_t e = net_elt(c);
+    if ((s = ensure(c, c->obs_keys, sizeof(uint64_t) * Ns)) || (s = ensure(c, c->obs_state, sizeof(uint64_t) * chains)) ||
+        (s = ensure(c, c->obs_lp, 2 * e * Ns)) || (s = ensure(c, c->obs_L, 2 * e * Ns)) ||
+        (s = ensure(c, c->obs_stats, sizeof(double) * 8)))
+        return s;
+    // the diagonal chain, restarted at every call (_sample_state!), its own random stream
+    samp_args a{};
+    a.d = c->d;
+    a.p = fp64_net(c) ? c->p64.p : c->p32.p;
+    a.chains = chains;
+    a.passes = passes;
+    a.burn = burn;
+    a.stored = chain_length;
+    a.restart = 1;
+    a.diag = 1;
+    a.tag = 1;
+    a.lpc = is_rbm(c) ? 1 : nq_sampler_lanes_per_chain(c->d);
+    a.seed = c->seed;
+    a.iter = c->iter;
+    a.chain_state = P<uint64_t>(c->obs_state);
+    a.samples = P<uint64_t>(c->obs_keys);
+    HIPCHK(c, launch_metropolis(c, a));
+    // O_loc(sigma) = sum <sigma|O|sigma'> rho(sigma', sigma) / rho(sigma, sigma): the
+    // operator's changes act on the row index of the doubled configuration
+    if (is_rbm(c)) {
+        rbm_args r = make_rbm(c, P<uint64_t>(c->obs_keys), Ns);
+        r.tab = c->obs_tab;
+        r.max_conn = std::max(maxc, 1);
+        r.tab_lds = c->obs_tab.bytes <= kTableLdsMax ? 1 : 0;
+        r.do_conn = 1;
+        r.lp = c->obs_lp.p;
+        r.Lloc = c->obs_L.p;
+        HIPCHK(c, nq_launch_rbm_estimate(r, fp64_net(c), est_grid_for(Ns), c->stream));
+    } else {
+        est_args r = make_est(c, P<uint64_t>(c->obs_keys), Ns);
+        r.tab = c->obs_tab;
+        r.max_conn = std::max(maxc, 1);
+        r.tab_lds = c->obs_tab.bytes <= kTableLdsMax ? 1 : 0;
+        r.do_conn = 1;
+        r.lp = c->obs_lp.p;
+        r.Lloc = c->obs_L.p;
+        HIPCHK(c, nq_launch_estimate(r, fp64_net(c), est_grid_for(Ns), c->stream));
+    }
+    HIPCHK(c, nq_launch_stats_cplx(c->obs_L.p, fp64_net(c), chains, chain_length, P<double>(c->obs_stats), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(stats_out, c->obs_stats.p, sizeof(double) * 6, hipMemcpyDeviceToHost));
+    return NQH_OK;
+}
+
+// the last observable's configurations (doubled keys, chain_length x chains) and local values
+nqh_status nqh_get_observable_samples(nqh_ctx* c, uint64_t* keys_out, double* local_out, int64_t n) {
+    if (!c || n < 0) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if ((size_t)n * sizeof(uint64_t) > c->obs_keys.bytes) return fail(c, NQH_E_ARG, "n exceeds the last observable's samples");
+    if (keys_out) HIPCHK(c, hipMemcpy(keys_out, c->obs_keys.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    if (local_out) return download_cplx(c, c->obs_L, local_out, 2 * n);
     return NQH_OK;
 }
 

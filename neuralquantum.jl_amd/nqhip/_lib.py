@@ -63,6 +63,9 @@ SIGNATURES = {
     "nqh_set_rbm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "nqh_get_stats": (C.c_int32, [C.c_void_p, _dp, C.c_int32]),
     "nqh_exact": (C.c_int32, [C.c_void_p, _dp, _dp]),
+    "nqh_observable": (C.c_int32, [C.c_void_p, C.POINTER(OpTable), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                   _dp]),
+    "nqh_get_observable_samples": (C.c_int32, [C.c_void_p, _u64p, _dp, C.c_int64]),
     "nqh_set_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_get_params": (C.c_int32, [C.c_void_p, _dp, C.c_int64]),
     "nqh_set_precision": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -213,6 +216,22 @@ class Context:
         self._chk(self.lib.nqh_exact(self.h, _p(out, _dp), _p(lp, _dp)))
         res = (float(out[0]), complex(out[1], out[2]), float(out[3]))
         return res + (lp,) if want_logp else res
+
+    def observable(self, flat_op, chains, chain_length, passes, burn):
+        """compute_observables for one ket operator (a FlatOpTable of kind-3
+        terms): [Re mean, error, variance, tau, R, Im mean]."""
+        t = Table(flat_op)
+        out = np.empty(6)
+        self._chk(self.lib.nqh_observable(self.h, C.byref(t.s), chains, chain_length, passes, burn, _p(out, _dp)))
+        self._obs_n = int(chains) * int(chain_length)
+        return out
+
+    def observable_samples(self):
+        n = self._obs_n
+        keys = np.empty(n, dtype=np.uint64)
+        L = np.empty(2 * n)
+        self._chk(self.lib.nqh_get_observable_samples(self.h, _p(keys, _u64p), _p(L, _dp), n))
+        return keys, L.view(np.complex128)
 
     def get_stats(self, n=6):
         out = np.empty(n)
