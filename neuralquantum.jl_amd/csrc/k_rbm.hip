@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
                     delta += nfr[r] - fr[r];
                 }
             }
-            delta = group_sum(delta, lpc);
+            delta = group_total(delta, lpc);
             delta += c * sp[(col ? oac : oar) + j].re;
             if (diag) delta += c * sp[oac + j].re;
             else delta *= T(2);

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                     }
                 }
             }
-            delta = group_sum(delta, lpc);
+            delta = group_total(delta, lpc);
             delta += c * sp[d.bla + j];
 
             bool accept;

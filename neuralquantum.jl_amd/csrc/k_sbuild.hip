@@ -70,12 +70,11 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
 #pragma unroll
         for (int kk = tid >> 6; kk < KC; kk += 4) {
             const int64_t s = k0 + kk;
-            double xa = 0.0, xb = 0.0;
-            if (s < a.Ns) {
-                const YT* row = Y + s * (int64_t)a.ldy + a.c0;
-                if (va) xa = (double)row[ca] - mua;
-                if (vb) xb = (double)row[cb] - mub;
-            }
+            // unconditional loads from clamped addresses, selected afterwards
+            const YT* row = Y + (s < a.Ns ? s : a.Ns - 1) * (int64_t)a.ldy + a.c0;
+            const double ya = (double)row[va ? ca : 0], yb = (double)row[vb ? cb : 0];
+            const double xa = (s < a.Ns && va) ? ya - mua : 0.0;
+            const double xb = (s < a.Ns && vb) ? yb - mub : 0.0;
             if constexpr (MODE == NQ_GRAM_BF16) {
                 As[lc * PITCH + kk] = (__bf16)(float)xa;
                 Bs[lc * PITCH + kk] = (__bf16)(float)xb;
@@ -162,7 +161,8 @@ __global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int l
     for (int r = 0; r < 16; ++r) {
         const int64_t si = sb + ty + 4 * r;
         float v = 0.f;
-        if (c < n && si < Ns) v = (float)((double)Y[si * ldy + c0 + c] - m);
+        const double y = (double)Y[min(si, Ns - 1) * ldy + c0 + min(c, n - 1)];   // unconditional load
+        if (c < n && si < Ns) v = (float)(y - m);
         tile[ty + 4 * r][tx] = v;
     }
     __syncthreads();

@@ -130,10 +130,15 @@ __global__ __launch_bounds__(64, 1) void k_chol_panel32(double* A, int n, int ld
     __shared__ double L[CB][CB + 1];
     __shared__ double rdiag[CB];
     const int t = threadIdx.x, tr = t & (CB - 1);
+    // loads are unconditional (clamped addresses) and selected afterwards: a
+    // load under a lane-dependent condition would be waited for one by one
     double r[CB];
+    const int64_t rrow = (int64_t)min(k + tr, n - 1);
 #pragma unroll
-    for (int c = 0; c < CB; ++c)
-        r[c] = (tr < kb && c < kb && tr >= c) ? A[(int64_t)(k + tr) + (int64_t)(k + c) * lda] : (tr == c ? 1.0 : 0.0);
+    for (int c = 0; c < CB; ++c) {
+        const double a = A[rrow + (int64_t)min(k + c, n - 1) * lda];
+        r[c] = (tr < kb && c < kb && tr >= c) ? a : (tr == c ? 1.0 : 0.0);
+    }
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
         const double v = rl_d(r[j], j);                      // current A[j][j]
@@ -142,11 +147,11 @@ __global__ __launch_bounds__(64, 1) void k_chol_panel32(double* A, int n, int ld
         const double rp = 1.0 / piv;
         const double lj = (tr == j) ? piv : r[j] * rp;      // L[tr][j] (zero above the diagonal)
         r[j] = lj;
+        // r[m] -= L[tr][j] L[m][j] for every m > j; in rows tr < m this writes
+        // the upper triangle, which no lane reads (lane m only ever hands out
+        // its own entries r[j] with j <= m), so no mask is needed
 #pragma unroll
-        for (int m = j + 1; m < CB; ++m) {
-            const double lmj = rl_d(lj, m);                  // L[m][j]
-            if (tr >= m) r[m] -= lj * lmj;
-        }
+        for (int m = j + 1; m < CB; ++m) r[m] -= lj * rl_d(lj, m);
     }
     if (t < CB) {
 #pragma unroll
@@ -167,7 +172,10 @@ __global__ __launch_bounds__(64, 1) void k_chol_panel32(double* A, int n, int ld
     if (i >= n) return;
     double x[CB];
 #pragma unroll
-    for (int j = 0; j < CB; ++j) x[j] = (j < kb) ? A[(int64_t)i + (int64_t)(k + j) * lda] : 0.0;
+    for (int j = 0; j < CB; ++j) {
+        const double a = A[(int64_t)i + (int64_t)min(k + j, n - 1) * lda];
+        x[j] = (j < kb) ? a : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
         if (j < kb) {
@@ -202,13 +210,11 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
     for (int e = threadIdx.x; e < 64 * KW; e += blockDim.x) {
         const int r = e & 63, m = e >> 6;
-        double vi = 0.0, vj = 0.0;
-        if (m < kb) {
-            if (i0 + r < n) vi = A[(int64_t)(i0 + r) + (int64_t)(k + m) * lda];
-            if (j0 + r < n) vj = A[(int64_t)(j0 + r) + (int64_t)(k + m) * lda];
-        }
-        Xi[m][r] = vi;
-        Xj[m][r] = vj;
+        const int64_t col = (int64_t)(k + min(m, kb - 1)) * lda;
+        const double vi = A[(int64_t)min(i0 + r, n - 1) + col];
+        const double vj = A[(int64_t)min(j0 + r, n - 1) + col];
+        Xi[m][r] = (m < kb && i0 + r < n) ? vi : 0.0;
+        Xj[m][r] = (m < kb && j0 + r < n) ? vj : 0.0;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -252,9 +258,12 @@ __global__ __launch_bounds__(64) void k_fwd_diag(const double* L, int lda, int k
     const int lane = threadIdx.x;
     double row[NB];   // row[j] = L[k + lane][k + j]
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-        row[j] = (lane < kb && j < kb) ? L[(int64_t)(k + lane) + (int64_t)(k + j) * lda] : 1.0;
-    double v = (lane < kb) ? y[k + lane] : 0.0;
+    for (int j = 0; j < NB; ++j) {
+        const double a = L[(int64_t)(k + min(lane, kb - 1)) + (int64_t)(k + min(j, kb - 1)) * lda];
+        row[j] = (lane < kb && j < kb) ? a : 1.0;
+    }
+    const double y0 = y[k + min(lane, kb - 1)];
+    double v = (lane < kb) ? y0 : 0.0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (j < kb) {
@@ -290,9 +299,12 @@ __global__ __launch_bounds__(64) void k_back_diag(const double* L, int lda, int 
     const int lane = threadIdx.x;
     double col[NB];   // col[j] = L[k + j][k + lane]
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-        col[j] = (lane < kb && j < kb) ? L[(int64_t)(k + j) + (int64_t)(k + lane) * lda] : 1.0;
-    double v = (lane < kb) ? y[k + lane] : 0.0;
+    for (int j = 0; j < NB; ++j) {
+        const double a = L[(int64_t)(k + min(j, kb - 1)) + (int64_t)(k + min(lane, kb - 1)) * lda];
+        col[j] = (lane < kb && j < kb) ? a : 1.0;
+    }
+    const double y0 = y[k + min(lane, kb - 1)];
+    double v = (lane < kb) ? y0 : 0.0;
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
         const int j = NB - 1 - jj;

@@ -145,6 +145,38 @@ __device__ __forceinline__ float wave_total(float v) {
     v += dpp_f<0x140>(v);
     return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
 }
+// Sum over aligned groups of `width` lanes (power of two <= 64), the same value
+// in every lane of the group: DPP within rows of 16, then xor shuffles for
+// the 32 / 64 steps.  All lanes must be active.
+template <typename T>
+__device__ __forceinline__ T dpp_any(T v, int step) {
+    if constexpr (sizeof(T) == 8) {
+        switch (step) {
+            case 0: return dpp_d<0xB1>(v);
+            case 1: return dpp_d<0x4E>(v);
+            case 2: return dpp_d<0x141>(v);
+            default: return dpp_d<0x140>(v);
+        }
+    } else {
+        switch (step) {
+            case 0: return dpp_f<0xB1>(v);
+            case 1: return dpp_f<0x4E>(v);
+            case 2: return dpp_f<0x141>(v);
+            default: return dpp_f<0x140>(v);
+        }
+    }
+}
+template <typename T>
+__device__ __forceinline__ T group_total(T v, int width) {
+    if (width >= 2) v += dpp_any(v, 0);
+    if (width >= 4) v += dpp_any(v, 1);
+    if (width >= 8) v += dpp_any(v, 2);
+    if (width >= 16) v += dpp_any(v, 3);
+    if (width >= 32) v += __shfl_xor(v, 16, 64);
+    if (width >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 __device__ __forceinline__ double wave_total(double v) {
     v += dpp_d<0xB1>(v);
     v += dpp_d<0x4E>(v);
