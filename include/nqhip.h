@@ -197,6 +197,15 @@ nqh_status nqh_update(nqh_ctx* ctx, double lr);
 nqh_status nqh_sr_iteration(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr,
                             double* stats);
 
+/* the same iteration enqueued on the context's stream without waiting
+ * (returns once the work is queued), and the wait for it (statistics, solve
+ * and finiteness checks as nqh_sr_iteration).  Many contexts on one device,
+ * each with its own stream, then run their iterations concurrently: a batch
+ * of independent problems (the survey's batched-instances case).  No
+ * reference counterpart (execution strategy). */
+nqh_status nqh_sr_iteration_async(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr);
+nqh_status nqh_wait(nqh_ctx* ctx, double* stats);
+
 /* device time per phase, measured with HIP events on the context's stream
  * while profiling is on (accumulated, in ms; counts = timed occurrences):
  * [0] sampler  [1] estimator (log rho, O, L_loc, force)  [2] means, statistics,

@@ -3,10 +3,11 @@
 //   stats ........ stat_analysis of |L_loc|^2 over chains x slices (stats.jl:52-77)
 //   force ........ F = Re(conj(L_loc) dL_loc)/Ns - C Re(Obar)   (BatchedGradSampler.jl:108-118)
 //   shift ........ S_ii += eps (sr_shift, SRDirect.jl:60-65) or eps * S_ii
-//   Cholesky ..... blocked right-looking, FP64, panels of 32 columns: diagonal
-//                  block factored in registers of one wavefront (v_readlane
-//                  broadcasts), panel triangular solve, trailing update on
-//                  FP64 MFMA (cholesky!(Hermitian(S)) + ldiv!, SRDirect.jl:78-81)
+//   Cholesky ..... right-looking, FP64, panels of 32 columns, one launch per
+//                  panel: panel transform and trailing update on FP64 MFMA,
+//                  look-ahead factorization of the next diagonal block in one
+//                  wavefront's registers (cholesky!(Hermitian(S)) + ldiv!,
+//                  SRDirect.jl:78-81)
 //   update ....... p -= lr * Delta (Descent, src/Optimisers/rules.jl:15-17)
 #include "nq_kernels.h"
 
@@ -117,77 +118,101 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
     s += (kind == 0) ? eps : eps * s;
 }
 
-// ---- Cholesky step k with a panel of CB = 32 columns, one wavefront per block.
-// Every block factors the kb x kb diagonal block (padded with the identity)
-// with lane t < 32 holding row t in registers; column j is broadcast with
-// v_readlane (no LDS and no barrier on the dependent chain).  Block 0 writes
-// the factor back; block b >= 1 solves rows k + kb + 64 (b - 1) + lane of the
-// panel, X = A[i, k:k+kb] L_kk^-T, by column-oriented substitution against
-// the factor in LDS.
+// ---- Cholesky in panels of CB = 32 columns.
+//
+// Storage during the factorization (S column-major, leading dimension lda,
+// nr = n or n + 1 rows, row n carrying the right-hand side):
+//   * diagonal blocks L_kk in the lower triangle of S;
+//   * off-diagonal panels L_ik (i > k) written TRANSPOSED into the upper
+//     triangle, A[k + c, i] = L[i, k + c]: the lower triangle of a panel is
+//     still being read by the other tiles of the same step, the upper
+//     triangle is read by nobody until the substitutions;
+//   * winv[b] = L_bb^-T (32 x 32, column-major) of every diagonal block.
+// Step k (one launch, k_chol_step): every trailing tile (I >= J, 64 x 64)
+// stages its rows of the panel A_ik, transforms them to L_ik = A_ik L_kk^-T
+// on the FP64 matrix cores, and applies A_IJ -= L_Ik L_Jk^T; the diagonal
+// tiles write L_Ik out (upper triangle); tile (0, 0), once updated, factors
+// the next diagonal block in one wavefront (look-ahead), so the serial
+// column chain of block k + 1 runs while the other tiles of step k update.
+// The right-hand side row takes part as an ordinary trailing row, which is
+// the forward substitution L y = b; y ends in column n of the upper triangle.
 constexpr int CB = 32;
 
-__global__ __launch_bounds__(64, 1) void k_chol_panel32(double* A, int n, int lda, int k, int kb, int* info) {
-    __shared__ double L[CB][CB + 1];
-    __shared__ double rdiag[CB];
-    const int t = threadIdx.x, tr = t & (CB - 1);
-    // loads are unconditional (clamped addresses) and selected afterwards: a
-    // load under a lane-dependent condition would be waited for one by one
+// sqrt(v) and 1/sqrt(v) from the hardware estimate refined by two Goldschmidt
+// steps and one Newton correction of the root (relative error ~1 ulp), a
+// much shorter dependent chain than the IEEE sqrt + divide expansions; v <= 0
+// yields NaN or inf, which the caller reports
+__device__ __forceinline__ void sqrt_rsqrt(double v, double* root, double* rinv) {
+    const double y = __builtin_amdgcn_rsq(v);
+    double g = v * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, v);
+    *root = fma(d, h, g);
+    *rinv = 2.0 * h;
+}
+
+// factor the kb x kb block at (k, k) (identity padding to 32) in one
+// wavefront: lane t < 32 holds row t of the block, lane 32 + t holds row t of
+// the identity appended below it, so the same column sweep that produces
+// L_kk turns the identity rows into I L_kk^-T = L_kk^-T (the bordering rule
+// L_21 = A_21 L_11^-T with A_21 = I).  Column j: the pivot is read from lane
+// j, the column L[., j] goes through LDS (Lc) to every lane for the
+// right-looking update; the entry the next pivot needs, L[j + 1, j], is read
+// directly from its lane.  Writes L_kk (lower triangle) and L_kk^-T to `w`.
+__device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb, int nmax, double* w, int* info,
+                                              bool report, double (*Lc)[CB]) {
+    const int t = threadIdx.x & 63, tr = t & (CB - 1);
+    const bool mrow = t < CB;
     double r[CB];
-    const int64_t rrow = (int64_t)min(k + tr, n - 1);
+    const int64_t rrow = (int64_t)min(k + tr, nmax - 1);
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
-        const double a = A[rrow + (int64_t)min(k + c, n - 1) * lda];
-        r[c] = (tr < kb && c < kb && tr >= c) ? a : (tr == c ? 1.0 : 0.0);
+        const double a = A[rrow + (int64_t)min(k + c, nmax - 1) * lda];   // unconditional load, then select
+        r[c] = (mrow && tr < kb && c < kb && tr >= c) ? a : (tr == c ? 1.0 : 0.0);
     }
+    int bad = 0;   // 1 + first column with a non-positive pivot (branch-free inside the sweep)
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
         const double v = rl_d(r[j], j);                      // current A[j][j]
-        if (j < kb && t == 0 && blockIdx.x == 0 && !(v > 0.0) && *info == 0) *info = k + j + 1;
-        const double piv = sqrt(v);
-        const double rp = 1.0 / piv;
-        const double lj = (tr == j) ? piv : r[j] * rp;      // L[tr][j] (zero above the diagonal)
+        bad = (bad == 0 && j < kb && !(v > 0.0)) ? j + 1 : bad;
+        double piv, rp;
+        sqrt_rsqrt(v, &piv, &rp);
+        const double lj = (mrow && tr == j) ? piv : r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
         r[j] = lj;
-        // r[m] -= L[tr][j] L[m][j] for every m > j; in rows tr < m this writes
-        // the upper triangle, which no lane reads (lane m only ever hands out
-        // its own entries r[j] with j <= m), so no mask is needed
+        if (j + 1 < CB) {
+            if (mrow) Lc[j][t] = lj;
+            __builtin_amdgcn_wave_barrier();
+            r[j + 1] -= lj * rl_d(lj, j + 1);
 #pragma unroll
-        for (int m = j + 1; m < CB; ++m) r[m] -= lj * rl_d(lj, m);
-    }
-    if (t < CB) {
+            for (int m = j + 2; m < CB; ++m) r[m] -= lj * Lc[j][m];
+            // keep the update right-looking: without this the compiler sinks every
+            // update to the column that consumes it, holding all of L live in registers
 #pragma unroll
-        for (int c = 0; c < CB; ++c) L[t][c] = r[c];
+            for (int m = j + 1; m < CB; ++m) asm volatile("" : "+v"(r[m]));
+        }
     }
-    __syncthreads();
-    if (t < CB) rdiag[t] = 1.0 / L[t][t];
-    __syncthreads();
-    if (blockIdx.x == 0) {
+    if (report && bad != 0 && t == 0 && *info == 0) *info = k + bad;
+    if (mrow) {
         if (t < kb) {
 #pragma unroll
             for (int c = 0; c < CB; ++c)
-                if (c < kb && t >= c) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = r[c];
+                if (c < kb && c <= t) A[(int64_t)(k + t) + (int64_t)(k + c) * lda] = r[c];
         }
-        return;
+    } else {
+        // (L^-T)[tr][c], column-major with ld 32
+#pragma unroll
+        for (int c = 0; c < CB; ++c) w[tr + CB * c] = r[c];
     }
-    const int i = k + kb + (blockIdx.x - 1) * 64 + t;
-    if (i >= n) return;
-    double x[CB];
-#pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        const double a = A[(int64_t)i + (int64_t)min(k + j, n - 1) * lda];
-        x[j] = (j < kb) ? a : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        if (j < kb) {
-            x[j] *= rdiag[j];
-#pragma unroll
-            for (int m = j + 1; m < CB; ++m)
-                if (m < kb) x[m] -= x[j] * L[m][j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < CB; ++j)
-        if (j < kb) A[(int64_t)i + (int64_t)(k + j) * lda] = x[j];
+}
+
+__global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info) {
+    __shared__ double Ls[CB][CB];
+    factor_diag32(A, lda, k, kb, n, winv + (size_t)(k / CB) * CB * CB, info, true, Ls);
 }
 
 __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
@@ -198,28 +223,91 @@ __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
     *I = b - j * (j + 1) / 2;
 }
 
-// ---- trailing update: A[i, j] -= sum_m X[i, m] X[j, m] for tiles I >= J of the
-// trailing matrix (origin o = k + kb), X = A[:, k:k+kb]; FP64 MFMA 16x16x4
-template <int KW>
-__global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda, int k, int kb) {
-    __shared__ double Xi[KW][65];   // [m][row]
-    __shared__ double Xj[KW][65];
+// step k: panel transform + trailing update of the lower tiles (I >= J) of
+// the rows/columns >= o = k + kb; n = columns of the matrix, nr = rows
+__global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int lda, int k, int kb, double* winv,
+                                                   int* info) {
+    __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
+    __shared__ double Wt[CB][CB + 1];           // L_kk^-T
+    __shared__ double Ls[CB][CB];
     int tI, tJ;
-    tri_decode(blockIdx.x, &tJ, &tI);   // tJ <= tI
+    tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
     const int o = k + kb;
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
-    for (int e = threadIdx.x; e < 64 * KW; e += blockDim.x) {
+    const double* w = winv + (size_t)(k / CB) * CB * CB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+    // every load of the kernel is issued up front, unconditionally (clamped
+    // addresses, out-of-range values replaced afterwards), so the round trips
+    // overlap instead of serializing one wait per guarded load
+    double old[2][2][4];   // this lane's 16 entries of the tile A_IJ
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = min(i0 + wr * 32 + a * 16 + lk + 4 * v, nr - 1);
+                const int c = min(j0 + wc * 32 + b * 16 + li, nr - 1);
+                old[a][b][v] = A[(int64_t)r + (int64_t)c * lda];
+            }
+    double si[64 * CB / 256], sj[64 * CB / 256], sw[CB * CB / 256];
+#pragma unroll
+    for (int q = 0; q < 64 * CB / 256; ++q) {
+        const int e = threadIdx.x + 256 * q;
         const int r = e & 63, m = e >> 6;
         const int64_t col = (int64_t)(k + min(m, kb - 1)) * lda;
-        const double vi = A[(int64_t)min(i0 + r, n - 1) + col];
-        const double vj = A[(int64_t)min(j0 + r, n - 1) + col];
-        Xi[m][r] = (m < kb && i0 + r < n) ? vi : 0.0;
-        Xj[m][r] = (m < kb && j0 + r < n) ? vj : 0.0;
+        si[q] = A[(int64_t)min(i0 + r, nr - 1) + col];
+        sj[q] = A[(int64_t)min(j0 + r, nr - 1) + col];
+    }
+#pragma unroll
+    for (int q = 0; q < CB * CB / 256; ++q) sw[q] = w[threadIdx.x + 256 * q];
+#pragma unroll
+    for (int q = 0; q < 64 * CB / 256; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        const int r = e & 63, m = e >> 6;
+        Xi[m][r] = (m < kb && i0 + r < nr) ? si[q] : 0.0;
+        Xj[m][r] = (m < kb && j0 + r < nr) ? sj[q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < CB * CB / 256; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        Wt[e & (CB - 1)][e >> 5] = sw[q];
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int li = lane & 15, lk = lane >> 4;
+    // L_Ik = A_Ik W and L_Jk = A_Jk W (W = L_kk^-T): 2 x (4 x 2) output tiles
+    // of 16 x 16, two per wave and side; f64 MFMA output D[lane/16 + 4v][lane%16];
+    // the results replace the staged panels once every wave has read them
+    d4 ai[2], aj[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tile = wave * 2 + q;           // 0..7
+        const int rt = tile >> 1, ct = tile & 1; // row tile (of 4), column tile (of 2)
+        ai[q] = d4{0.0, 0.0, 0.0, 0.0};
+        aj[q] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < CB; m += 4) {
+            const double b = Wt[m + lk][ct * 16 + li];
+            ai[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xi[m + lk][rt * 16 + li], b, ai[q], 0, 0, 0);
+            aj[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xj[m + lk][rt * 16 + li], b, aj[q], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tile = wave * 2 + q;
+        const int rt = tile >> 1, ct = tile & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Xi[ct * 16 + li][rt * 16 + lk + 4 * v] = ai[q][v];
+            Xj[ct * 16 + li][rt * 16 + lk + 4 * v] = aj[q][v];
+        }
+    }
+    __syncthreads();
+    double (*Pi)[65] = Xi;
+    double (*Pj)[65] = Xj;
+    // A_IJ -= L_Ik L_Jk^T (lower part)
     d4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -227,63 +315,91 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(double* A, int n, int lda
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
-    for (int m = 0; m < KW; m += 4) {
+#pragma unroll
+    for (int m = 0; m < CB; m += 4) {
         double av[2], bv[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) av[a] = Xi[m + lk][wr * 32 + a * 16 + li];
+        for (int a = 0; a < 2; ++a) av[a] = Pi[m + lk][wr * 32 + a * 16 + li];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) bv[b] = Xj[m + lk][wc * 32 + b * 16 + li];
+        for (int b = 0; b < 2; ++b) bv[b] = Pj[m + lk][wc * 32 + b * 16 + li];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    // f64 MFMA output: D[lane / 16 + 4 v][lane % 16]
-    const int lj = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = i0 + wr * 32 + a * 16 + lg + 4 * v;
-                const int c = j0 + wc * 32 + b * 16 + lj;
-                if (r < n && c < n && r >= c) A[(int64_t)r + (int64_t)c * lda] -= acc[a][b][v];
+                const int r = i0 + wr * 32 + a * 16 + lk + 4 * v;
+                const int c = j0 + wc * 32 + b * 16 + li;
+                if (r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = old[a][b][v] - acc[a][b][v];
             }
-}
-
-// ---- forward substitution L y = b, one block of 64 per step
-__global__ __launch_bounds__(64) void k_fwd_diag(const double* L, int lda, int k, int kb, double* y) {
-    const int lane = threadIdx.x;
-    double row[NB];   // row[j] = L[k + lane][k + j]
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const double a = L[(int64_t)(k + min(lane, kb - 1)) + (int64_t)(k + min(j, kb - 1)) * lda];
-        row[j] = (lane < kb && j < kb) ? a : 1.0;
-    }
-    const double y0 = y[k + min(lane, kb - 1)];
-    double v = (lane < kb) ? y0 : 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        if (j < kb) {
-            if (lane == j) v = v / row[j];
-            const double yj = __shfl(v, j, 64);
-            if (lane > j && lane < kb) v -= row[j] * yj;
+    if (tI == tJ) {
+        // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c]
+        for (int e = threadIdx.x; e < 64 * CB; e += blockDim.x) {
+            const int c = e & (CB - 1), r = e >> 5;
+            if (c < kb && i0 + r < nr) A[(int64_t)(k + c) + (int64_t)(i0 + r) * lda] = Pi[c][r];
         }
     }
-    if (lane < kb) y[k + lane] = v;
+    if (blockIdx.x == 0 && o < n) {
+        // look-ahead: tile (0, 0) now holds the final values of the next diagonal block
+        __syncthreads();
+        if (wave == 0)
+            factor_diag32(A, lda, o, min(CB, n - o), n, winv + (size_t)(o / CB) * CB * CB, info, true, Ls);
+    }
 }
 
-// y[i] -= sum_{m < kb} L[i][k + m] y[k + m] for k + kb <= i < n (coalesced over i)
-__global__ __launch_bounds__(256) void k_fwd_update(const double* L, int n, int lda, int k, int kb, double* y) {
-    __shared__ double ys[NB];
-    if (threadIdx.x < kb) ys[threadIdx.x] = y[k + threadIdx.x];
+// ---- substitutions with the stored factor
+// back step: x_k = L_kk^-T y_k (from winv), then y_i -= sum_m L[k + m, i] x_{k+m}
+// for i < k, the factor read from the upper triangle A[i, k + m] (coalesced in i)
+__global__ __launch_bounds__(256) void k_back_step(const double* A, int lda, int k, int kb, const double* winv,
+                                                   double* y, double* x) {
+    __shared__ double ys[CB], xs[CB];
+    const double* w = winv + (size_t)(k / CB) * CB * CB;
+    if (threadIdx.x < CB) ys[threadIdx.x] = (threadIdx.x < kb) ? y[k + min((int)threadIdx.x, kb - 1)] : 0.0;
+    __syncthreads();
+    if (threadIdx.x < CB) {
+        const int c = threadIdx.x;
+        double s = 0.0;
+#pragma unroll 8
+        for (int m = 0; m < CB; ++m) s += w[c + CB * m] * ys[m];
+        xs[c] = (c < kb) ? s : 0.0;
+        if (blockIdx.x == 0 && c < kb) x[k + c] = s;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    double acc = 0.0;
+    for (int m = 0; m < kb; ++m) acc += A[(int64_t)i + (int64_t)(k + m) * lda] * xs[m];
+    y[i] -= acc;
+}
+
+// forward step (refinement solves): z_k = L_kk^-1 y_k, then y_i -= sum_m L[i, k + m] z_m
+// for i >= k + kb, the factor read from the upper triangle A[k + m, i]
+__global__ __launch_bounds__(256) void k_fwd_step(const double* A, int n, int lda, int k, int kb, const double* winv,
+                                                  double* y, double* z) {
+    __shared__ double ys[CB], zs[CB];
+    const double* w = winv + (size_t)(k / CB) * CB * CB;
+    if (threadIdx.x < CB) ys[threadIdx.x] = (threadIdx.x < kb) ? y[k + min((int)threadIdx.x, kb - 1)] : 0.0;
+    __syncthreads();
+    if (threadIdx.x < CB) {
+        const int c = threadIdx.x;
+        double s = 0.0;
+#pragma unroll 8
+        for (int m = 0; m < CB; ++m) s += w[m + CB * c] * ys[m];   // (L^-1)[c][m] = (L^-T)[m][c]
+        zs[c] = (c < kb) ? s : 0.0;
+        if (blockIdx.x == 0 && c < kb) z[k + c] = s;
+    }
     __syncthreads();
     const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double* col = A + (int64_t)i * lda + k;
     double acc = 0.0;
-    for (int m = 0; m < kb; ++m) acc += L[(int64_t)i + (int64_t)(k + m) * lda] * ys[m];
+    for (int m = 0; m < kb; ++m) acc += col[m] * zs[m];
     y[i] -= acc;
 }
 
@@ -293,49 +409,10 @@ __global__ void k_set_row(double* A, int n, int lda, const double* b) {
     if (j < n) A[(int64_t)n + (int64_t)j * lda] = b[j];
 }
 
-// ---- back substitution L^T x = y, one block of 64 per step (last block first)
-// diag step: x[k:k+kb] = L_kk^-T y[k:k+kb] (one wave, the block's columns in registers)
-__global__ __launch_bounds__(64) void k_back_diag(const double* L, int lda, int k, int kb, double* y) {
-    const int lane = threadIdx.x;
-    double col[NB];   // col[j] = L[k + j][k + lane]
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const double a = L[(int64_t)(k + min(j, kb - 1)) + (int64_t)(k + min(lane, kb - 1)) * lda];
-        col[j] = (lane < kb && j < kb) ? a : 1.0;
-    }
-    const double y0 = y[k + min(lane, kb - 1)];
-    double v = (lane < kb) ? y0 : 0.0;
-#pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
-        const int j = NB - 1 - jj;
-        if (j < kb) {
-            if (lane == j) v = v / col[j];
-            const double xj = __shfl(v, j, 64);
-            if (lane < j) v -= col[j] * xj;
-        }
-    }
-    if (lane < kb) y[k + lane] = v;
-}
-
-// update step: y[i] -= sum_{m < kb} L[k + m][i] x[k + m] for i < k; one row i
-// per thread, its 64 factor entries are contiguous (column i, rows k..k+kb-1)
-__global__ __launch_bounds__(256) void k_back_update(const double* L, int lda, int k, int kb, double* y) {
-    __shared__ double xs[NB];
-    if (threadIdx.x < kb) xs[threadIdx.x] = y[k + threadIdx.x];
-    __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= k) return;
-    const double* c = L + (int64_t)i * lda + k;
-    double acc = 0.0;
-#pragma unroll 8
-    for (int m = 0; m < kb; ++m) acc += c[m] * xs[m];
-    y[i] -= acc;
-}
-
-// the forward half of the solve is the factorization's extra row n: copy it out
-__global__ void k_copy_row(const double* A, int n, int lda, double* y) {
+// y[j] = A[j, n]: the forward-substituted right-hand side (upper column n)
+__global__ void k_copy_col(const double* A, int n, int lda, double* y) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) y[j] = A[(int64_t)n + (int64_t)j * lda];
+    if (j < n) y[j] = A[(int64_t)j + (int64_t)n * lda];
 }
 
 __global__ void k_residual(const double* S, int n, int lds, double eps, int kind, const double* x,
@@ -445,45 +522,47 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
     return hipGetLastError();
 }
 
-hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st) {
-    // rows >= n (at most one extra row, the right-hand side) are carried along:
-    // the panel solves and trailing updates turn row n into (L^-1 b)^T
+size_t nq_cholesky_winv_doubles(int n) { return (size_t)((n + CB - 1) / CB) * CB * CB; }
+
+hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st) {
+    // rows >= n (at most one extra row, the right-hand side) are carried along
     const int nr = (lds > n) ? n + 1 : n;
+    hipLaunchKernelGGL(k_chol_diag32, dim3(1), dim3(64), 0, st, S, n, lds, 0, (n < CB ? n : CB), winv, info);
     for (int k = 0; k < n; k += CB) {
         const int kb = (n - k < CB) ? n - k : CB;
         const int rest = nr - k - kb;
-        hipLaunchKernelGGL(k_chol_panel32, dim3(1 + nblk(rest, 64)), dim3(64), 0, st, S, nr, lds, k, kb, info);
-        if (rest > 0) {
-            const int T = (rest + 63) / 64;
-            hipLaunchKernelGGL(k_syrk_trailing<CB>, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, nr, lds, k, kb);
-        }
+        if (rest <= 0) break;
+        const int T = (rest + 63) / 64;
+        hipLaunchKernelGGL(k_chol_step, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, n, nr, lds, k, kb, winv, info);
     }
     return hipGetLastError();
 }
 
-hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, double* y, hipStream_t st) {
-    const int last = ((n - 1) / NB) * NB;
-    for (int k = last; k >= 0; k -= NB) {
-        const int kb = (n - k < NB) ? n - k : NB;
-        hipLaunchKernelGGL(k_back_diag, dim3(1), dim3(64), 0, st, L, lds, k, kb, y);
-        if (k > 0) hipLaunchKernelGGL(k_back_update, dim3(nblk(k, 256)), dim3(256), 0, st, L, lds, k, kb, y);
+hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, const double* winv, double* y, double* x,
+                                 hipStream_t st) {
+    const int last = ((n - 1) / CB) * CB;
+    for (int k = last; k >= 0; k -= CB) {
+        const int kb = (n - k < CB) ? n - k : CB;
+        hipLaunchKernelGGL(k_back_step, dim3(k > 0 ? nblk(k, 256) : 1), dim3(256), 0, st, L, lds, k, kb, winv, y, x);
     }
     return hipGetLastError();
 }
 
-hipError_t nq_copy_aug_row(const double* A, int n, int lds, double* y, hipStream_t st) {
-    hipLaunchKernelGGL(k_copy_row, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, lds, y);
+hipError_t nq_copy_aug_col(const double* A, int n, int lds, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_col, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, lds, y);
     return hipGetLastError();
 }
 
-hipError_t nq_cholesky_solve(const double* L, int n, int lds, double* y, hipStream_t st) {
-    for (int k = 0; k < n; k += NB) {
-        const int kb = (n - k < NB) ? n - k : NB;
-        hipLaunchKernelGGL(k_fwd_diag, dim3(1), dim3(64), 0, st, L, lds, k, kb, y);
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* winv, double* y, double* tmp,
+                             hipStream_t st) {
+    // forward L z = y (z in tmp), then back L^T y = z
+    for (int k = 0; k < n; k += CB) {
+        const int kb = (n - k < CB) ? n - k : CB;
         const int rest = n - k - kb;
-        if (rest > 0) hipLaunchKernelGGL(k_fwd_update, dim3(nblk(rest, 256)), dim3(256), 0, st, L, n, lds, k, kb, y);
+        hipLaunchKernelGGL(k_fwd_step, dim3(rest > 0 ? nblk(rest, 256) : 1), dim3(256), 0, st, L, n, lds, k, kb, winv,
+                           y, tmp);
     }
-    return nq_cholesky_backsolve(L, n, lds, y, st);
+    return nq_cholesky_backsolve(L, n, lds, winv, tmp, y, st);
 }
 
 hipError_t nq_set_aug_row(double* A, int n, int lds, const double* b, hipStream_t st) {

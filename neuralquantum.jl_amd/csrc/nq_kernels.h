@@ -97,16 +97,23 @@ hipError_t nq_launch_force_finish(const double* Fpart, int nblk, double* part2, 
 hipError_t nq_launch_force_part(const double* Fpart, int nb, int n, double* part2, hipStream_t st);
 // S[k,k] += eps (kind 0) or eps * S[k,k] (kind 1)
 hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st);
-// in-place lower Cholesky of S (column-major, n x n, leading dimension lds);
-// *info = first failing column + 1 or 0.  With lds > n, row n holds a right-hand
-// side b^T on entry and (L^-1 b)^T on exit (the forward substitution).
-hipError_t nq_cholesky(double* S, int n, int lds, int* info, hipStream_t st);
-// y <- L^-T y (back substitution with the factor)
-hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, double* y, hipStream_t st);
-// y[j] = A[n, j] (read the forward-substituted right-hand side out of row n)
-hipError_t nq_copy_aug_row(const double* A, int n, int lds, double* y, hipStream_t st);
-// y <- (L L^T)^-1 y with the factor from nq_cholesky (blocked forward + back substitution)
-hipError_t nq_cholesky_solve(const double* L, int n, int lds, double* y, hipStream_t st);
+// in-place Cholesky of S (column-major, n x n, leading dimension lds): the
+// diagonal 32 x 32 blocks of L in the lower triangle, the panels below them
+// transposed into the upper triangle (A[j, i] = L[i, j] for i beyond j's block),
+// winv[b] = L_bb^-T (32 x 32 column-major, nq_cholesky_winv_doubles(n) doubles);
+// *info = first failing column + 1 or 0.  With lds > n, row n holds a
+// right-hand side b^T on entry and L^-1 b ends in column n (read it with
+// nq_copy_aug_col).
+size_t nq_cholesky_winv_doubles(int n);
+hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st);
+// x = L^-T y with the factor from nq_cholesky (y is overwritten)
+hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, const double* winv, double* y, double* x,
+                                 hipStream_t st);
+// y[j] = A[j, n] (the forward-substituted right-hand side)
+hipError_t nq_copy_aug_col(const double* A, int n, int lds, double* y, hipStream_t st);
+// y <- (L L^T)^-1 y with the factor from nq_cholesky; tmp: n doubles
+hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* winv, double* y, double* tmp,
+                             hipStream_t st);
 // A[n, j] = b[j]: the right-hand side as the extra row nq_cholesky carries along
 hipError_t nq_set_aug_row(double* A, int n, int lds, const double* b, hipStream_t st);
 // r = b - (S + shift) x   (S the unfactored matrix, shift as in nq_launch_shift)

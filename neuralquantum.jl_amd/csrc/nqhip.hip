@@ -58,7 +58,7 @@ struct nqh_ctx {
     int est_grid = 0;
     dbuf Fpart;
     // iteration buffers
-    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2;
+    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp;
     int64_t last_ns = 0;
     bool grad_ready = false;
 
@@ -238,6 +238,8 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * nsys(c)))) return s;
     if ((s = ensure(c, c->delta, sizeof(double) * nsys(c)))) return s;
     if ((s = ensure(c, c->resid, sizeof(double) * nsys(c)))) return s;
+    if ((s = ensure(c, c->tmp, sizeof(double) * nsys(c)))) return s;
+    if ((s = ensure(c, c->winv, sizeof(double) * nq_cholesky_winv_doubles(nsys(c))))) return s;
     if ((s = ensure(c, c->info, sizeof(int) * 4))) return s;
     if (c->s_prec == NQH_PREC_BF16) {
         const int64_t nsp = (Ns + 63) / 64 * 64;
@@ -307,6 +309,8 @@ hipError_t launch_metropolis(nqh_ctx* c, const samp_args& a) {
     r.nh = c->rbm_nh;
     r.npc = c->rbm_npc;
     r.ket = is_ket(c) ? 1 : 0;
+    r.diag = a.diag;
+    r.tag = a.tag;
     r.p = a.p;
     r.chains = a.chains;
     r.passes = a.passes;
@@ -537,7 +541,7 @@ void nqh_destroy(nqh_ctx* c) {
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->made_th, &c->made_m, &c->made_v,
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
                    &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats};
@@ -1156,13 +1160,16 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     HIPCHK(c, nq_launch_shift(S, np, ldS, eps, kind, c->stream));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, c->stream));
     HIPCHK(c, nq_set_aug_row(S, np, ldS, P<double>(c->F), c->stream));
-    HIPCHK(c, nq_cholesky(S, np, ldS, P<int>(c->info), c->stream));       // also row np <- (L^-1 F)^T
-    HIPCHK(c, nq_copy_aug_row(S, np, ldS, P<double>(c->delta), c->stream));
-    HIPCHK(c, nq_cholesky_backsolve(S, np, ldS, P<double>(c->delta), c->stream));
+    // the factorization also forward-substitutes F (carried as row np)
+    HIPCHK(c, nq_cholesky(S, np, ldS, P<int>(c->info), P<double>(c->winv), c->stream));
+    HIPCHK(c, nq_copy_aug_col(S, np, ldS, P<double>(c->resid), c->stream));
+    HIPCHK(c, nq_cholesky_backsolve(S, np, ldS, P<double>(c->winv), P<double>(c->resid), P<double>(c->delta),
+                                    c->stream));
     for (int it = 0; it < refine; ++it) {
         HIPCHK(c, nq_residual(P<double>(c->S0), np, ldS, eps, kind, P<double>(c->delta), P<double>(c->F),
                               P<double>(c->resid), c->stream));
-        HIPCHK(c, nq_cholesky_solve(S, np, ldS, P<double>(c->resid), c->stream));
+        HIPCHK(c, nq_cholesky_solve(S, np, ldS, P<double>(c->winv), P<double>(c->resid), P<double>(c->tmp),
+                                    c->stream));
         HIPCHK(c, nq_launch_axpy(P<double>(c->delta), P<double>(c->resid), np, c->stream));
     }
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, c->stream));
@@ -1290,9 +1297,8 @@ nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
     return NQH_OK;
 }
 
-nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
-    if (!c) return NQH_E_ARG;
-    (void)hipSetDevice(c->device);
+// enqueue one SR iteration on the context's stream without waiting for it
+static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
     if (nqh_status sm = made_maybe_retrain(c)) return sm;
     if (c->use_graph && !c->profiling) {
         nqh_status s0 = need_net(c);
@@ -1301,15 +1307,7 @@ nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
         if (!s0 && (shift_kind != NQH_SHIFT_ADD && shift_kind != NQH_SHIFT_DIAG)) s0 = fail(c, NQH_E_ARG, "bad shift kind");
         if (!s0 && (!(eps >= 0.0) || refine < 0 || refine > 8)) s0 = fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
         if (s0) return s0;
-        nqh_status sg = graph_iteration(c, eps, shift_kind, refine, lr);
-        if (sg) return sg;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        if ((sg = check_solve(c))) return sg;
-        double st[5];
-        HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
-        if (stats) std::memcpy(stats, st, sizeof(st));
-        if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
-        return NQH_OK;
+        return graph_iteration(c, eps, shift_kind, refine, lr);
     }
     nqh_status s = sample_grad_async(c);
     if (s) return s;
@@ -1317,14 +1315,40 @@ nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
     phase_begin(c, PH_UPDATE);
     HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
+    return NQH_OK;
+}
+
+// wait for the enqueued iteration; statistics and solve checks
+static nqh_status sr_finish(nqh_ctx* c, double* stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
+    nqh_status s;
     if ((s = check_solve(c))) return s;
     double st[5];
     HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
     if (stats) std::memcpy(stats, st, sizeof(st));
     if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
     return NQH_OK;
+}
+
+nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s = sr_enqueue(c, eps, shift_kind, refine, lr);
+    if (s) return s;
+    return sr_finish(c, stats);
+}
+
+nqh_status nqh_sr_iteration_async(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    return sr_enqueue(c, eps, shift_kind, refine, lr);
+}
+
+nqh_status nqh_wait(nqh_ctx* c, double* stats) {
+    if (!c) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    return sr_finish(c, stats);
 }
 
 nqh_status nqh_get_samples(nqh_ctx* c, uint64_t* keys_out, int64_t n) {

@@ -86,6 +86,8 @@ SIGNATURES = {
     "nqh_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "nqh_get_phase_times": (C.c_int32, [C.c_void_p, _dp, _i64p]),
     "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "nqh_sr_iteration_async": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double]),
+    "nqh_wait": (C.c_int32, [C.c_void_p, _dp]),
     "nqh_made_num_params": (C.c_int64, [C.c_int32]),
     "nqh_made_setup": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                    C.c_int64]),
@@ -329,6 +331,14 @@ class Context:
     def sr_iteration(self, eps, lr, shift_kind=SHIFT_ADD, refine=0):
         st = np.empty(5)
         self._chk(self.lib.nqh_sr_iteration(self.h, eps, shift_kind, refine, lr, _p(st, _dp)))
+        return st
+
+    def sr_iteration_async(self, eps, lr, shift_kind=SHIFT_ADD, refine=0):
+        self._chk(self.lib.nqh_sr_iteration_async(self.h, eps, shift_kind, refine, lr))
+
+    def wait(self):
+        st = np.empty(5)
+        self._chk(self.lib.nqh_wait(self.h, _p(st, _dp)))
         return st
 
     def get_samples(self):
