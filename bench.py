@@ -94,6 +94,82 @@ def time_form(form, steps, warmup, cfg=CFG):
     return ctx, dt, phases, losses
 
 
+C5 = dict(instances=64, nmin=6, nmax=16, gmin=0.5, gmax=2.0, seed=2024, chains=256, chain_length=8, burn=50,
+          eps=0.01, lr=0.01, sigma=1e-4)
+
+
+def build_instances(cfg=C5):
+    """The batched-instances workload: dissipative Ising chains (periodic,
+    V=2, gamma=1) with N ~ U{6..16} and g ~ U[0.5, 2], NDM alpha = 1,
+    256 chains x 8 samples, 2N+1 passes, in the chosen form."""
+    rng = np.random.default_rng(cfg["seed"])
+    ns = rng.integers(cfg["nmin"], cfg["nmax"] + 1, size=cfg["instances"])
+    gs = rng.uniform(cfg["gmin"], cfg["gmax"], size=cfg["instances"])
+    members = []
+    for i, (N, g) in enumerate(zip(ns, gs)):
+        N = int(N)
+        _, _, _, L = problems.dissipative_ising_chain(N, V=2.0, g=float(g), gamma=1.0, periodic=True)
+        ctx = _lib.Context(0, cfg["seed"] + i)
+        ctx.set_problem(operators.flatten(L))
+        ctx.set_ndm(N, N, N)
+        prng = np.random.default_rng(cfg["seed"] + 1000 + i)
+        ctx.set_params(prng.standard_normal(ctx.np_) * math.sqrt(cfg["sigma"]))
+        ctx.set_precision(_lib.PREC_FP32, _lib.PREC_BF16)
+        ctx.set_sampler(cfg["chains"], cfg["chain_length"], 2 * N + 1, cfg["burn"], _lib.SAMPLER_PERSISTENT)
+        members.append(ctx)
+    return members, ns
+
+
+def time_instances(steps, warmup, cfg=C5):
+    """Aggregate SR iterations/s of the batched instances, three ways: one
+    after another (each member replaying its own graph), all members queued
+    on their own streams then awaited, and one batch graph per iteration."""
+    eps, lr = cfg["eps"], cfg["lr"]
+    res, losses = {}, {}
+    for mode in ("sequential", "streams", "batch_graph"):
+        members, ns = build_instances(cfg)
+        batch = None
+        if mode == "batch_graph":
+            batch = _lib.Batch(members)
+        else:
+            for c in members:
+                c.set_graph(True)
+
+        def one_iteration():
+            if mode == "sequential":
+                return np.array([c.sr_iteration(eps, lr) for c in members])
+            if mode == "streams":
+                for c in members:
+                    c.sr_iteration_async(eps, lr)
+                return np.array([c.wait() for c in members])
+            return batch.sr_iteration(eps, lr)
+
+        for _ in range(warmup):
+            one_iteration()
+        hip_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = one_iteration()
+        hip_sync()
+        dt = time.perf_counter() - t0
+        res[mode] = cfg["instances"] * steps / dt
+        losses[mode] = st[:, 0].copy()
+        if batch is not None:
+            batch.close()
+        for c in members:
+            c.close()
+    d = float(np.max(np.abs(losses["batch_graph"] - losses["sequential"]) / np.abs(losses["sequential"])))
+    np_total = int(sum(5 * n + 4 * n * n for n in ns))
+    return {"workload": (f"{cfg['instances']} dissipative Ising chains (periodic, V=2, gamma=1), N~U{{6..16}}, "
+                         f"g~U[0.5,2] (seed {cfg['seed']}), NDM alpha=1 (sum Np = {np_total}), "
+                         f"{cfg['chains']} chains x {cfg['chain_length']} samples, 2N+1 passes, persistent "
+                         f"chains, FP32 network + BF16 S build + FP64 solve, shift 0.01, lr 0.01"),
+            "unit": "SR iters/s summed over instances",
+            "aggregate_it_s": {k: round(v, 1) for k, v in res.items()},
+            "steps": steps, "warmup": warmup,
+            "max_rel_loss_diff_batch_vs_sequential": d}
+
+
 def tau_int(series):
     """Integrated autocorrelation time (Sokal automatic window, c = 5) of
     per-chain series (chains x T), in units of stored samples; tau = 1/2 for
@@ -173,7 +249,7 @@ def kernel_rooflines(phases, form, d, pairs):
                          algorithmic=f"4 Np^2 Ns = {fl:.4g} flop", ms=per["sbuild"])
     t = per["solve"] * 1e-3
     fl = np_ ** 3 / 3 + 2 * np_ ** 2
-    out["solve"] = dict(kernel="k_chol_panel + k_syrk_trailing + solves", bound="mfma", achieved=fl / t / 1e12,
+    out["solve"] = dict(kernel="k_chol_step + k_back_step", bound="mfma", achieved=fl / t / 1e12,
                         peak=PEAK_TFLOPS["fp64"], unit="TFLOP/s", algorithmic=f"Np^3/3 + 2 Np^2 = {fl:.4g} flop",
                         ms=per["solve"])
     t = per["post"] * 1e-3
@@ -283,6 +359,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tau", action="store_true")
+    ap.add_argument("--no-instances", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (the known-size kernel the --pmc byte counters are "
                          "scaled by)")
@@ -327,9 +404,10 @@ def main():
                    "forms": {"fp64": "FP64 network + FP64 S build, restart + 50 burn-in calls per iteration",
                              "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"}},
         "fp64_form": {"it_s": round(res["fp64"]["it_s"], 3), "ms_per_step": round(res["fp64"]["ms"], 4),
-                      "phase_ms": {k: round(v, 4) for k, v in res["fp64"]["per"].items()}},
+                      "phase_ms": {k: round(v, 4) for k, v in res["fp64"]["per"].items()},
+                      "loss_last": res["fp64"]["loss_last"]},
         "chosen_form": {"phase_ms": {k: round(v, 4) for k, v in ch["per"].items()},
-                        "pairs_per_iteration": ch["pairs"]},
+                        "pairs_per_iteration": ch["pairs"], "loss_last": ch["loss_last"]},
         "speedup_chosen_vs_fp64": round(ch["it_s"] / res["fp64"]["it_s"], 3),
     }
     if taus:
@@ -350,6 +428,8 @@ def main():
         out["roofline"]["hbm_algorithmic_gbs"] = round(r["hbm_gbs"], 2)
     out["kernels"] = {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in roofs.items()}
+    if not args.no_instances:
+        out["batched_instances"] = time_instances(max(2, args.steps // 2), 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

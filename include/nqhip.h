@@ -206,6 +206,26 @@ nqh_status nqh_sr_iteration(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
 nqh_status nqh_sr_iteration_async(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr);
 nqh_status nqh_wait(nqh_ctx* ctx, double* stats);
 
+/* A batch of independent problems, e.g. many Liouvillians of uneven size:
+ * every member's SR iteration is captured, each on its member's stream,
+ * forked from and joined into the batch's own stream, as ONE hipGraph that
+ * is launched once per iteration (members with persistent chains run their
+ * first, restart + burn-in, iteration eagerly and concurrently).  Members
+ * are fully set-up contexts on one device (problem, network, sampler; no
+ * learned proposal); they must not be reconfigured (nqh_set_problem,
+ * nqh_set_ndm / rbmsplit / rbm, nqh_set_sampler, nqh_set_precision) nor
+ * destroyed while the batch exists; reading parameters or samples between
+ * iterations is fine, and a member's own calls stay ordered with the
+ * batch's iterations.  stats: n x 5, member i at 5 i in nqh_sr_iteration's
+ * layout; on a member's failure the message names the member.  No reference
+ * counterpart (the reference runs one problem per process). */
+typedef struct nqh_batch nqh_batch;
+nqh_status nqh_batch_create(nqh_batch** out, nqh_ctx* const* members, int32_t n);
+void nqh_batch_destroy(nqh_batch* batch);
+const char* nqh_batch_last_error(const nqh_batch* batch);
+nqh_status nqh_batch_sr_iteration(nqh_batch* batch, double eps, int32_t shift_kind, int32_t refine, double lr,
+                                  double* stats);
+
 /* device time per phase, measured with HIP events on the context's stream
  * while profiling is on (accumulated, in ms; counts = timed occurrences):
  * [0] sampler  [1] estimator (log rho, O, L_loc, force)  [2] means, statistics,

@@ -1247,9 +1247,13 @@ static nqh_status enqueue_iteration(nqh_ctx* c, double eps, int kind, int refine
 static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, double lr) {
     const bool persistent = (c->flags & NQH_SAMPLER_PERSISTENT) != 0;
     if (persistent && !c->chains_initialised) {
-        // the restart + burn-in iteration runs eagerly; later ones replay a graph
+        // the restart + burn-in iteration runs eagerly (iteration number from
+        // the host, not from a possibly stale device copy); later ones replay a graph
         drop_graph(c);
-        return enqueue_iteration(c, eps, kind, refine, lr);
+        c->use_graph = false;
+        const nqh_status s = enqueue_iteration(c, eps, kind, refine, lr);
+        c->use_graph = true;
+        return s;
     }
     if (!c->graph_valid || c->g_eps != eps || c->g_kind != kind || c->g_refine != refine || c->g_lr != lr) {
         drop_graph(c);
@@ -1349,6 +1353,207 @@ nqh_status nqh_wait(nqh_ctx* c, double* stats) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
     return sr_finish(c, stats);
+}
+
+// ---- batches of independent problems: one graph launch per SR iteration
+struct nqh_batch {
+    int device = 0;
+    std::vector<nqh_ctx*> m;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join, pre;
+    hipEvent_t post = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool valid = false;
+    double eps = 0.0, lr = 0.0;
+    int kind = -1, refine = -1;
+    std::string err;
+};
+
+namespace {
+
+nqh_status bfail(nqh_batch* b, nqh_status s, const std::string& msg) {
+    b->err = msg;
+    return s;
+}
+
+nqh_status bmember(nqh_batch* b, size_t i, nqh_status s) {
+    return bfail(b, s, "member " + std::to_string(i) + ": " + b->m[i]->err);
+}
+
+void batch_drop_graph(nqh_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->valid = false;
+}
+
+#define BHIPCHK(b, expr)                                                                           \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) return bfail(b, NQH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// capture every member's iteration, each on its own stream forked from and
+// joined back into the batch stream, as one graph
+nqh_status batch_build(nqh_batch* b, double eps, int kind, int refine, double lr) {
+    batch_drop_graph(b);
+    const size_t n = b->m.size();
+    std::vector<uint32_t> iter0(n);
+    std::vector<char> prof(n);
+    for (size_t i = 0; i < n; ++i) {
+        nqh_ctx* c = b->m[i];
+        nqh_status s;
+        if ((s = need_net(c)) || (s = need_problem(c))) return bmember(b, i, s);
+        if (!c->has_sampler) return bmember(b, i, fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)"));
+        if ((s = ensure_samples(c, c->chains * c->chain_len))) return bmember(b, i, s);
+        if ((s = ensure(c, c->iter_dev, sizeof(uint32_t) * 4))) return bmember(b, i, s);
+        BHIPCHK(b, hipMemcpyAsync(c->iter_dev.p, &c->iter, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        BHIPCHK(b, hipStreamSynchronize(c->stream));
+        iter0[i] = c->iter;
+        prof[i] = c->profiling;
+        c->profiling = false;
+    }
+    nqh_status s = NQH_OK;
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bfail(b, NQH_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    e = hipEventRecord(b->fork, b->stream);
+    for (size_t i = 0; i < n && e == hipSuccess && !s; ++i) {
+        nqh_ctx* c = b->m[i];
+        e = hipStreamWaitEvent(c->stream, b->fork, 0);
+        if (e == hipSuccess) {
+            s = enqueue_iteration(c, eps, kind, refine, lr);
+            if (s) s = bmember(b, i, s);
+        }
+        if (e == hipSuccess && !s) e = nq_launch_bump(P<uint32_t>(c->iter_dev), c->stream);
+        if (e == hipSuccess && !s) e = hipEventRecord(b->join[i], c->stream);
+        if (e == hipSuccess && !s) e = hipStreamWaitEvent(b->stream, b->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    for (size_t i = 0; i < n; ++i) {
+        b->m[i]->iter = iter0[i];      // the capture ran nothing
+        b->m[i]->profiling = prof[i];
+    }
+    if (s || e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (s) return s;
+        return bfail(b, NQH_E_HIP, std::string("batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    BHIPCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->valid = true;
+    b->eps = eps; b->kind = kind; b->refine = refine; b->lr = lr;
+    return NQH_OK;
+}
+
+}  // namespace
+
+nqh_status nqh_batch_create(nqh_batch** out, nqh_ctx* const* members, int32_t n) {
+    if (!out || !members || n < 1) return NQH_E_ARG;
+    *out = nullptr;
+    std::vector<nqh_ctx*> m(members, members + n);
+    for (int i = 0; i < n; ++i) {
+        if (!m[i] || m[i]->device != m[0]->device) return NQH_E_ARG;
+        for (int j = 0; j < i; ++j)
+            if (m[j] == m[i]) return NQH_E_ARG;
+    }
+    if (hipSetDevice(m[0]->device) != hipSuccess) return NQH_E_HIP;
+    nqh_batch* b = new nqh_batch();
+    b->device = m[0]->device;
+    b->m = m;
+    b->join.assign(n, nullptr);
+    b->pre.assign(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->post, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < n && ok; ++i) {
+        ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        nqh_batch_destroy(b);
+        return NQH_E_HIP;
+    }
+    for (nqh_ctx* c : m) {
+        // members read the iteration number from device memory, as in their own graphs
+        drop_graph(c);
+        c->use_graph = true;
+    }
+    *out = b;
+    return NQH_OK;
+}
+
+void nqh_batch_destroy(nqh_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->pre)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->post) (void)hipEventDestroy(b->post);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* nqh_batch_last_error(const nqh_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_t refine, double lr, double* stats) {
+    if (!b) return NQH_E_ARG;
+    (void)hipSetDevice(b->device);
+    if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return bfail(b, NQH_E_ARG, "bad shift kind");
+    if (!(eps >= 0.0) || refine < 0 || refine > 8) return bfail(b, NQH_E_ARG, "eps >= 0, refine in 0..8");
+    const size_t n = b->m.size();
+    bool eager = false;
+    for (size_t i = 0; i < n; ++i) {
+        nqh_ctx* c = b->m[i];
+        if (c->made_ready && c->made_mix > 0.0)
+            return bfail(b, NQH_E_STATE, "member " + std::to_string(i) + ": learned proposals are not batched");
+        if ((c->flags & NQH_SAMPLER_PERSISTENT) && !c->chains_initialised) eager = true;
+    }
+    if (eager) {
+        // the restart + burn-in iteration runs eagerly, members concurrently on their streams
+        batch_drop_graph(b);
+        for (size_t i = 0; i < n; ++i) {
+            nqh_ctx* c = b->m[i];
+            c->use_graph = false;    // iteration number from the host
+            const nqh_status s = enqueue_iteration(c, eps, kind, refine, lr);
+            c->use_graph = true;
+            if (s) return bmember(b, i, s);
+        }
+    } else {
+        if (!b->valid || b->eps != eps || b->kind != kind || b->refine != refine || b->lr != lr) {
+            nqh_status s = batch_build(b, eps, kind, refine, lr);
+            if (s) return s;
+        }
+        // work already queued on a member's stream precedes the batch's graph,
+        // and the member's later work follows it
+        for (size_t i = 0; i < n; ++i) {
+            BHIPCHK(b, hipEventRecord(b->pre[i], b->m[i]->stream));
+            BHIPCHK(b, hipStreamWaitEvent(b->stream, b->pre[i], 0));
+        }
+        BHIPCHK(b, hipGraphLaunch(b->gexec, b->stream));
+        BHIPCHK(b, hipEventRecord(b->post, b->stream));
+        for (size_t i = 0; i < n; ++i) {
+            nqh_ctx* c = b->m[i];
+            BHIPCHK(b, hipStreamWaitEvent(c->stream, b->post, 0));
+            c->iter += 1;
+            c->chains_initialised = true;
+            c->grad_ready = true;
+            c->last_ns = c->chains * c->chain_len;
+        }
+    }
+    nqh_status first = NQH_OK;
+    for (size_t i = 0; i < n; ++i) {
+        nqh_status s = sr_finish(b->m[i], stats ? stats + 5 * i : nullptr);
+        if (s && !first) first = bmember(b, i, s);
+    }
+    return first;
 }
 
 nqh_status nqh_get_samples(nqh_ctx* c, uint64_t* keys_out, int64_t n) {

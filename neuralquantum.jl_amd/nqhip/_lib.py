@@ -88,6 +88,10 @@ SIGNATURES = {
     "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
     "nqh_sr_iteration_async": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double]),
     "nqh_wait": (C.c_int32, [C.c_void_p, _dp]),
+    "nqh_batch_create": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32]),
+    "nqh_batch_destroy": (None, [C.c_void_p]),
+    "nqh_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "nqh_batch_sr_iteration": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double, _dp]),
     "nqh_made_num_params": (C.c_int64, [C.c_int32]),
     "nqh_made_setup": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                    C.c_int64]),
@@ -410,3 +414,39 @@ class Context:
         a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
         self._chk(self.lib.nqh_made_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return int(a.value), int(b.value), int(c.value)
+
+
+class Batch:
+    """Owns one nqh_batch: the SR iterations of several contexts (independent
+    problems on one device) captured as one graph, launched once per
+    iteration.  The member contexts must stay configured and alive while the
+    batch exists; close the batch first."""
+
+    def __init__(self, members):
+        self.lib = load()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        st = self.lib.nqh_batch_create(C.byref(h), arr, len(self.members))
+        if st != NQH_OK:
+            raise NQHError(st, "nqh_batch_create failed (members on different devices, duplicated or null)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nqh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sr_iteration(self, eps, lr, shift_kind=SHIFT_ADD, refine=0):
+        """One SR iteration of every member; returns an (n, 5) array of statistics."""
+        st = np.empty((len(self.members), 5))
+        r = self.lib.nqh_batch_sr_iteration(self.h, eps, shift_kind, refine, lr, _p(st, _dp))
+        if r != NQH_OK:
+            raise NQHError(r, self.lib.nqh_batch_last_error(self.h).decode())
+        return st

@@ -1,0 +1,85 @@
+"""Batches of independent problems (nqh_batch_*): every member's SR iteration
+captured into one graph and launched once per iteration gives the same
+statistics, samples and parameters as the same contexts run one by one."""
+import numpy as np
+import pytest
+
+from _helpers import ising, params, rel_err
+from nqhip import Batch, Context, NQHError, _lib
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(6, 1.0), (9, 0.6), (7, 1.7), (11, 1.2), (8, 0.9)]
+
+
+def member(ref, N, g, seed, flags, prec):
+    flat, _, _ = ising(N, periodic=True, g=g)
+    ctx = Context(0, seed)
+    ctx.set_problem(flat)
+    ctx.set_ndm(N, N, N)
+    ctx.set_params(params(ref.ndm_np(N, N, N), seed=seed, sigma=0.05))
+    ctx.set_precision(*prec)
+    ctx.set_sampler(64, 8, 2 * N + 1, 10, flags)
+    ctx.set_iteration(0)
+    return ctx
+
+
+@pytest.mark.parametrize("flags", [0, _lib.SAMPLER_PERSISTENT])
+@pytest.mark.parametrize("prec", [(_lib.PREC_FP64, _lib.PREC_FP64), (_lib.PREC_FP32, _lib.PREC_BF16)])
+def test_batch_matches_members_run_alone(gpu_ok, ref, flags, prec):
+    alone = [member(ref, N, g, 100 + i, flags, prec) for i, (N, g) in enumerate(SIZES)]
+    inb = [member(ref, N, g, 100 + i, flags, prec) for i, (N, g) in enumerate(SIZES)]
+    batch = Batch(inb)
+    for it in range(4):
+        st = batch.sr_iteration(0.01, 0.01)
+        for i, c in enumerate(alone):
+            sa = c.sr_iteration(0.01, 0.01)
+            assert rel_err(st[i], sa) < 1e-12, (it, i)
+            assert np.array_equal(inb[i].get_samples(), c.get_samples()), (it, i)
+    for a, b in zip(alone, inb):
+        assert rel_err(b.get_params(), a.get_params()) < 1e-12
+    batch.close()
+    for c in alone + inb:
+        c.close()
+
+
+def test_batch_member_keeps_working_alone(gpu_ok, ref):
+    """A member's own iterations interleave with the batch's in order."""
+    prec = (_lib.PREC_FP64, _lib.PREC_FP64)
+    a = [member(ref, 6, 1.0, 7, _lib.SAMPLER_PERSISTENT, prec), member(ref, 8, 1.4, 8, _lib.SAMPLER_PERSISTENT, prec)]
+    b = [member(ref, 6, 1.0, 7, _lib.SAMPLER_PERSISTENT, prec), member(ref, 8, 1.4, 8, _lib.SAMPLER_PERSISTENT, prec)]
+    batch = Batch(b)
+    for it in range(3):
+        batch.sr_iteration(0.01, 0.01)
+        for c in a:
+            c.sr_iteration(0.01, 0.01)
+        sb = b[1].sr_iteration(0.01, 0.01)       # member 1 alone, between batch iterations
+        sa = a[1].sr_iteration(0.01, 0.01)
+        assert rel_err(sb, sa) < 1e-12, it
+        a[0].sr_iteration(0.01, 0.01)
+        b[0].sr_iteration(0.01, 0.01)
+    for x, y in zip(a, b):
+        assert rel_err(y.get_params(), x.get_params()) < 1e-12
+    batch.close()
+    for c in a + b:
+        c.close()
+
+
+def test_batch_rejects_learned_proposals(gpu_ok, ref):
+    prec = (_lib.PREC_FP32, _lib.PREC_BF16)
+    c0 = member(ref, 6, 1.0, 1, 0, prec)
+    c1 = member(ref, 6, 1.0, 2, 0, prec)
+    c1.made_setup(0.5, 0, 0, 64, 1e-3, 0)
+    batch = Batch([c0, c1])
+    with pytest.raises(NQHError, match="member 1"):
+        batch.sr_iteration(0.01, 0.01)
+    batch.close()
+    c0.close()
+    c1.close()
+
+
+def test_batch_create_rejects_duplicates(gpu_ok):
+    c = Context(0, 1)
+    with pytest.raises(NQHError):
+        Batch([c, c])
+    c.close()
