@@ -59,7 +59,7 @@ __device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
 }
 
 template <typename T, int NMAX, int RS, int PS>
-__global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
+__global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_estimate(est_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ndm_dims d = a.d;
     const int N = d.N, nh = d.nh, na = d.na;
@@ -527,9 +527,14 @@ __global__ __launch_bounds__(256, 2) void k_estimate(est_args a) {
 
 template <typename T, int NMAX, int RS, int PS>
 hipError_t launch_t(const est_args& a, int grid, hipStream_t st) {
-    const size_t lds = est_lds_bytes(a, sizeof(T));
-    // the parity mode (per-sample gradients) runs one wave per block
-    hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS>), dim3(grid), dim3(a.G ? 64 : 256), lds, st, a);
+    // 8 waves per block share one copy of the parameters, the force
+    // accumulator and the table (two blocks per CU while that fits in half
+    // the LDS: 4 waves per SIMD); otherwise 4.  The parity mode (per-sample
+    // gradients) runs one wave per block.
+    int wpb = a.G ? 1 : 8;
+    if (!a.G && est_lds_bytes(a, sizeof(T), wpb) > 80 * 1024) wpb = 4;
+    hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS>), dim3(grid), dim3(64 * wpb), est_lds_bytes(a, sizeof(T), wpb), st,
+                       a);
     return hipGetLastError();
 }
 
@@ -543,16 +548,16 @@ hipError_t launch_n(const est_args& a, int grid, hipStream_t st) {
 
 }  // namespace
 
-size_t est_lds_bytes(const est_args& a, size_t elt) {
-    // params | FP64 force | 4 waves x (W cache + flip masks) | table (16-aligned)
+size_t est_lds_bytes(const est_args& a, size_t elt, int waves) {
+    // params | FP64 force | waves x (W cache + flip masks) | table (16-aligned)
     const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? sizeof(double) * (size_t)a.d.np : 0) +
-                     4 * (size_t)a.max_conn * (2 * elt + 8);
+                     (size_t)waves * (size_t)a.max_conn * (2 * elt + 8);
     return ((b + 15) & ~(size_t)15) + (a.tab_lds ? (size_t)a.tab.bytes : 0);
 }
 
 hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st) {
     if (a.d.N > 16) return hipErrorInvalidValue;
-    if (est_lds_bytes(a, fp64 ? 8 : 4) > 160 * 1024) return hipErrorInvalidValue;
+    if (est_lds_bytes(a, fp64 ? 8 : 4, a.G ? 1 : 4) > 160 * 1024) return hipErrorInvalidValue;
     if (fp64) {
         if (a.d.N <= 8) return launch_n<double, 8>(a, grid, st);
         return launch_n<double, 16>(a, grid, st);

@@ -24,7 +24,7 @@ struct est_args {
     double* Fpart;            // gridDim x np: per-workgroup force partials (P order), nullable
     void* G;                  // Ns x np complex FP64: per-sample dL_loc (reference order), nullable, pre-zeroed
 };
-size_t est_lds_bytes(const est_args& a, size_t elt);
+size_t est_lds_bytes(const est_args& a, size_t elt, int waves);
 hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st);
 
 // ---- Metropolis sampler (k_sampler.hip)
