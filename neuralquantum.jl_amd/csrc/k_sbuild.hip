@@ -247,6 +247,164 @@ __global__ __launch_bounds__(256) void k_gram_bf16t(const __bf16* Yt, int64_t ns
             }
 }
 
+// ---- split-K BF16 Gram.  Pass 1: each workgroup takes one 128x128 upper
+// tile of a Gram block (block A = Yt rows [0, nA), block B = rows
+// [nA, nA + nB)) over one chunk of the samples and writes its FP32 partial
+// sums (row-major 128 x 128) to `part`; splitting the sample axis keeps
+// every CU busy with a few tiles in flight.  LDS is double-buffered with a
+// one-step register prefetch (one barrier per k-step of 32).
+// Pass 2 (k_gram_finish): sums the chunks in a fixed order (deterministic,
+// no atomics), scales, embeds block A at S[0, nA) and block B at
+// S[offB, offB + nB) (adding where they overlap), writes every element of
+// the (np + 1)^2 matrix (zeros outside the blocks) and mirrors the lower
+// triangle through an LDS transpose so both writes are coalesced.
+__global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_t nsp, int nA, int TA, int nB,
+                                                        int64_t kchunk, int tiles, float* part) {
+    constexpr int KS = 32, PITCH = 40, BUF = 128 * PITCH;
+    __shared__ __align__(16) __bf16 As[2 * BUF];
+    __shared__ __align__(16) __bf16 Bs[2 * BUF];
+    const int tilesA = TA * (TA + 1) / 2;
+    const int tile = blockIdx.x;
+    const __bf16* Y = Yt;
+    int n = nA, tl = tile;
+    if (tile >= tilesA) {
+        Y = Yt + (int64_t)nA * nsp;
+        n = nB;
+        tl = tile - tilesA;
+    }
+    int I, J;
+    tri_decode(tl, &I, &J);
+    const int r0 = I * 128, q0 = J * 128;
+    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+    const int64_t kend = (kbeg + kchunk < nsp) ? kbeg + kchunk : nsp;
+    const int nsteps = kend > kbeg ? (int)((kend - kbeg) / KS) : 0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.f;
+
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const int row0 = tid >> 2, ch = tid & 3;            // rows row0 and row0 + 64, 16 B chunk ch
+    const bool va0 = r0 + row0 < n, va1 = r0 + row0 + 64 < n, vb0 = q0 + row0 < n, vb1 = q0 + row0 + 64 < n;
+    const __bf16* ya0 = Y + (int64_t)min(r0 + row0, n - 1) * nsp + ch * 8;
+    const __bf16* ya1 = Y + (int64_t)min(r0 + row0 + 64, n - 1) * nsp + ch * 8;
+    const __bf16* yb0 = Y + (int64_t)min(q0 + row0, n - 1) * nsp + ch * 8;
+    const __bf16* yb1 = Y + (int64_t)min(q0 + row0 + 64, n - 1) * nsp + ch * 8;
+    const u4 z4 = u4{0u, 0u, 0u, 0u};
+    u4 pa0 = z4, pa1 = z4, pb0 = z4, pb1 = z4;
+    auto load = [&](int64_t k0) {   // unconditional (clamped rows), zeroed when out of range
+        const u4 a0 = *reinterpret_cast<const u4*>(ya0 + k0), a1 = *reinterpret_cast<const u4*>(ya1 + k0);
+        const u4 b0 = *reinterpret_cast<const u4*>(yb0 + k0), b1 = *reinterpret_cast<const u4*>(yb1 + k0);
+        pa0 = va0 ? a0 : z4;
+        pa1 = va1 ? a1 : z4;
+        pb0 = vb0 ? b0 : z4;
+        pb1 = vb1 ? b1 : z4;
+    };
+    auto store = [&](int buf) {
+        __bf16* A = As + buf * BUF;
+        __bf16* B = Bs + buf * BUF;
+        *reinterpret_cast<u4*>(&A[row0 * PITCH + ch * 8]) = pa0;
+        *reinterpret_cast<u4*>(&A[(row0 + 64) * PITCH + ch * 8]) = pa1;
+        *reinterpret_cast<u4*>(&B[row0 * PITCH + ch * 8]) = pb0;
+        *reinterpret_cast<u4*>(&B[(row0 + 64) * PITCH + ch * 8]) = pb1;
+    };
+    if (nsteps > 0) {
+        load(kbeg);
+        store(0);
+        if (nsteps > 1) load(kbeg + KS);
+        __syncthreads();
+        for (int i = 0; i < nsteps; ++i) {
+            const __bf16* A = As + (i & 1) * BUF;
+            const __bf16* B = Bs + (i & 1) * BUF;
+            bf8 av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                av[a] = *reinterpret_cast<const bf8*>(&A[(wr * 64 + a * 16 + li) * PITCH + lk * 8]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bv[b] = *reinterpret_cast<const bf8*>(&B[(wc * 64 + b * 16 + li) * PITCH + lk * 8]);
+            if (i + 1 < nsteps) store((i + 1) & 1);       // step i + 1, loaded one step ago
+            if (i + 2 < nsteps) load(kbeg + (int64_t)(i + 2) * KS);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+            __syncthreads();
+        }
+    }
+    float* P = part + ((int64_t)blockIdx.y * tiles + tile) * (128 * 128);
+    const int lj = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                P[(wr * 64 + a * 16 + 4 * lg + v) * 128 + wc * 64 + b * 16 + lj] = acc[a][b][v];
+}
+
+// sum over the chunks of element (rr, cc) of the Gram block whose tiles start at toff
+__device__ __forceinline__ double gram_part_sum(const float* part, int splits, int tiles, int toff, int rr, int cc) {
+    int I = rr >> 7, J = cc >> 7, x = rr & 127, y = cc & 127;
+    if (I > J) {
+        const int t0 = I; I = J; J = t0;
+        const int t1 = x; x = y; y = t1;
+    }
+    const float* p = part + (int64_t)(toff + J * (J + 1) / 2 + I) * (128 * 128) + x * 128 + y;
+    const int64_t stride = (int64_t)tiles * (128 * 128);
+    double s = 0.0;
+    for (int sp = 0; sp < splits; ++sp) s += (double)p[sp * stride];
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish(const float* part, int splits, int tiles, int nA, int TA, int nB,
+                                                     int offB, double scale, double* S, int np, int lds) {
+    __shared__ double T[64][65];
+    int bi, bj;
+    tri_decode(blockIdx.x, &bi, &bj);   // bi <= bj: 64 x 64 tiles of the (np + 1)^2 matrix
+    const int y = threadIdx.x & 63, x0 = threadIdx.x >> 6;
+    const int c = bj * 64 + y;
+    const int tilesA = TA * (TA + 1) / 2;
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int r = bi * 64 + x0 + 4 * m;
+        double s = 0.0;
+        if (r < np && c < np) {
+            if (r < nA && c < nA) s += gram_part_sum(part, splits, tiles, 0, r, c);
+            if (nB > 0 && r >= offB && c >= offB && r - offB < nB && c - offB < nB)
+                s += gram_part_sum(part, splits, tiles, tilesA, r - offB, c - offB);
+        }
+        v[m] = scale * s;
+    }
+    // S[c, r] (coalesced in c); on a diagonal tile this is every element
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int r = bi * 64 + x0 + 4 * m;
+        if (r <= np && c <= np) S[(int64_t)c + (int64_t)r * lds] = v[m];
+    }
+    if (bi != bj) {
+        // S[r, c] through an LDS transpose (coalesced in r)
+#pragma unroll
+        for (int m = 0; m < 16; ++m) T[x0 + 4 * m][y] = v[m];
+        __syncthreads();
+        const int r = bi * 64 + y;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int cc = bj * 64 + x0 + 4 * m;
+            if (r <= np && cc <= np) S[(int64_t)r + (int64_t)cc * lds] = T[y][x0 + 4 * m];
+        }
+    }
+}
+
 template <typename YT>
 __global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, double* part, int chunks) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,6 +465,44 @@ hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S,
     const int blocks = T * (T + 1) / 2;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gram_bf16t, dim3(blocks), dim3(256), 0, st, Yt, nsp, n, S, lds, o, scale);
+    return hipGetLastError();
+}
+
+int nq_gram_split_plan(int nA, int nB, int64_t nsp, int64_t* kchunk) {
+    const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
+    const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
+    int s = tiles > 0 ? (3 * 256 + tiles - 1) / tiles : 1;   // about three tiles in flight per CU
+    const int maxs = nsp >= 64 ? (int)(nsp / 64) : 1;
+    s = s < maxs ? s : maxs;
+    s = s < 64 ? s : 64;
+    s = s > 1 ? s : 1;
+    int64_t kc = (nsp + s - 1) / s;
+    kc = (kc + 63) / 64 * 64;
+    if (kc < 64) kc = 64;
+    *kchunk = kc;
+    return (int)((nsp + kc - 1) / kc);
+}
+
+size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp) {
+    int64_t kc;
+    const int splits = nq_gram_split_plan(nA, nB, nsp, &kc);
+    const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
+    const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
+    return (size_t)splits * tiles * 128 * 128;
+}
+
+hipError_t nq_launch_gram2_bf16(const __bf16* Yt, int64_t nsp, int nA, int nB, int offB, double scale, float* part,
+                                double* S, int np, int lds, hipStream_t st) {
+    int64_t kc;
+    const int splits = nq_gram_split_plan(nA, nB, nsp, &kc);
+    const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
+    const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
+    if (tiles > 0)
+        hipLaunchKernelGGL(k_gram_bf16_part, dim3(tiles, splits), dim3(256), 0, st, Yt, nsp, nA, TA, nB, kc, tiles,
+                           part);
+    const int TS = (np + 1 + 63) / 64;
+    hipLaunchKernelGGL(k_gram_finish, dim3(TS * (TS + 1) / 2), dim3(256), 0, st, part, splits, tiles, nA, TA, nB, offB,
+                       scale, S, np, lds);
     return hipGetLastError();
 }
 

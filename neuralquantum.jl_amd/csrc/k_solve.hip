@@ -138,34 +138,32 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
 // the forward substitution L y = b; y ends in column n of the upper triangle.
 constexpr int CB = 32;
 
-// sqrt(v) and 1/sqrt(v) from the hardware estimate refined by two Goldschmidt
-// steps and one Newton correction of the root (relative error ~1 ulp), a
-// much shorter dependent chain than the IEEE sqrt + divide expansions; v <= 0
-// yields NaN or inf, which the caller reports
-__device__ __forceinline__ void sqrt_rsqrt(double v, double* root, double* rinv) {
-    const double y = __builtin_amdgcn_rsq(v);
-    double g = v * y, h = 0.5 * y;
-    double e = fma(-g, h, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    e = fma(-g, h, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    const double d = fma(-g, g, v);
-    *root = fma(d, h, g);
-    *rinv = 2.0 * h;
+// 1/sqrt(v) from the hardware estimate refined by two Newton steps
+// y <- y (3/2 - (v/2) y^2) (relative error ~1 ulp), a much shorter dependent
+// chain than the IEEE sqrt + divide expansions; v <= 0 yields NaN or inf,
+// which the caller reports
+__device__ __forceinline__ double rsqrt_nr(double v) {
+    const double h = 0.5 * v;
+    double y = __builtin_amdgcn_rsq(v);
+    y = y * fma(-h, y * y, 1.5);
+    y = y * fma(-h, y * y, 1.5);
+    return y;
 }
 
 // factor the kb x kb block at (k, k) (identity padding to 32) in one
 // wavefront: lane t < 32 holds row t of the block, lane 32 + t holds row t of
 // the identity appended below it, so the same column sweep that produces
 // L_kk turns the identity rows into I L_kk^-T = L_kk^-T (the bordering rule
-// L_21 = A_21 L_11^-T with A_21 = I).  Column j: the pivot is read from lane
-// j, the column L[., j] goes through LDS (Lc) to every lane for the
-// right-looking update; the entry the next pivot needs, L[j + 1, j], is read
-// directly from its lane.  Writes L_kk (lower triangle) and L_kk^-T to `w`.
+// L_21 = A_21 L_11^-T with A_21 = I).
+// The sweep runs in sub-panels of 8 columns: inside a sub-panel each column
+// updates only the sub-panel's later columns (values broadcast from their
+// lanes by v_readlane), so the serial chain from one pivot to the next stays
+// short; the rest of the block then takes one rank-8 update, its broadcast
+// operands read from LDS (Lc[column][lane]).  Writes L_kk (lower triangle)
+// and L_kk^-T to `w`.
+constexpr int SUBP = 8;
 __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb, int nmax, double* w, int* info,
-                                              bool report, double (*Lc)[CB]) {
+                                              bool report, double (*Lc)[64]) {
     const int t = threadIdx.x & 63, tr = t & (CB - 1);
     const bool mrow = t < CB;
     double r[CB];
@@ -177,23 +175,39 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
     }
     int bad = 0;   // 1 + first column with a non-positive pivot (branch-free inside the sweep)
 #pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        const double v = rl_d(r[j], j);                      // current A[j][j]
-        bad = (bad == 0 && j < kb && !(v > 0.0)) ? j + 1 : bad;
-        double piv, rp;
-        sqrt_rsqrt(v, &piv, &rp);
-        const double lj = (mrow && tr == j) ? piv : r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
-        r[j] = lj;
-        if (j + 1 < CB) {
-            if (mrow) Lc[j][t] = lj;
+    for (int p0 = 0; p0 < CB; p0 += SUBP) {
+#pragma unroll
+        for (int j = p0; j < p0 + SUBP; ++j) {
+            const double v = rl_d(r[j], j);                      // current A[j][j]
+            bad = (bad == 0 && j < kb && !(v > 0.0)) ? j + 1 : bad;
+            const double rp = rsqrt_nr(v);
+            const double piv = v * rp;
+            const double lj = (mrow && tr == j) ? piv : r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
+            r[j] = lj;
+#pragma unroll
+            for (int m = j + 1; m < p0 + SUBP; ++m) r[m] -= lj * rl_d(lj, m);
+#pragma unroll
+            for (int m = j + 1; m < p0 + SUBP; ++m) asm volatile("" : "+v"(r[m]));
+        }
+        if (p0 + SUBP < CB) {
+#pragma unroll
+            for (int q = p0; q < p0 + SUBP; ++q) Lc[q][t] = r[q];
             __builtin_amdgcn_wave_barrier();
-            r[j + 1] -= lj * rl_d(lj, j + 1);
+            // column pairs, each pinned in place: otherwise the compiler sinks
+            // every column's update to its first use (or hoists all the LDS
+            // reads), holding hundreds of operands live
 #pragma unroll
-            for (int m = j + 2; m < CB; ++m) r[m] -= lj * Lc[j][m];
-            // keep the update right-looking: without this the compiler sinks every
-            // update to the column that consumes it, holding all of L live in registers
+            for (int c = p0 + SUBP; c < CB; c += 2) {
+                double acc0 = r[c], acc1 = r[c + 1];
 #pragma unroll
-            for (int m = j + 1; m < CB; ++m) asm volatile("" : "+v"(r[m]));
+                for (int q = p0; q < p0 + SUBP; ++q) {
+                    acc0 -= r[q] * Lc[q][c];
+                    acc1 -= r[q] * Lc[q][c + 1];
+                }
+                r[c] = acc0;
+                r[c + 1] = acc1;
+                asm volatile("" : "+v"(r[c]), "+v"(r[c + 1]));
+            }
         }
     }
     if (report && bad != 0 && t == 0 && *info == 0) *info = k + bad;
@@ -211,7 +225,7 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
 }
 
 __global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info) {
-    __shared__ double Ls[CB][CB];
+    __shared__ double Ls[CB][64];
     factor_diag32(A, lda, k, kb, n, winv + (size_t)(k / CB) * CB * CB, info, true, Ls);
 }
 
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                                                    int* info) {
     __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
-    __shared__ double Ls[CB][CB];
+    __shared__ double Ls[CB][64];
     int tI, tJ;
     tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
     const int o = k + kb;
@@ -354,27 +368,42 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 }
 
 // ---- substitutions with the stored factor
+// Every load of a step is issued up front (clamped addresses, unused values
+// zeroed), so a step costs about one memory round trip rather than one per
+// guarded load.
+//
 // back step: x_k = L_kk^-T y_k (from winv), then y_i -= sum_m L[k + m, i] x_{k+m}
 // for i < k, the factor read from the upper triangle A[i, k + m] (coalesced in i)
 __global__ __launch_bounds__(256) void k_back_step(const double* A, int lda, int k, int kb, const double* winv,
                                                    double* y, double* x) {
     __shared__ double ys[CB], xs[CB];
     const double* w = winv + (size_t)(k / CB) * CB * CB;
-    if (threadIdx.x < CB) ys[threadIdx.x] = (threadIdx.x < kb) ? y[k + min((int)threadIdx.x, kb - 1)] : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ic = (int64_t)min(i, max(k - 1, 0));
+    double a[CB];
+#pragma unroll
+    for (int m = 0; m < CB; ++m) a[m] = A[ic + (int64_t)(k + min(m, kb - 1)) * lda];
+    const int c = threadIdx.x & (CB - 1);
+    const double yv = y[k + min(c, kb - 1)];
+    double wv[CB];
+    if (threadIdx.x < CB) {
+#pragma unroll
+        for (int m = 0; m < CB; ++m) wv[m] = w[c + CB * m];
+        ys[c] = (c < kb) ? yv : 0.0;
+    }
     __syncthreads();
     if (threadIdx.x < CB) {
-        const int c = threadIdx.x;
         double s = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < CB; ++m) s += w[c + CB * m] * ys[m];
+#pragma unroll
+        for (int m = 0; m < CB; ++m) s += wv[m] * ys[m];
         xs[c] = (c < kb) ? s : 0.0;
         if (blockIdx.x == 0 && c < kb) x[k + c] = s;
     }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
     double acc = 0.0;
-    for (int m = 0; m < kb; ++m) acc += A[(int64_t)i + (int64_t)(k + m) * lda] * xs[m];
+#pragma unroll
+    for (int m = 0; m < CB; ++m) acc += a[m] * xs[m];
     y[i] -= acc;
 }
 
@@ -384,22 +413,32 @@ __global__ __launch_bounds__(256) void k_fwd_step(const double* A, int n, int ld
                                                   double* y, double* z) {
     __shared__ double ys[CB], zs[CB];
     const double* w = winv + (size_t)(k / CB) * CB * CB;
-    if (threadIdx.x < CB) ys[threadIdx.x] = (threadIdx.x < kb) ? y[k + min((int)threadIdx.x, kb - 1)] : 0.0;
+    const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
+    const double* col = A + (int64_t)min(i, n - 1) * lda + k;
+    double a[CB];
+#pragma unroll
+    for (int m = 0; m < CB; ++m) a[m] = col[min(m, kb - 1)];
+    const int c = threadIdx.x & (CB - 1);
+    const double yv = y[k + min(c, kb - 1)];
+    double wv[CB];
+    if (threadIdx.x < CB) {
+#pragma unroll
+        for (int m = 0; m < CB; ++m) wv[m] = w[m + CB * c];   // (L^-1)[c][m] = (L^-T)[m][c]
+        ys[c] = (c < kb) ? yv : 0.0;
+    }
     __syncthreads();
     if (threadIdx.x < CB) {
-        const int c = threadIdx.x;
         double s = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < CB; ++m) s += w[m + CB * c] * ys[m];   // (L^-1)[c][m] = (L^-T)[m][c]
+#pragma unroll
+        for (int m = 0; m < CB; ++m) s += wv[m] * ys[m];
         zs[c] = (c < kb) ? s : 0.0;
         if (blockIdx.x == 0 && c < kb) z[k + c] = s;
     }
     __syncthreads();
-    const int i = k + kb + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double* col = A + (int64_t)i * lda + k;
     double acc = 0.0;
-    for (int m = 0; m < kb; ++m) acc += col[m] * zs[m];
+#pragma unroll
+    for (int m = 0; m < CB; ++m) acc += a[m] * zs[m];
     y[i] -= acc;
 }
 

@@ -73,6 +73,14 @@ hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, in
                               __bf16* Yt, int64_t nsp, hipStream_t st);
 hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds, int o, double scale,
                                 hipStream_t st);
+// split-K BF16 Gram of up to two column blocks of Yt (block A: rows [0, nA),
+// block B: rows [nA, nA + nB)) written into all of the (np + 1)^2 matrix S:
+// scale * A^T A at [0, nA) plus scale * B^T B at [offB, offB + nB), zeros
+// elsewhere (row and column np included); `part` holds
+// nq_gram_split_part_floats(nA, nB, nsp) floats of partial sums
+size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp);
+hipError_t nq_launch_gram2_bf16(const __bf16* Yt, int64_t nsp, int nA, int nB, int offB, double scale, float* part,
+                                double* S, int np, int lds, hipStream_t st);
 
 // column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]
 hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double* part, int chunks,

@@ -58,7 +58,7 @@ struct nqh_ctx {
     int est_grid = 0;
     dbuf Fpart;
     // iteration buffers
-    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp;
+    dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp, gpart;
     int64_t last_ns = 0;
     bool grad_ready = false;
 
@@ -216,6 +216,20 @@ int est_grid_for(int64_t Ns) {
     return (int)g;
 }
 
+// the Gram blocks of the S build: block A = columns [0, nA) of the compact
+// O rows, block B = the next nB columns, embedded in S at offB
+void gram_blocks(const nqh_ctx* c, int* nA, int* nB, int* offB) {
+    if (is_rbm(c)) {
+        *nA = ycols(c);
+        *nB = 0;
+        *offB = 0;
+    } else {
+        *nA = c->d.nlam + c->d.npi;
+        *nB = c->d.npi + c->d.nmu;
+        *offB = c->d.nlam;
+    }
+}
+
 // per-sample buffers for Ns samples
 nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     nqh_status s;
@@ -244,6 +258,9 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if (c->s_prec == NQH_PREC_BF16) {
         const int64_t nsp = (Ns + 63) / 64 * 64;
         if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)ycols(c) * nsp))) return s;
+        int nA, nB, offB;
+        gram_blocks(c, &nA, &nB, &offB);
+        if ((s = ensure(c, c->gpart, sizeof(float) * nq_gram_split_part_floats(nA, nB, nsp)))) return s;
     }
     if ((s = ensure(c, c->flag, sizeof(int) * 4))) return s;
     return NQH_OK;
@@ -365,13 +382,14 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
     phase_end(c, PH_POST);
     phase_begin(c, PH_SBUILD);
     const int ldS = n + 1;
-    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
     if (c->s_prec == NQH_PREC_BF16) {
         const int64_t nsp = (Ns + 63) / 64 * 64;
         __bf16* Yt = reinterpret_cast<__bf16*>(c->Yt.p);
         HIPCHK(c, nq_launch_center_t(c->Y.p, f64, Ns, R, 0, R, P<double>(c->mu), Yt, nsp, c->stream));
-        HIPCHK(c, nq_launch_gram_bf16t(Yt, nsp, R, P<double>(c->S), ldS, 0, 1.0 / (double)Ns, c->stream));
+        HIPCHK(c, nq_launch_gram2_bf16(Yt, nsp, R, 0, 0, 1.0 / (double)Ns, P<float>(c->gpart), P<double>(c->S), n,
+                                       ldS, c->stream));
     } else {
+        HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
         gram_args g{};
         g.Y = c->Y.p;
         g.y_fp64 = f64;
@@ -421,21 +439,22 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     const int np = d.np;
     phase_begin(c, PH_SBUILD);
     const int ldS = np + 1;
-    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
     if (c->s_prec == NQH_PREC_BF16) {
-        // centred BF16 copy with samples contiguous, then the two Gram blocks
+        // centred BF16 copy with samples contiguous, then both Gram blocks in
+        // one split-K launch and one pass that writes all of S
         const int64_t nsp = (Ns + 63) / 64 * 64;
         __bf16* Yt = reinterpret_cast<__bf16*>(c->Yt.p);
         HIPCHK(c, nq_launch_center_t(c->Y.p, f64, Ns, d.R, 0, d.R, P<double>(c->mu), Yt, nsp, c->stream));
-        const int nA = d.nlam + d.npi, nB = d.npi + d.nmu;
-        HIPCHK(c, nq_launch_gram_bf16t(Yt, nsp, nA, P<double>(c->S), ldS, 0, 1.0 / (double)Ns, c->stream));
-        HIPCHK(c, nq_launch_gram_bf16t(Yt + (size_t)nA * nsp, nsp, nB, P<double>(c->S), ldS, d.nlam,
-                                       1.0 / (double)Ns, c->stream));
+        int nA, nB, offB;
+        gram_blocks(c, &nA, &nB, &offB);
+        HIPCHK(c, nq_launch_gram2_bf16(Yt, nsp, nA, nB, offB, 1.0 / (double)Ns, P<float>(c->gpart), P<double>(c->S),
+                                       np, ldS, c->stream));
         phase_end(c, PH_SBUILD);
         c->last_ns = Ns;
         c->grad_ready = true;
         return NQH_OK;
     }
+    HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
     const int mode = c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : NQ_GRAM_F32;
     gram_args g{};
     g.Y = c->Y.p;
@@ -541,7 +560,7 @@ void nqh_destroy(nqh_ctx* c) {
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->made_th, &c->made_m, &c->made_v,
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
                    &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats};
