@@ -5,12 +5,14 @@
 // B = [Im Pi | Im mu]; S in P order is A^T A embedded at [0, nA) plus B^T B
 // embedded at [nlam, np) (the lambda-mu block is identically zero).
 //
-// Each workgroup computes one 64x64 upper-triangular tile of a Gram block;
-// the 4 waves own 32x32 quadrants of 2x2 MFMA 16x16 tiles.  Columns are
-// centred (Y - mean) while staging through LDS, which is what the reference's
-// O - Obar does (BaseIterativeSampler.jl:19-26), before any down-conversion.
+// FP64 / FP32 (k_gram): each workgroup computes one 64x64 upper-triangular
+// tile of a Gram block; the 4 waves own 32x32 quadrants of 2x2 MFMA 16x16
+// tiles.  Columns are centred (Y - mean) while staging through LDS, which is
+// what the reference's O - Obar does (BaseIterativeSampler.jl:19-26).
 //   FP64: v_mfma_f64_16x16x4f64     FP32: v_mfma_f32_16x16x4f32
-//   BF16: v_mfma_f32_16x16x32_bf16 (FP32 accumulation)
+// BF16 (k_center_t + k_gram_bf16_part + k_gram_finish): a centred, transposed
+// BF16 copy, then a split-K Gram of 128x128 tiles with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation) and one pass that writes S.
 #include "nq_kernels.h"
 
 namespace {
@@ -22,7 +24,6 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 template <int MODE> struct gram_traits;
 template <> struct gram_traits<NQ_GRAM_F64> { typedef double st; typedef d4 acc; static constexpr int KC = 16; };
 template <> struct gram_traits<NQ_GRAM_F32> { typedef float st; typedef f4 acc; static constexpr int KC = 16; };
-template <> struct gram_traits<NQ_GRAM_BF16> { typedef __bf16 st; typedef f4 acc; static constexpr int KC = 32; };
 
 __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
     int j = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
@@ -37,11 +38,11 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
     typedef typename gram_traits<MODE>::st ST;
     typedef typename gram_traits<MODE>::acc AT;
     constexpr int KC = gram_traits<MODE>::KC;
-    // f64/f32: [k][64 + 1] (k-major, rows contiguous); bf16: [row][KC + 8] (k contiguous)
-    constexpr int PITCH = (MODE == NQ_GRAM_BF16) ? (KC + 8) : 65;
-    constexpr int TILE = (MODE == NQ_GRAM_BF16) ? 64 * PITCH : KC * PITCH;
-    __shared__ ST As[TILE];
-    __shared__ ST Bs[TILE];
+    constexpr int PITCH = 65;                 // [k][64 + 1]: k-major, rows contiguous
+    constexpr int TILE = KC * PITCH;
+    constexpr int NJ = KC / 4;                // sample rows staged per thread and k-step
+    __shared__ ST As[2 * TILE];               // double-buffered: one barrier per k-step
+    __shared__ ST Bs[2 * TILE];
 
     int I, J;
     tri_decode(blockIdx.x, &I, &J);
@@ -65,69 +66,59 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
     const bool va = ca < a.n, vb = cb < a.n;
     const double mua = (va && a.mu) ? a.mu[a.c0 + ca] : 0.0;
     const double mub = (vb && a.mu) ? a.mu[a.c0 + cb] : 0.0;
-
-    for (int64_t k0 = 0; k0 < a.Ns; k0 += KC) {
+    double ya[NJ], yb[NJ];
+    // unconditional loads from clamped addresses (one k-step ahead), centred
+    // and selected when stored
+    auto load = [&](int64_t k0) {
 #pragma unroll
-        for (int kk = tid >> 6; kk < KC; kk += 4) {
-            const int64_t s = k0 + kk;
-            // unconditional loads from clamped addresses, selected afterwards
+        for (int j = 0; j < NJ; ++j) {
+            const int64_t s = k0 + (tid >> 6) + 4 * j;
             const YT* row = Y + (s < a.Ns ? s : a.Ns - 1) * (int64_t)a.ldy + a.c0;
-            const double ya = (double)row[va ? ca : 0], yb = (double)row[vb ? cb : 0];
-            const double xa = (s < a.Ns && va) ? ya - mua : 0.0;
-            const double xb = (s < a.Ns && vb) ? yb - mub : 0.0;
-            if constexpr (MODE == NQ_GRAM_BF16) {
-                As[lc * PITCH + kk] = (__bf16)(float)xa;
-                Bs[lc * PITCH + kk] = (__bf16)(float)xb;
-            } else {
-                As[kk * PITCH + lc] = (ST)xa;
-                Bs[kk * PITCH + lc] = (ST)xb;
-            }
+            ya[j] = (double)row[va ? ca : 0];
+            yb[j] = (double)row[vb ? cb : 0];
         }
-        __syncthreads();
-        const int li = lane & 15, lk = lane >> 4;
-        if constexpr (MODE == NQ_GRAM_BF16) {
-            bf8 av[2], bv[2];
+    };
+    auto store = [&](int buf, int64_t k0) {
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                const int row = wr * 32 + m * 16 + li;
+        for (int j = 0; j < NJ; ++j) {
+            const int kk = (tid >> 6) + 4 * j;
+            const int64_t s = k0 + kk;
+            As[buf * TILE + kk * PITCH + lc] = (ST)((s < a.Ns && va) ? ya[j] - mua : 0.0);
+            Bs[buf * TILE + kk * PITCH + lc] = (ST)((s < a.Ns && vb) ? yb[j] - mub : 0.0);
+        }
+    };
+    const int64_t nsteps = (a.Ns + KC - 1) / KC;
+    load(0);
+    store(0, 0);
+    if (nsteps > 1) load(KC);
+    __syncthreads();
+    const int li = lane & 15, lk = lane >> 4;
+    for (int64_t i = 0; i < nsteps; ++i) {
+        if (i + 1 < nsteps) store((int)((i + 1) & 1), (i + 1) * KC);
+        if (i + 2 < nsteps) load((i + 2) * KC);
+        const ST* A = As + (i & 1) * TILE;
+        const ST* B = Bs + (i & 1) * TILE;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) av[m][e] = As[row * PITCH + lk * 8 + e];
-            }
+        for (int k = 0; k < KC; k += 4) {
+            ST av[2], bv[2];
 #pragma unroll
-            for (int n = 0; n < 2; ++n) {
-                const int col = wc * 32 + n * 16 + li;
+            for (int m = 0; m < 2; ++m) av[m] = A[(k + lk) * PITCH + wr * 32 + m * 16 + li];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) bv[n][e] = Bs[col * PITCH + lk * 8 + e];
-            }
+            for (int n = 0; n < 2; ++n) bv[n] = B[(k + lk) * PITCH + wc * 32 + n * 16 + li];
 #pragma unroll
             for (int m = 0; m < 2; ++m)
 #pragma unroll
-                for (int n = 0; n < 2; ++n)
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
-        } else {
-#pragma unroll
-            for (int k = 0; k < KC; k += 4) {
-                ST av[2], bv[2];
-#pragma unroll
-                for (int m = 0; m < 2; ++m) av[m] = As[(k + lk) * PITCH + wr * 32 + m * 16 + li];
-#pragma unroll
-                for (int n = 0; n < 2; ++n) bv[n] = Bs[(k + lk) * PITCH + wc * 32 + n * 16 + li];
-#pragma unroll
-                for (int m = 0; m < 2; ++m)
-#pragma unroll
-                    for (int n = 0; n < 2; ++n) {
-                        if constexpr (MODE == NQ_GRAM_F64)
-                            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
-                        else
-                            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bv[n], acc[m][n], 0, 0, 0);
-                    }
-            }
+                for (int n = 0; n < 2; ++n) {
+                    if constexpr (MODE == NQ_GRAM_F64)
+                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
+                    else
+                        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bv[n], acc[m][n], 0, 0, 0);
+                }
         }
         __syncthreads();
     }
 
-    // D[i][j] of a 16x16 tile, j = lane % 16:
-    //   f32 / bf16: i = 4 (lane / 16) + v;   f64: i = lane / 16 + 4 v
+    // D[i][j] of a 16x16 tile, j = lane % 16:  f32: i = 4 (lane / 16) + v;  f64: i = lane / 16 + 4 v
     const int lj = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int m = 0; m < 2; ++m)
@@ -172,79 +163,6 @@ __global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int l
         const int64_t si = sb + tx;
         if (cc < n && si < nsp) Yt[(int64_t)cc * nsp + si] = (__bf16)tile[tx][ty + 4 * r];
     }
-}
-
-// S[o + r, o + c] += scale * sum_k Yt[r][k] Yt[c][k] over upper 128x128 tiles;
-// 4 waves in 2x2, each 64x64 = 4x4 tiles of v_mfma_f32_16x16x32_bf16.
-__global__ __launch_bounds__(256) void k_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds,
-                                                    int o, double scale) {
-    constexpr int KS = 32, PITCH = 40;   // bf16 per LDS row (80 B: 16-byte aligned, staggered banks)
-    __shared__ __align__(16) __bf16 As[128 * PITCH];
-    __shared__ __align__(16) __bf16 Bs[128 * PITCH];
-    int I, J;
-    tri_decode(blockIdx.x, &I, &J);
-    const int r0 = I * 128, q0 = J * 128;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int li = lane & 15, lk = lane >> 4;
-
-    f4 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.f;
-
-    // staging: 128 rows x 4 chunks of 16 B per operand = 512 chunks, 2 per thread
-    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-    const int row0 = tid >> 2, ch = tid & 3;            // rows row0 and row0 + 64
-    auto gload = [&](int base, int row, int64_t k0) -> u4 {
-        const int rr = base + row;
-        if (rr >= n) return u4{0u, 0u, 0u, 0u};
-        return *reinterpret_cast<const u4*>(Yt + (int64_t)rr * nsp + k0 + ch * 8);
-    };
-    u4 pa0 = gload(r0, row0, 0), pa1 = gload(r0, row0 + 64, 0);
-    u4 pb0 = gload(q0, row0, 0), pb1 = gload(q0, row0 + 64, 0);
-    for (int64_t k0 = 0; k0 < nsp; k0 += KS) {
-        __syncthreads();
-        *reinterpret_cast<u4*>(&As[row0 * PITCH + ch * 8]) = pa0;
-        *reinterpret_cast<u4*>(&As[(row0 + 64) * PITCH + ch * 8]) = pa1;
-        *reinterpret_cast<u4*>(&Bs[row0 * PITCH + ch * 8]) = pb0;
-        *reinterpret_cast<u4*>(&Bs[(row0 + 64) * PITCH + ch * 8]) = pb1;
-        __syncthreads();
-        if (k0 + KS < nsp) {   // prefetch the next k-step while the MFMAs run
-            pa0 = gload(r0, row0, k0 + KS); pa1 = gload(r0, row0 + 64, k0 + KS);
-            pb0 = gload(q0, row0, k0 + KS); pb1 = gload(q0, row0 + 64, k0 + KS);
-        }
-        bf8 av[4], bv[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-            av[a] = *reinterpret_cast<const bf8*>(&As[(wr * 64 + a * 16 + li) * PITCH + lk * 8]);
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            bv[b] = *reinterpret_cast<const bf8*>(&Bs[(wc * 64 + b * 16 + li) * PITCH + lk * 8]);
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
-    const int lj = lane & 15, lg = lane >> 4;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = r0 + wr * 64 + a * 16 + 4 * lg + v;
-                const int c = q0 + wc * 64 + b * 16 + lj;
-                if (r < n && c < n) {
-                    const double val = scale * (double)acc[a][b][v];
-                    S[(int64_t)(o + r) + (int64_t)(o + c) * lds] += val;
-                    if (I != J) S[(int64_t)(o + c) + (int64_t)(o + r) * lds] += val;
-                }
-            }
 }
 
 // ---- split-K BF16 Gram.  Pass 1: each workgroup takes one 128x128 upper
@@ -444,7 +362,6 @@ hipError_t nq_launch_gram(const gram_args& a, int mode, hipStream_t st) {
     switch (mode) {
         case NQ_GRAM_F64: return launch_mode<NQ_GRAM_F64>(a, st);
         case NQ_GRAM_F32: return launch_mode<NQ_GRAM_F32>(a, st);
-        case NQ_GRAM_BF16: return launch_mode<NQ_GRAM_BF16>(a, st);
     }
     return hipErrorInvalidValue;
 }
@@ -456,15 +373,6 @@ hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, in
         hipLaunchKernelGGL((k_center_t<double>), grid, dim3(256), 0, st, (const double*)Y, Ns, ldy, c0, n, mu, Yt, nsp);
     else
         hipLaunchKernelGGL((k_center_t<float>), grid, dim3(256), 0, st, (const float*)Y, Ns, ldy, c0, n, mu, Yt, nsp);
-    return hipGetLastError();
-}
-
-hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds, int o, double scale,
-                                hipStream_t st) {
-    const int T = (n + 127) / 128;
-    const int blocks = T * (T + 1) / 2;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gram_bf16t, dim3(blocks), dim3(256), 0, st, Yt, nsp, n, S, lds, o, scale);
     return hipGetLastError();
 }
 

@@ -51,7 +51,7 @@ int nq_sampler_lanes_per_chain(const ndm_dims& d);
 hipError_t nq_launch_metropolis(const samp_args& a, bool fp64, hipStream_t st);
 
 // ---- S build (k_sbuild.hip)
-enum { NQ_GRAM_F64 = 0, NQ_GRAM_F32 = 1, NQ_GRAM_BF16 = 2 };
+enum { NQ_GRAM_F64 = 0, NQ_GRAM_F32 = 1 };
 // S[o + r, o + c] += scale * sum_s (Y[s, c0 + r] - mu[c0 + r]) (Y[s, c0 + c] - mu[c0 + c])
 // for r, c < n; S is column-major with leading dimension lds.  Y is Ns x ldy of
 // type double (y_fp64) or float.
@@ -68,11 +68,9 @@ struct gram_args {
 hipError_t nq_launch_gram(const gram_args& a, int mode, hipStream_t st);
 
 // BF16 S build: Yt[c][s] = bf16(Y[s, c0 + c] - mu[c0 + c]) for c < n, s < nsp
-// (zero beyond Ns; nsp a multiple of 64), then S[o + r, o + c] += scale Yt Yt^T
+// (zero beyond Ns; nsp a multiple of 64), then the split-K Gram below
 hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, int c0, int n, const double* mu,
                               __bf16* Yt, int64_t nsp, hipStream_t st);
-hipError_t nq_launch_gram_bf16t(const __bf16* Yt, int64_t nsp, int n, double* S, int lds, int o, double scale,
-                                hipStream_t st);
 // split-K BF16 Gram of up to two column blocks of Yt (block A: rows [0, nA),
 // block B: rows [nA, nA + nB)) written into all of the (np + 1)^2 matrix S:
 // scale * A^T A at [0, nA) plus scale * B^T B at [offB, offB + nB), zeros
