@@ -190,6 +190,27 @@ nqh_status nqh_sample_grad(nqh_ctx* ctx, double* stats);
  * delta_out (nullable): Np, reference order */
 nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double* delta_out);
 
+/* SR solver, the reference's SR(algorithm=...) (SR.jl:37-57):
+ *   NQH_SOLVER_CHOLESKY  sr_cholesky (SRDirect.jl:26-90): S built on the
+ *                        matrix cores and factored (default);
+ *   NQH_SOLVER_CG        sr_cg through SRIterative.jl:71-152 with the
+ *                        matrix-free S of SrMatrixR (SR_notfull.jl:84-148):
+ *                        S is never formed (no Np^2 memory); conjugate
+ *                        gradients on S v = Re(Oc^H Oc v)/Ns + shift, two
+ *                        passes over the O rows per product, until
+ *                        ||r|| <= tol ||F|| or maxiter products (0: 10 Np,
+ *                        the reference's limit); without convergence the
+ *                        step is zero (SRIterative.jl:134-150).  NDM networks
+ *                        only; the solve synchronises with the host once per
+ *                        product, so iterations with it are not graph-captured
+ *                        and are not batched.
+ * Takes effect from the next gradient (nqh_sample_grad / grad_from_samples). */
+#define NQH_SOLVER_CHOLESKY 0
+#define NQH_SOLVER_CG 1
+nqh_status nqh_set_sr_solver(nqh_ctx* ctx, int32_t solver, double tol, int32_t maxiter);
+/* products and final relative residual ||r|| / ||F|| of the last CG solve */
+nqh_status nqh_get_sr_solver_info(nqh_ctx* ctx, int32_t* iters, double* rel_resid);
+
 /* Optimisers.update!(Descent(lr), net, Delta) (apply.jl:25-31, rules.jl:15-17) */
 nqh_status nqh_update(nqh_ctx* ctx, double lr);
 

@@ -80,6 +80,25 @@ size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp);
 hipError_t nq_launch_gram2_bf16(const __bf16* Yt, int64_t nsp, int nA, int nB, int offB, double scale, float* part,
                                 double* S, int np, int lds, hipStream_t st);
 
+// matrix-free SR (k_cg.hip): q = S v + shift(v) with S = Re(Oc^H Oc) / Ns from
+// the compact rows Y (block A = columns [0, nA) -> parameters [0, nA), block
+// B = columns [nA, R) -> parameters [offB, offB + R - nA)); w: 2 Ns doubles,
+// part: chunks x R doubles; kind 0: eps v, 1: eps diag(S) v (diag from
+// nq_launch_sdiag)
+hipError_t nq_launch_sv(const void* Y, int y_fp64, int64_t Ns, int R, int nA, int offB, int np, const double* mu,
+                        const double* v, double eps, int kind, const double* diag, double* w, double* part,
+                        int chunks, double* q, hipStream_t st);
+hipError_t nq_launch_sdiag(const void* Y, int y_fp64, int64_t Ns, int R, int nA, int offB, int np, const double* mu,
+                           double* part, int chunks, double* diag, hipStream_t st);
+// *out = x . y (part: nparts doubles)
+hipError_t nq_launch_dot(const double* x, const double* y, int n, double* part, int nparts, double* out,
+                         hipStream_t st);
+// CG steps with the scalars in device memory sc = [r.r, p.q, new r.r]
+hipError_t nq_launch_cg_xr(double* x, double* r, const double* p, const double* q, int n, const double* sc,
+                           hipStream_t st);
+hipError_t nq_launch_cg_p(double* p, const double* r, int n, const double* sc, hipStream_t st);
+hipError_t nq_launch_copy_scalar(const double* src, double* dst, hipStream_t st);
+
 // column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]
 hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double* part, int chunks,
                             hipStream_t st);

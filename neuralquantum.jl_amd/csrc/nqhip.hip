@@ -59,6 +59,15 @@ struct nqh_ctx {
     dbuf Fpart;
     // iteration buffers
     dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp, gpart;
+
+    // SR solver: NQH_SOLVER_CHOLESKY (S built and factored) or NQH_SOLVER_CG
+    // (matrix-free conjugate gradients on the O rows, S never formed)
+    int solver = NQH_SOLVER_CHOLESKY;
+    double cg_tol = 1e-4;
+    int cg_maxiter = 0;
+    int cg_iters = 0;
+    double cg_resid = 0.0;
+    dbuf cg_w, cg_part, cg_p, cg_q, cg_r, cg_sc, cg_diag, cg_dpart;
     int64_t last_ns = 0;
     bool grad_ready = false;
 
@@ -245,17 +254,20 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->stats, sizeof(double) * 8))) return s;
     if ((s = ensure(c, c->F, sizeof(double) * nsys(c)))) return s;
     // S has one extra row and column: row np carries the right-hand side
-    // through the factorization (forward substitution for free)
+    // through the factorization (forward substitution for free); the
+    // matrix-free solver never forms it
     const size_t ldS = (size_t)nsys(c) + 1;
-    if ((s = ensure(c, c->S, sizeof(double) * ldS * ldS))) return s;
-    if ((s = ensure(c, c->S0, sizeof(double) * ldS * ldS))) return s;
+    if (c->solver == NQH_SOLVER_CHOLESKY) {
+        if ((s = ensure(c, c->S, sizeof(double) * ldS * ldS))) return s;
+        if ((s = ensure(c, c->S0, sizeof(double) * ldS * ldS))) return s;
+    }
     if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * nsys(c)))) return s;
     if ((s = ensure(c, c->delta, sizeof(double) * nsys(c)))) return s;
     if ((s = ensure(c, c->resid, sizeof(double) * nsys(c)))) return s;
     if ((s = ensure(c, c->tmp, sizeof(double) * nsys(c)))) return s;
     if ((s = ensure(c, c->winv, sizeof(double) * nq_cholesky_winv_doubles(nsys(c))))) return s;
     if ((s = ensure(c, c->info, sizeof(int) * 4))) return s;
-    if (c->s_prec == NQH_PREC_BF16) {
+    if (c->s_prec == NQH_PREC_BF16 && c->solver == NQH_SOLVER_CHOLESKY) {
         const int64_t nsp = (Ns + 63) / 64 * 64;
         if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)ycols(c) * nsp))) return s;
         int nA, nB, offB;
@@ -434,6 +446,11 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
                                      P<double>(c->stats), c->d, P<double>(c->F), c->stream));
     phase_end(c, PH_POST);
+    if (c->solver == NQH_SOLVER_CG) {   // matrix-free: the solve works on Y and mu directly
+        c->last_ns = Ns;
+        c->grad_ready = true;
+        return NQH_OK;
+    }
     // S in P order: A-gram at [0, nA), B-gram at [nlam, np)
     const ndm_dims& d = c->d;
     const int np = d.np;
@@ -560,7 +577,8 @@ void nqh_destroy(nqh_ctx* c) {
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->made_th, &c->made_m, &c->made_v,
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->cg_w,
+                   &c->cg_part, &c->cg_p, &c->cg_q, &c->cg_r, &c->cg_sc, &c->cg_diag, &c->cg_dpart, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
                    &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats};
@@ -949,6 +967,8 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
     (void)hipSetDevice(c->device);
     nqh_status s;
     if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    if (S && c->solver != NQH_SOLVER_CHOLESKY)
+        return fail(c, NQH_E_STATE, "S is not formed with the matrix-free solver (pass S = NULL)");
     const int64_t Ns = chains * Lc;
     if ((s = check_keys(c, keys, Ns))) return s;
     if ((s = ensure_samples(c, Ns))) return s;
@@ -1167,10 +1187,84 @@ nqh_status nqh_sample_grad(nqh_ctx* c, double* stats) {
     return NQH_OK;
 }
 
+// matrix-free conjugate gradients on (S + shift) x = F (SRIterative.jl:71-152
+// with sr_cg; S v from SrMatrixR, SR_notfull.jl:123-148).  Converged when
+// ||r|| <= tol ||F|| (IterativeSolvers' relative-residual rule).  Without
+// convergence in maxiter products the step is zero, as the reference sets
+// Delta w = 0 after its retries (SRIterative.jl:134-150).  Synchronises with
+// the host once per product (the convergence test).
+static nqh_status cg_solve(nqh_ctx* c, double eps, int kind) {
+    if (is_rbm(c)) return fail(c, NQH_E_STATE, "the matrix-free CG solver supports NDM networks only");
+    const ndm_dims& d = c->d;
+    const int np = d.np, R = d.R;
+    const int64_t Ns = c->last_ns;
+    const bool f64 = c->net_prec == NQH_PREC_FP64;
+    int nA, nB, offB;
+    gram_blocks(c, &nA, &nB, &offB);
+    constexpr int chunks = 64, nparts = 64;
+    nqh_status s;
+    if ((s = ensure(c, c->cg_w, sizeof(double) * 2 * (size_t)Ns)) ||
+        (s = ensure(c, c->cg_part, sizeof(double) * (size_t)chunks * R)) ||
+        (s = ensure(c, c->cg_p, sizeof(double) * np)) || (s = ensure(c, c->cg_q, sizeof(double) * np)) ||
+        (s = ensure(c, c->cg_r, sizeof(double) * np)) || (s = ensure(c, c->cg_diag, sizeof(double) * np)) ||
+        (s = ensure(c, c->cg_sc, sizeof(double) * 8)) || (s = ensure(c, c->cg_dpart, sizeof(double) * nparts)))
+        return s;
+    double *x = P<double>(c->delta), *r = P<double>(c->cg_r), *p = P<double>(c->cg_p), *q = P<double>(c->cg_q);
+    double* sc = P<double>(c->cg_sc);
+    double* diag = P<double>(c->cg_diag);
+    const double* mu = P<double>(c->mu);
+    const double* F = P<double>(c->F);
+    hipStream_t st = c->stream;
+    HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    HIPCHK(c, hipMemcpyAsync(r, F, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(p, F, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    if (kind == NQH_SHIFT_DIAG)
+        HIPCHK(c, nq_launch_sdiag(c->Y.p, f64, Ns, R, nA, offB, np, mu, P<double>(c->cg_part), chunks, diag, st));
+    HIPCHK(c, nq_launch_dot(F, F, np, P<double>(c->cg_dpart), nparts, sc, st));
+    double f2 = 0.0, rr = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&f2, sc, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int maxit = c->cg_maxiter > 0 ? c->cg_maxiter : 10 * np;
+    const double target = c->cg_tol * c->cg_tol * f2;
+    int it = 0;
+    rr = f2;
+    bool ok = f2 == 0.0;
+    while (!ok && it < maxit) {
+        HIPCHK(c, nq_launch_sv(c->Y.p, f64, Ns, R, nA, offB, np, mu, p, eps, kind, diag, P<double>(c->cg_w),
+                               P<double>(c->cg_part), chunks, q, st));
+        HIPCHK(c, nq_launch_dot(p, q, np, P<double>(c->cg_dpart), nparts, sc + 1, st));
+        HIPCHK(c, nq_launch_cg_xr(x, r, p, q, np, sc, st));
+        HIPCHK(c, nq_launch_dot(r, r, np, P<double>(c->cg_dpart), nparts, sc + 2, st));
+        HIPCHK(c, hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        ++it;
+        if (!std::isfinite(rr)) break;
+        if (rr <= target) {
+            ok = true;
+            break;
+        }
+        HIPCHK(c, nq_launch_cg_p(p, r, np, sc, st));
+        HIPCHK(c, nq_launch_copy_scalar(sc + 2, sc, st));
+    }
+    c->cg_iters = it;
+    c->cg_resid = f2 > 0.0 ? std::sqrt(rr / f2) : 0.0;
+    if (!ok && std::isfinite(rr)) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
+    HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
+    return NQH_OK;
+}
+
 static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine) {
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
+    if (c->solver == NQH_SOLVER_CG) {
+        phase_begin(c, PH_SOLVE);
+        const nqh_status s = cg_solve(c, eps, kind);
+        phase_end(c, PH_SOLVE);
+        return s;
+    }
     const int np = nsys(c), ldS = np + 1;
     double* S = P<double>(c->S);
     phase_begin(c, PH_SOLVE);
@@ -1312,6 +1406,27 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
     return NQH_OK;
 }
 
+nqh_status nqh_set_sr_solver(nqh_ctx* c, int32_t solver, double tol, int32_t maxiter) {
+    if (!c) return NQH_E_ARG;
+    if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG) return fail(c, NQH_E_ARG, "unknown solver");
+    if (solver == NQH_SOLVER_CG && !(tol > 0.0)) return fail(c, NQH_E_ARG, "tol must be > 0");
+    if (maxiter < 0) return fail(c, NQH_E_ARG, "maxiter must be >= 0");
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    c->solver = solver;
+    c->cg_tol = tol;
+    c->cg_maxiter = maxiter;
+    c->grad_ready = false;   // the next gradient is prepared for this solver
+    return NQH_OK;
+}
+
+nqh_status nqh_get_sr_solver_info(nqh_ctx* c, int32_t* iters, double* rel_resid) {
+    if (!c) return NQH_E_ARG;
+    if (iters) *iters = c->cg_iters;
+    if (rel_resid) *rel_resid = c->cg_resid;
+    return NQH_OK;
+}
+
 nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
@@ -1323,7 +1438,7 @@ nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
 // enqueue one SR iteration on the context's stream without waiting for it
 static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
     if (nqh_status sm = made_maybe_retrain(c)) return sm;
-    if (c->use_graph && !c->profiling) {
+    if (c->use_graph && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY) {
         nqh_status s0 = need_net(c);
         if (!s0) s0 = need_problem(c);
         if (!s0 && !c->has_sampler) s0 = fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
@@ -1533,6 +1648,8 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
         nqh_ctx* c = b->m[i];
         if (c->made_ready && c->made_mix > 0.0)
             return bfail(b, NQH_E_STATE, "member " + std::to_string(i) + ": learned proposals are not batched");
+        if (c->solver != NQH_SOLVER_CHOLESKY)
+            return bfail(b, NQH_E_STATE, "member " + std::to_string(i) + ": the iterative solver is not batched");
         if ((c->flags & NQH_SAMPLER_PERSISTENT) && !c->chains_initialised) eager = true;
     }
     if (eager) {

@@ -24,6 +24,7 @@ NQH_E_CAPACITY = -7
 
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 SHIFT_ADD, SHIFT_DIAG = 0, 1
+SOLVER_CHOLESKY, SOLVER_CG = 0, 1
 SAMPLER_COMPAT_STALE_LAST_SLICE = 1
 SAMPLER_PERSISTENT = 2
 
@@ -88,6 +89,8 @@ SIGNATURES = {
     "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
     "nqh_sr_iteration_async": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double]),
     "nqh_wait": (C.c_int32, [C.c_void_p, _dp]),
+    "nqh_set_sr_solver": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
+    "nqh_get_sr_solver_info": (C.c_int32, [C.c_void_p, _i32p, _dp]),
     "nqh_batch_create": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32]),
     "nqh_batch_destroy": (None, [C.c_void_p]),
     "nqh_batch_last_error": (C.c_char_p, [C.c_void_p]),
@@ -349,6 +352,14 @@ class Context:
         keys = np.empty(self.chains * self.chain_length, dtype=np.uint64)
         self._chk(self.lib.nqh_get_samples(self.h, _p(keys, _u64p), len(keys)))
         return keys
+
+    def set_sr_solver(self, solver=SOLVER_CHOLESKY, tol=1e-4, maxiter=0):
+        self._chk(self.lib.nqh_set_sr_solver(self.h, solver, tol, maxiter))
+
+    def sr_solver_info(self):
+        it, res = C.c_int32(0), C.c_double(0.0)
+        self._chk(self.lib.nqh_get_sr_solver_info(self.h, C.byref(it), C.byref(res)))
+        return int(it.value), float(res.value)
 
     def set_graph(self, on=True):
         self._chk(self.lib.nqh_set_graph(self.h, 1 if on else 0))

@@ -112,20 +112,30 @@ class MetropolisSampler:
 
 
 class SR:
-    """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift)`
-    (SR.jl:50-57).  `precondition_type` "shift" adds eps*I (reference);
-    "diag" adds eps*diag(S).  `refine` extra FP64 refinement steps."""
+    """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift,
+    precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky" (SRDirect.jl) or
+    "cg" (SRIterative.jl with the matrix-free S of SR_notfull.jl: S is never
+    formed; `precision` is the relative-residual tolerance, `maxiter` 0 means
+    the reference's 10 Np).  `precondition_type` "shift" adds eps*I
+    (reference); "diag" adds eps*diag(S).  `refine` extra FP64 refinement
+    steps of the Cholesky solve."""
 
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
-                 refine: int = 0):
-        if algorithm != "cholesky":
-            raise NotImplementedError("only sr_cholesky (the default) is implemented")
+                 refine: int = 0, precision: float = 1e-4, maxiter: int = 0):
+        if algorithm not in ("cholesky", "cg"):
+            raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky) or 'cg' (sr_cg)")
         if precondition_type not in ("shift", "diag"):
             raise ValueError("precondition_type must be 'shift' or 'diag'")
         self.eps = float(eps)
         self.algorithm = algorithm
         self.precondition_type = precondition_type
         self.refine = int(refine)
+        self.precision = float(precision)
+        self.maxiter = int(maxiter)
+
+    @property
+    def solver(self):
+        return _lib.SOLVER_CHOLESKY if self.algorithm == "cholesky" else _lib.SOLVER_CG
 
     @property
     def shift_kind(self):
@@ -182,6 +192,7 @@ class HIPBatchedSampler:
         self.ctx.set_precision(*_PREC[precision])
         self.ctx.set_params(net.flat)
         self.ctx.set_sampler(self.batch_sz, sampler.chain_length, sampler.passes, sampler.burn, sampler.flags)
+        self.ctx.set_sr_solver(algo.solver, algo.precision, algo.maxiter)
         self._pending = False
 
     @property

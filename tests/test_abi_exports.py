@@ -58,3 +58,14 @@ def test_ndm_flat_layout_and_init():
     assert net.u_la[-1, -1] == net.n_params - 1
     net.init_random_pars(np.random.default_rng(0), sigma=1e-4)
     assert abs(np.std(net.flat) - 1e-2) < 3e-3
+
+
+def test_sr_mirror_algorithms():
+    """SR(algorithm=...) accepts the two solvers built (sr_cholesky, sr_cg)
+    and maps them to the ABI's solver codes; others are refused."""
+    from nqhip import SR
+    assert SR(0.01).solver == _lib.SOLVER_CHOLESKY
+    cg = SR(0.01, algorithm="cg", precision=1e-6)
+    assert cg.solver == _lib.SOLVER_CG and cg.precision == 1e-6 and cg.maxiter == 0
+    with pytest.raises(NotImplementedError):
+        SR(0.01, algorithm="qlp")
