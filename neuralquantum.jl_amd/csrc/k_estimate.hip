@@ -471,10 +471,13 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                     rho2[r] += vp.im;
                 }
             }
-            // dense parts: x_s times the per-unit sums
+            // dense parts: x_s times the per-unit sums (the opaque copy of the
+            // unit index keeps the 16 addresses per unit from being hoisted
+            // out of the connection loop and spilled)
 #pragma unroll
             for (int r = 0; r < RS; ++r) {
-                const int u = lane + 64 * r;
+                int u = lane + 64 * r;
+                asm volatile("" : "+v"(u));
                 if (u < 2 * nh) {
                     const bool lam = u < nh;
                     const int h = lam ? u : u - nh;
@@ -488,7 +491,8 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
             }
 #pragma unroll
             for (int r = 0; r < PS; ++r) {
-                const int q = lane + 64 * r;
+                int q = lane + 64 * r;
+                asm volatile("" : "+v"(q));
                 if (q < na) {
                     fadd(PB_d + q, (double)rho1[r]);
 #pragma unroll
