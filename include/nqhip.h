@@ -250,9 +250,10 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* batch, double eps, int32_t shift_ki
 /* device time per phase, measured with HIP events on the context's stream
  * while profiling is on (accumulated, in ms; counts = timed occurrences):
  * [0] sampler  [1] estimator (log rho, O, L_loc, force)  [2] means, statistics,
- * force finish  [3] S build  [4] shift + Cholesky + solves  [5] update.
+ * force finish  [3] S build  [4] shift + Cholesky + solves  [5] update
+ * [6] the Cholesky factorization alone (inside [4]).
  * No reference counterpart (the reference has no timers). */
-#define NQH_NUM_PHASES 6
+#define NQH_NUM_PHASES 7
 nqh_status nqh_set_profiling(nqh_ctx* ctx, int32_t on);
 nqh_status nqh_get_phase_times(nqh_ctx* ctx, double* ms_out, int64_t* counts_out);
 

@@ -124,7 +124,7 @@ nqh_status hip_fail(nqh_ctx* c, hipError_t e, const char* where) {
         if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
     } while (0)
 
-enum { PH_SAMPLER = 0, PH_ESTIMATE, PH_POST, PH_SBUILD, PH_SOLVE, PH_UPDATE };
+enum { PH_SAMPLER = 0, PH_ESTIMATE, PH_POST, PH_SBUILD, PH_SOLVE, PH_UPDATE, PH_FACTOR };
 
 void phase_begin(nqh_ctx* c, int ph) {
     if (c->profiling) (void)hipEventRecord(c->ev[ph][0], c->stream);
@@ -1274,7 +1274,9 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, c->stream));
     HIPCHK(c, nq_set_aug_row(S, np, ldS, P<double>(c->F), c->stream));
     // the factorization also forward-substitutes F (carried as row np)
+    phase_begin(c, PH_FACTOR);
     HIPCHK(c, nq_cholesky(S, np, ldS, P<int>(c->info), P<double>(c->winv), c->stream));
+    phase_end(c, PH_FACTOR);
     HIPCHK(c, nq_copy_aug_col(S, np, ldS, P<double>(c->resid), c->stream));
     HIPCHK(c, nq_cholesky_backsolve(S, np, ldS, P<double>(c->winv), P<double>(c->resid), P<double>(c->delta),
                                     c->stream));

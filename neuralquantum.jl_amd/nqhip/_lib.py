@@ -107,8 +107,8 @@ SIGNATURES = {
     "nqh_made_stats": (C.c_int32, [C.c_void_p, _i64p, _i64p, _i64p]),
 }
 
-NUM_PHASES = 6
-PHASES = ("sampler", "estimator", "post", "sbuild", "solve", "update")
+NUM_PHASES = 7
+PHASES = ("sampler", "estimator", "post", "sbuild", "solve", "update", "factor")
 
 _lib = None
 
