@@ -170,6 +170,75 @@ def time_instances(steps, warmup, cfg=C5):
             "max_rel_loss_diff_batch_vs_sequential": d}
 
 
+C4 = dict(mix=0.5, retrain_every=20, train_steps=50, batch=1024, lr=1e-3, buffer=65536)
+
+
+def learned_proposal(steps, cfg=CFG, c4=C4):
+    """The learned-proposal workload: the chosen form on the 4x4 XYZ NDM with a
+    global MADE move before half of the stored slices (retrained every 20 SR
+    iterations: 50 Adam steps, batch 1024, lr 1e-3, on the last 65536
+    samples), next to the same run without it; then, at the parameters
+    reached, tau_int of log p along chains that make one local proposal per
+    stored sample, without and with a move of the trained proposal before
+    every stored sample (1024 chains x 48 slices)."""
+    eps, lr = cfg["eps"], cfg["lr"]
+    warm = 2 * c4["retrain_every"]           # two retrainings before the timed region
+    out = {}
+    for name in ("local_only", "learned"):
+        ctx, _ = build_ctx("chosen", cfg)
+        if name == "learned":
+            ctx.made_setup(c4["mix"], c4["retrain_every"], c4["train_steps"], c4["batch"], c4["lr"], c4["buffer"])
+        ctx.set_graph(True)
+        for _ in range(warm):
+            ctx.sr_iteration(eps, lr)
+        hip_sync()
+        t0 = time.perf_counter()
+        losses = [ctx.sr_iteration(eps, lr)[0] for _ in range(steps)]
+        hip_sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"it_s": round(steps / dt, 2), "loss_mean_last": float(np.mean(losses[-10:]))}
+        if name == "learned":
+            prop, acc, fill = ctx.made_stats()
+            out[name]["global_moves_last_call"] = {"proposed": prop, "accepted": acc, "buffer": fill}
+            theta = ctx.made_get_params()
+            p = ctx.get_params()
+        ctx.close()
+    N = cfg["lx"] * cfg["ly"]
+    taus = {}
+    for name, mix in (("local_only", 0.0), ("learned", 1.0)):
+        c2, _ = build_ctx("chosen", cfg)
+        c2.set_params(p)
+        c2.set_sampler(1024, 48, 1, 50, 0)
+        c2.made_setup(mix, 0, 0, c4["batch"], c4["lr"], 0)
+        c2.made_set_params(theta)
+        keys, _ = c2.sample()
+        lp = 2.0 * c2.logpsi(keys).real.reshape(48, 1024).T
+        taus[name] = round(tau_int(lp), 3)
+        if mix > 0:
+            prop, acc, _ = c2.made_stats()
+            taus["learned_acceptance"] = round(acc / max(prop, 1), 4)
+        c2.close()
+    out["tau_int_logp_one_local_proposal_per_sample"] = taus
+    out["config"] = (f"MADE {2 * N}->128->128->{2 * N}, mix {c4['mix']}, retrain every {c4['retrain_every']} "
+                     f"iterations ({c4['train_steps']} Adam steps, batch {c4['batch']}, lr {c4['lr']}, buffer "
+                     f"{c4['buffer']}); {warm} warm-up + {steps} timed SR iterations per run")
+    return out
+
+
+def equal_loss(iters, cfg=CFG):
+    """Both forms from the same parameters and seeds for `iters` SR
+    iterations: the mean loss of the last 20 and their relative difference."""
+    res = {}
+    for form in ("fp64", "chosen"):
+        ctx, _ = build_ctx(form, cfg)
+        ctx.set_graph(True)
+        losses = [ctx.sr_iteration(cfg["eps"], cfg["lr"])[0] for _ in range(iters)]
+        res[form] = float(np.mean(losses[-20:]))
+        ctx.close()
+    return {"iterations": iters, "loss_mean_last20": res,
+            "rel_diff": abs(res["chosen"] - res["fp64"]) / abs(res["fp64"])}
+
+
 def tau_int(series):
     """Integrated autocorrelation time (Sokal automatic window, c = 5) of
     per-chain series (chains x T), in units of stored samples; tau = 1/2 for
@@ -365,6 +434,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tau", action="store_true")
     ap.add_argument("--no-instances", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal measurement")
+    ap.add_argument("--no-equal-loss", action="store_true", help="skip the equal-loss run of both forms")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (the known-size kernel the --pmc byte counters are "
                          "scaled by)")
@@ -439,6 +510,10 @@ def main():
         out["roofline"]["hbm_algorithmic_gbs"] = round(r["hbm_gbs"], 2)
     out["kernels"] = {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in roofs.items()}
+    if not args.no_equal_loss:
+        out["equal_loss"] = equal_loss(150)
+    if not args.no_learned:
+        out["learned_proposal"] = learned_proposal(20)
     if not args.no_instances:
         out["batched_instances"] = time_instances(max(2, args.steps // 2), 2)
     if not args.no_cpu_baseline:

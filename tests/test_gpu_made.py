@@ -189,3 +189,48 @@ def test_mixed_sampler_graph_replay_matches_eager(gpu_ok, ref):
         assert xa == xb
     assert rel_err(pb, pa) < 1e-12
     assert np.array_equal(ma, mb)
+
+
+def tau_int(series):
+    """Integrated autocorrelation time (Sokal window, c = 5) of per-chain
+    series (chains x T), in stored samples; 1/2 for independent samples."""
+    x = series - series.mean(axis=1, keepdims=True)
+    var = np.mean(x * x)
+    tau = 0.5
+    for t in range(1, x.shape[1]):
+        tau += np.mean(x[:, :-t] * x[:, t:]) / var
+        if t >= 5 * tau:
+            break
+    return max(tau, 0.5)
+
+
+def test_learned_proposal_shortens_autocorrelation(gpu_ok, ref):
+    """What the trained network gains: with one local proposal per stored
+    sample, chains on a peaked 2N = 8 distribution are strongly correlated;
+    a global move from a proposal fitted to the distribution before every
+    stored sample cuts the integrated autocorrelation time of log p by more
+    than half, with the same stationary distribution (TV of 65536 pooled
+    samples over the 256 configurations below 0.04)."""
+    N, nh, na = 4, 4, 4
+    flat, _, _ = ising(N)
+    p = params(ref.ndm_np(N, nh, na), seed=23, sigma=0.6)
+    pex = exact_pdf(ref, N, nh, na, p)
+    rng = np.random.default_rng(3)
+    fit = rng.choice(len(pex), size=32768, p=pex).astype(np.uint64)
+    taus, tvs = [], []
+    for mix in (0.0, 1.0):
+        ctx = Context(0, 17)
+        ctx.set_problem(flat)
+        ctx.set_ndm(N, nh, na)
+        ctx.set_params(p)
+        ctx.set_sampler(1024, 64, 1, 50, 0)
+        ctx.made_setup(mix=mix, retrain_every=0, train_steps=0, batch=512, buffer_cap=0)
+        ctx.made_train(fit, steps=400, batch=512, lr=3e-3)
+        keys, _ = ctx.sample()
+        lp = 2.0 * ctx.logpsi(keys).real.reshape(64, 1024).T
+        taus.append(tau_int(lp))
+        h = np.bincount(keys.astype(np.int64), minlength=len(pex)) / len(keys)
+        tvs.append(0.5 * np.abs(h - pex).sum())
+        ctx.close()
+    assert taus[1] < 0.5 * taus[0], taus
+    assert tvs[1] < 0.04, tvs
