@@ -330,18 +330,38 @@ __global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, 
     if (c >= R) return;
     const int64_t per = (Ns + chunks - 1) / chunks;
     const int64_t s0 = chunk * per, s1 = (s0 + per < Ns) ? s0 + per : Ns;
-    double acc = 0.0;
-    for (int64_t s = s0; s < s1; ++s) acc += (double)Y[s * R + c];
-    part[(int64_t)chunk * R + c] = acc;
+    // four independent sums, unrolled: many loads in flight per thread
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t s = s0;
+#pragma unroll 4
+    for (; s + 4 <= s1; s += 4) {
+        a0 += (double)Y[s * R + c];
+        a1 += (double)Y[(s + 1) * R + c];
+        a2 += (double)Y[(s + 2) * R + c];
+        a3 += (double)Y[(s + 3) * R + c];
+    }
+    for (; s < s1; ++s) a0 += (double)Y[s * R + c];
+    part[(int64_t)chunk * R + c] = (a0 + a1) + (a2 + a3);
 }
 
+// mu[c] = scale * sum_k part[k][c]: 64 columns per workgroup, the chunks split
+// over 4 thread groups (coalesced along the columns), combined in a fixed order
 __global__ __launch_bounds__(256) void k_colsum_finish(const double* part, int chunks, int R, double scale,
                                                        double* mu) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= R) return;
-    double acc = 0.0;
-    for (int k = 0; k < chunks; ++k) acc += part[(int64_t)k * R + c];
-    mu[c] = acc * scale;
+    __shared__ double red[4][64];
+    const int x = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + x;
+    const int cc = c < R ? c : R - 1;
+    double a0 = 0.0, a1 = 0.0;
+    int k = g;
+    for (; k + 4 < chunks; k += 8) {
+        a0 += part[(int64_t)k * R + cc];
+        a1 += part[(int64_t)(k + 4) * R + cc];
+    }
+    if (k < chunks) a0 += part[(int64_t)k * R + cc];
+    red[g][x] = a0 + a1;
+    __syncthreads();
+    if (g == 0 && c < R) mu[c] = ((red[0][x] + red[1][x]) + (red[2][x] + red[3][x])) * scale;
 }
 
 template <int MODE>
@@ -426,6 +446,6 @@ hipError_t nq_launch_colsum(const void* Y, int y_fp64, int64_t Ns, int R, double
 
 hipError_t nq_launch_colsum_finish(const double* part, int chunks, int R, double scale, double* mu,
                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_colsum_finish, dim3((R + 255) / 256), dim3(256), 0, st, part, chunks, R, scale, mu);
+    hipLaunchKernelGGL(k_colsum_finish, dim3((R + 63) / 64), dim3(256), 0, st, part, chunks, R, scale, mu);
     return hipGetLastError();
 }

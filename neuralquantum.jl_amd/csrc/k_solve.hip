@@ -48,7 +48,7 @@ __device__ double block_sum(double v, double* red) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_stats(const T* L, int64_t B, int64_t Lc, double* out) {
+__global__ __launch_bounds__(1024) void k_stats(const T* L, int64_t B, int64_t Lc, double* out) {
     __shared__ double red[16];
     const double invL = 1.0 / (double)Lc;
     // pass 1: mean of chain means (= overall mean, chains have equal length)
@@ -534,9 +534,9 @@ hipError_t nq_launch_bump(uint32_t* x, hipStream_t st) {
 
 hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st) {
     if (fp64)
-        hipLaunchKernelGGL((k_stats<double>), dim3(1), dim3(256), 0, st, (const double*)Lloc, B, Lc, out);
+        hipLaunchKernelGGL((k_stats<double>), dim3(1), dim3(1024), 0, st, (const double*)Lloc, B, Lc, out);
     else
-        hipLaunchKernelGGL((k_stats<float>), dim3(1), dim3(256), 0, st, (const float*)Lloc, B, Lc, out);
+        hipLaunchKernelGGL((k_stats<float>), dim3(1), dim3(1024), 0, st, (const float*)Lloc, B, Lc, out);
     return hipGetLastError();
 }
 

@@ -106,7 +106,7 @@ struct nqh_ctx {
 
 namespace {
 
-constexpr int kColChunks = 64;
+constexpr int kColChunks = 256;
 constexpr int kTableLdsMax = 32 * 1024;   // connection tables up to this size are staged in LDS
 
 nqh_status fail(nqh_ctx* c, nqh_status s, const std::string& msg) {
