@@ -161,16 +161,20 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
 // short; the rest of the block then takes one rank-8 update, its broadcast
 // operands read from LDS (Lc[column][lane]).  Writes L_kk (lower triangle)
 // and L_kk^-T to `w`.
+// The block comes from A (global) or, when `src` is given, from LDS
+// (src[row][column], rows and columns of the block).
 constexpr int SUBP = 8;
+template <bool FROM_LDS>
 __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb, int nmax, double* w, int* info,
-                                              bool report, double (*Lc)[64]) {
+                                              bool report, double (*Lc)[64], const double (*src)[CB + 1]) {
     const int t = threadIdx.x & 63, tr = t & (CB - 1);
     const bool mrow = t < CB;
     double r[CB];
     const int64_t rrow = (int64_t)min(k + tr, nmax - 1);
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
-        const double a = A[rrow + (int64_t)min(k + c, nmax - 1) * lda];   // unconditional load, then select
+        // unconditional load, then select
+        const double a = FROM_LDS ? src[tr][c] : A[rrow + (int64_t)min(k + c, nmax - 1) * lda];
         r[c] = (mrow && tr < kb && c < kb && tr >= c) ? a : (tr == c ? 1.0 : 0.0);
     }
     int bad = 0;   // 1 + first column with a non-positive pivot (branch-free inside the sweep)
@@ -226,7 +230,7 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
 
 __global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info) {
     __shared__ double Ls[CB][64];
-    factor_diag32(A, lda, k, kb, n, winv + (size_t)(k / CB) * CB * CB, info, true, Ls);
+    factor_diag32<false>(A, lda, k, kb, n, winv + (size_t)(k / CB) * CB * CB, info, true, Ls, nullptr);
 }
 
 __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
@@ -244,6 +248,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ double Ls[CB][64];
+    __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
     int tI, tJ;
     tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
     const int o = k + kb;
@@ -342,6 +347,30 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
+    // look-ahead: in tile (0, 0) wave 0 owns rows and columns [o, o + 32), the
+    // next diagonal block; it keeps its update in LDS and factors it at once,
+    // while the other waves store the rest of the tile and the panel
+    const bool la = blockIdx.x == 0 && o < n;
+    if (la && wave == 0) {
+        // rows of the quarter past the diagonal block (only the right-hand
+        // side row n, when fewer than 32 columns remain) still go to A
+        const int kbn = min(CB, n - o);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int lr = a * 16 + lk + 4 * v, lc = b * 16 + li;
+                    const double val = old[a][b][v] - acc[a][b][v];
+                    Ld[lr][lc] = val;
+                    const int r = o + lr, c = o + lc;
+                    if (lr >= kbn && r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
+                }
+        __builtin_amdgcn_wave_barrier();
+        factor_diag32<true>(A, lda, o, min(CB, n - o), n, winv + (size_t)(o / CB) * CB * CB, info, true, Ls, Ld);
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -354,16 +383,13 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
             }
     if (tI == tJ) {
         // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c]
-        for (int e = threadIdx.x; e < 64 * CB; e += blockDim.x) {
+        // (in the look-ahead block without wave 0)
+        const int t0 = la ? (int)threadIdx.x - 64 : (int)threadIdx.x;
+        const int nt = la ? (int)blockDim.x - 64 : (int)blockDim.x;
+        for (int e = t0; e < 64 * CB; e += nt) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && i0 + r < nr) A[(int64_t)(k + c) + (int64_t)(i0 + r) * lda] = Pi[c][r];
         }
-    }
-    if (blockIdx.x == 0 && o < n) {
-        // look-ahead: tile (0, 0) now holds the final values of the next diagonal block
-        __syncthreads();
-        if (wave == 0)
-            factor_diag32(A, lda, o, min(CB, n - o), n, winv + (size_t)(o / CB) * CB * CB, info, true, Ls);
     }
 }
 
