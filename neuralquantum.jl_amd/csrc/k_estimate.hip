@@ -58,7 +58,9 @@ __device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
     return d.wmu + k;
 }
 
-template <typename T, int NMAX, int RS, int PS>
+// TAB_LDS: the connection table is staged in LDS (a compile-time choice, so
+// every table access is a known LDS or global access, never a flat one)
+template <typename T, int NMAX, int RS, int PS, bool TAB_LDS>
 __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_estimate(est_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ndm_dims d = a.d;
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
     for (int k = threadIdx.x; k < d.np; k += blockDim.x) sp[k] = gp[k];
     if (useF)
         for (int k = threadIdx.x; k < d.np; k += blockDim.x) sF[k] = 0.0;
-    const dev_table tab = a.tab_lds ? table_to_lds(a.tab, sT) : a.tab;
+    const dev_table tab = TAB_LDS ? table_to_lds(a.tab, sT) : a.tab;
     __syncthreads();
     auto fadd = [&](int pidx, double v) { atomicAdd(&sF[pidx], v); };
 
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                             nq_sincos(z.im, &sn, &cs);
                             const T ex = nq_exp(z.re);
                             const T wr = ex * cs, wi = ex * sn, dr = T(1) + wr;
-                            const T den = T(1) / (dr * dr + wi * wi);
+                            const T den = nq_rcp(dr * dr + wi * wi);
                             p2 = cmk((wr * dr + wi * wi) * den, wi * den);
                         }
                         const cplx<T> vp = cmul(VW, p2);
@@ -537,8 +539,12 @@ hipError_t launch_t(const est_args& a, int grid, hipStream_t st) {
     // gradients) runs one wave per block.
     int wpb = a.G ? 1 : 8;
     if (!a.G && est_lds_bytes(a, sizeof(T), wpb) > 80 * 1024) wpb = 4;
-    hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS>), dim3(grid), dim3(64 * wpb), est_lds_bytes(a, sizeof(T), wpb), st,
-                       a);
+    if (a.tab_lds)
+        hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, true>), dim3(grid), dim3(64 * wpb),
+                           est_lds_bytes(a, sizeof(T), wpb), st, a);
+    else
+        hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false>), dim3(grid), dim3(64 * wpb),
+                           est_lds_bytes(a, sizeof(T), wpb), st, a);
     return hipGetLastError();
 }
 

@@ -38,16 +38,39 @@ __device__ __forceinline__ float nq_exp(float x) { return __expf(x); }
 __device__ __forceinline__ double nq_log1p(double x) { return log1p(x); }
 __device__ __forceinline__ float nq_log1p(float x) { return __logf(1.0f + x); }
 __device__ __forceinline__ double nq_atan2(double y, double x) { return atan2(y, x); }
-__device__ __forceinline__ float nq_atan2(float y, float x) { return atan2f(y, x); }
+// FP32: octant reduction to a = min(|x|,|y|) / max(|x|,|y|) in [0, 1] and
+// atan(a) = a P(a^2), P of degree 7 fitted for this file by weighted least
+// squares on [0, 1] (max abs error 6e-8 rad); atan2(0, 0) = 0
+__device__ __forceinline__ float nq_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = (mx > 0.0f) ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = a * a;
+    float p = -0.00405455f;
+    p = fmaf(p, s, 0.02186288f);
+    p = fmaf(p, s, -0.05591222f);
+    p = fmaf(p, s, 0.0964219f);
+    p = fmaf(p, s, -0.13908626f);
+    p = fmaf(p, s, 0.19946565f);
+    p = fmaf(p, s, -0.3332986f);
+    p = fmaf(p, s, 0.99999934f);
+    float r = a * p;
+    r = (ay > ax) ? 1.57079633f - r : r;
+    r = (x < 0.0f) ? 3.14159265f - r : r;
+    return copysignf(r, y);
+}
 __device__ __forceinline__ void nq_sincos(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ void nq_sincos(float x, float* s, float* c) { *s = __sinf(x); *c = __cosf(x); }
 
 // softplus f(x) = log(1 + e^x) and its derivative 1/(1 + e^-x)
 // (src/Networks/activation.jl:14-18), sharing one exponential.
 // e / (1 + e) is accurate for every finite e; e = inf gives the limit 1.
+// FP32 quotients use the hardware reciprocal (1 ulp), not the IEEE divide
+__device__ __forceinline__ double nq_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float nq_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 template <typename T>
 __device__ __forceinline__ T sigmoid_of_exp(T e) {
-    return __builtin_isinf(e) ? T(1) : e / (T(1) + e);
+    return __builtin_isinf(e) ? T(1) : e * nq_rcp(T(1) + e);
 }
 template <typename T>
 __device__ __forceinline__ void softplus_and_sigmoid(T x, T* f, T* df) {
@@ -72,7 +95,7 @@ __device__ __forceinline__ void csoftplus_and_sigmoid(cplx<T> z, cplx<T>* f, cpl
     f->re = T(0.5) * nq_log1p(ex * (T(2) * c + ex));
     f->im = nq_atan2(wi, dr);
     // w / (1 + w) = (wr + i wi)(dr - i wi) / (dr^2 + wi^2)
-    T den = T(1) / (dr * dr + wi * wi);
+    T den = nq_rcp(dr * dr + wi * wi);
     df->re = (wr * dr + wi * wi) * den;
     df->im = (wi * dr - wr * wi) * den;
 }
