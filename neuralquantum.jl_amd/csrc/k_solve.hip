@@ -228,9 +228,14 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
     }
 }
 
-__global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info) {
+// the first diagonal block of a column range [k0, ...): winv block 0;
+// blockIdx.y = instance (A and winv at the given strides)
+__global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info,
+                                                    int64_t sA, int64_t sW) {
     __shared__ double Ls[CB][64];
-    factor_diag32<false>(A, lda, k, kb, n, winv + (size_t)(k / CB) * CB * CB, info, true, Ls, nullptr);
+    A += blockIdx.y * sA;
+    winv += blockIdx.y * sW;
+    factor_diag32<false>(A, lda, k, kb, n, winv, info, true, Ls, nullptr);
 }
 
 __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
@@ -243,17 +248,22 @@ __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
 
 // step k: panel transform + trailing update of the lower tiles (I >= J) of
 // the rows/columns >= o = k + kb; n = columns of the matrix, nr = rows
+// columns [k0, n) are factored (n <= nr: a partial factorization leaves the
+// Schur complement of the trailing rows / columns [n, nr)); winv block
+// (k - k0) / 32 holds panel k; blockIdx.y = instance (A, winv at strides)
 __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int lda, int k, int kb, double* winv,
-                                                   int* info) {
+                                                   int* info, int k0, int64_t sA, int64_t sW) {
     __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
     int tI, tJ;
     tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
+    A += blockIdx.y * sA;
+    winv += blockIdx.y * sW;
     const int o = k + kb;
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
-    const double* w = winv + (size_t)(k / CB) * CB * CB;
+    const double* w = winv + (size_t)((k - k0) / CB) * CB * CB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int wr = wave >> 1, wc = wave & 1;
@@ -368,7 +378,8 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                     if (lr >= kbn && r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
                 }
         __builtin_amdgcn_wave_barrier();
-        factor_diag32<true>(A, lda, o, min(CB, n - o), n, winv + (size_t)(o / CB) * CB * CB, info, true, Ls, Ld);
+        factor_diag32<true>(A, lda, o, min(CB, n - o), n, winv + (size_t)((o - k0) / CB) * CB * CB, info, true, Ls,
+                            Ld);
         return;
     }
 #pragma unroll
@@ -401,9 +412,13 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 // back step: x_k = L_kk^-T y_k (from winv), then y_i -= sum_m L[k + m, i] x_{k+m}
 // for i < k, the factor read from the upper triangle A[i, k + m] (coalesced in i)
 __global__ __launch_bounds__(256) void k_back_step(const double* A, int lda, int k, int kb, const double* winv,
-                                                   double* y, double* x) {
+                                                   double* y, double* x, int k0, int64_t sA, int64_t sW,
+                                                   int64_t sY) {
     __shared__ double ys[CB], xs[CB];
-    const double* w = winv + (size_t)(k / CB) * CB * CB;
+    A += blockIdx.y * sA;
+    y += blockIdx.y * sY;
+    x += blockIdx.y * sY;
+    const double* w = winv + blockIdx.y * sW + (size_t)((k - k0) / CB) * CB * CB;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ic = (int64_t)min(i, max(k - 1, 0));
     double a[CB];
@@ -466,6 +481,88 @@ __global__ __launch_bounds__(256) void k_fwd_step(const double* A, int n, int ld
 #pragma unroll
     for (int m = 0; m < CB; ++m) acc += a[m] * zs[m];
     y[i] -= acc;
+}
+
+// ---- split factorization of the NDM's S.  In P order S = [[Sll, Slp, 0],
+// [Spl, Spp, Spm], [0, Smp, Smm]] (lambda, Pi, mu; the lambda-mu block is
+// zero: O_lambda is real and O_mu imaginary).  Instance 0 = the [lambda | Pi]
+// block and instance 1 = the [mu | Pi] block (mu first), each with its own
+// right-hand side row; partially factoring the first nl columns of both (in
+// the same launches) leaves two Schur complements of the Pi block whose sum
+// minus the original is the Pi block's Schur complement in the whole matrix.
+// So the serial pivot chain is nl + npi long instead of 2 nl + npi.
+__device__ __forceinline__ int split_p(int inst, int i, int nl, int npi) {
+    return (inst == 0 || i >= nl) ? i : nl + npi + i;
+}
+
+// lower triangle (with the shift on the diagonal) and the right-hand side row
+// of both instances (m = nl + npi, lda = m + 1); grid (rows, m, 2)
+__global__ void k_split_copy(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
+                             double* A2, int lda) {
+    const int m = nl + npi;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, inst = blockIdx.z;
+    if (i > m || i < j) return;
+    double* A = A2 + (int64_t)inst * lda * lda;
+    const int pj = split_p(inst, j, nl, npi);
+    double v;
+    if (i == m) {
+        v = F[pj];
+    } else {
+        v = S[(int64_t)split_p(inst, i, nl, npi) + (int64_t)pj * lds];
+        if (i == j) v += (kind == 0) ? eps : eps * v;
+    }
+    A[(int64_t)i + (int64_t)j * lda] = v;
+}
+
+// instance 0's Pi block and right-hand side entries += instance 1's - original
+__global__ void k_split_combine(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
+                                double* A2, int lda) {
+    const int m = nl + npi;
+    const int i = nl + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = nl + blockIdx.y;
+    if (i > m || i < j) return;
+    double orig;
+    if (i == m) {
+        orig = F[j];
+    } else {
+        orig = S[(int64_t)i + (int64_t)j * lds];
+        if (i == j) orig += (kind == 0) ? eps : eps * orig;
+    }
+    const int64_t e = (int64_t)i + (int64_t)j * lda;
+    A2[e] += A2[(int64_t)lda * lda + e] - orig;
+}
+
+// y1[i] -= sum_{c in [nl, m)} A1[i, c] x0[c] for i < nl (the mu rows' share of
+// the Pi solution; A1's upper triangle holds L[c, i] transposed): 64 rows x 16
+// column groups per workgroup, fixed-order combination
+__global__ __launch_bounds__(1024) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
+                                                    double* y1) {
+    __shared__ double red[16][64];
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + r;
+    const int ic = i < nl ? i : nl - 1;
+    double a0 = 0.0, a1 = 0.0;
+    int c = nl + g;
+    for (; c + 16 < m; c += 32) {
+        a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
+        a1 += A1[(int64_t)ic + (int64_t)(c + 16) * lda] * x0[c + 16];
+    }
+    if (c < m) a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
+    red[g][r] = a0 + a1;
+    __syncthreads();
+    if (g == 0 && i < nl) {
+        double t = 0.0;
+        for (int q = 0; q < 16; ++q) t += red[q][r];
+        y1[i] -= t;
+    }
+}
+
+// delta in P order from the two instances' solutions
+__global__ void k_split_gather(const double* x0, const double* x1, int nl, int npi, int np, double* delta) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    delta[p] = (p < nl + npi) ? x0[p] : x1[p - nl - npi];
 }
 
 // A[n, j] = b[j] (the right-hand side as the extra row of the factorization)
@@ -589,28 +686,42 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 
 size_t nq_cholesky_winv_doubles(int n) { return (size_t)((n + CB - 1) / CB) * CB * CB; }
 
-hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st) {
-    // rows >= n (at most one extra row, the right-hand side) are carried along
-    const int nr = (lds > n) ? n + 1 : n;
-    hipLaunchKernelGGL(k_chol_diag32, dim3(1), dim3(64), 0, st, S, n, lds, 0, (n < CB ? n : CB), winv, info);
-    for (int k = 0; k < n; k += CB) {
+hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* info, double* winv, int ninst,
+                              int64_t sA, int64_t sW, hipStream_t st) {
+    if (k0 >= n) return hipSuccess;
+    hipLaunchKernelGGL(k_chol_diag32, dim3(1, ninst), dim3(64), 0, st, A, n, lda, k0, (n - k0 < CB ? n - k0 : CB),
+                       winv, info, sA, sW);
+    for (int k = k0; k < n; k += CB) {
         const int kb = (n - k < CB) ? n - k : CB;
         const int rest = nr - k - kb;
         if (rest <= 0) break;
         const int T = (rest + 63) / 64;
-        hipLaunchKernelGGL(k_chol_step, dim3(T * (T + 1) / 2), dim3(256), 0, st, S, n, nr, lds, k, kb, winv, info);
+        hipLaunchKernelGGL(k_chol_step, dim3(T * (T + 1) / 2, ninst), dim3(256), 0, st, A, n, nr, lda, k, kb, winv,
+                           info, k0, sA, sW);
+    }
+    return hipGetLastError();
+}
+
+hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st) {
+    // rows >= n (at most one extra row, the right-hand side) are carried along
+    return nq_cholesky_range(S, 0, n, (lds > n) ? n + 1 : n, lds, info, winv, 1, 0, 0, st);
+}
+
+hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
+                                       double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st) {
+    if (k0 >= n) return hipSuccess;
+    const int last = k0 + ((n - 1 - k0) / CB) * CB;
+    for (int k = last; k >= k0; k -= CB) {
+        const int kb = (n - k < CB) ? n - k : CB;
+        hipLaunchKernelGGL(k_back_step, dim3(k > 0 ? nblk(k, 256) : 1, ninst), dim3(256), 0, st, L, lds, k, kb, winv,
+                           y, x, k0, sA, sW, sY);
     }
     return hipGetLastError();
 }
 
 hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, const double* winv, double* y, double* x,
                                  hipStream_t st) {
-    const int last = ((n - 1) / CB) * CB;
-    for (int k = last; k >= 0; k -= CB) {
-        const int kb = (n - k < CB) ? n - k : CB;
-        hipLaunchKernelGGL(k_back_step, dim3(k > 0 ? nblk(k, 256) : 1), dim3(256), 0, st, L, lds, k, kb, winv, y, x);
-    }
-    return hipGetLastError();
+    return nq_cholesky_backsolve_range(L, 0, n, lds, winv, y, x, 1, 0, 0, 0, st);
 }
 
 hipError_t nq_copy_aug_col(const double* A, int n, int lds, double* y, hipStream_t st) {
@@ -628,6 +739,45 @@ hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* winv
                            y, tmp);
     }
     return nq_cholesky_backsolve(L, n, lds, winv, tmp, y, st);
+}
+
+size_t nq_split_winv_doubles(int nl, int npi) {
+    return 2 * nq_cholesky_winv_doubles(nl) + nq_cholesky_winv_doubles(npi);
+}
+
+hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
+                           double* A2, double* winv, int* info, hipStream_t st) {
+    const int m = nl + npi, lda = m + 1;
+    const int64_t sA = (int64_t)lda * lda;
+    const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
+    hipLaunchKernelGGL(k_split_copy, dim3(nblk(m + 1, 256), m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind,
+                       A2, lda);
+    // both [lambda | Pi] and [mu | Pi] factored over their first nl columns together
+    hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, lda, info, winv, 2, sA, sW, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_split_combine, dim3(nblk(npi + 1, 256), npi), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind,
+                       A2, lda);
+    // the Pi block of instance 0 now holds the whole matrix's Schur complement
+    return nq_cholesky_range(A2, nl, m, m + 1, lda, info, winv + 2 * sW, 1, 0, 0, st);
+}
+
+hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,
+                               double* delta, hipStream_t st) {
+    const int m = nl + npi, lda = m + 1;
+    const int64_t sA = (int64_t)lda * lda, sY = lda;
+    const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
+    const double* winvP = winv + 2 * sW;
+    hipError_t e;
+    double *y0 = y2, *y1 = y2 + sY, *x0 = x2, *x1 = x2 + sY;
+    if ((e = nq_copy_aug_col(A2, m, lda, y0, st)) != hipSuccess) return e;
+    if ((e = nq_copy_aug_col(A2 + sA, m, lda, y1, st)) != hipSuccess) return e;
+    // x_Pi, with the lambda rows of y updated on the way; then the mu rows
+    if ((e = nq_cholesky_backsolve_range(A2, nl, m, lda, winvP, y0, x0, 1, 0, 0, 0, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_split_upd, dim3(nblk(nl, 64)), dim3(1024), 0, st, A2 + sA, lda, nl, m, x0, y1);
+    // x_lambda and x_mu together
+    if ((e = nq_cholesky_backsolve_range(A2, 0, nl, lda, winv, y0, x0, 2, sA, sW, sY, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_split_gather, dim3(nblk(m + nl, 256)), dim3(256), 0, st, x0, x1, nl, npi, m + nl, delta);
+    return hipGetLastError();
 }
 
 hipError_t nq_set_aug_row(double* A, int n, int lds, const double* b, hipStream_t st) {

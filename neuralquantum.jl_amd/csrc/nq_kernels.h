@@ -130,6 +130,21 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 // right-hand side b^T on entry and L^-1 b ends in column n (read it with
 // nq_copy_aug_col).
 size_t nq_cholesky_winv_doubles(int n);
+// factor columns [k0, n) of the nr x nr lower matrix A (rows past n are carried
+// along: a partial factorization leaves their Schur complement); winv block
+// (k - k0) / 32 holds panel k; ninst instances at strides sA (A), sW (winv)
+hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* info, double* winv, int ninst,
+                              int64_t sA, int64_t sW, hipStream_t st);
+hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
+                                       double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st);
+// the NDM's shifted SR solve through the lambda / mu split (k_solve.hip):
+// A2 2 (m+1)^2 doubles, winv nq_split_winv_doubles, y2 and x2 2 (m+1), m = nl + npi;
+// delta in P order (np = 2 nl + npi)
+size_t nq_split_winv_doubles(int nl, int npi);
+hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
+                           double* A2, double* winv, int* info, hipStream_t st);
+hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,
+                               double* delta, hipStream_t st);
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st);
 // x = L^-T y with the factor from nq_cholesky (y is overwritten)
 hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, const double* winv, double* y, double* x,

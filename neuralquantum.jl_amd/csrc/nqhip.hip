@@ -59,6 +59,7 @@ struct nqh_ctx {
     dbuf Fpart;
     // iteration buffers
     dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp, gpart;
+    dbuf A2, winv2, y2, x2;   // the split lambda / mu factorization of the NDM's S
 
     // SR solver: NQH_SOLVER_CHOLESKY (S built and factored) or NQH_SOLVER_CG
     // (matrix-free conjugate gradients on the O rows, S never formed)
@@ -239,6 +240,11 @@ void gram_blocks(const nqh_ctx* c, int* nA, int* nB, int* offB) {
     }
 }
 
+// the NDM's S has a zero lambda-mu block: factor it through the split
+bool use_split(const nqh_ctx* c) {
+    return !is_rbm(c) && c->d.nlam > 0 && c->d.nlam == c->d.nmu && c->d.npi > 0;
+}
+
 // per-sample buffers for Ns samples
 nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     nqh_status s;
@@ -260,6 +266,13 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if (c->solver == NQH_SOLVER_CHOLESKY) {
         if ((s = ensure(c, c->S, sizeof(double) * ldS * ldS))) return s;
         if ((s = ensure(c, c->S0, sizeof(double) * ldS * ldS))) return s;
+        if (use_split(c)) {
+            const size_t lda = (size_t)c->d.nlam + c->d.npi + 1;
+            if ((s = ensure(c, c->A2, sizeof(double) * 2 * lda * lda))) return s;
+            if ((s = ensure(c, c->winv2, sizeof(double) * nq_split_winv_doubles(c->d.nlam, c->d.npi)))) return s;
+            if ((s = ensure(c, c->y2, sizeof(double) * 2 * lda))) return s;
+            if ((s = ensure(c, c->x2, sizeof(double) * 2 * lda))) return s;
+        }
     }
     if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * nsys(c)))) return s;
     if ((s = ensure(c, c->delta, sizeof(double) * nsys(c)))) return s;
@@ -577,7 +590,8 @@ void nqh_destroy(nqh_ctx* c) {
     hipStreamSynchronize(c->stream);
     dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
-                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->cg_w,
+                   &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->A2,
+                   &c->winv2, &c->y2, &c->x2, &c->cg_w,
                    &c->cg_part, &c->cg_p, &c->cg_q, &c->cg_r, &c->cg_sc, &c->cg_diag, &c->cg_dpart, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
@@ -1268,6 +1282,20 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     const int np = nsys(c), ldS = np + 1;
     double* S = P<double>(c->S);
     phase_begin(c, PH_SOLVE);
+    if (refine == 0 && use_split(c)) {
+        // lambda and mu factored side by side, then the Pi block (no refinement)
+        HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, c->stream));
+        phase_begin(c, PH_FACTOR);
+        HIPCHK(c, nq_split_factor(S, ldS, P<double>(c->F), c->d.nlam, c->d.npi, eps, kind, P<double>(c->A2),
+                                  P<double>(c->winv2), P<int>(c->info), c->stream));
+        phase_end(c, PH_FACTOR);
+        HIPCHK(c, nq_split_substitute(c->d.nlam, c->d.npi, P<double>(c->A2), P<double>(c->winv2), P<double>(c->y2),
+                                      P<double>(c->x2), P<double>(c->delta), c->stream));
+        HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, c->stream));
+        HIPCHK(c, nq_launch_check_finite(P<double>(c->delta), np, P<int>(c->flag), c->stream));
+        phase_end(c, PH_SOLVE);
+        return NQH_OK;
+    }
     if (refine > 0)
         HIPCHK(c, hipMemcpyAsync(c->S0.p, S, sizeof(double) * (size_t)ldS * ldS, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, nq_launch_shift(S, np, ldS, eps, kind, c->stream));

@@ -6,9 +6,10 @@ Tolerance (north star): 1e-10 relative for the FP64 kernels; configuration
 and connection indexing bit-exact."""
 import numpy as np
 import pytest
+import scipy.linalg
 
 from _helpers import ising, params, random_keys, rel_err, xyz
-from nqhip import Context
+from nqhip import Context, _lib
 from oracle import nqref
 
 pytestmark = pytest.mark.gpu
@@ -141,3 +142,26 @@ def test_sr_solve_matches_lapack(gpu_ok, ref):
     delta = ctx.precondition(0.01)
     delta_r = nqref.sr_solve(S, F, 0.01)
     assert rel_err(delta, delta_r) < 1e-9
+
+
+@pytest.mark.parametrize("kind", [_lib.SHIFT_ADD, _lib.SHIFT_DIAG])
+def test_split_solve_matches_lapack_c2_shapes(gpu_ok, ref, kind):
+    """The NDM solve factors [lambda | Pi] and [mu | Pi] side by side and then
+    the Pi Schur complement (S's lambda-mu block is zero).  At the N = 12,
+    alpha = 2 shapes (nl = 324, npi = 600: ragged panels in both phases) it
+    matches LAPACK on the full S, and the unsplit path (taken with
+    refinement) gives the same step."""
+    N, nh, na = 12, 24, 24
+    chains, Lc = 128, 16
+    flat, tab, _ = ising(N)
+    p = params(ref.ndm_np(N, nh, na), seed=52, sigma=0.05)
+    keys = random_keys(N, chains * Lc, seed=9)
+    ctx = make_ctx(flat, N, nh, na, p)
+    _, S, F, _ = ctx.grad_from_samples(keys, chains, Lc)
+    Ssh = S + (0.01 * np.eye(len(F)) if kind == _lib.SHIFT_ADD else 0.01 * np.diag(np.diag(S)))
+    delta_r = scipy.linalg.cho_solve(scipy.linalg.cho_factor(Ssh, lower=True), F)
+    d_split = ctx.precondition(0.01, kind, 0)
+    assert rel_err(d_split, delta_r) < 1e-9
+    d_full = ctx.precondition(0.01, kind, 1)
+    assert rel_err(d_split, d_full) < 1e-9
+    ctx.close()
