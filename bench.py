@@ -317,15 +317,20 @@ def kernel_rooflines(phases, form, d, pairs):
                          bound="mfma",
                          achieved=fl / t / 1e12, peak=PEAK_TFLOPS[sp], unit="TFLOP/s",
                          algorithmic=f"4 Np^2 Ns = {fl:.4g} flop", ms=per["sbuild"])
-    # the factorization (k_chol_diag32 + k_chol_step x ceil(Np / 32)), timed on
-    # its own, against Np^3/3 flop; the whole solve phase (shift, factor,
-    # back substitution, checks) is reported beside it
+    # the factorization, timed on its own, against Np^3/3 flop (the dense
+    # Cholesky of S); the whole solve phase (shift, factor, substitutions,
+    # checks) is reported beside it.  S's lambda-mu block is zero, so the
+    # lambda and mu columns are factored side by side (one k_chol_step launch
+    # per 32 columns of lambda for both) before the Pi block: the chain is
+    # ceil(nl / 32) + ceil(npi / 32) launches instead of ceil(Np / 32)
     t = per.get("factor", per["solve"]) * 1e-3
     fl = np_ ** 3 / 3
-    out["solve"] = dict(kernel="k_chol_step (+ k_chol_diag32)", bound="mfma", achieved=fl / t / 1e12,
+    nl, npi = N + nh + nh * N, na + 2 * na * N
+    out["solve"] = dict(kernel="k_chol_step (+ k_chol_diag32, k_split_combine)", bound="mfma",
+                        achieved=fl / t / 1e12,
                         peak=PEAK_TFLOPS["fp64"], unit="TFLOP/s", algorithmic=f"Np^3/3 = {fl:.4g} flop",
                         ms=per.get("factor", per["solve"]), phase_ms=per["solve"],
-                        launches=(np_ + 31) // 32)
+                        launches=(nl + 31) // 32 + (npi + 31) // 32)
     t = per["post"] * 1e-3
     by = 2.0 * Ns * R * elt
     out["post"] = dict(kernel="k_colsum + k_stats + k_force", bound="hbm", achieved=by / t / 1e9,
@@ -493,7 +498,7 @@ def main():
             k: round(Ns / (2 * max(taus[k])) * res[k]["it_s"], 1) for k in ("fp64", "chosen")}
     # the dominant kernel of the chosen form: the kernel with the most device
     # time per SR iteration.  Every phase but the solve is one main kernel;
-    # the solve phase is ~140 launches of several kernels, of which the
+    # the solve phase is ~110 launches of several kernels, of which the
     # factorization (k_chol_step, timed on its own) is the largest, so the
     # solve competes with its factorization time
     roofs = ch["roofs"]

@@ -58,6 +58,9 @@ __device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
     return d.wmu + k;
 }
 
+// changed connections evaluated per transposed reduction in pass 1
+constexpr int EB = 8;
+
 // TAB_LDS: the connection table is staged in LDS (a compile-time choice, so
 // every table access is a known LDS or global access, never a flat one)
 template <typename T, int NMAX, int RS, int PS, bool TAB_LDS>
@@ -233,8 +236,12 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
 
         if (!a.do_conn) continue;
 
-        // ---- pass 1: W_c and L_loc; the changed connections (W, masks) are
-        // cached for pass 2, the unchanged ones only summed (Wdiag)
+        // ---- pass 1: W_c and L_loc.  The walk sums the unchanged connections
+        // (W = mel) and caches the changed ones (mel, flip masks) in LDS; those
+        // are then evaluated EB at a time: every lane adds its units' share of
+        // dlog rho for each of them, one transposed reduction sums all 2 EB
+        // shares, and lanes 4j..4j+3 finish connection j (bias terms, e^dlog rho,
+        // W = mel e^dlog rho) in parallel; W replaces mel in the cache for pass 2
         cplx<T> Lloc = cmk(T(0), T(0));
         cplx<T> Wdiag = cmk(T(0), T(0));
         int nc = 0;
@@ -246,82 +253,107 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                 Wdiag = cadd(Wdiag, mel);
                 return;
             }
-            T dre = T(0), dim = T(0);
-#pragma unroll
-            for (int r = 0; r < RS; ++r) {
-                const int u = lane + 64 * r;
-                if (u < 2 * nh) {
-                    const bool lam = u < nh;
-                    const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
-                    T dfs = T(0), dfe = T(0);
-                    if (ms) {
-                        T t = th_s[r];
-                        for (uint32_t m = ms; m; m &= m - 1) {
-                            const int i = __builtin_ctz(m);
-                            t += cflip<T>(ks, i) * w[nh * i];
-                        }
-                        dfs = softplus(t) - f_s[r];
-                    }
-                    if (me) {
-                        T t = th_e[r];
-                        for (uint32_t m = me; m; m &= m - 1) {
-                            const int i = __builtin_ctz(m);
-                            t += cflip<T>(ke, i) * w[nh * i];
-                        }
-                        dfe = softplus(t) - f_e[r];
-                    }
-                    if (lam) dre += half * (dfs + dfe);
-                    else dim += half * (dfs - dfe);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < PS; ++r) {
-                const int q = lane + 64 * r;
-                if (q < na && (ms | me)) {
-                    cplx<T> z = pi_z[r];
-                    for (uint32_t m = ms; m; m &= m - 1) {
-                        const int i = __builtin_ctz(m);
-                        const T c = cflip<T>(ks, i) * half;
-                        z.re += c * sp[d.ula + q + na * i];
-                        z.im += c * sp[d.umu + q + na * i];
-                    }
-                    for (uint32_t m = me; m; m &= m - 1) {
-                        const int i = __builtin_ctz(m);
-                        const T c = cflip<T>(ke, i) * half;
-                        z.re += c * sp[d.ula + q + na * i];
-                        z.im -= c * sp[d.umu + q + na * i];
-                    }
-                    cplx<T> f2, p2;
-                    csoftplus_and_sigmoid(z, &f2, &p2);
-                    dre += f2.re - pi_f[r].re;
-                    dim += f2.im - pi_f[r].im;
-                }
-            }
-            dre = wave_total(dre);
-            dim = wave_total(dim);
-            for (uint32_t m = ms; m; m &= m - 1) {
-                const int i = __builtin_ctz(m);
-                const T c = cflip<T>(ks, i) * half;
-                dre += c * sp[d.bla + i];
-                dim += c * sp[d.bmu + i];
-            }
-            for (uint32_t m = me; m; m &= m - 1) {
-                const int i = __builtin_ctz(m);
-                const T c = cflip<T>(ke, i) * half;
-                dre += c * sp[d.bla + i];
-                dim -= c * sp[d.bmu + i];
-            }
-            T sn, cs;
-            nq_sincos(dim, &sn, &cs);
-            const T er = nq_exp(dre);
-            const cplx<T> W = cmul(mel, cmk(er * cs, er * sn));
-            Lloc = cadd(Lloc, W);
             if (nc < a.max_conn) {   // every lane writes the same values
-                wbuf[nc] = W;
+                wbuf[nc] = mel;
                 mbuf[nc] = make_uint2(ms, me);
             }
             ++nc;
         });
+        __builtin_amdgcn_wave_barrier();
+        const int ncc = nc < a.max_conn ? nc : a.max_conn;
+        cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of sum_c W_c
+        for (int c0 = 0; c0 < ncc; c0 += EB) {
+            T part[2 * EB];
+#pragma unroll
+            for (int j = 0; j < EB; ++j) {
+                T dre = T(0), dim = T(0);
+                if (c0 + j < ncc) {
+                    const uint2 mm = mbuf[c0 + j];
+                    const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        const int u = lane + 64 * r;
+                        if (u < 2 * nh) {
+                            const bool lam = u < nh;
+                            const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
+                            T dfs = T(0), dfe = T(0);
+                            if (ms) {
+                                T t = th_s[r];
+                                for (uint32_t m = ms; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    t += cflip<T>(ks, i) * w[nh * i];
+                                }
+                                dfs = softplus(t) - f_s[r];
+                            }
+                            if (me) {
+                                T t = th_e[r];
+                                for (uint32_t m = me; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    t += cflip<T>(ke, i) * w[nh * i];
+                                }
+                                dfe = softplus(t) - f_e[r];
+                            }
+                            if (lam) dre += half * (dfs + dfe);
+                            else dim += half * (dfs - dfe);
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < PS; ++r) {
+                        const int q = lane + 64 * r;
+                        if (q < na) {
+                            cplx<T> z = pi_z[r];
+                            for (uint32_t m = ms; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T c = cflip<T>(ks, i) * half;
+                                z.re += c * sp[d.ula + q + na * i];
+                                z.im += c * sp[d.umu + q + na * i];
+                            }
+                            for (uint32_t m = me; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T c = cflip<T>(ke, i) * half;
+                                z.re += c * sp[d.ula + q + na * i];
+                                z.im -= c * sp[d.umu + q + na * i];
+                            }
+                            cplx<T> f2, p2;
+                            csoftplus_and_sigmoid(z, &f2, &p2);
+                            dre += f2.re - pi_f[r].re;
+                            dim += f2.im - pi_f[r].im;
+                        }
+                    }
+                }
+                part[j] = dre;
+                part[EB + j] = dim;
+            }
+            // lane L < 32: sum of dre of connection c0 + L/4; lane 32 + L: its dim
+            T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
+            plswap<true>(dre, dim);   // lanes 0-31: dre in `dre`, dim in `dim`
+            const int c = c0 + (lane0 >> 2);
+            if (lane0 < 32 && c < ncc) {
+                const uint2 mm = mbuf[c];
+                for (uint32_t m = mm.x; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cc = cflip<T>(ks, i) * half;
+                    dre += cc * sp[d.bla + i];
+                    dim += cc * sp[d.bmu + i];
+                }
+                for (uint32_t m = mm.y; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cc = cflip<T>(ke, i) * half;
+                    dre += cc * sp[d.bla + i];
+                    dim -= cc * sp[d.bmu + i];
+                }
+                T sn, cs;
+                nq_sincos(dim, &sn, &cs);
+                const T er = nq_exp(dre);
+                const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
+                if ((lane0 & 3) == 0) {
+                    wbuf[c] = W;
+                    Lacc = cadd(Lacc, W);
+                }
+            }
+        }
+        Lloc = cadd(Lloc, cmk(wave_total(Lacc.re), wave_total(Lacc.im)));
+        __builtin_amdgcn_wave_barrier();
 
         if (lane == 0 && a.Lloc) {
             T* lo = reinterpret_cast<T*>(a.Lloc);
@@ -346,7 +378,6 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
             // unchanged configurations contribute V W = V mel with O(c) = O(s):
             // only the dense parts, added once below
             const cplx<T> VWdiag = cmul(V, Wdiag);
-            const int ncc = nc < a.max_conn ? nc : a.max_conn;
             for (int c = 0; c < ncc; ++c) {
                 const cplx<T> VW = cmul(V, wbuf[c]);
                 const uint2 mm = mbuf[c];

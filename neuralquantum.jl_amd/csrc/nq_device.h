@@ -208,6 +208,88 @@ __device__ __forceinline__ double wave_total(double v) {
     return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
 }
 
+// Cross-half exchanges (gfx950): permlane32 swap leaves a = (a lanes 0-31 |
+// b lanes 0-31) and b = (a lanes 32-63 | b lanes 32-63); permlane16 swap
+// exchanges a's odd rows with b's even rows (rows of 16 lanes), leaving
+// a = (a.r0, b.r0, a.r2, b.r2) and b = (a.r1, b.r1, a.r3, b.r3).
+__device__ __forceinline__ void pl32_swap(uint32_t& a, uint32_t& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+}
+__device__ __forceinline__ void pl16_swap(uint32_t& a, uint32_t& b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+}
+template <bool W32>
+__device__ __forceinline__ void plswap(float& a, float& b) {
+    uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+    if (W32) pl32_swap(x, y); else pl16_swap(x, y);
+    a = __uint_as_float(x);
+    b = __uint_as_float(y);
+}
+template <bool W32>
+__device__ __forceinline__ void plswap(double& a, double& b) {
+    uint32_t al = (uint32_t)__double2loint(a), ah = (uint32_t)__double2hiint(a);
+    uint32_t bl = (uint32_t)__double2loint(b), bh = (uint32_t)__double2hiint(b);
+    if (W32) { pl32_swap(al, bl); pl32_swap(ah, bh); } else { pl16_swap(al, bl); pl16_swap(ah, bh); }
+    a = __hiloint2double((int)ah, (int)al);
+    b = __hiloint2double((int)bh, (int)bl);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v) { return dpp_f<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v) { return dpp_d<CTRL>(v); }
+
+// K = 16 or 32 wave sums at once (a transposed reduction).  On return lane L
+// holds the sum over all 64 lanes of v[L >> 2] (K = 16) or v[L >> 1]
+// (K = 32).  Each step pairs lanes that differ in one lane bit and halves the
+// values every lane holds: the lane with the bit clear keeps the lower index
+// of each pair (i, i + n/2), its partner the upper one (bit 5: permlane32
+// swap, bit 4: permlane16 swap, bit 3: row mirror, bit 2: half-row mirror,
+// bit 1: quad_perm [2,3,0,1]); the lane bits left over are summed plainly.
+// About 2K instructions for K sums instead of K reductions of ~15 each.
+// All 64 lanes must be active; v is clobbered.
+template <int K, typename T>
+__device__ __forceinline__ T wave_totals(T (&v)[K], int lane) {
+    static_assert(K == 16 || K == 32, "16 or 32 sums");
+#pragma unroll
+    for (int i = 0; i < K / 2; ++i) {
+        T a = v[i], b = v[i + K / 2];
+        plswap<true>(a, b);
+        v[i] = a + b;
+    }
+#pragma unroll
+    for (int i = 0; i < K / 4; ++i) {
+        T a = v[i], b = v[i + K / 4];
+        plswap<false>(a, b);
+        v[i] = a + b;
+    }
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0, b1 = (lane & 2) != 0;
+#pragma unroll
+    for (int i = 0; i < K / 8; ++i) {
+        const T lo = v[i] + dpp_t<0x140>(v[i]);
+        const T hi = v[i + K / 8] + dpp_t<0x140>(v[i + K / 8]);
+        v[i] = b3 ? hi : lo;
+    }
+#pragma unroll
+    for (int i = 0; i < K / 16; ++i) {
+        const T lo = v[i] + dpp_t<0x141>(v[i]);
+        const T hi = v[i + K / 16] + dpp_t<0x141>(v[i + K / 16]);
+        v[i] = b2 ? hi : lo;
+    }
+    T r;
+    if constexpr (K == 32) {
+        const T lo = v[0] + dpp_t<0x4E>(v[0]);
+        const T hi = v[1] + dpp_t<0x4E>(v[1]);
+        r = b1 ? hi : lo;
+    } else {
+        r = v[0] + dpp_t<0x4E>(v[0]);
+    }
+    return r + dpp_t<0xB1>(r);
+}
+
 // ---------------------------------------------------------------- NDM layout
 // Offsets of the NDM blocks in the reference's flat order (NDM.jl:17-18):
 // b_mu[N] h_mu[nh] w_mu[nh x N] u_mu[na x N] b_la[N] h_la[nh] d_la[na] w_la[nh x N] u_la[na x N]
