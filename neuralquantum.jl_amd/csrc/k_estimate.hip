@@ -77,10 +77,10 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
     unsigned char* sT = smem + ((reinterpret_cast<unsigned char*>(sM + wpb0 * a.max_conn) - smem + 15) & ~15);
     const T* gp = reinterpret_cast<const T*>(a.p);
     const bool useF = a.Fpart != nullptr || a.G != nullptr;
+    const dev_table tab = TAB_LDS ? table_to_lds(a.tab, sT) : a.tab;
     for (int k = threadIdx.x; k < d.np; k += blockDim.x) sp[k] = gp[k];
     if (useF)
         for (int k = threadIdx.x; k < d.np; k += blockDim.x) sF[k] = 0.0;
-    const dev_table tab = TAB_LDS ? table_to_lds(a.tab, sT) : a.tab;
     __syncthreads();
     auto fadd = [&](int pidx, double v) { atomicAdd(&sF[pidx], v); };
 
@@ -236,32 +236,32 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
 
         if (!a.do_conn) continue;
 
-        // ---- pass 1: W_c and L_loc.  The walk sums the unchanged connections
-        // (W = mel) and caches the changed ones (mel, flip masks) in LDS; those
-        // are then evaluated EB at a time: every lane adds its units' share of
-        // dlog rho for each of them, one transposed reduction sums all 2 EB
-        // shares, and lanes 4j..4j+3 finish connection j (bias terms, e^dlog rho,
-        // W = mel e^dlog rho) in parallel; W replaces mel in the cache for pass 2
-        cplx<T> Lloc = cmk(T(0), T(0));
-        cplx<T> Wdiag = cmk(T(0), T(0));
-        int nc = 0;
-        table_walk<T>(tab, ks, ke, [&](cplx<T> mel, uint32_t ms, uint32_t me) {
-            if ((ms | me) == 0u) {
+        // ---- pass 1: W_c and L_loc.  The walk (terms spread over the lanes)
+        // sums the unchanged connections (W = mel) per lane and caches the
+        // changed ones (mel, flip masks) in LDS; those are then evaluated EB at
+        // a time: every lane adds its units' share of dlog rho for each of
+        // them, one transposed reduction sums all 2 EB shares, and lanes
+        // 4j..4j+3 finish connection j (bias terms, e^dlog rho, W = mel
+        // e^dlog rho) in parallel; W replaces mel in the cache for pass 2
+        cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of L_loc = sum_c W_c
+        cplx<T> Wd = cmk(T(0), T(0));     // and of the unchanged connections' sum
+        const int nc = table_walk_lanes<T>(
+            tab, ks, ke, lane0,
+            [&](cplx<T> mel) {
                 // unchanged configuration: exp(0) = 1, W = mel exactly (the
                 // reference evaluates the network anyway, AccumulatorObsGrad.jl:72)
-                Lloc = cadd(Lloc, mel);
-                Wdiag = cadd(Wdiag, mel);
-                return;
-            }
-            if (nc < a.max_conn) {   // every lane writes the same values
-                wbuf[nc] = mel;
-                mbuf[nc] = make_uint2(ms, me);
-            }
-            ++nc;
-        });
+                Lacc = cadd(Lacc, mel);
+                Wd = cadd(Wd, mel);
+            },
+            [&](int slot, cplx<T> mel, uint32_t ms, uint32_t me) {
+                if (slot < a.max_conn) {
+                    wbuf[slot] = mel;
+                    mbuf[slot] = make_uint2(ms, me);
+                }
+            });
+        const cplx<T> Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
         __builtin_amdgcn_wave_barrier();
         const int ncc = nc < a.max_conn ? nc : a.max_conn;
-        cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of sum_c W_c
         for (int c0 = 0; c0 < ncc; c0 += EB) {
             T part[2 * EB];
 #pragma unroll
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                 }
             }
         }
-        Lloc = cadd(Lloc, cmk(wave_total(Lacc.re), wave_total(Lacc.im)));
+        const cplx<T> Lloc = cmk(wave_total(Lacc.re), wave_total(Lacc.im));
         __builtin_amdgcn_wave_barrier();
 
         if (lane == 0 && a.Lloc) {

@@ -17,7 +17,7 @@ struct est_args {
     int do_conn;              // walk connections (L_loc, force)
     int lds_param_bytes;      // LDS bytes reserved for the staged parameters
     int max_conn;             // upper bound of connections per configuration (LDS W cache)
-    int tab_lds;              // stage the connection table in LDS (tab.bytes small enough)
+    int tab_lds;              // unused by k_estimate (the scalar walk reads the global table)
     void* lp;                 // 2 Ns T: log rho (re, im), nullable
     void* Y;                  // Ns x R T: compact O rows, nullable
     void* Lloc;               // 2 Ns T, nullable

@@ -17,6 +17,11 @@ struct dev_table {
     const uint32_t* ent_flip;    // physical flip mask (N <= 32)
     const void* base;            // the one allocation all arrays above live in
     int bytes;                   // its size (multiple of 16)
+    // per term 8 words (part of the same allocation): kind | nsites_l << 8 |
+    // nsites_r << 16, row base of op_l, row base of op_r, 0, the sites of
+    // op_l as bytes (2 words), those of op_r (2 words)
+    const uint32_t* term_desc;
+    int emax;                    // most connections one term yields (row entries, or their product)
 };
 
 // Copy the table into LDS (all threads of the block take part; the caller
@@ -38,6 +43,7 @@ __device__ __forceinline__ dev_table table_to_lds(const dev_table& g, unsigned c
     t.ent_mel = reinterpret_cast<const double*>(rb(g.ent_mel));
     t.ent_mel_f = reinterpret_cast<const float*>(rb(g.ent_mel_f));
     t.ent_flip = reinterpret_cast<const uint32_t*>(rb(g.ent_flip));
+    t.term_desc = reinterpret_cast<const uint32_t*>(rb(g.term_desc));
     return t;
 }
 
@@ -57,6 +63,77 @@ __device__ __forceinline__ cplx<double> table_mel<double>(const dev_table& t, in
 template <>
 __device__ __forceinline__ cplx<float> table_mel<float>(const dev_table& t, int k) {
     return cmk(t.ent_mel_f[2 * k], t.ent_mel_f[2 * k + 1]);
+}
+
+// The connections of (row bits ks, column bits ke) with the terms spread over
+// the lanes (term q on lane q mod 64, ceil(n_terms / 64) rounds): each lane
+// gathers its own term's local rows and entries, so a round costs a few
+// dependent loads for 64 terms instead of a chain of loads per term.
+// Unchanged configurations (no flips) go to diag(mel) in the lane that found
+// them; the changed ones are numbered across the wave in (round, entry, lane)
+// order and handed to put(slot, mel, ms, me) in their lane.  Returns the
+// number of changed connections (wave-uniform).  The same connections and
+// mels as table_walk (exact-zero mels skipped), in another order.
+// ks, ke wave-uniform; every lane must be active.
+template <typename T, class Diag, class Put>
+__device__ __forceinline__ int table_walk_lanes(const dev_table& t, uint32_t ks, uint32_t ke, int lane, Diag&& diag,
+                                                Put&& put) {
+    int nc = 0;
+    for (int q0 = 0; q0 < t.n_terms; q0 += 64) {
+        const int q = q0 + lane;
+        const bool act = q < t.n_terms;
+        uint4 d0 = make_uint4(0u, 0u, 0u, 0u), d1 = d0;
+        if (act) {
+            d0 = reinterpret_cast<const uint4*>(t.term_desc)[2 * q];
+            d1 = reinterpret_cast<const uint4*>(t.term_desc)[2 * q + 1];
+        }
+        const uint32_t kind = d0.x & 255u, nsl = (d0.x >> 8) & 255u, nsr = (d0.x >> 16) & 255u;
+        const uint64_t sl = (uint64_t)d1.x | ((uint64_t)d1.y << 32);
+        const uint64_t sr = (uint64_t)d1.z | ((uint64_t)d1.w << 32);
+        uint32_t rl = 0, rr = 0;
+        for (uint32_t j = 0; j < nsl; ++j) rl |= ((ks >> (uint32_t)((sl >> (8 * j)) & 255u)) & 1u) << j;
+        for (uint32_t j = 0; j < nsr; ++j) rr |= ((ke >> (uint32_t)((sr >> (8 * j)) & 255u)) & 1u) << j;
+        int kl0 = 0, cl = 0, kr0 = 0, cr = 0;
+        if (act && kind != 1u) {
+            kl0 = t.row_ent_off[d0.y + rl];
+            cl = t.row_ent_off[d0.y + rl + 1] - kl0;
+        }
+        if (act && (kind == 1u || kind == 2u)) {
+            kr0 = t.row_ent_off[d0.z + rr];
+            cr = t.row_ent_off[d0.z + rr + 1] - kr0;
+        }
+        const int cnt = kind == 2u ? cl * cr : (kind == 1u ? cr : cl);
+        int el = 0, er = 0;   // the pair (row entry, column entry) of a tensor term
+        for (int e = 0; e < t.emax; ++e) {
+            bool chg = false;
+            cplx<T> mel = cmk(T(0), T(0));
+            uint32_t ms = 0u, me = 0u;
+            if (e < cnt) {
+                if (kind == 2u) {
+                    mel = cmul(table_mel<T>(t, kl0 + el), table_mel<T>(t, kr0 + er));
+                    ms = t.ent_flip[kl0 + el];
+                    me = t.ent_flip[kr0 + er];
+                    if (++er == cr) { er = 0; ++el; }
+                } else if (kind == 1u) {
+                    mel = table_mel<T>(t, kr0 + e);
+                    me = t.ent_flip[kr0 + e];
+                } else {
+                    mel = table_mel<T>(t, kl0 + e);
+                    ms = t.ent_flip[kl0 + e];
+                }
+                if (!(mel.re == T(0) && mel.im == T(0))) {
+                    if ((ms | me) == 0u) diag(mel);
+                    else chg = true;
+                }
+            }
+            const uint64_t m = __ballot(chg);
+            if (chg)
+                put(nc + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)),
+                    mel, ms, me);
+            nc += __popcll(m);
+        }
+    }
+    return nc;
 }
 
 // Visit the connections of (row bits ks, column bits ke) in the reference's

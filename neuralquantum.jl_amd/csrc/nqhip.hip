@@ -633,7 +633,7 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
     const uint64_t lim = (N >= 64) ? ~0ull : ((1ull << N) - 1);
     for (int e = 0; e < t->n_entries; ++e)
         if (t->ent_flip[e] & ~lim) return fail(c, NQH_E_ARG, "flip mask beyond n_sites");
-    int maxc = 0, n_ket = 0;
+    int maxc = 0, n_ket = 0, emax = 0;
     for (int q = 0; q < t->n_terms; ++q) n_ket += t->term_kind[q] == 3;
     if (n_ket != 0 && n_ket != t->n_terms)
         return fail(c, NQH_E_ARG, "ket terms (kind 3) cannot be mixed with row/column terms");
@@ -646,12 +646,15 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
                 m = std::max(m, t->row_ent_off[r + 1] - t->row_ent_off[r]);
             return m;
         };
-        if (k == 0 || k == 3) { if (!ok(ol)) return fail(c, NQH_E_ARG, "bad row operator"); maxc += maxrow(ol); }
-        else if (k == 1) { if (!ok(orr)) return fail(c, NQH_E_ARG, "bad column operator"); maxc += maxrow(orr); }
+        int e = 0;
+        if (k == 0 || k == 3) { if (!ok(ol)) return fail(c, NQH_E_ARG, "bad row operator"); e = maxrow(ol); }
+        else if (k == 1) { if (!ok(orr)) return fail(c, NQH_E_ARG, "bad column operator"); e = maxrow(orr); }
         else if (k == 2) {
             if (!ok(ol) || !ok(orr)) return fail(c, NQH_E_ARG, "bad tensor operators");
-            maxc += maxrow(ol) * maxrow(orr);
+            e = maxrow(ol) * maxrow(orr);
         } else return fail(c, NQH_E_ARG, "bad term kind");
+        maxc += e;
+        emax = std::max(emax, e);
     }
     // one device allocation: int32 arrays, then doubles, floats, flips
     const size_t ni = (size_t)3 * t->n_terms + (t->n_ops + 1) + nsites_total + (t->n_ops + 1) + (t->n_rows + 1);
@@ -659,7 +662,10 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
     const size_t bytes_d = (size_t)2 * t->n_entries * sizeof(double);
     const size_t bytes_f = ((size_t)2 * t->n_entries * sizeof(float) + 15) / 16 * 16;
     const size_t bytes_m = (size_t)t->n_entries * sizeof(uint32_t);
-    std::vector<unsigned char> host((bytes_i + bytes_d + bytes_f + bytes_m + 16 + 15) / 16 * 16, 0);
+    const size_t bytes_tab = (bytes_i + bytes_d + bytes_f + bytes_m + 16 + 15) / 16 * 16;
+    // term descriptors (8 words per term) after the table proper
+    const size_t bytes_desc = (size_t)8 * sizeof(uint32_t) * std::max(t->n_terms, 1);
+    std::vector<unsigned char> host(bytes_tab + bytes_desc, 0);
     int32_t* hi = reinterpret_cast<int32_t*>(host.data());
     size_t off = 0;
     auto put = [&](const int32_t* src, int n) { std::memcpy(hi + off, src, sizeof(int32_t) * n); size_t o = off; off += n; return o; };
@@ -679,6 +685,24 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
     for (int e = 0; e < 2 * t->n_entries; ++e) hf[e] = (float)t->ent_mel[e];
     uint32_t* hm = reinterpret_cast<uint32_t*>(host.data() + bytes_i + bytes_d + bytes_f);
     for (int e = 0; e < t->n_entries; ++e) hm[e] = (uint32_t)t->ent_flip[e];
+    uint32_t* hdesc = reinterpret_cast<uint32_t*>(host.data() + bytes_tab);
+    for (int q = 0; q < t->n_terms; ++q) {
+        const int k = hi[o_kind + q];
+        const bool use_l = k != 1, use_r = k == 1 || k == 2;
+        auto pack = [&](int o, uint32_t* w) {
+            for (int m = t->op_site_off[o]; m < t->op_site_off[o + 1]; ++m) {
+                const int j = m - t->op_site_off[o];
+                w[j >> 2] |= (uint32_t)t->op_sites[m] << (8 * (j & 3));
+            }
+            return (uint32_t)(t->op_site_off[o + 1] - t->op_site_off[o]);
+        };
+        uint32_t* dq = hdesc + 8 * q;
+        const uint32_t nsl = use_l ? pack(t->term_op_l[q], dq + 4) : 0u;
+        const uint32_t nsr = use_r ? pack(t->term_op_r[q], dq + 6) : 0u;
+        dq[0] = (uint32_t)k | (nsl << 8) | (nsr << 16);
+        dq[1] = use_l ? (uint32_t)t->op_row_off[t->term_op_l[q]] : 0u;
+        dq[2] = use_r ? (uint32_t)t->op_row_off[t->term_op_r[q]] : 0u;
+    }
     nqh_status s = ensure(c, mem, host.size());
     if (s) return s;
     HIPCHK(c, hipMemcpy(mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
@@ -698,6 +722,8 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
     tab.ent_flip = reinterpret_cast<const uint32_t*>(base + bytes_i + bytes_d + bytes_f);
     tab.base = base;
     tab.bytes = (int)host.size();
+    tab.term_desc = reinterpret_cast<const uint32_t*>(base + bytes_tab);
+    tab.emax = emax;
     *maxc_out = maxc;
     *nket_out = n_ket;
     return NQH_OK;
