@@ -58,8 +58,10 @@ __device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
     return d.wmu + k;
 }
 
-// changed connections evaluated per transposed reduction in pass 1
-constexpr int EB = 8;
+// changed connections evaluated per transposed reduction in pass 1 (fewer
+// in FP64, where the shares take twice the registers); lanes per connection
+// after the reduction
+template <typename T> constexpr int est_batch() { return sizeof(T) == 8 ? 4 : 8; }
 
 // TAB_LDS: the connection table is staged in LDS (a compile-time choice, so
 // every table access is a known LDS or global access, never a flat one)
@@ -240,9 +242,9 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
         // sums the unchanged connections (W = mel) per lane and caches the
         // changed ones (mel, flip masks) in LDS; those are then evaluated EB at
         // a time: every lane adds its units' share of dlog rho for each of
-        // them, one transposed reduction sums all 2 EB shares, and lanes
-        // 4j..4j+3 finish connection j (bias terms, e^dlog rho, W = mel
-        // e^dlog rho) in parallel; W replaces mel in the cache for pass 2
+        // them, one transposed reduction sums all 2 EB shares, and a group of
+        // lanes per connection finishes it (bias terms, e^dlog rho, W = mel
+        // e^dlog rho), all EB in parallel; W replaces mel in the cache for pass 2
         cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of L_loc = sum_c W_c
         cplx<T> Wd = cmk(T(0), T(0));     // and of the unchanged connections' sum
         const int nc = table_walk_lanes<T>(
@@ -262,6 +264,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
         const cplx<T> Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
         __builtin_amdgcn_wave_barrier();
         const int ncc = nc < a.max_conn ? nc : a.max_conn;
+        constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
         for (int c0 = 0; c0 < ncc; c0 += EB) {
             T part[2 * EB];
 #pragma unroll
@@ -324,10 +327,10 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                 part[j] = dre;
                 part[EB + j] = dim;
             }
-            // lane L < 32: sum of dre of connection c0 + L/4; lane 32 + L: its dim
+            // lane L < 32: sum of dre of connection c0 + L/LG; lane 32 + L: its dim
             T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
             plswap<true>(dre, dim);   // lanes 0-31: dre in `dre`, dim in `dim`
-            const int c = c0 + (lane0 >> 2);
+            const int c = c0 + lane0 / LG;
             if (lane0 < 32 && c < ncc) {
                 const uint2 mm = mbuf[c];
                 for (uint32_t m = mm.x; m; m &= m - 1) {
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                 nq_sincos(dim, &sn, &cs);
                 const T er = nq_exp(dre);
                 const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
-                if ((lane0 & 3) == 0) {
+                if (lane0 % LG == 0) {
                     wbuf[c] = W;
                     Lacc = cadd(Lacc, W);
                 }

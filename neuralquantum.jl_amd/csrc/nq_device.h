@@ -242,18 +242,18 @@ __device__ __forceinline__ float dpp_t(float v) { return dpp_f<CTRL>(v); }
 template <int CTRL>
 __device__ __forceinline__ double dpp_t(double v) { return dpp_d<CTRL>(v); }
 
-// K = 16 or 32 wave sums at once (a transposed reduction).  On return lane L
-// holds the sum over all 64 lanes of v[L >> 2] (K = 16) or v[L >> 1]
-// (K = 32).  Each step pairs lanes that differ in one lane bit and halves the
-// values every lane holds: the lane with the bit clear keeps the lower index
-// of each pair (i, i + n/2), its partner the upper one (bit 5: permlane32
-// swap, bit 4: permlane16 swap, bit 3: row mirror, bit 2: half-row mirror,
-// bit 1: quad_perm [2,3,0,1]); the lane bits left over are summed plainly.
+// K = 8, 16 or 32 wave sums at once (a transposed reduction).  On return
+// lane L holds the sum over all 64 lanes of v[L / (64 / K)].  Each step pairs
+// lanes that differ in one lane bit and halves the values every lane holds:
+// the lane with the bit clear keeps the lower index of each pair
+// (i, i + n/2), its partner the upper one (bit 5: permlane32 swap, bit 4:
+// permlane16 swap, bit 3: row mirror, bit 2: half-row mirror, bit 1:
+// quad_perm [2,3,0,1]); the lane bits left over are summed plainly.
 // About 2K instructions for K sums instead of K reductions of ~15 each.
 // All 64 lanes must be active; v is clobbered.
 template <int K, typename T>
 __device__ __forceinline__ T wave_totals(T (&v)[K], int lane) {
-    static_assert(K == 16 || K == 32, "16 or 32 sums");
+    static_assert(K == 8 || K == 16 || K == 32, "8, 16 or 32 sums");
 #pragma unroll
     for (int i = 0; i < K / 2; ++i) {
         T a = v[i], b = v[i + K / 2];
@@ -273,13 +273,17 @@ __device__ __forceinline__ T wave_totals(T (&v)[K], int lane) {
         const T hi = v[i + K / 8] + dpp_t<0x140>(v[i + K / 8]);
         v[i] = b3 ? hi : lo;
     }
-#pragma unroll
-    for (int i = 0; i < K / 16; ++i) {
-        const T lo = v[i] + dpp_t<0x141>(v[i]);
-        const T hi = v[i + K / 16] + dpp_t<0x141>(v[i + K / 16]);
-        v[i] = b2 ? hi : lo;
-    }
     T r;
+    if constexpr (K >= 16) {
+#pragma unroll
+        for (int i = 0; i < K / 16; ++i) {
+            const T lo = v[i] + dpp_t<0x141>(v[i]);
+            const T hi = v[i + K / 16] + dpp_t<0x141>(v[i + K / 16]);
+            v[i] = b2 ? hi : lo;
+        }
+    } else {
+        v[0] += dpp_t<0x141>(v[0]);
+    }
     if constexpr (K == 32) {
         const T lo = v[0] + dpp_t<0x4E>(v[0]);
         const T hi = v[1] + dpp_t<0x4E>(v[1]);
