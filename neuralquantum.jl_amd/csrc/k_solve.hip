@@ -409,43 +409,90 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 // zeroed), so a step costs about one memory round trip rather than one per
 // guarded load.
 //
-// back step: x_k = L_kk^-T y_k (from winv), then y_i -= sum_m L[k + m, i] x_{k+m}
-// for i < k, the factor read from the upper triangle A[i, k + m] (coalesced in i)
-__global__ __launch_bounds__(256) void k_back_step(const double* A, int lda, int k, int kb, const double* winv,
-                                                   double* y, double* x, int k0, int64_t sA, int64_t sW,
-                                                   int64_t sY) {
-    __shared__ double ys[CB], xs[CB];
+// back substitution over a group of up to BG panels [k, kend) in one launch.
+// Every workgroup first solves the group itself (redundantly, all in LDS and
+// registers): for p = last .. 0, x_p = L_pp^-T y_p, then y_q -= L_pq^T x_p for
+// the group's earlier panels q < p (the blocks A[q rows, p columns] of the
+// upper triangle, staged once); then rows i < k take y_i -= sum_m
+// L[k + m, i] x_m over all the group's columns (64 rows per workgroup, four
+// lanes per row with 32 columns each), their factor entries loaded before the
+// group solve so that the two overlap.  One launch instead of BG.
+constexpr int BG = 4;
+__global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, int k, int kend, int n,
+                                                    const double* winv, double* y, double* x, int k0, int64_t sA,
+                                                    int64_t sW, int64_t sY) {
+    __shared__ double Lq[BG * (BG - 1) / 2][CB][CB + 1];   // (p, q) block pairs, q < p: [row in q][column in p]
+    __shared__ double ys[BG * CB], xs[BG * CB];
     A += blockIdx.y * sA;
     y += blockIdx.y * sY;
     x += blockIdx.y * sY;
-    const double* w = winv + blockIdx.y * sW + (size_t)((k - k0) / CB) * CB * CB;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = threadIdx.x;
+    const int ng = kend - k;                       // columns of the group (<= BG * CB)
+    const int npan = (ng + CB - 1) / CB;
+    const int i = blockIdx.x * 64 + (t >> 2), h = t & 3;   // row below, its column quarter
+    // every load is issued before the first use: the group's upper-triangle
+    // blocks (q < p), this thread's row of L_pp^-T (threads 32p .. 32p + 31),
+    // y of the group, then the factor entries A[i, k + m] of the rows below
+    // (clamped; unused ones multiply x = 0)
+    constexpr int NQB = BG * (BG - 1) / 2, PER = CB * CB / 256;
+    double lq[NQB][PER];
+#pragma unroll
+    for (int p = 1; p < BG; ++p)
+#pragma unroll
+        for (int q = 0; q < p; ++q)
+#pragma unroll
+            for (int v = 0; v < PER; ++v) {
+                const int e = v * 256 + t, r = e & (CB - 1), m = e >> 5;
+                lq[p * (p - 1) / 2 + q][v] =
+                    A[(int64_t)min(k + q * CB + r, n - 1) + (int64_t)min(k + p * CB + m, n - 1) * lda];
+            }
+    const int pt = t / CB, c = t & (CB - 1);
+    double wv[CB];
+    if (pt < npan) {
+        const double* w = winv + blockIdx.y * sW + (size_t)((k + pt * CB - k0) / CB) * CB * CB;
+#pragma unroll
+        for (int m = 0; m < CB; ++m) wv[m] = w[c + CB * m];
+    }
+    const double yv = y[k + min(t & (BG * CB - 1), ng - 1)];
     const int64_t ic = (int64_t)min(i, max(k - 1, 0));
     double a[CB];
 #pragma unroll
-    for (int m = 0; m < CB; ++m) a[m] = A[ic + (int64_t)(k + min(m, kb - 1)) * lda];
-    const int c = threadIdx.x & (CB - 1);
-    const double yv = y[k + min(c, kb - 1)];
-    double wv[CB];
-    if (threadIdx.x < CB) {
+    for (int m = 0; m < CB; ++m) a[m] = A[ic + (int64_t)min(k + h * CB + m, n - 1) * lda];
 #pragma unroll
-        for (int m = 0; m < CB; ++m) wv[m] = w[c + CB * m];
-        ys[c] = (c < kb) ? yv : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x < CB) {
-        double s = 0.0;
+    for (int p = 1; p < BG; ++p)
 #pragma unroll
-        for (int m = 0; m < CB; ++m) s += wv[m] * ys[m];
-        xs[c] = (c < kb) ? s : 0.0;
-        if (blockIdx.x == 0 && c < kb) x[k + c] = s;
-    }
+        for (int q = 0; q < p; ++q)
+#pragma unroll
+            for (int v = 0; v < PER; ++v) {
+                const int e = v * 256 + t, r = e & (CB - 1), m = e >> 5;
+                Lq[p * (p - 1) / 2 + q][r][m] = (p < npan) ? lq[p * (p - 1) / 2 + q][v] : 0.0;
+            }
+    if (t < BG * CB) ys[t] = (t < ng) ? yv : 0.0;
     __syncthreads();
-    if (i >= k) return;
+    for (int p = npan - 1; p >= 0; --p) {
+        if (pt == p) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < CB; ++m) s += wv[m] * ys[p * CB + m];
+            xs[t] = (t < ng) ? s : 0.0;
+        }
+        __syncthreads();
+        if (t < p * CB) {
+            const int q = t / CB, r = t & (CB - 1), b = p * (p - 1) / 2 + q;
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < CB; ++m) s += Lq[b][r][m] * xs[p * CB + m];
+            ys[t] -= s;
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && t < ng) x[k + t] = xs[t];
     double acc = 0.0;
 #pragma unroll
-    for (int m = 0; m < CB; ++m) acc += a[m] * xs[m];
-    y[i] -= acc;
+    for (int m = 0; m < CB; ++m) acc += a[m] * (h * CB + m < ng ? xs[h * CB + m] : 0.0);
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (h == 0 && i < k) y[i] -= acc;
 }
 
 // forward step (refinement solves): z_k = L_kk^-1 y_k, then y_i -= sum_m L[i, k + m] z_m
@@ -710,11 +757,12 @@ hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipSt
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
                                        double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st) {
     if (k0 >= n) return hipSuccess;
-    const int last = k0 + ((n - 1 - k0) / CB) * CB;
-    for (int k = last; k >= k0; k -= CB) {
-        const int kb = (n - k < CB) ? n - k : CB;
-        hipLaunchKernelGGL(k_back_step, dim3(k > 0 ? nblk(k, 256) : 1, ninst), dim3(256), 0, st, L, lds, k, kb, winv,
-                           y, x, k0, sA, sW, sY);
+    // groups of BG panels from k0, the last group first
+    const int groups = (n - k0 + BG * CB - 1) / (BG * CB);
+    for (int g = groups - 1; g >= 0; --g) {
+        const int k = k0 + g * BG * CB, kend = (n - k < BG * CB) ? n : k + BG * CB;
+        hipLaunchKernelGGL(k_back_group, dim3(k > 0 ? nblk(k, 64) : 1, ninst), dim3(256), 0, st, L, lds, k, kend, n,
+                           winv, y, x, k0, sA, sW, sY);
     }
     return hipGetLastError();
 }
