@@ -138,16 +138,15 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
 // the forward substitution L y = b; y ends in column n of the upper triangle.
 constexpr int CB = 32;
 
-// 1/sqrt(v) from the hardware estimate refined by two Newton steps
-// y <- y (3/2 - (v/2) y^2) (relative error ~1 ulp), a much shorter dependent
-// chain than the IEEE sqrt + divide expansions; v <= 0 yields NaN or inf,
-// which the caller reports
+// 1/sqrt(v) from the hardware estimate y refined by one third-order step:
+// with e = 1 - v y^2, 1/sqrt(v) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...),
+// so y (1 + e (1/2 + 3e/8)) leaves a relative error O(e^3) plus rounding
+// (~1 ulp): four dependent operations after the estimate instead of six for
+// two Newton steps.  v <= 0 yields NaN or inf, which the caller reports
 __device__ __forceinline__ double rsqrt_nr(double v) {
-    const double h = 0.5 * v;
-    double y = __builtin_amdgcn_rsq(v);
-    y = y * fma(-h, y * y, 1.5);
-    y = y * fma(-h, y * y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-v * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
 // factor the kb x kb block at (k, k) (identity padding to 32) in one
@@ -180,36 +179,55 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
     int bad = 0;   // 1 + first column with a non-positive pivot (branch-free inside the sweep)
 #pragma unroll
     for (int p0 = 0; p0 < CB; p0 += SUBP) {
+        // column j updates the next column at once (on the pivot chain) and
+        // the sub-panel's later columns one pivot later, after column j + 1's
+        // square root has started; every column still receives its updates
+        // in pivot order
+        double lprev = 0.0;
 #pragma unroll
         for (int j = p0; j < p0 + SUBP; ++j) {
             const double v = rl_d(r[j], j);                      // current A[j][j]
             bad = (bad == 0 && j < kb && !(v > 0.0)) ? j + 1 : bad;
             const double rp = rsqrt_nr(v);
+            if (j > p0) {
+#pragma unroll
+                for (int m = j + 1; m < p0 + SUBP; ++m) r[m] -= lprev * rl_d(lprev, m);
+            }
             const double piv = v * rp;
             const double lj = (mrow && tr == j) ? piv : r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
             r[j] = lj;
-#pragma unroll
-            for (int m = j + 1; m < p0 + SUBP; ++m) r[m] -= lj * rl_d(lj, m);
-#pragma unroll
-            for (int m = j + 1; m < p0 + SUBP; ++m) asm volatile("" : "+v"(r[m]));
+            if (j + 1 < p0 + SUBP) r[j + 1] -= lj * rl_d(lj, j + 1);
+            lprev = lj;
         }
         if (p0 + SUBP < CB) {
 #pragma unroll
             for (int q = p0; q < p0 + SUBP; ++q) Lc[q][t] = r[q];
             __builtin_amdgcn_wave_barrier();
-            // column pairs, each pinned in place: otherwise the compiler sinks
-            // every column's update to its first use (or hoists all the LDS
-            // reads), holding hundreds of operands live
+            // column pairs, each pinned in place (otherwise the compiler sinks
+            // every column's update to its first use, or hoists all the LDS
+            // reads, holding hundreds of operands live); the next pair's
+            // operands are loaded before the current pair's products, and each
+            // column sums in two chains, so LDS latency and the FMA chains overlap
+            double2 lv[2][SUBP];
+#pragma unroll
+            for (int q = 0; q < SUBP; ++q) lv[0][q] = *reinterpret_cast<const double2*>(&Lc[p0 + q][p0 + SUBP]);
 #pragma unroll
             for (int c = p0 + SUBP; c < CB; c += 2) {
-                double acc0 = r[c], acc1 = r[c + 1];
+                const int b = ((c - p0 - SUBP) >> 1) & 1;
+                if (c + 2 < CB) {
 #pragma unroll
-                for (int q = p0; q < p0 + SUBP; ++q) {
-                    acc0 -= r[q] * Lc[q][c];
-                    acc1 -= r[q] * Lc[q][c + 1];
+                    for (int q = 0; q < SUBP; ++q) lv[b ^ 1][q] = *reinterpret_cast<const double2*>(&Lc[p0 + q][c + 2]);
                 }
-                r[c] = acc0;
-                r[c + 1] = acc1;
+                double a0 = r[c], a1 = r[c + 1], e0 = 0.0, e1 = 0.0;
+#pragma unroll
+                for (int q = 0; q < SUBP; q += 2) {
+                    a0 -= r[p0 + q] * lv[b][q].x;
+                    a1 -= r[p0 + q] * lv[b][q].y;
+                    e0 -= r[p0 + q + 1] * lv[b][q + 1].x;
+                    e1 -= r[p0 + q + 1] * lv[b][q + 1].y;
+                }
+                r[c] = a0 + e0;
+                r[c + 1] = a1 + e1;
                 asm volatile("" : "+v"(r[c]), "+v"(r[c + 1]));
             }
         }
@@ -232,7 +250,7 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
 // blockIdx.y = instance (A and winv at the given strides)
 __global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info,
                                                     int64_t sA, int64_t sW) {
-    __shared__ double Ls[CB][64];
+    __shared__ __align__(16) double Ls[CB][64];
     A += blockIdx.y * sA;
     winv += blockIdx.y * sW;
     factor_diag32<false>(A, lda, k, kb, n, winv, info, true, Ls, nullptr);
@@ -255,7 +273,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                                                    int* info, int k0, int64_t sA, int64_t sW) {
     __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
-    __shared__ double Ls[CB][64];
+    __shared__ __align__(16) double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
     int tI, tJ;
     tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
