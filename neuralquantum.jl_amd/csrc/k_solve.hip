@@ -271,7 +271,8 @@ __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
 // (k - k0) / 32 holds panel k; blockIdx.y = instance (A, winv at strides)
 __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int lda, int k, int kb, double* winv,
                                                    int* info, int k0, int64_t sA, int64_t sW) {
-    __shared__ double Xi[CB][65], Xj[CB][65];   // staged A_Ik, A_Jk: [m][row], then L_Ik, L_Jk
+    __shared__ double Xi[CB][65];               // staged A_Ik: [m][row], then L_Ik
+    __shared__ double Xj[CB][65];               // the same for A_Jk (off-diagonal tiles only)
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ __align__(16) double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
@@ -281,6 +282,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     winv += blockIdx.y * sW;
     const int o = k + kb;
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
+    const bool dg = tI == tJ;                   // diagonal tile: A_Jk = A_Ik
     const double* w = winv + (size_t)((k - k0) / CB) * CB * CB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
         const int r = e & 63, m = e >> 6;
         const int64_t col = (int64_t)(k + min(m, kb - 1)) * lda;
         si[q] = A[(int64_t)min(i0 + r, nr - 1) + col];
-        sj[q] = A[(int64_t)min(j0 + r, nr - 1) + col];
+        if (!dg) sj[q] = A[(int64_t)min(j0 + r, nr - 1) + col];
     }
 #pragma unroll
     for (int q = 0; q < CB * CB / 256; ++q) sw[q] = w[threadIdx.x + 256 * q];
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
         const int e = threadIdx.x + 256 * q;
         const int r = e & 63, m = e >> 6;
         Xi[m][r] = (m < kb && i0 + r < nr) ? si[q] : 0.0;
-        Xj[m][r] = (m < kb && j0 + r < nr) ? sj[q] : 0.0;
+        if (!dg) Xj[m][r] = (m < kb && j0 + r < nr) ? sj[q] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < CB * CB / 256; ++q) {
@@ -324,8 +326,9 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     }
     __syncthreads();
     // L_Ik = A_Ik W and L_Jk = A_Jk W (W = L_kk^-T): 2 x (4 x 2) output tiles
-    // of 16 x 16, two per wave and side; f64 MFMA output D[lane/16 + 4v][lane%16];
-    // the results replace the staged panels once every wave has read them
+    // of 16 x 16, two per wave and side (one side on diagonal tiles); f64 MFMA
+    // output D[lane/16 + 4v][lane%16]; the results replace the staged panels
+    // once every wave has read them
     d4 ai[2], aj[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -337,7 +340,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
         for (int m = 0; m < CB; m += 4) {
             const double b = Wt[m + lk][ct * 16 + li];
             ai[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xi[m + lk][rt * 16 + li], b, ai[q], 0, 0, 0);
-            aj[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xj[m + lk][rt * 16 + li], b, aj[q], 0, 0, 0);
+            if (!dg) aj[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xj[m + lk][rt * 16 + li], b, aj[q], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -348,12 +351,16 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             Xi[ct * 16 + li][rt * 16 + lk + 4 * v] = ai[q][v];
-            Xj[ct * 16 + li][rt * 16 + lk + 4 * v] = aj[q][v];
+            if (!dg) Xj[ct * 16 + li][rt * 16 + lk + 4 * v] = aj[q][v];
         }
     }
     __syncthreads();
     double (*Pi)[65] = Xi;
-    double (*Pj)[65] = Xj;
+    double (*Pj)[65] = dg ? Xi : Xj;
+    const bool la = blockIdx.x == 0 && o < n;
+    // the quarter above the diagonal of a diagonal tile is never stored: its
+    // wave has no update to make
+    if (dg && wr == 0 && wc == 1) return;
     // A_IJ -= L_Ik L_Jk^T (lower part)
     d4 acc[2][2];
 #pragma unroll
@@ -377,8 +384,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     }
     // look-ahead: in tile (0, 0) wave 0 owns rows and columns [o, o + 32), the
     // next diagonal block; it keeps its update in LDS and factors it at once,
-    // while the other waves store the rest of the tile and the panel
-    const bool la = blockIdx.x == 0 && o < n;
+    // while waves 2 and 3 store the rest of the tile and the panel
     if (la && wave == 0) {
         // rows of the quarter past the diagonal block (only the right-hand
         // side row n, when fewer than 32 columns remain) still go to A
@@ -396,8 +402,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                     if (lr >= kbn && r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
                 }
         __builtin_amdgcn_wave_barrier();
-        factor_diag32<true>(A, lda, o, min(CB, n - o), n, winv + (size_t)((o - k0) / CB) * CB * CB, info, true, Ls,
-                            Ld);
+        factor_diag32<true>(A, lda, o, kbn, n, winv + (size_t)((o - k0) / CB) * CB * CB, info, true, Ls, Ld);
         return;
     }
 #pragma unroll
@@ -410,11 +415,11 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                 const int c = j0 + wc * 32 + b * 16 + li;
                 if (r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = old[a][b][v] - acc[a][b][v];
             }
-    if (tI == tJ) {
+    if (dg) {
         // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c]
-        // (in the look-ahead block without wave 0)
-        const int t0 = la ? (int)threadIdx.x - 64 : (int)threadIdx.x;
-        const int nt = la ? (int)blockDim.x - 64 : (int)blockDim.x;
+        // by the waves still running: 0, 2 and 3, or 2 and 3 in the look-ahead tile
+        const int t0 = la ? (int)threadIdx.x - 128 : (wave == 0 ? lane : (int)threadIdx.x - 64);
+        const int nt = la ? 128 : 192;
         for (int e = t0; e < 64 * CB; e += nt) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && i0 + r < nr) A[(int64_t)(k + c) + (int64_t)(i0 + r) * lda] = Pi[c][r];
