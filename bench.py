@@ -353,15 +353,18 @@ def count_pairs(ctx, cap=512, chunk=2048):
     return total
 
 
-def load_traffic(form):
-    """HBM bytes per k_estimate launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), if a
-    measurement for this workload is committed."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_estimate_xyz44.json")
+def load_traffic(form, kernel="estimator"):
+    """HBM bytes per k_estimate launch, or per factorization (its 51
+    k_chol_step launches), measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE scaled by a copy kernel of known size), if a measurement for
+    this workload is committed."""
+    name = "traffic_k_estimate_xyz44.json" if kernel == "estimator" else "traffic_k_chol_step_xyz44.json"
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             t = json.load(f)
-        return t.get(form, {}).get("bytes_per_launch")
+        if kernel == "estimator":
+            return t.get(form, {}).get("bytes_per_launch")
+        return t.get("bytes_per_factorization")
     except (OSError, ValueError):
         return None
 
@@ -508,7 +511,7 @@ def main():
     r = roofs[dom]
     out["roofline"] = {"kernel": r["kernel"], "bound": r["bound"], "achieved": round(r["achieved"], 4),
                        "peak": r["peak"], "unit": r["unit"], "frac": round(r["frac"], 5),
-                       "traffic": load_traffic("chosen") if dom == "estimator" else None,
+                       "traffic": load_traffic("chosen", dom) if dom in ("estimator", "solve") else None,
                        "algorithmic": r["algorithmic"], "ms_per_launch": round(r["ms"], 4)}
     if dom == "estimator":
         out["roofline"]["valu_lane_slot_frac"] = round(r["valu_frac"], 4)
