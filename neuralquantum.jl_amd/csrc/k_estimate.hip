@@ -245,6 +245,23 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
         // them, one transposed reduction sums all 2 EB shares, and a group of
         // lanes per connection finishes it (bias terms, e^dlog rho, W = mel
         // e^dlog rho), all EB in parallel; W replaces mel in the cache for pass 2
+        int ncc;
+        cplx<T> Wdiag, Lloc;
+        if (a.rec_mode == 2) {
+            // forces only: the connections of pass 1 from the earlier launch
+            const int64_t rb = s * (int64_t)a.max_conn;
+            ncc = a.cnum[s];
+            const T* wd = reinterpret_cast<const T*>(a.cwd) + 2 * s;
+            const T* ll = reinterpret_cast<const T*>(a.Lloc) + 2 * s;
+            Wdiag = cmk(wd[0], wd[1]);
+            Lloc = cmk(ll[0], ll[1]);
+            const cplx<T>* rw = reinterpret_cast<const cplx<T>*>(a.crw) + rb;
+            for (int c = lane0; c < ncc; c += 64) {
+                wbuf[c] = rw[c];
+                mbuf[c] = a.crm[rb + c];
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else {
         cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of L_loc = sum_c W_c
         cplx<T> Wd = cmk(T(0), T(0));     // and of the unchanged connections' sum
         const int nc = table_walk_lanes<T>(
@@ -261,9 +278,9 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                     mbuf[slot] = make_uint2(ms, me);
                 }
             });
-        const cplx<T> Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
+        Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
         __builtin_amdgcn_wave_barrier();
-        const int ncc = nc < a.max_conn ? nc : a.max_conn;
+        ncc = nc < a.max_conn ? nc : a.max_conn;
         constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
         for (int c0 = 0; c0 < ncc; c0 += EB) {
             T part[2 * EB];
@@ -355,7 +372,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                 }
             }
         }
-        const cplx<T> Lloc = cmk(wave_total(Lacc.re), wave_total(Lacc.im));
+        Lloc = cmk(wave_total(Lacc.re), wave_total(Lacc.im));
         __builtin_amdgcn_wave_barrier();
 
         if (lane == 0 && a.Lloc) {
@@ -363,6 +380,22 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
             lo[2 * s] = Lloc.re;
             lo[2 * s + 1] = Lloc.im;
         }
+        if (a.rec_mode == 1) {
+            // keep the changed connections (W, flip masks) for the forces launch
+            const int64_t rb = s * (int64_t)a.max_conn;
+            cplx<T>* rw = reinterpret_cast<cplx<T>*>(a.crw) + rb;
+            for (int c = lane0; c < ncc; c += 64) {
+                rw[c] = wbuf[c];
+                a.crm[rb + c] = mbuf[c];
+            }
+            if (lane0 == 0) {
+                a.cnum[s] = ncc;
+                T* wd = reinterpret_cast<T*>(a.cwd) + 2 * s;
+                wd[0] = Wdiag.re;
+                wd[1] = Wdiag.im;
+            }
+        }
+        }   // rec_mode != 2
         if (!a.Fpart && !a.G) continue;
 
         // ---- pass 2: force contributions Re(V W_c O_k(c))

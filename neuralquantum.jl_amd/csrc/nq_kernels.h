@@ -17,12 +17,21 @@ struct est_args {
     int do_conn;              // walk connections (L_loc, force)
     int lds_param_bytes;      // LDS bytes reserved for the staged parameters
     int max_conn;             // upper bound of connections per configuration (LDS W cache)
-    int tab_lds;              // unused by k_estimate (the scalar walk reads the global table)
+    int tab_lds;              // stage the connection table in LDS (tab.bytes small enough)
     void* lp;                 // 2 Ns T: log rho (re, im), nullable
     void* Y;                  // Ns x R T: compact O rows, nullable
     void* Lloc;               // 2 Ns T, nullable
     double* Fpart;            // gridDim x np: per-workgroup force partials (P order), nullable
     void* G;                  // Ns x np complex FP64: per-sample dL_loc (reference order), nullable, pre-zeroed
+    // two launches instead of one: rec_mode 1 walks the connections and
+    // writes L_loc and, per sample, the changed connections (W, flip masks),
+    // their count and the unchanged connections' mel sum; rec_mode 2 reads
+    // those (and L_loc) and adds the forces only.  0: everything in one launch
+    int rec_mode;
+    void* crw;                // Ns x max_conn complex T: W of the changed connections
+    uint2* crm;               // Ns x max_conn: their (row, column) flip masks
+    int* cnum;                // Ns: changed connections per sample
+    void* cwd;                // 2 Ns T: sum of the unchanged connections' mels
 };
 size_t est_lds_bytes(const est_args& a, size_t elt, int waves);
 hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st);

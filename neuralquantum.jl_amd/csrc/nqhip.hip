@@ -24,6 +24,10 @@ struct dbuf {
 struct nqh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for the forces launch beside the S
+    // build, forked from and joined back into `stream` by the two events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t seed = 0;
     std::string err;
 
@@ -55,6 +59,7 @@ struct nqh_ctx {
     // per-sample buffers (capacity Ns_cap)
     int64_t ns_cap = 0;
     dbuf keys_in, lp, Y, Lloc, G, Yt;
+    dbuf crw, crm, cnum, cwd;          // changed connections per sample, between the two estimator launches
     int est_grid = 0;
     dbuf Fpart;
     // iteration buffers
@@ -253,6 +258,12 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->lp, 2 * e * Ns))) return s;
     if ((s = ensure(c, c->Y, e * (size_t)ycols(c) * Ns))) return s;
     if ((s = ensure(c, c->Lloc, 2 * e * Ns))) return s;
+    if (!is_rbm(c)) {   // the changed connections between the two estimator launches
+        const size_t nrec = (size_t)Ns * (size_t)std::max(c->max_conn, 1);
+        if ((s = ensure(c, c->crw, nrec * 2 * e)) || (s = ensure(c, c->crm, nrec * sizeof(uint2))) ||
+            (s = ensure(c, c->cnum, (size_t)Ns * sizeof(int))) || (s = ensure(c, c->cwd, (size_t)Ns * 2 * e)))
+            return s;
+    }
     c->est_grid = est_grid_for(Ns);
     if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * nsys(c)))) return s;
     if ((s = ensure(c, c->colpart, sizeof(double) * (size_t)kColChunks * ycols(c)))) return s;
@@ -436,29 +447,63 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
     return NQH_OK;
 }
 
-// log rho, O rows, L_loc, force partials, then column means, stats, force, S
+static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64);
+
+// log rho, O rows, L_loc and the changed connections (first estimator
+// launch), column means; then the forces (second launch), stats and F on the
+// side stream while the main stream builds S; joined before the solve
 nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chains, int64_t Lc) {
     if (is_rbm(c)) return run_grad_pipeline_rbm(c, keys_dev, chains, Lc);
     const int64_t Ns = chains * Lc;
     const bool f64 = fp64_net(c);
+    nqh_status s;
 
     est_args a = make_est(c, keys_dev, Ns);
     a.do_conn = 1;
-    a.lp = c->lp.p;
-    a.Y = c->Y.p;
     a.Lloc = c->Lloc.p;
-    a.Fpart = P<double>(c->Fpart);
+    a.crw = c->crw.p;
+    a.crm = P<uint2>(c->crm);
+    a.cnum = P<int>(c->cnum);
+    a.cwd = c->cwd.p;
+    est_args a1 = a, a2 = a;
+    a1.rec_mode = 1;
+    a1.lp = c->lp.p;
+    a1.Y = c->Y.p;
+    a2.rec_mode = 2;
+    a2.Fpart = P<double>(c->Fpart);
+    // the forces launch runs beside the S build unless per-phase times are
+    // recorded (then everything runs in order) or S is not built (CG)
+    const bool beside = !c->profiling && c->solver != NQH_SOLVER_CG;
     phase_begin(c, PH_ESTIMATE);
-    HIPCHK(c, nq_launch_estimate(a, f64, c->est_grid, c->stream));
+    HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
+    if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
     phase_end(c, PH_ESTIMATE);
     phase_begin(c, PH_POST);
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
     HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, c->d.R, 1.0 / (double)Ns,
                                       P<double>(c->mu), c->stream));
-    HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), c->stream));
+    hipStream_t fs = c->stream;
+    if (beside) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        fs = c->side;
+        HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, fs));
+    }
+    HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), fs));
     HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
-                                     P<double>(c->stats), c->d, P<double>(c->F), c->stream));
+                                     P<double>(c->stats), c->d, P<double>(c->F), fs));
+    if (beside) HIPCHK(c, hipEventRecord(c->ev_join, c->side));
     phase_end(c, PH_POST);
+    s = build_s(c, Ns, f64);
+    if (beside) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (s) return s;
+    c->last_ns = Ns;
+    c->grad_ready = true;
+    return NQH_OK;
+}
+
+// S (P order: A-gram at [0, nA), B-gram at [nlam, np)) on the main stream
+static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
     if (c->solver == NQH_SOLVER_CG) {   // matrix-free: the solve works on Y and mu directly
         c->last_ns = Ns;
         c->grad_ready = true;
@@ -580,6 +625,12 @@ nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
         delete c;
         return NQH_E_HIP;
     }
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        nqh_destroy(c);
+        return NQH_E_HIP;
+    }
     *out = c;
     return NQH_OK;
 }
@@ -595,13 +646,19 @@ void nqh_destroy(nqh_ctx* c) {
                    &c->cg_part, &c->cg_p, &c->cg_q, &c->cg_r, &c->cg_sc, &c->cg_diag, &c->cg_dpart, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
-                   &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats};
+                   &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats, &c->crw, &c->crm, &c->cnum, &c->cwd};
     drop_graph(c);
     release(c->iter_dev);
     for (dbuf* b : all) release(*b);
     for (int ph = 0; ph < NQH_NUM_PHASES; ++ph)
         for (int k = 0; k < 2; ++k)
             if (c->ev[ph][k]) (void)hipEventDestroy(c->ev[ph][k]);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
