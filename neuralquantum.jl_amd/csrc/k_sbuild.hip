@@ -270,17 +270,15 @@ __global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_
 }
 
 // sum over the chunks of element (rr, cc) of the Gram block whose tiles start at toff
-__device__ __forceinline__ double gram_part_sum(const float* part, int splits, int tiles, int toff, int rr, int cc) {
+// offset of element (rr, cc) of a Gram block in split 0 of the partials
+// (upper tiles only: the lower half is read transposed)
+__device__ __forceinline__ int gram_part_off(int toff, int rr, int cc) {
     int I = rr >> 7, J = cc >> 7, x = rr & 127, y = cc & 127;
     if (I > J) {
         const int t0 = I; I = J; J = t0;
         const int t1 = x; x = y; y = t1;
     }
-    const float* p = part + (int64_t)(toff + J * (J + 1) / 2 + I) * (128 * 128) + x * 128 + y;
-    const int64_t stride = (int64_t)tiles * (128 * 128);
-    double s = 0.0;
-    for (int sp = 0; sp < splits; ++sp) s += (double)p[sp * stride];
-    return s;
+    return (toff + J * (J + 1) / 2 + I) * (128 * 128) + x * 128 + y;
 }
 
 __global__ __launch_bounds__(256) void k_gram_finish(const float* part, int splits, int tiles, int nA, int TA, int nB,
@@ -291,18 +289,39 @@ __global__ __launch_bounds__(256) void k_gram_finish(const float* part, int spli
     const int y = threadIdx.x & 63, x0 = threadIdx.x >> 6;
     const int c = bj * 64 + y;
     const int tilesA = TA * (TA + 1) / 2;
-    double v[16];
+    // the 16 elements' offsets in each block (-1: outside it), then the
+    // chunks in order with all 16 (or 32) loads of a chunk in flight
+    int oa[16], ob[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int r = bi * 64 + x0 + 4 * m;
-        double s = 0.0;
-        if (r < np && c < np) {
-            if (r < nA && c < nA) s += gram_part_sum(part, splits, tiles, 0, r, c);
-            if (nB > 0 && r >= offB && c >= offB && r - offB < nB && c - offB < nB)
-                s += gram_part_sum(part, splits, tiles, tilesA, r - offB, c - offB);
-        }
-        v[m] = scale * s;
+        const bool in = r < np && c < np;
+        oa[m] = (in && r < nA && c < nA) ? gram_part_off(0, r, c) : -1;
+        ob[m] = (in && nB > 0 && r >= offB && c >= offB && r - offB < nB && c - offB < nB)
+                    ? gram_part_off(tilesA, r - offB, c - offB)
+                    : -1;
     }
+    const int64_t stride = (int64_t)tiles * (128 * 128);
+    double sa[16], sb[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) sa[m] = sb[m] = 0.0;
+    for (int sp = 0; sp < splits; ++sp) {
+        const float* p = part + sp * stride;
+        float la[16], lb[16];   // unconditional loads (offset 0 stands in), selected after
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            la[m] = p[oa[m] >= 0 ? oa[m] : 0];
+            lb[m] = p[ob[m] >= 0 ? ob[m] : 0];
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            sa[m] += oa[m] >= 0 ? (double)la[m] : 0.0;
+            sb[m] += ob[m] >= 0 ? (double)lb[m] : 0.0;
+        }
+    }
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = scale * (sa[m] + sb[m]);
     // S[c, r] (coalesced in c); on a diagonal tile this is every element
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
