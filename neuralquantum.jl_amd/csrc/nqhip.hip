@@ -113,6 +113,7 @@ struct nqh_ctx {
 namespace {
 
 constexpr int kColChunks = 256;
+constexpr int64_t kSplitSamples = 4096;   // batches from this size run the estimator as two launches
 constexpr int kTableLdsMax = 32 * 1024;   // connection tables up to this size are staged in LDS
 
 nqh_status fail(nqh_ctx* c, nqh_status s, const std::string& msg) {
@@ -258,11 +259,19 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->lp, 2 * e * Ns))) return s;
     if ((s = ensure(c, c->Y, e * (size_t)ycols(c) * Ns))) return s;
     if ((s = ensure(c, c->Lloc, 2 * e * Ns))) return s;
-    if (!is_rbm(c)) {   // the changed connections between the two estimator launches
+    if (!is_rbm(c) && Ns >= kSplitSamples) {
+        // the changed connections between the two estimator launches, and the
+        // side stream (created on first use, so that contexts that never fork
+        // do not take a hardware queue from the others)
         const size_t nrec = (size_t)Ns * (size_t)std::max(c->max_conn, 1);
         if ((s = ensure(c, c->crw, nrec * 2 * e)) || (s = ensure(c, c->crm, nrec * sizeof(uint2))) ||
             (s = ensure(c, c->cnum, (size_t)Ns * sizeof(int))) || (s = ensure(c, c->cwd, (size_t)Ns * 2 * e)))
             return s;
+        if (!c->side) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        }
     }
     c->est_grid = est_grid_for(Ns);
     if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * nsys(c)))) return s;
@@ -471,12 +480,24 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     a1.Y = c->Y.p;
     a2.rec_mode = 2;
     a2.Fpart = P<double>(c->Fpart);
-    // the forces launch runs beside the S build unless per-phase times are
-    // recorded (then everything runs in order) or S is not built (CG)
-    const bool beside = !c->profiling && c->solver != NQH_SOLVER_CG;
+    // Large batches: two launches, the forces beside the S build (in order
+    // when per-phase times are recorded, or when S is not built (CG)).
+    // Below kSplitSamples the kernels are short and one launch does it all:
+    // a second launch and stream cost more than the overlap gains (e.g. many
+    // small problems each on their own stream)
+    const bool split = Ns >= kSplitSamples;
+    const bool beside = split && !c->profiling && c->solver != NQH_SOLVER_CG;
     phase_begin(c, PH_ESTIMATE);
-    HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
-    if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
+    if (split) {
+        HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
+        if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
+    } else {
+        est_args a0 = a;
+        a0.lp = c->lp.p;
+        a0.Y = c->Y.p;
+        a0.Fpart = P<double>(c->Fpart);
+        HIPCHK(c, nq_launch_estimate(a0, f64, c->est_grid, c->stream));
+    }
     phase_end(c, PH_ESTIMATE);
     phase_begin(c, PH_POST);
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
@@ -623,12 +644,6 @@ nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
     c->seed = seed;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
-        return NQH_E_HIP;
-    }
-    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
-        nqh_destroy(c);
         return NQH_E_HIP;
     }
     *out = c;

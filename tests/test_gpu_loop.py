@@ -64,6 +64,32 @@ def test_graph_replay_matches_eager(gpu_ok, ref, flags):
     b.close()
 
 
+@pytest.mark.parametrize("prec", [(_lib.PREC_FP64, _lib.PREC_FP64), (_lib.PREC_FP32, _lib.PREC_BF16)])
+def test_forces_beside_s_build_match_in_order(gpu_ok, ref, prec):
+    """4096 samples: the forces launch on the side stream beside the S build
+    (graph replays and eager) gives what the same launches give in order (the
+    profiled path): no race between the two streams."""
+    N, nh, na = 6, 6, 6
+    flat, _, _ = ising(N)
+    p = params(ref.ndm_np(N, nh, na), seed=8, sigma=0.05)
+    args = (flat, N, nh, na, p, 512, 8, 13, 10, 21)
+    a, b, c = make(*args), make(*args), make(*args)
+    for x in (a, b, c):
+        x.set_precision(*prec)
+    a.set_profiling(True)
+    c.set_graph(True)
+    for _ in range(3):
+        sa = a.sr_iteration(0.01, 0.01)
+        sb = b.sr_iteration(0.01, 0.01)
+        sc = c.sr_iteration(0.01, 0.01)
+        assert rel_err(sb, sa) < 1e-12
+        assert rel_err(sc, sa) < 1e-12
+    assert rel_err(b.get_params(), a.get_params()) < 1e-12
+    assert rel_err(c.get_params(), a.get_params()) < 1e-12
+    for x in (a, b, c):
+        x.close()
+
+
 def test_fp32_network_errors_stated(gpu_ok, ref):
     """FP32 network kernels against the FP64 oracle at the 4x4 XYZ shapes:
     log rho within 1e-5 absolute, O within 1e-5 relative, L_loc within 1e-4

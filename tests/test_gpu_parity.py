@@ -100,7 +100,9 @@ def test_local_estimator_xyz44(gpu_ok, ref):
     assert rel_err(G, G_r) < TOL
 
 
-@pytest.mark.parametrize("N,nh,na,chains,Lc", [(4, 4, 4, 16, 8), (6, 6, 6, 16, 20)])
+# 4096 samples and more take the estimator's two-launch path (walk, then
+# forces beside the S build); fewer take the single launch
+@pytest.mark.parametrize("N,nh,na,chains,Lc", [(4, 4, 4, 16, 8), (6, 6, 6, 16, 20), (4, 4, 4, 512, 8)])
 def test_grad_from_samples(gpu_ok, ref, N, nh, na, chains, Lc):
     flat, tab, _ = ising(N)
     p = params(ref.ndm_np(N, nh, na), seed=31)
