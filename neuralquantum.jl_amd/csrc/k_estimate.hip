@@ -282,6 +282,9 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
         __builtin_amdgcn_wave_barrier();
         ncc = nc < a.max_conn ? nc : a.max_conn;
         constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
+        // FP32 with one Pi unit per lane (na <= 32): two connections' Pi units
+        // per pass (in FP64 the second set of operands would spill)
+        constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
         for (int c0 = 0; c0 < ncc; c0 += EB) {
             T part[2 * EB];
 #pragma unroll
@@ -317,6 +320,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                             else dim += half * (dfs - dfe);
                         }
                     }
+                    if (!PAIR_PI) {
 #pragma unroll
                     for (int r = 0; r < PS; ++r) {
                         const int q = lane + 64 * r;
@@ -340,9 +344,41 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                             dim += f2.im - pi_f[r].im;
                         }
                     }
+                    }
                 }
                 part[j] = dre;
                 part[EB + j] = dim;
+                if (PAIR_PI && (j & 1)) {
+                    // the Pi units of connections j - 1 and j at once: lanes
+                    // [0, na) take j - 1 and lanes [na, 2 na), which hold the
+                    // same units' angles, take j (flip masks per lane)
+                    const bool hi = lane0 >= na;
+                    const int cj = c0 + (hi ? j : j - 1);
+                    if (lane0 < 2 * na && cj < ncc) {
+                        const int q = hi ? lane0 - na : lane0;
+                        const uint2 mm = mbuf[cj];
+                        cplx<T> z = pi_z[0];
+                        for (uint32_t m = mm.x; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T c = cflip<T>(ks, i) * half;
+                            z.re += c * sp[d.ula + q + na * i];
+                            z.im += c * sp[d.umu + q + na * i];
+                        }
+                        for (uint32_t m = mm.y; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T c = cflip<T>(ke, i) * half;
+                            z.re += c * sp[d.ula + q + na * i];
+                            z.im -= c * sp[d.umu + q + na * i];
+                        }
+                        cplx<T> f2, p2;
+                        csoftplus_and_sigmoid(z, &f2, &p2);
+                        const T vr = f2.re - pi_f[0].re, vi = f2.im - pi_f[0].im;
+                        part[j - 1] += hi ? T(0) : vr;
+                        part[EB + j - 1] += hi ? T(0) : vi;
+                        part[j] += hi ? vr : T(0);
+                        part[EB + j] += hi ? vi : T(0);
+                    }
+                }
             }
             // lane L < 32: sum of dre of connection c0 + L/LG; lane 32 + L: its dim
             T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
