@@ -435,6 +435,7 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
         if (!a.Fpart && !a.G) continue;
 
         // ---- pass 2: force contributions Re(V W_c O_k(c))
+        constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
         const int nv = a.G ? 2 : 1;
         for (int vi = 0; vi < nv; ++vi) {
             // Re(1 G) = Re G, Re(-i G) = Im G in the parity mode
@@ -450,7 +451,9 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
             // unchanged configurations contribute V W = V mel with O(c) = O(s):
             // only the dense parts, added once below
             const cplx<T> VWdiag = cmul(V, Wdiag);
-            for (int c = 0; c < ncc; ++c) {
+            // one changed connection's terms; p2in: its Pi units' f'(z) when
+            // already known (paired evaluation), else recomputed here
+            auto conn = [&](int c, bool have_p2, cplx<T> p2in) {
                 const cplx<T> VW = cmul(V, wbuf[c]);
                 const uint2 mm = mbuf[c];
                 const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
@@ -499,8 +502,8 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                     if (q < 2 * na) {
                         const bool rowhalf = q < na;
                         const int au = rowhalf ? q : q - na;
-                        cplx<T> p2 = pi_p[r];
-                        if (ms | me) {
+                        cplx<T> p2 = have_p2 ? p2in : pi_p[r];
+                        if (!have_p2 && (ms | me)) {
                             cplx<T> z = pi_z[r];
                             for (uint32_t m = ms; m; m &= m - 1) {
                                 const int i = __builtin_ctz(m);
@@ -554,6 +557,47 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1) ? 4 : 2) void k_es
                         fadd(PB_bmu + lane, (double)(cc * VW.im));
                     }
                 }
+            };
+            if (PAIR_PI) {
+                // connections in pairs: lanes [0, na) compute f'(z) of c's Pi
+                // units, lanes [na, 2 na) (the same units) that of c + 1, then
+                // each half takes the other's values by a lane exchange
+                for (int c = 0; c < ncc; c += 2) {
+                    const bool hi = lane0 >= na;
+                    const int cj = c + (hi ? 1 : 0);
+                    cplx<T> p2 = pi_p[0];
+                    if (lane0 < 2 * na && cj < ncc) {
+                        const uint2 mm = mbuf[cj];
+                        if (mm.x | mm.y) {
+                            const int au = hi ? lane0 - na : lane0;
+                            cplx<T> z = pi_z[0];
+                            for (uint32_t m = mm.x; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T cc = cflip<T>(ks, i) * half;
+                                z.re += cc * sp[d.ula + au + na * i];
+                                z.im += cc * sp[d.umu + au + na * i];
+                            }
+                            for (uint32_t m = mm.y; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T cc = cflip<T>(ke, i) * half;
+                                z.re += cc * sp[d.ula + au + na * i];
+                                z.im -= cc * sp[d.umu + au + na * i];
+                            }
+                            T sn, cs;
+                            nq_sincos(z.im, &sn, &cs);
+                            const T ex = nq_exp(z.re);
+                            const T wr = ex * cs, wi = ex * sn, dr = T(1) + wr;
+                            const T den = nq_rcp(dr * dr + wi * wi);
+                            p2 = cmk((wr * dr + wi * wi) * den, wi * den);
+                        }
+                    }
+                    const int src = lane0 < 2 * na ? (hi ? lane0 - na : lane0 + na) : lane0;
+                    const cplx<T> po = cmk(__shfl(p2.re, src, 64), __shfl(p2.im, src, 64));
+                    conn(c, true, hi ? po : p2);
+                    if (c + 1 < ncc) conn(c + 1, true, hi ? p2 : po);
+                }
+            } else {
+                for (int c = 0; c < ncc; ++c) conn(c, false, pi_p[0]);
             }
 
             // unchanged configurations contribute O(s) with weight sum VWdiag
