@@ -381,7 +381,7 @@ def copy_calibration(nbytes=256 << 20):
     del src, dst
 
 
-def cpu_baseline(budget_s=20.0):
+def cpu_baseline():
     """The oracle in reference-structured mode (full network recompute per
     proposal, full Np gradient per connection, complex S = Oc Oc^H, LAPACK
     Cholesky) on a bounded sample of one SR iteration, scaled to the full
@@ -402,17 +402,17 @@ def cpu_baseline(budget_s=20.0):
     ctxm = threadpool_limits(limits=1) if threadpool_limits else None
     try:
         t0 = time.perf_counter()
-        ch_s = 48
+        ch_s = 512
         keys, _ = ref.metropolis(N, nh, na, p, ch_s, Lc, cfg["passes"], cfg["burn"], cfg["sampler_seed"],
                                  compat=True)
         t_samp = (time.perf_counter() - t0) * chains / ch_s
-        nsamp = 192
+        nsamp = 2048
         ks = np.random.default_rng(1).integers(0, 2 ** (2 * N), size=nsamp, dtype=np.uint64)
         t0 = time.perf_counter()
         ref.logpsi_grad(N, nh, na, p, ks)
         ref.local_estimator(tab, nh, na, p, ks, grad=True)
         t_est = (time.perf_counter() - t0) * Ns / nsamp
-        ns_s = 4096
+        ns_s = Ns
         X = (np.random.default_rng(2).standard_normal((np_, ns_s)) +
              1j * np.random.default_rng(3).standard_normal((np_, ns_s)))
         t0 = time.perf_counter()
