@@ -203,12 +203,20 @@ nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
  *                        step is zero (SRIterative.jl:134-150).  NDM networks
  *                        only; the solve synchronises with the host once per
  *                        product, so iterations with it are not graph-captured
- *                        and are not batched.
+ *                        and are not batched;
+ *   NQH_SOLVER_MINRES    sr_minres (SRIterative.jl:101-108, IterativeSolvers.jl
+ *                        0.8.1 `minres`) on the same matrix-free S: Lanczos
+ *                        steps with Givens rotations, until the recurrence's
+ *                        residual norm <= tol ||F|| or maxiter products; the
+ *                        same zero step without convergence, host
+ *                        synchronisation and limits as NQH_SOLVER_CG.
  * Takes effect from the next gradient (nqh_sample_grad / grad_from_samples). */
 #define NQH_SOLVER_CHOLESKY 0
 #define NQH_SOLVER_CG 1
+#define NQH_SOLVER_MINRES 2
 nqh_status nqh_set_sr_solver(nqh_ctx* ctx, int32_t solver, double tol, int32_t maxiter);
-/* products and final relative residual ||r|| / ||F|| of the last CG solve */
+/* products and final relative residual ||r|| / ||F|| of the last iterative
+ * solve (CG: the recurrence's r; MINRES: its residual-norm estimate) */
 nqh_status nqh_get_sr_solver_info(nqh_ctx* ctx, int32_t* iters, double* rel_resid);
 
 /* Optimisers.update!(Descent(lr), net, Delta) (apply.jl:25-31, rules.jl:15-17) */

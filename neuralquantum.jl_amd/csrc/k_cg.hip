@@ -130,6 +130,38 @@ __global__ __launch_bounds__(256) void k_cg_p(double* p, const double* r, int n,
 
 __global__ void k_copy_scalar(const double* src, double* dst) { *dst = *src; }
 
+// MINRES vectors (the Lanczos basis v and the directions w = V R^-1)
+// y = a y
+__global__ __launch_bounds__(256) void k_mr_scale(double* y, int n, double a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] *= a;
+}
+
+// v_next -= h v_prev (h on the host), then v_next -= sc[0] v_curr (the
+// projection, on the device): the two orthogonalisations of one Lanczos step,
+// the second applied by k_mr_orth after the dot product
+__global__ __launch_bounds__(256) void k_mr_axpy(double* y, const double* x, int n, double a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_mr_orth(double* vn, const double* vc, int n, const double* proj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) vn[i] -= proj[0] * vc[i];
+}
+
+// v_next /= h4; w_next = (v_curr - h2 w_curr - h1 w_prev) / h3; x += rhs w_next
+__global__ __launch_bounds__(256) void k_mr_step(double* vn, const double* vc, double* wn, const double* wc,
+                                                 const double* wp, double* x, int n, double inv_h4, double h2,
+                                                 double h1, double inv_h3, double rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vn[i] *= inv_h4;
+    const double w = (vc[i] - h2 * wc[i] - h1 * wp[i]) * inv_h3;
+    wn[i] = w;
+    x[i] += rhs * w;
+}
+
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -183,5 +215,27 @@ hipError_t nq_launch_cg_p(double* p, const double* r, int n, const double* sc, h
 
 hipError_t nq_launch_copy_scalar(const double* src, double* dst, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, st, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_mr_scale(double* y, int n, double a, hipStream_t st) {
+    hipLaunchKernelGGL(k_mr_scale, dim3(nblk(n, 256)), dim3(256), 0, st, y, n, a);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_mr_axpy(double* y, const double* x, int n, double a, hipStream_t st) {
+    hipLaunchKernelGGL(k_mr_axpy, dim3(nblk(n, 256)), dim3(256), 0, st, y, x, n, a);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_mr_orth(double* vn, const double* vc, int n, const double* proj, hipStream_t st) {
+    hipLaunchKernelGGL(k_mr_orth, dim3(nblk(n, 256)), dim3(256), 0, st, vn, vc, n, proj);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_mr_step(double* vn, const double* vc, double* wn, const double* wc, const double* wp, double* x,
+                             int n, double inv_h4, double h2, double h1, double inv_h3, double rhs, hipStream_t st) {
+    hipLaunchKernelGGL(k_mr_step, dim3(nblk(n, 256)), dim3(256), 0, st, vn, vc, wn, wc, wp, x, n, inv_h4, h2, h1,
+                       inv_h3, rhs);
     return hipGetLastError();
 }

@@ -106,6 +106,13 @@ hipError_t nq_launch_dot(const double* x, const double* y, int n, double* part, 
 hipError_t nq_launch_cg_xr(double* x, double* r, const double* p, const double* q, int n, const double* sc,
                            hipStream_t st);
 hipError_t nq_launch_cg_p(double* p, const double* r, int n, const double* sc, hipStream_t st);
+// MINRES vector updates (k_cg.hip): y *= a; y += a x; vn -= proj[0] vc;
+// vn /= h4, wn = (vc - h2 wc - h1 wp) / h3, x += rhs wn
+hipError_t nq_launch_mr_scale(double* y, int n, double a, hipStream_t st);
+hipError_t nq_launch_mr_axpy(double* y, const double* x, int n, double a, hipStream_t st);
+hipError_t nq_launch_mr_orth(double* vn, const double* vc, int n, const double* proj, hipStream_t st);
+hipError_t nq_launch_mr_step(double* vn, const double* vc, double* wn, const double* wc, const double* wp, double* x,
+                             int n, double inv_h4, double h2, double h1, double inv_h3, double rhs, hipStream_t st);
 hipError_t nq_launch_copy_scalar(const double* src, double* dst, hipStream_t st);
 
 // column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]

@@ -66,14 +66,15 @@ struct nqh_ctx {
     dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp, gpart;
     dbuf A2, winv2, y2, x2;   // the split lambda / mu factorization of the NDM's S
 
-    // SR solver: NQH_SOLVER_CHOLESKY (S built and factored) or NQH_SOLVER_CG
-    // (matrix-free conjugate gradients on the O rows, S never formed)
+    // SR solver: NQH_SOLVER_CHOLESKY (S built and factored), NQH_SOLVER_CG or
+    // NQH_SOLVER_MINRES (matrix-free on the O rows, S never formed)
     int solver = NQH_SOLVER_CHOLESKY;
     double cg_tol = 1e-4;
     int cg_maxiter = 0;
     int cg_iters = 0;
     double cg_resid = 0.0;
     dbuf cg_w, cg_part, cg_p, cg_q, cg_r, cg_sc, cg_diag, cg_dpart;
+    dbuf mr_vw;   // MINRES: three Lanczos vectors and three directions, Np doubles each
     int64_t last_ns = 0;
     bool grad_ready = false;
 
@@ -486,7 +487,7 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     // a second launch and stream cost more than the overlap gains (e.g. many
     // small problems each on their own stream)
     const bool split = Ns >= kSplitSamples;
-    const bool beside = split && !c->profiling && c->solver != NQH_SOLVER_CG;
+    const bool beside = split && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY;
     phase_begin(c, PH_ESTIMATE);
     if (split) {
         HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
@@ -525,7 +526,7 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
 
 // S (P order: A-gram at [0, nA), B-gram at [nlam, np)) on the main stream
 static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
-    if (c->solver == NQH_SOLVER_CG) {   // matrix-free: the solve works on Y and mu directly
+    if (c->solver != NQH_SOLVER_CHOLESKY) {   // matrix-free: the solve works on Y and mu directly
         c->last_ns = Ns;
         c->grad_ready = true;
         return NQH_OK;
@@ -658,7 +659,7 @@ void nqh_destroy(nqh_ctx* c) {
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
                    &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->A2,
                    &c->winv2, &c->y2, &c->x2, &c->cg_w,
-                   &c->cg_part, &c->cg_p, &c->cg_q, &c->cg_r, &c->cg_sc, &c->cg_diag, &c->cg_dpart, &c->made_th, &c->made_m, &c->made_v,
+                   &c->cg_part, &c->cg_p, &c->cg_q, &c->cg_r, &c->cg_sc, &c->cg_diag, &c->cg_dpart, &c->mr_vw, &c->made_th, &c->made_m, &c->made_v,
                    &c->made_gpart, &c->made_lpart, &c->made_g, &c->made_nll, &c->made_keys, &c->made_lp,
                    &c->made_lq, &c->ring, &c->ring_posd, &c->made_tmp, &c->obs_table_mem, &c->obs_keys,
                    &c->obs_state, &c->obs_lp, &c->obs_L, &c->obs_stats, &c->crw, &c->crm, &c->cnum, &c->cwd};
@@ -1367,13 +1368,129 @@ static nqh_status cg_solve(nqh_ctx* c, double eps, int kind) {
     return NQH_OK;
 }
 
+// Givens rotation [c s; -s c] [f; g] = [r; 0] with LAPACK dlartg's sign
+// convention (c > 0 when |f| > |g|), as LinearAlgebra.givensAlgorithm
+static void givens(double f, double g, double* cs, double* sn, double* r) {
+    if (g == 0.0) {
+        *cs = 1.0, *sn = 0.0, *r = f;
+    } else if (f == 0.0) {
+        *cs = 0.0, *sn = 1.0, *r = g;
+    } else {
+        const double h = std::hypot(f, g);
+        *cs = f / h, *sn = g / h, *r = h;
+        if (std::fabs(f) > std::fabs(g) && *cs < 0.0) *cs = -*cs, *sn = -*sn, *r = -*r;
+    }
+}
+
+// matrix-free MINRES on (S + shift) x = F (SRIterative.jl:101-108, sr_minres:
+// IterativeSolvers.jl 0.8.1 `minres`, the Lanczos recurrence with Givens
+// rotations of Paige and Saunders): per product one Lanczos step (two dot
+// products, read by the host together), the rotations on the host, and one
+// fused update of v, w and x.  Converged when the recurrence's residual norm
+// |rhs_2| <= tol ||F||; without convergence in maxiter products the step is
+// zero, as with CG.  Synchronises with the host once per product.
+static nqh_status minres_solve(nqh_ctx* c, double eps, int kind) {
+    if (is_rbm(c)) return fail(c, NQH_E_STATE, "the matrix-free MINRES solver supports NDM networks only");
+    const ndm_dims& d = c->d;
+    const int np = d.np, R = d.R;
+    const int64_t Ns = c->last_ns;
+    const bool f64 = c->net_prec == NQH_PREC_FP64;
+    int nA, nB, offB;
+    gram_blocks(c, &nA, &nB, &offB);
+    constexpr int chunks = 64, nparts = 64;
+    nqh_status s;
+    if ((s = ensure(c, c->cg_w, sizeof(double) * 2 * (size_t)Ns)) ||
+        (s = ensure(c, c->cg_part, sizeof(double) * (size_t)chunks * R)) ||
+        (s = ensure(c, c->mr_vw, sizeof(double) * 6 * (size_t)np)) ||
+        (s = ensure(c, c->cg_diag, sizeof(double) * np)) || (s = ensure(c, c->cg_sc, sizeof(double) * 8)) ||
+        (s = ensure(c, c->cg_dpart, sizeof(double) * nparts)))
+        return s;
+    double* vw = P<double>(c->mr_vw);
+    double *vp = vw, *vc = vw + np, *vn = vw + 2 * (size_t)np;
+    double *wp = vw + 3 * (size_t)np, *wc = vw + 4 * (size_t)np, *wn = vw + 5 * (size_t)np;
+    double* x = P<double>(c->delta);
+    double* sc = P<double>(c->cg_sc);
+    double* diag = P<double>(c->cg_diag);
+    const double* mu = P<double>(c->mu);
+    const double* F = P<double>(c->F);
+    hipStream_t st = c->stream;
+    HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    HIPCHK(c, nq_launch_fill(vw, 6 * np, 0.0, st));   // w_prev, w_curr enter with zero weight: keep them finite
+    HIPCHK(c, hipMemcpyAsync(vc, F, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    if (kind == NQH_SHIFT_DIAG)
+        HIPCHK(c, nq_launch_sdiag(c->Y.p, f64, Ns, R, nA, offB, np, mu, P<double>(c->cg_part), chunks, diag, st));
+    HIPCHK(c, nq_launch_dot(F, F, np, P<double>(c->cg_dpart), nparts, sc, st));
+    double f2 = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&f2, sc, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const double res0 = std::sqrt(f2);
+    const int maxit = c->cg_maxiter > 0 ? c->cg_maxiter : 10 * np;
+    const double tolerance = c->cg_tol * res0;
+    double resnorm = res0;
+    if (res0 > 0.0) HIPCHK(c, nq_launch_mr_scale(vc, np, 1.0 / res0, st));
+    // H = the active column of the tridiagonal (h1 .. h4), rhs = the rotated right-hand side
+    double h1 = 0.0, h2 = 0.0, h3 = 0.0, h4 = 0.0, rhs1 = res0, rhs2 = 0.0;
+    double c_prev = 0.0, s_prev = 0.0, c_curr = 0.0, s_curr = 0.0;
+    int it = 1;
+    bool finite = std::isfinite(res0);
+    while (finite && !(it > maxit || resnorm <= tolerance)) {
+        HIPCHK(c, nq_launch_sv(c->Y.p, f64, Ns, R, nA, offB, np, mu, vc, eps, kind, diag, P<double>(c->cg_w),
+                               P<double>(c->cg_part), chunks, vn, st));
+        if (it > 1) HIPCHK(c, nq_launch_mr_axpy(vn, vp, np, -h2, st));
+        HIPCHK(c, nq_launch_dot(vc, vn, np, P<double>(c->cg_dpart), nparts, sc, st));
+        HIPCHK(c, nq_launch_mr_orth(vn, vc, np, sc, st));
+        HIPCHK(c, nq_launch_dot(vn, vn, np, P<double>(c->cg_dpart), nparts, sc + 1, st));
+        double hv[2];
+        HIPCHK(c, hipMemcpyAsync(hv, sc, sizeof(hv), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (!std::isfinite(hv[0]) || !std::isfinite(hv[1])) {
+            finite = false;
+            break;
+        }
+        h3 = hv[0];
+        h4 = std::sqrt(hv[1]);
+        if (it > 2) {
+            h1 = s_prev * h2;
+            h2 = c_prev * h2;
+        }
+        if (it > 1) {
+            const double t = -s_curr * h2 + c_curr * h3;
+            h2 = c_curr * h2 + s_curr * h3;
+            h3 = t;
+        }
+        double cn, sn;
+        givens(h3, h4, &cn, &sn, &h3);
+        rhs2 = -sn * rhs1;
+        rhs1 = cn * rhs1;
+        HIPCHK(c, nq_launch_mr_step(vn, vc, wn, wc, wp, x, np, h4 > 0.0 ? 1.0 / h4 : 0.0, it > 1 ? h2 : 0.0,
+                                    it > 2 ? h1 : 0.0, 1.0 / h3, rhs1, st));
+        double* t = vp;
+        vp = vc, vc = vn, vn = t;
+        t = wp;
+        wp = wc, wc = wn, wn = t;
+        c_prev = c_curr, s_prev = s_curr, c_curr = cn, s_curr = sn;
+        rhs1 = rhs2;
+        h2 = h4;
+        resnorm = std::fabs(rhs2);
+        ++it;
+    }
+    const bool ok = finite && resnorm <= tolerance;
+    c->cg_iters = it - 1;
+    c->cg_resid = res0 > 0.0 ? resnorm / res0 : 0.0;
+    if (!ok && finite) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
+    HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
+    return NQH_OK;
+}
+
 static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine) {
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
-    if (c->solver == NQH_SOLVER_CG) {
+    if (c->solver != NQH_SOLVER_CHOLESKY) {
         phase_begin(c, PH_SOLVE);
-        const nqh_status s = cg_solve(c, eps, kind);
+        const nqh_status s = c->solver == NQH_SOLVER_CG ? cg_solve(c, eps, kind) : minres_solve(c, eps, kind);
         phase_end(c, PH_SOLVE);
         return s;
     }
@@ -1536,8 +1653,9 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
 
 nqh_status nqh_set_sr_solver(nqh_ctx* c, int32_t solver, double tol, int32_t maxiter) {
     if (!c) return NQH_E_ARG;
-    if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG) return fail(c, NQH_E_ARG, "unknown solver");
-    if (solver == NQH_SOLVER_CG && !(tol > 0.0)) return fail(c, NQH_E_ARG, "tol must be > 0");
+    if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG && solver != NQH_SOLVER_MINRES)
+        return fail(c, NQH_E_ARG, "unknown solver");
+    if (solver != NQH_SOLVER_CHOLESKY && !(tol > 0.0)) return fail(c, NQH_E_ARG, "tol must be > 0");
     if (maxiter < 0) return fail(c, NQH_E_ARG, "maxiter must be >= 0");
     (void)hipSetDevice(c->device);
     drop_graph(c);
