@@ -209,14 +209,23 @@ nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
  *                        steps with Givens rotations, until the recurrence's
  *                        residual norm <= tol ||F|| or maxiter products; the
  *                        same zero step without convergence, host
- *                        synchronisation and limits as NQH_SOLVER_CG.
+ *                        synchronisation and limits as NQH_SOLVER_CG;
+ *   NQH_SOLVER_MINRES_QLP sr_qlp (SRIterative.jl:92-100, the reference's
+ *                        minresqlp.jl) on the same matrix-free S: the same
+ *                        Lanczos vectors, the file's rotations, norm
+ *                        estimates and stopping flags (relres = ||r|| /
+ *                        (||A|| ||x|| + ||F||) <= tol, relAres <= tol, maxiter,
+ *                        ...); as in that file a run that reaches maxiter
+ *                        counts as converged and keeps its x.
  * Takes effect from the next gradient (nqh_sample_grad / grad_from_samples). */
 #define NQH_SOLVER_CHOLESKY 0
 #define NQH_SOLVER_CG 1
 #define NQH_SOLVER_MINRES 2
+#define NQH_SOLVER_MINRES_QLP 3
 nqh_status nqh_set_sr_solver(nqh_ctx* ctx, int32_t solver, double tol, int32_t maxiter);
-/* products and final relative residual ||r|| / ||F|| of the last iterative
- * solve (CG: the recurrence's r; MINRES: its residual-norm estimate) */
+/* products and final relative residual of the last iterative solve (CG:
+ * ||r|| / ||F|| of the recurrence's r; MINRES: its residual-norm estimate over
+ * ||F||; MINRES-QLP: relres = ||r|| / (||A|| ||x|| + ||F||) of its estimates) */
 nqh_status nqh_get_sr_solver_info(nqh_ctx* ctx, int32_t* iters, double* rel_resid);
 
 /* Optimisers.update!(Descent(lr), net, Delta) (apply.jl:25-31, rules.jl:15-17) */

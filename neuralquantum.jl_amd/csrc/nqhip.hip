@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1382,6 +1383,188 @@ static void givens(double f, double g, double* cs, double* sn, double* r) {
     }
 }
 
+// The scalar part of the reference's MINRES-QLP (minresqlp.jl, sr_qlp),
+// real, unpreconditioned, unshifted: one Lanczos step's (alfa, betan) in, the
+// solution-update coefficients and the stopping flags out.  As that file is
+// written: its MINRES/QLP branch test compares `flag != flag0`, so it always
+// updates x through the QLP branch, whose iterates equal those of the MINRES
+// branch used here in exact arithmetic; gmin keeps its first-step value
+// (minresqlp.jl:344 assigns a local); flags 2, 4, 6, 7 take the step count
+// back; any flag counts as converged, maxiter (flag 8) included.
+struct qlp_scalars {
+    static constexpr double kTranCond = 10e6, kMaxXnorm = 10e6, kAcondLim = 10e14;
+    double tol;
+    int maxiter;
+    int flag0 = -2, flag = -2, iters = 0;
+    double beta = 0, tau = 0, taul = 0, taul2 = 0, phi, beta1, betan, gmin = 0;
+    double cs = -1, sn = 0, cr1 = -1, sr1 = 0, cr2 = -1, sr2 = 0, dltan = 0, eplnn = 0;
+    double gama = 0, gamal = 0, gamal2 = 0, eta = 0, etal = 0, etal2 = 0, vepln = 0, veplnl = 0, veplnl2 = 0;
+    double ul3 = 0, ul2 = 0, ul = 0, u = 0, rnorm, xnorm = 0, relares = 0, xl2norm = 0, anorm = 0, acond = 1;
+    double relres, resnorm = 123, pnorm = 0, gbar = 0;
+
+    qlp_scalars(double b1, double t, int mi)
+        : tol(t), maxiter(mi), phi(b1), beta1(b1), betan(b1), rnorm(b1), relres(b1 / (b1 + 1e-50)) {}
+
+    // SymGivens (minresqlp.jl:484-509): r >= 0
+    static void sym_givens(double a, double b, double* c, double* s, double* r) {
+        if (b == 0.0) {
+            *c = a == 0.0 ? 1.0 : std::copysign(1.0, a), *s = 0.0, *r = std::fabs(a);
+        } else if (a == 0.0) {
+            *c = 0.0, *s = std::copysign(1.0, b), *r = std::fabs(b);
+        } else if (std::fabs(b) > std::fabs(a)) {
+            const double t = a / b;
+            *s = std::copysign(1.0, b) / std::sqrt(1 + t * t), *c = *s * t, *r = b / *s;
+        } else {
+            const double t = b / a;
+            *c = std::copysign(1.0, a) / std::sqrt(1 + t * t), *s = *c * t, *r = a / *c;
+        }
+    }
+    bool converged() const { return resnorm <= tol || flag != flag0; }
+    bool done(int iteration) const { return iteration > maxiter || converged(); }
+    void begin(double* betal, double* b) {
+        ++iters;
+        *betal = beta;
+        beta = betan;
+        *b = beta;
+    }
+    // minresqlp.jl:200-261; returns w = (v - epln w_{k-2} - dlta w_{k-1}) / gama, x += tau w
+    void rotate(double betal, double alfa, double bn, double* epln, double* dlta_qlp, double* gama_w, double* tau_w) {
+        betan = bn;
+        pnorm = std::sqrt(betal * betal + alfa * alfa + betan * betan);
+        const double dbar = dltan;
+        double dlta = cs * dbar + sn * alfa;
+        *epln = eplnn;
+        gbar = sn * dbar - cs * alfa;
+        eplnn = sn * betan;
+        dltan = -cs * betan;
+        *dlta_qlp = dlta;
+        gamal2 = gamal;
+        gamal = gama;
+        sym_givens(gbar, betan, &cs, &sn, &gama);
+        *gama_w = gama;
+        taul2 = taul;
+        taul = tau;
+        tau = cs * phi;
+        phi = sn * phi;
+        *tau_w = tau;
+        if (iters > 2) {
+            veplnl2 = veplnl;
+            etal2 = etal;
+            etal = eta;
+            const double d = sr2 * vepln - cr2 * dlta;
+            veplnl = cr2 * vepln + sr2 * dlta;
+            dlta = d;
+            eta = sr2 * gama;
+            gama = -cr2 * gama;
+        }
+        if (iters > 1) {
+            sym_givens(gamal, dlta, &cr1, &sr1, &gamal);
+            vepln = sr1 * gama;
+            gama = -cr1 * gama;
+        }
+        const double ul4 = ul3;
+        ul3 = ul2;
+        if (iters > 2) ul2 = (taul2 - etal2 * ul4 - veplnl2 * ul3) / gamal2;
+        if (iters > 1) ul = (taul - etal * ul3 - veplnl * ul2) / gamal;
+        const double xn = std::sqrt(xl2norm * xl2norm + ul2 * ul2 + ul * ul);
+        if (std::fabs(gama) > DBL_MIN && xn < kMaxXnorm) {
+            u = (tau - eta * ul2 - vepln * ul) / gama;
+            if (std::sqrt(xn * xn + u * u) > kMaxXnorm) u = 0.0, flag = 6;
+        } else {
+            u = 0.0, flag = 9;
+        }
+        xl2norm = std::sqrt(xl2norm * xl2norm + ul2 * ul2);
+        xnorm = std::sqrt(xl2norm * xl2norm + ul * ul + u * u);
+    }
+    // minresqlp.jl:328-396
+    void finish() {
+        sym_givens(gamal, eplnn, &cr2, &sr2, &gamal);
+        anorm = std::max(std::max(anorm, pnorm), std::max(gamal, std::fabs(gama)));
+        if (iters == 1) gmin = gama;
+        const double acondl = acond, rnorml = rnorm, relresl = relres;
+        acond = anorm / gmin;
+        if (flag != 9) rnorm = phi;
+        relres = rnorm / (anorm * xnorm + beta1);
+        const double rootl = std::sqrt(gbar * gbar + dltan * dltan);
+        relares = rootl / anorm;
+        const double epsx = anorm * xnorm * DBL_EPSILON;
+        if (flag == flag0 || flag == 9) {
+            const double t1 = 1 + relres, t2 = 1 + relares;
+            if (iters >= maxiter) flag = 8;
+            if (acond >= kAcondLim) flag = 7;
+            if (xnorm >= kMaxXnorm) flag = 6;
+            if (epsx >= beta1) flag = 5;
+            if (t2 <= 1) flag = 4;
+            if (t1 <= 1) flag = 3;
+            if (relares <= tol) flag = 2;
+            if (relres <= tol) flag = 1;
+        }
+        if (flag == 2 || flag == 4 || flag == 6 || flag == 7) {
+            --iters;
+            acond = acondl, rnorm = rnorml, relres = relresl;
+        }
+        resnorm = std::min(relres, relares);
+    }
+};
+
+// sr_qlp: the Lanczos vectors and the x update of minres_solve, the
+// coefficients and the stopping rule of qlp_scalars; x = F / alfa when F is an
+// eigenvector (betan = 0 at the first step, the file's flag -1 intent)
+static nqh_status qlp_loop(nqh_ctx* c, double eps, int kind, double res0, int maxit, double* vw, double* x,
+                           const double* diag, const double* mu, bool f64, int64_t Ns, int R, int nA, int offB,
+                           int chunks, int nparts) {
+    const int np = c->d.np;
+    double *vp = vw, *vc = vw + np, *vn = vw + 2 * (size_t)np;
+    double *wp = vw + 3 * (size_t)np, *wc = vw + 4 * (size_t)np, *wn = vw + 5 * (size_t)np;
+    double* sc = P<double>(c->cg_sc);
+    hipStream_t st = c->stream;
+    qlp_scalars q(res0, c->cg_tol, maxit);
+    bool finite = std::isfinite(res0);
+    int iteration = 1;
+    if (res0 == 0.0) q.flag = 0;   // F = 0: x = 0
+    while (finite && !q.done(iteration)) {
+        double betal, beta;
+        q.begin(&betal, &beta);
+        HIPCHK(c, nq_launch_sv(c->Y.p, f64, Ns, R, nA, offB, np, mu, vc, eps, kind, diag, P<double>(c->cg_w),
+                               P<double>(c->cg_part), chunks, vn, st));
+        if (q.iters > 1) HIPCHK(c, nq_launch_mr_axpy(vn, vp, np, -beta, st));
+        HIPCHK(c, nq_launch_dot(vc, vn, np, P<double>(c->cg_dpart), nparts, sc, st));
+        HIPCHK(c, nq_launch_mr_orth(vn, vc, np, sc, st));
+        HIPCHK(c, nq_launch_dot(vn, vn, np, P<double>(c->cg_dpart), nparts, sc + 1, st));
+        double hv[2];
+        HIPCHK(c, hipMemcpyAsync(hv, sc, sizeof(hv), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (!std::isfinite(hv[0]) || !std::isfinite(hv[1])) {
+            finite = false;
+            break;
+        }
+        const double alfa = hv[0], betan = std::sqrt(hv[1]);
+        if (q.iters == 1 && betan == 0.0) {
+            HIPCHK(c, nq_launch_mr_axpy(x, vc, np, res0 / alfa, st));   // vc = F / ||F||
+            q.flag = -1;
+            break;
+        }
+        double epln, dlta, gama, tau;
+        q.rotate(betal, alfa, betan, &epln, &dlta, &gama, &tau);
+        double rhs = tau;
+        if (!(q.xnorm < qlp_scalars::kMaxXnorm)) rhs = 0.0, q.flag = 6;
+        HIPCHK(c, nq_launch_mr_step(vn, vc, wn, wc, wp, x, np, betan > 0.0 ? 1.0 / betan : 0.0, dlta, epln,
+                                    1.0 / gama, rhs, st));
+        double* t = vp;
+        vp = vc, vc = vn, vn = t;
+        t = wp;
+        wp = wc, wc = wn, wn = t;
+        q.finish();
+        iteration = q.iters;
+    }
+    c->cg_iters = q.iters;
+    c->cg_resid = q.relres;
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
+    HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
+    return NQH_OK;
+}
+
 // matrix-free MINRES on (S + shift) x = F (SRIterative.jl:101-108, sr_minres:
 // IterativeSolvers.jl 0.8.1 `minres`, the Lanczos recurrence with Givens
 // rotations of Paige and Saunders): per product one Lanczos step (two dot
@@ -1389,8 +1572,8 @@ static void givens(double f, double g, double* cs, double* sn, double* r) {
 // fused update of v, w and x.  Converged when the recurrence's residual norm
 // |rhs_2| <= tol ||F||; without convergence in maxiter products the step is
 // zero, as with CG.  Synchronises with the host once per product.
-static nqh_status minres_solve(nqh_ctx* c, double eps, int kind) {
-    if (is_rbm(c)) return fail(c, NQH_E_STATE, "the matrix-free MINRES solver supports NDM networks only");
+static nqh_status minres_solve(nqh_ctx* c, double eps, int kind, bool qlp) {
+    if (is_rbm(c)) return fail(c, NQH_E_STATE, "the matrix-free MINRES solvers support NDM networks only");
     const ndm_dims& d = c->d;
     const int np = d.np, R = d.R;
     const int64_t Ns = c->last_ns;
@@ -1428,6 +1611,7 @@ static nqh_status minres_solve(nqh_ctx* c, double eps, int kind) {
     const double tolerance = c->cg_tol * res0;
     double resnorm = res0;
     if (res0 > 0.0) HIPCHK(c, nq_launch_mr_scale(vc, np, 1.0 / res0, st));
+    if (qlp) return qlp_loop(c, eps, kind, res0, maxit, vw, x, diag, mu, f64, Ns, R, nA, offB, chunks, nparts);
     // H = the active column of the tridiagonal (h1 .. h4), rhs = the rotated right-hand side
     double h1 = 0.0, h2 = 0.0, h3 = 0.0, h4 = 0.0, rhs1 = res0, rhs2 = 0.0;
     double c_prev = 0.0, s_prev = 0.0, c_curr = 0.0, s_curr = 0.0;
@@ -1490,7 +1674,8 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
     if (c->solver != NQH_SOLVER_CHOLESKY) {
         phase_begin(c, PH_SOLVE);
-        const nqh_status s = c->solver == NQH_SOLVER_CG ? cg_solve(c, eps, kind) : minres_solve(c, eps, kind);
+        const nqh_status s = c->solver == NQH_SOLVER_CG ? cg_solve(c, eps, kind)
+                                                           : minres_solve(c, eps, kind, c->solver == NQH_SOLVER_MINRES_QLP);
         phase_end(c, PH_SOLVE);
         return s;
     }
@@ -1653,7 +1838,8 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
 
 nqh_status nqh_set_sr_solver(nqh_ctx* c, int32_t solver, double tol, int32_t maxiter) {
     if (!c) return NQH_E_ARG;
-    if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG && solver != NQH_SOLVER_MINRES)
+    if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG && solver != NQH_SOLVER_MINRES &&
+        solver != NQH_SOLVER_MINRES_QLP)
         return fail(c, NQH_E_ARG, "unknown solver");
     if (solver != NQH_SOLVER_CHOLESKY && !(tol > 0.0)) return fail(c, NQH_E_ARG, "tol must be > 0");
     if (maxiter < 0) return fail(c, NQH_E_ARG, "maxiter must be >= 0");

@@ -114,17 +114,17 @@ class MetropolisSampler:
 class SR:
     """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift,
     precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky" (SRDirect.jl),
-    "cg" or "minres" (SRIterative.jl with the matrix-free S of
-    SR_notfull.jl: S is never formed; `precision` is the relative-residual
-    tolerance, `maxiter` 0 means the reference's 10 Np).  `precondition_type` "shift" adds eps*I
+    "cg", "minres" or "qlp" (SRIterative.jl with the matrix-free S of
+    SR_notfull.jl: S is never formed; `precision` is the solver's tolerance,
+    `maxiter` 0 means the reference's 10 Np).  `precondition_type` "shift" adds eps*I
     (reference); "diag" adds eps*diag(S).  `refine` extra FP64 refinement
     steps of the Cholesky solve."""
 
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
                  refine: int = 0, precision: float = 1e-4, maxiter: int = 0):
-        if algorithm not in ("cholesky", "cg", "minres"):
-            raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'cg' (sr_cg) or "
-                                      "'minres' (sr_minres)")
+        if algorithm not in ("cholesky", "cg", "minres", "qlp"):
+            raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'cg' (sr_cg), "
+                                      "'minres' (sr_minres) or 'qlp' (sr_qlp)")
         if precondition_type not in ("shift", "diag"):
             raise ValueError("precondition_type must be 'shift' or 'diag'")
         self.eps = float(eps)
@@ -136,7 +136,8 @@ class SR:
 
     @property
     def solver(self):
-        return {"cholesky": _lib.SOLVER_CHOLESKY, "cg": _lib.SOLVER_CG, "minres": _lib.SOLVER_MINRES}[self.algorithm]
+        return {"cholesky": _lib.SOLVER_CHOLESKY, "cg": _lib.SOLVER_CG, "minres": _lib.SOLVER_MINRES,
+                "qlp": _lib.SOLVER_MINRES_QLP}[self.algorithm]
 
     @property
     def shift_kind(self):

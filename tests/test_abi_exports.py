@@ -61,12 +61,13 @@ def test_ndm_flat_layout_and_init():
 
 
 def test_sr_mirror_algorithms():
-    """SR(algorithm=...) accepts the solvers built (sr_cholesky, sr_cg, sr_minres)
+    """SR(algorithm=...) accepts the solvers built (sr_cholesky, sr_cg, sr_minres, sr_qlp)
     and maps them to the ABI's solver codes; others are refused."""
     from nqhip import SR
     assert SR(0.01).solver == _lib.SOLVER_CHOLESKY
     cg = SR(0.01, algorithm="cg", precision=1e-6)
     assert cg.solver == _lib.SOLVER_CG and cg.precision == 1e-6 and cg.maxiter == 0
     assert SR(0.01, algorithm="minres").solver == _lib.SOLVER_MINRES
+    assert SR(0.01, algorithm="qlp").solver == _lib.SOLVER_MINRES_QLP
     with pytest.raises(NotImplementedError):
-        SR(0.01, algorithm="qlp")
+        SR(0.01, algorithm="lsq")

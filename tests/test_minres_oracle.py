@@ -74,3 +74,71 @@ def test_maxiter_and_zero_rhs():
     assert k == 2 and not ok and r > 1e-14 * np.linalg.norm(b)
     x, k, r, ok = minres(A, np.zeros(20), 1e-6, 10)
     assert k == 0 and ok and not np.any(x)
+
+
+# ---- MINRES-QLP as the reference's minresqlp.jl runs it ---------------------
+from oracle.minresqlp_ref import minresqlp, sym_givens
+
+
+def test_sym_givens():
+    for a, b in [(3.0, 4.0), (-3.0, 4.0), (4.0, -3.0), (-4.0, -3.0), (0.0, -2.0), (2.0, 0.0), (-2.0, 0.0)]:
+        c, s, r = sym_givens(a, b)
+        assert r >= 0 and abs(c * c + s * s - 1.0) < 1e-15
+        assert np.allclose([c * a + s * b, s * a - c * b], [r, 0.0], atol=1e-15)
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e4])
+def test_qlp_iterates_are_minres_iterates(cond):
+    n, K = 30, 14
+    A = spd(n, cond, 11)
+    b = np.random.default_rng(12).standard_normal(n)
+    h_ref, h_mr = [], []
+    minresqlp(A, b, 0.0, K, "reference", h_ref)
+    minresqlp(A, b, 0.0, K, "minres", h_mr)
+    assert len(h_ref) == len(h_mr) == K
+    for k in range(K):
+        xs, _ = scipy_minres(A, b, rtol=0.0, maxiter=k + 1)
+        scale = np.linalg.norm(xs)
+        # two implementations of the Lanczos process part by rounding once
+        # orthogonality is lost (after ~10 steps on this spectrum); the two
+        # update branches of one Lanczos run agree to rounding throughout
+        if k < 10:
+            assert np.linalg.norm(h_ref[k][0] - xs) <= 1e-11 * scale
+        assert np.linalg.norm(h_mr[k][0] - h_ref[k][0]) <= 1e-12 * scale
+        x, st = h_ref[k]
+        assert abs(st["xnorm"] - np.linalg.norm(x)) <= 1e-9 * scale
+        assert abs(st["rnorm"] - np.linalg.norm(b - A @ x)) <= 1e-9 * np.linalg.norm(b)
+        assert 0 < st["anorm"] <= np.linalg.norm(A, 2) * (1 + 1e-12)
+
+
+def test_qlp_stopping_rule_and_solution():
+    n = 50
+    A = spd(n, 1e3, 13)
+    b = np.random.default_rng(14).standard_normal(n)
+    x, sc = minresqlp(A, b, 1e-10, 10 * n)
+    assert sc.flag == 1 and sc.converged()
+    assert sc.relres <= 1e-10
+    assert abs(sc.relres - np.linalg.norm(b - A @ x) / (sc.anorm * sc.xnorm + np.linalg.norm(b))) < 1e-12
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-6 * np.linalg.norm(x)
+    # relres is relative to ||A|| ||x|| + ||b||: a looser tolerance stops before ||r|| <= tol ||b||
+    hist = []
+    x6, sc6 = minresqlp(A, b, 1e-4, 10 * n, history=hist)
+    assert sc6.flag == 1 and len(hist) == sc6.iters
+    xm, km, _, _ = minres(A, b, 1e-4, 10 * n)
+    assert sc6.iters <= km
+
+
+def test_qlp_maxiter_reports_convergence():
+    A = spd(20, 1e3, 15)
+    b = np.random.default_rng(16).standard_normal(20)
+    x, sc = minresqlp(A, b, 1e-14, 3)
+    assert sc.iters == 3 and sc.flag == 8 and sc.converged() and np.any(x)
+
+
+def test_qlp_eigenvector_rhs():
+    A = np.diag(np.geomspace(1.0, 1e2, 10))
+    e = np.zeros(10)
+    e[3] = 2.0
+    x, sc = minresqlp(A, e, 1e-8, 100)
+    assert sc.flag == -1 and sc.converged()
+    assert np.allclose(x, e / A[3, 3], rtol=1e-15, atol=0)
