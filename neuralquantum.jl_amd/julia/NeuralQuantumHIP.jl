@@ -134,11 +134,15 @@ end
 
 # precondition!(cache, SR, iter) -> gradient (SRDirect.jl:51-90); owned by the
 # cache and overwritten by the next call, as in the reference (SRDirect.jl:89)
+# sr_shift: eps I; sr_multiplicative: max(λ0 b^iter, λmin) diag(S) (SRDirect.jl:56-70); sr_none: no shift
 function precondition!(is::HIPBatchedSampler, algo::SR, iter)
-    kind = algo.precondition_type == NeuralQuantum.sr_shift ? 0 : 1
+    pt = algo.precondition_type
+    eps, kind = pt == NeuralQuantum.sr_shift ? (Float64(algo.sr_diag_shift), 0) :
+                pt == NeuralQuantum.sr_multiplicative ? (Float64(max(algo.λ0 * algo.b^iter, algo.λmin)), 1) :
+                (0.0, 0)
     check(is.ctx, ccall((:nqh_precondition, libnqhip), Int32,
                         (Ptr{Cvoid}, Float64, Int32, Int32, Ptr{Float64}),
-                        is.ctx, algo.sr_diag_shift, kind, 0, is.delta))
+                        is.ctx, eps, kind, 0, is.delta))
     return is.delta
 end
 

@@ -14,6 +14,8 @@ BatchedSampler(net, sampl, liouv, algo;              BatchedSampler(net, sampl, 
 sample!(is) -> (L_stat, prec)          BatchedGradSampler.jl:77     sample_(is)
 precondition!(prec, algo, i) -> grad   SRDirect.jl:51-90            precondition_(prec, algo, i)
 Optimisers.update!(opt, net, grad)     apply.jl:9                   update_(opt, net, grad)
+add_observable!(is, name, op)          BatchedGradSampler.jl:129    add_observable_(is, name, op)
+compute_observables(is)                BatchedObsDMSampler.jl:59    compute_observables(is)
 
 The loss statistics come back as `Measurement` (stats.jl:1-13).  Device state
 (samples, O, S, the factor, the update) stays on the GPU; `net.flat` is the
@@ -32,7 +34,7 @@ from .operators import KLocalLiouvillian, flatten
 
 __all__ = ["NDM", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
            "BatchedSampler", "HIPBatchedSampler", "sample_", "precondition_", "update_",
-           "init_random_pars_"]
+           "init_random_pars_", "add_observable_", "compute_observables"]
 
 
 class NDM:
@@ -116,23 +118,39 @@ class SR:
     precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky" (SRDirect.jl),
     "cg", "minres" or "qlp" (SRIterative.jl with the matrix-free S of
     SR_notfull.jl: S is never formed; `precision` is the solver's tolerance,
-    `maxiter` 0 means the reference's 10 Np).  `precondition_type` "shift" adds eps*I
-    (reference); "diag" adds eps*diag(S).  `refine` extra FP64 refinement
-    steps of the Cholesky solve."""
+    `maxiter` 0 means the reference's 10 Np).  `precondition_type`
+    (SR.jl:4, SRDirect.jl:56-70): "none" solves S itself; "shift" adds eps*I;
+    "multiplicative" adds lambda*diag(S) with lambda = max(lambda0 * b^n,
+    lambda_min) at iteration n (the reference's Cholesky path drops this
+    shift, SRDirect.jl:67-78; here every solver applies it); "diag" adds a
+    fixed eps*diag(S).  `refine` extra FP64 refinement steps of the Cholesky
+    solve."""
 
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
-                 refine: int = 0, precision: float = 1e-4, maxiter: int = 0):
+                 refine: int = 0, precision: float = 1e-4, maxiter: int = 0, lambda0: float = 100.0,
+                 b: float = 0.95, lambda_min: float = 1e-4):
         if algorithm not in ("cholesky", "cg", "minres", "qlp"):
             raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'cg' (sr_cg), "
                                       "'minres' (sr_minres) or 'qlp' (sr_qlp)")
-        if precondition_type not in ("shift", "diag"):
-            raise ValueError("precondition_type must be 'shift' or 'diag'")
+        if precondition_type not in ("none", "shift", "multiplicative", "diag"):
+            raise ValueError("precondition_type must be 'none', 'shift', 'multiplicative' or 'diag'")
         self.eps = float(eps)
         self.algorithm = algorithm
         self.precondition_type = precondition_type
         self.refine = int(refine)
         self.precision = float(precision)
         self.maxiter = int(maxiter)
+        self.lambda0, self.b, self.lambda_min = float(lambda0), float(b), float(lambda_min)
+
+    def shift(self, it: int = 0):
+        """(eps, shift kind) of iteration `it` (SRDirect.jl:56-70)."""
+        if self.precondition_type == "none":
+            return 0.0, _lib.SHIFT_ADD
+        if self.precondition_type == "shift":
+            return self.eps, _lib.SHIFT_ADD
+        if self.precondition_type == "multiplicative":
+            return max(self.lambda0 * self.b ** it, self.lambda_min), _lib.SHIFT_DIAG
+        return self.eps, _lib.SHIFT_DIAG
 
     @property
     def solver(self):
@@ -141,7 +159,7 @@ class SR:
 
     @property
     def shift_kind(self):
-        return _lib.SHIFT_ADD if self.precondition_type == "shift" else _lib.SHIFT_DIAG
+        return self.shift(0)[1]
 
 
 class Descent:
@@ -154,7 +172,7 @@ class Descent:
 @dataclass
 class Measurement:
     """`Measurement` of stats.jl:1-13."""
-    mean: float
+    mean: float | complex
     error: float
     variance: float
     tau: float
@@ -163,6 +181,11 @@ class Measurement:
     @classmethod
     def from_array(cls, st):
         return cls(*(float(x) for x in st[:5]))
+
+    @classmethod
+    def from_observable(cls, st):
+        """[Re mean, error, variance, tau, R, Im mean] of a complex observable."""
+        return cls(complex(st[0], st[5]), *(float(x) for x in st[1:5]))
 
     def __str__(self):
         return (f"({self.mean:6.4f}) ± {self.error:6.4f} [var={self.variance:6.4f}, "
@@ -196,6 +219,8 @@ class HIPBatchedSampler:
         self.ctx.set_sampler(self.batch_sz, sampler.chain_length, sampler.passes, sampler.burn, sampler.flags)
         self.ctx.set_sr_solver(algo.solver, algo.precision, algo.maxiter)
         self._pending = False
+        self._iter = 0
+        self.observables = {}   # name -> flattened ket operator
 
     @property
     def n_samples(self):
@@ -212,7 +237,24 @@ class HIPBatchedSampler:
     def precondition(self, algo: SR, it: int = 0):
         if not self._pending:
             raise RuntimeError("call sample_(is) before precondition_")
-        return self.ctx.precondition(algo.eps, algo.shift_kind, algo.refine)
+        eps, kind = algo.shift(it)
+        return self.ctx.precondition(eps, kind, algo.refine)
+
+    def add_observable(self, name: str, op):
+        """`add_observable!(is, name, Ô)` (BatchedGradSampler.jl:129-131): a
+        ket operator on the physical space, evaluated by compute_observables."""
+        self.observables[name] = flatten(op)
+
+    def compute_observables(self):
+        """`compute_observables(is)` (BatchedObsDMSampler.jl:59-76): a fresh
+        diagonal chain with the gradient sampler's settings, then every added
+        observable's statistics; None when none was added."""
+        if not self.observables:
+            return None
+        s = self.sampler
+        return {name: Measurement.from_observable(
+                    self.ctx.observable(op, self.batch_sz, s.chain_length, s.passes, s.burn))
+                for name, op in self.observables.items()}
 
     def update(self, opt: Descent, grad=None):
         self.ctx.update(opt.eta)
@@ -222,7 +264,9 @@ class HIPBatchedSampler:
     def step(self, opt: Descent):
         """One fused SR iteration (sample!, precondition!, update!) without
         host round trips; returns the loss Measurement."""
-        st = self.ctx.sr_iteration(self.algo.eps, opt.eta, self.algo.shift_kind, self.algo.refine)
+        eps, kind = self.algo.shift(self._iter)
+        st = self.ctx.sr_iteration(eps, opt.eta, kind, self.algo.refine)
+        self._iter += 1
         return Measurement.from_array(st)
 
     def samples(self):
@@ -257,3 +301,13 @@ def update_(opt: Descent, is_or_net, grad=None):
         is_or_net.update(opt, grad)
         return is_or_net.net
     raise TypeError("pass the HIPBatchedSampler that owns the network")
+
+
+def add_observable_(is_: HIPBatchedSampler, name: str, op):
+    """`add_observable!(is, name, Ô)`."""
+    is_.add_observable(name, op)
+
+
+def compute_observables(is_: HIPBatchedSampler):
+    """`compute_observables(is)` -> {name: Measurement} (None without observables)."""
+    return is_.compute_observables()

@@ -71,3 +71,17 @@ def test_sr_mirror_algorithms():
     assert SR(0.01, algorithm="qlp").solver == _lib.SOLVER_MINRES_QLP
     with pytest.raises(NotImplementedError):
         SR(0.01, algorithm="lsq")
+
+
+def test_sr_mirror_shift_schedules():
+    """precondition_type (SR.jl:4, SRDirect.jl:56-70): none, shift and the
+    multiplicative lambda = max(lambda0 b^n, lambda_min) on diag(S)."""
+    from nqhip import SR
+    assert SR(precondition_type="none").shift(5) == (0.0, _lib.SHIFT_ADD)
+    assert SR(0.02).shift(5) == (0.02, _lib.SHIFT_ADD)
+    m = SR(precondition_type="multiplicative")     # the reference's defaults 100, 0.95, 1e-4
+    assert m.shift(0) == (100.0, _lib.SHIFT_DIAG)
+    assert abs(m.shift(10)[0] - 100.0 * 0.95 ** 10) < 1e-12
+    assert m.shift(1000) == (1e-4, _lib.SHIFT_DIAG)
+    with pytest.raises(ValueError):
+        SR(precondition_type="sr_shift")
