@@ -216,16 +216,24 @@ nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
  *                        estimates and stopping flags (relres = ||r|| /
  *                        (||A|| ||x|| + ||F||) <= tol, relAres <= tol, maxiter,
  *                        ...); as in that file a run that reaches maxiter
- *                        counts as converged and keeps its x.
+ *                        counts as converged and keeps its x;
+ *   NQH_SOLVER_LSQR      sr_lsq (SRIterative.jl:109-116, IterativeSolvers.jl
+ *                        0.8.1 `lsqr`) on the same matrix-free S: two products
+ *                        per step, the package's default tolerances (atol =
+ *                        btol = sqrt(eps), conlim = 1/sqrt(eps); tol is not
+ *                        used, as the reference passes none); zero step when
+ *                        it stops on the condition estimate or maxiter.
  * Takes effect from the next gradient (nqh_sample_grad / grad_from_samples). */
 #define NQH_SOLVER_CHOLESKY 0
 #define NQH_SOLVER_CG 1
 #define NQH_SOLVER_MINRES 2
 #define NQH_SOLVER_MINRES_QLP 3
+#define NQH_SOLVER_LSQR 4
 nqh_status nqh_set_sr_solver(nqh_ctx* ctx, int32_t solver, double tol, int32_t maxiter);
 /* products and final relative residual of the last iterative solve (CG:
  * ||r|| / ||F|| of the recurrence's r; MINRES: its residual-norm estimate over
- * ||F||; MINRES-QLP: relres = ||r|| / (||A|| ||x|| + ||F||) of its estimates) */
+ * ||F||; MINRES-QLP: relres = ||r|| / (||A|| ||x|| + ||F||) of its estimates;
+ * LSQR: its ||r|| estimate over ||F||) */
 nqh_status nqh_get_sr_solver_info(nqh_ctx* ctx, int32_t* iters, double* rel_resid);
 
 /* Optimisers.update!(Descent(lr), net, Delta) (apply.jl:25-31, rules.jl:15-17) */

@@ -162,6 +162,17 @@ __global__ __launch_bounds__(256) void k_mr_step(double* vn, const double* vc, d
     x[i] += rhs * w;
 }
 
+// LSQR: v *= inv_alfa (the new right vector); x += t1 w; w = v + t2 w
+__global__ __launch_bounds__(256) void k_ls_step(double* v, double* w, double* x, int n, double inv_alfa, double t1,
+                                                 double t2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vi = v[i] * inv_alfa, wi = w[i];
+    v[i] = vi;
+    x[i] += t1 * wi;
+    w[i] = vi + t2 * wi;
+}
+
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -237,5 +248,11 @@ hipError_t nq_launch_mr_step(double* vn, const double* vc, double* wn, const dou
                              int n, double inv_h4, double h2, double h1, double inv_h3, double rhs, hipStream_t st) {
     hipLaunchKernelGGL(k_mr_step, dim3(nblk(n, 256)), dim3(256), 0, st, vn, vc, wn, wc, wp, x, n, inv_h4, h2, h1,
                        inv_h3, rhs);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_ls_step(double* v, double* w, double* x, int n, double inv_alfa, double t1, double t2,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_ls_step, dim3(nblk(n, 256)), dim3(256), 0, st, v, w, x, n, inv_alfa, t1, t2);
     return hipGetLastError();
 }

@@ -113,6 +113,9 @@ hipError_t nq_launch_mr_axpy(double* y, const double* x, int n, double a, hipStr
 hipError_t nq_launch_mr_orth(double* vn, const double* vc, int n, const double* proj, hipStream_t st);
 hipError_t nq_launch_mr_step(double* vn, const double* vc, double* wn, const double* wc, const double* wp, double* x,
                              int n, double inv_h4, double h2, double h1, double inv_h3, double rhs, hipStream_t st);
+// LSQR step (k_cg.hip): v *= inv_alfa; x += t1 w; w = v + t2 w
+hipError_t nq_launch_ls_step(double* v, double* w, double* x, int n, double inv_alfa, double t1, double t2,
+                             hipStream_t st);
 hipError_t nq_launch_copy_scalar(const double* src, double* dst, hipStream_t st);
 
 // column partial sums: part[chunk][c] = sum_{s in chunk} Y[s, c]

@@ -1668,14 +1668,153 @@ static nqh_status minres_solve(nqh_ctx* c, double eps, int kind, bool qlp) {
     return NQH_OK;
 }
 
+// matrix-free LSQR on (S + shift) x = F (SRIterative.jl:109-116, sr_lsq:
+// IterativeSolvers.jl 0.8.1 `lsqr` with its defaults, which the reference
+// does not override: atol = btol = sqrt(eps), conlim = 1/sqrt(eps); the
+// published method of Paige and Saunders without damping).  Per step two
+// products (A v and A^T u = A u: S is symmetric), two host reads (beta, then
+// alfa with ||w||^2), the rotation and the stopping tests on the host, one
+// fused update of v, x and w.  Converged unless the run stops on the
+// condition estimate or maxiter (istop 3, 6, 7): then the step is zero, as
+// with the other iterative solvers.
+static nqh_status lsqr_solve(nqh_ctx* c, double eps, int kind) {
+    if (is_rbm(c)) return fail(c, NQH_E_STATE, "the matrix-free LSQR solver supports NDM networks only");
+    const ndm_dims& d = c->d;
+    const int np = d.np, R = d.R;
+    const int64_t Ns = c->last_ns;
+    const bool f64 = c->net_prec == NQH_PREC_FP64;
+    int nA, nB, offB;
+    gram_blocks(c, &nA, &nB, &offB);
+    constexpr int chunks = 64, nparts = 64;
+    nqh_status s;
+    if ((s = ensure(c, c->cg_w, sizeof(double) * 2 * (size_t)Ns)) ||
+        (s = ensure(c, c->cg_part, sizeof(double) * (size_t)chunks * R)) ||
+        (s = ensure(c, c->mr_vw, sizeof(double) * 6 * (size_t)np)) ||
+        (s = ensure(c, c->cg_diag, sizeof(double) * np)) || (s = ensure(c, c->cg_sc, sizeof(double) * 8)) ||
+        (s = ensure(c, c->cg_dpart, sizeof(double) * nparts)))
+        return s;
+    double* vw = P<double>(c->mr_vw);
+    double *u = vw, *v = vw + np, *w = vw + 2 * (size_t)np, *t = vw + 3 * (size_t)np;
+    double* x = P<double>(c->delta);
+    double* sc = P<double>(c->cg_sc);
+    double* diag = P<double>(c->cg_diag);
+    const double* mu = P<double>(c->mu);
+    const double* F = P<double>(c->F);
+    hipStream_t st = c->stream;
+    auto sv = [&](const double* in, double* out) {
+        return nq_launch_sv(c->Y.p, f64, Ns, R, nA, offB, np, mu, in, eps, kind, diag, P<double>(c->cg_w),
+                            P<double>(c->cg_part), chunks, out, st);
+    };
+    auto read = [&](double* h, int n) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(h, sc, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+        return e != hipSuccess ? e : hipStreamSynchronize(st);
+    };
+    const double atol = std::sqrt(DBL_EPSILON), btol = atol, ctol = atol;   // ctol = 1 / conlim
+    HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    if (kind == NQH_SHIFT_DIAG)
+        HIPCHK(c, nq_launch_sdiag(c->Y.p, f64, Ns, R, nA, offB, np, mu, P<double>(c->cg_part), chunks, diag, st));
+    // u = F / beta, v = A u / alfa, w = v
+    HIPCHK(c, hipMemcpyAsync(u, F, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    HIPCHK(c, nq_launch_dot(F, F, np, P<double>(c->cg_dpart), nparts, sc, st));
+    double h[3];
+    HIPCHK(c, read(h, 1));
+    double beta = std::sqrt(h[0]), alfa = 0.0;
+    const double bnorm = beta;
+    if (beta > 0.0) {
+        HIPCHK(c, nq_launch_mr_scale(u, np, 1.0 / beta, st));
+        HIPCHK(c, sv(u, v));
+        HIPCHK(c, nq_launch_dot(v, v, np, P<double>(c->cg_dpart), nparts, sc, st));
+        HIPCHK(c, read(h, 1));
+        alfa = std::sqrt(h[0]);
+    }
+    if (alfa > 0.0) HIPCHK(c, nq_launch_mr_scale(v, np, 1.0 / alfa, st));
+    HIPCHK(c, hipMemcpyAsync(w, v, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    double anorm = 0.0, ddnorm = 0.0, xxnorm = 0.0, z = 0.0, cs2 = -1.0, sn2 = 0.0, rnorm = beta;
+    double rhobar = alfa, phibar = beta;
+    const int maxit = c->cg_maxiter > 0 ? c->cg_maxiter : 10 * np;
+    int itn = 0, istop = 0;
+    bool finite = std::isfinite(alfa) && std::isfinite(beta);
+    if (finite && alfa * beta != 0.0) {
+        while (itn < maxit) {
+            ++itn;
+            // beta u = A v - alfa u
+            HIPCHK(c, sv(v, t));
+            HIPCHK(c, nq_launch_mr_axpy(t, u, np, -alfa, st));
+            std::swap(u, t);
+            HIPCHK(c, nq_launch_dot(u, u, np, P<double>(c->cg_dpart), nparts, sc, st));
+            HIPCHK(c, read(h, 1));
+            beta = std::sqrt(h[0]);
+            double inv_alfa = 1.0;
+            if (beta > 0.0) {
+                // alfa v = A u - beta v (v normalised by the step kernel)
+                HIPCHK(c, nq_launch_mr_scale(u, np, 1.0 / beta, st));
+                anorm = std::sqrt(anorm * anorm + alfa * alfa + beta * beta);
+                HIPCHK(c, sv(u, t));
+                HIPCHK(c, nq_launch_mr_axpy(t, v, np, -beta, st));
+                std::swap(v, t);
+                HIPCHK(c, nq_launch_dot(v, v, np, P<double>(c->cg_dpart), nparts, sc, st));
+                HIPCHK(c, nq_launch_dot(w, w, np, P<double>(c->cg_dpart), nparts, sc + 1, st));
+                HIPCHK(c, read(h, 2));
+                alfa = std::sqrt(h[0]);
+                if (alfa > 0.0) inv_alfa = 1.0 / alfa;
+            } else {
+                HIPCHK(c, nq_launch_dot(w, w, np, P<double>(c->cg_dpart), nparts, sc + 1, st));
+                HIPCHK(c, read(h, 2));
+            }
+            if (!std::isfinite(beta) || !std::isfinite(alfa) || !std::isfinite(h[1])) {
+                finite = false;
+                break;
+            }
+            const double rho = std::hypot(rhobar, beta), cs = rhobar / rho, sn = beta / rho;
+            const double theta = sn * alfa;
+            rhobar = -cs * alfa;
+            const double phi = cs * phibar;
+            phibar = sn * phibar;
+            const double tau = sn * phi;
+            const double t1 = phi / rho, t2 = -theta / rho;
+            ddnorm += h[1] / (rho * rho);
+            HIPCHK(c, nq_launch_ls_step(v, w, x, np, inv_alfa, t1, t2, st));
+            const double delta = sn2 * rho, gambar = -cs2 * rho, rhs = phi - delta * z, zbar = rhs / gambar;
+            const double xnorm = std::sqrt(xxnorm + zbar * zbar), gamma = std::hypot(gambar, theta);
+            cs2 = gambar / gamma, sn2 = theta / gamma;
+            z = rhs / gamma;
+            xxnorm += z * z;
+            const double acond = anorm * std::sqrt(ddnorm);
+            rnorm = std::fabs(phibar);
+            const double arnorm = alfa * std::fabs(tau);
+            const double test1 = rnorm / bnorm;
+            const double test2 = anorm * rnorm > 0.0 ? arnorm / (anorm * rnorm) : 0.0;
+            const double test3 = acond > 0.0 ? 1.0 / acond : HUGE_VAL;
+            const double t1n = test1 / (1 + anorm * xnorm / bnorm), rtol = btol + atol * anorm * xnorm / bnorm;
+            if (itn >= maxit) istop = 7;
+            if (1 + test3 <= 1) istop = 6;
+            if (1 + test2 <= 1) istop = 5;
+            if (1 + t1n <= 1) istop = 4;
+            if (test3 <= ctol) istop = 3;
+            if (test2 <= atol) istop = 2;
+            if (test1 <= rtol) istop = 1;
+            if (istop) break;
+        }
+    }
+    const bool ok = finite && istop != 3 && istop != 6 && istop != 7;
+    c->cg_iters = itn;
+    c->cg_resid = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+    if (!ok && finite) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
+    HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
+    return NQH_OK;
+}
+
 static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine) {
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
     if (c->solver != NQH_SOLVER_CHOLESKY) {
         phase_begin(c, PH_SOLVE);
-        const nqh_status s = c->solver == NQH_SOLVER_CG ? cg_solve(c, eps, kind)
-                                                           : minres_solve(c, eps, kind, c->solver == NQH_SOLVER_MINRES_QLP);
+        const nqh_status s = c->solver == NQH_SOLVER_CG     ? cg_solve(c, eps, kind)
+                             : c->solver == NQH_SOLVER_LSQR ? lsqr_solve(c, eps, kind)
+                                                            : minres_solve(c, eps, kind, c->solver == NQH_SOLVER_MINRES_QLP);
         phase_end(c, PH_SOLVE);
         return s;
     }
@@ -1839,7 +1978,7 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
 nqh_status nqh_set_sr_solver(nqh_ctx* c, int32_t solver, double tol, int32_t maxiter) {
     if (!c) return NQH_E_ARG;
     if (solver != NQH_SOLVER_CHOLESKY && solver != NQH_SOLVER_CG && solver != NQH_SOLVER_MINRES &&
-        solver != NQH_SOLVER_MINRES_QLP)
+        solver != NQH_SOLVER_MINRES_QLP && solver != NQH_SOLVER_LSQR)
         return fail(c, NQH_E_ARG, "unknown solver");
     if (solver != NQH_SOLVER_CHOLESKY && !(tol > 0.0)) return fail(c, NQH_E_ARG, "tol must be > 0");
     if (maxiter < 0) return fail(c, NQH_E_ARG, "maxiter must be >= 0");

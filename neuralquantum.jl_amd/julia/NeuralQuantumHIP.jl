@@ -114,10 +114,10 @@ function HIPBatchedSampler(net::NDM, sampl::MetropolisSampler, liouv, algo::SR;
     # passes: MetropolisSampler already bumped an even value to odd (Metropolis.jl:30-38)
     check(ctx, ccall((:nqh_set_sampler, libnqhip), Int32, (Ptr{Cvoid}, Int64, Int32, Int32, Int32, Int32),
                      ctx, batch_sz, sampl.chain_length, sampl.passes, sampl.burn_length, 0))
-    # SR(algorithm=sr_cg / sr_minres / sr_qlp): the matrix-free iterative
+    # SR(algorithm=sr_cg / sr_minres / sr_qlp / sr_lsq): the matrix-free iterative
     # solves (SRIterative.jl:71-152); every other algorithm factors S (SRDirect.jl)
     solver = algo.algorithm == NeuralQuantum.sr_cg ? 1 : algo.algorithm == NeuralQuantum.sr_minres ? 2 :
-             algo.algorithm == NeuralQuantum.sr_qlp ? 3 : 0
+             algo.algorithm == NeuralQuantum.sr_qlp ? 3 : algo.algorithm == NeuralQuantum.sr_lsq ? 4 : 0
     check(ctx, ccall((:nqh_set_sr_solver, libnqhip), Int32, (Ptr{Cvoid}, Int32, Float64, Int32),
                      ctx, solver, Float64(algo.sr_precision), 0))
     is = HIPBatchedSampler(ctx, net, algo, np, zeros(np), t)

@@ -24,7 +24,7 @@ NQH_E_CAPACITY = -7
 
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 SHIFT_ADD, SHIFT_DIAG = 0, 1
-SOLVER_CHOLESKY, SOLVER_CG, SOLVER_MINRES, SOLVER_MINRES_QLP = 0, 1, 2, 3
+SOLVER_CHOLESKY, SOLVER_CG, SOLVER_MINRES, SOLVER_MINRES_QLP, SOLVER_LSQR = 0, 1, 2, 3, 4
 SAMPLER_COMPAT_STALE_LAST_SLICE = 1
 SAMPLER_PERSISTENT = 2
 

@@ -116,9 +116,10 @@ class MetropolisSampler:
 class SR:
     """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift,
     precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky" (SRDirect.jl),
-    "cg", "minres" or "qlp" (SRIterative.jl with the matrix-free S of
+    "cg", "minres", "qlp" or "lsq" (SRIterative.jl with the matrix-free S of
     SR_notfull.jl: S is never formed; `precision` is the solver's tolerance,
-    `maxiter` 0 means the reference's 10 Np).  `precondition_type`
+    unused by "lsq" as in the reference; `maxiter` 0 means the reference's
+    10 Np).  `precondition_type`
     (SR.jl:4, SRDirect.jl:56-70): "none" solves S itself; "shift" adds eps*I;
     "multiplicative" adds lambda*diag(S) with lambda = max(lambda0 * b^n,
     lambda_min) at iteration n (the reference's Cholesky path drops this
@@ -129,9 +130,9 @@ class SR:
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
                  refine: int = 0, precision: float = 1e-4, maxiter: int = 0, lambda0: float = 100.0,
                  b: float = 0.95, lambda_min: float = 1e-4):
-        if algorithm not in ("cholesky", "cg", "minres", "qlp"):
+        if algorithm not in ("cholesky", "cg", "minres", "qlp", "lsq"):
             raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'cg' (sr_cg), "
-                                      "'minres' (sr_minres) or 'qlp' (sr_qlp)")
+                                      "'minres' (sr_minres), 'qlp' (sr_qlp) or 'lsq' (sr_lsq)")
         if precondition_type not in ("none", "shift", "multiplicative", "diag"):
             raise ValueError("precondition_type must be 'none', 'shift', 'multiplicative' or 'diag'")
         self.eps = float(eps)
@@ -155,7 +156,7 @@ class SR:
     @property
     def solver(self):
         return {"cholesky": _lib.SOLVER_CHOLESKY, "cg": _lib.SOLVER_CG, "minres": _lib.SOLVER_MINRES,
-                "qlp": _lib.SOLVER_MINRES_QLP}[self.algorithm]
+                "qlp": _lib.SOLVER_MINRES_QLP, "lsq": _lib.SOLVER_LSQR}[self.algorithm]
 
     @property
     def shift_kind(self):

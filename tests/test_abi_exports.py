@@ -61,7 +61,7 @@ def test_ndm_flat_layout_and_init():
 
 
 def test_sr_mirror_algorithms():
-    """SR(algorithm=...) accepts the solvers built (sr_cholesky, sr_cg, sr_minres, sr_qlp)
+    """SR(algorithm=...) accepts the solvers built (sr_cholesky, sr_cg, sr_minres, sr_qlp, sr_lsq)
     and maps them to the ABI's solver codes; others are refused."""
     from nqhip import SR
     assert SR(0.01).solver == _lib.SOLVER_CHOLESKY
@@ -69,8 +69,9 @@ def test_sr_mirror_algorithms():
     assert cg.solver == _lib.SOLVER_CG and cg.precision == 1e-6 and cg.maxiter == 0
     assert SR(0.01, algorithm="minres").solver == _lib.SOLVER_MINRES
     assert SR(0.01, algorithm="qlp").solver == _lib.SOLVER_MINRES_QLP
+    assert SR(0.01, algorithm="lsq").solver == _lib.SOLVER_LSQR
     with pytest.raises(NotImplementedError):
-        SR(0.01, algorithm="lsq")
+        SR(0.01, algorithm="pivcholesky")
 
 
 def test_sr_mirror_shift_schedules():

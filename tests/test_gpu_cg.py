@@ -11,6 +11,7 @@ from _helpers import ising, params, random_keys, rel_err
 from nqhip import Context, NQHError, _lib
 from oracle.minres_ref import minres as minres_oracle
 from oracle.minresqlp_ref import minresqlp as minresqlp_oracle
+from oracle.lsqr_ref import lsqr as lsqr_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -166,4 +167,31 @@ def test_minres_qlp_maxiter_keeps_the_iterate(gpu_ok):
     xr, sc = minresqlp_oracle(A, F, 1e-14, 3)
     assert it == sc.iters == 3 and sc.flag == 8
     assert np.any(x) and rel_err(x, xr) < 1e-10
+    b.close()
+
+
+@pytest.mark.parametrize("kind", [_lib.SHIFT_ADD, _lib.SHIFT_DIAG])
+def test_lsqr_follows_the_oracle(gpu_ok, kind):
+    """sr_lsq against the LSQR oracle on the explicit S + shift with the
+    reference's default tolerances: the same stopping reason's outcome
+    (converged), a step count within the oracle's own spread under a
+    rounding-level perturbation of S (LSQR works on the normal equations,
+    so runs part by rounding early), and the iterate after the same number of
+    steps within 10x that spread."""
+    N, eps = 6, 0.01
+    keys = random_keys(N, 128 * 8, seed=37)
+    A, F = explicit_system(N, 11, kind, eps, keys)
+    b = ndm(N, 1, 11, solver=_lib.SOLVER_LSQR, tol=1e-4)
+    b.grad_from_samples(keys, 128, 8, want_S=False)
+    x = b.precondition(eps, kind)
+    it, res = b.sr_solver_info()
+    n = len(F)
+    xr, itr, istop, ok = lsqr_oracle(A, F, 10 * n)
+    assert ok and np.any(x)
+    counts = [itr] + [lsqr_oracle(perturbed(A, s), F, 10 * n)[1] for s in range(4)]
+    assert abs(it - itr) <= max(2, 2 * (max(counts) - min(counts)))
+    xk = lsqr_oracle(A, F, it, atol=0.0, btol=0.0, conlim=0.0)[0]
+    spread = max(rel_err(lsqr_oracle(perturbed(A, s), F, it, atol=0.0, btol=0.0, conlim=0.0)[0], xk)
+                 for s in range(3))
+    assert rel_err(x, xk) <= 10 * spread + 1e-10
     b.close()

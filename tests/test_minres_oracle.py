@@ -142,3 +142,52 @@ def test_qlp_eigenvector_rhs():
     x, sc = minresqlp(A, e, 1e-8, 100)
     assert sc.flag == -1 and sc.converged()
     assert np.allclose(x, e / A[3, 3], rtol=1e-15, atol=0)
+
+
+# ---- LSQR (IterativeSolvers.jl 0.8.1 lsqr, the published Paige-Saunders method)
+from scipy.sparse.linalg import lsqr as scipy_lsqr
+
+from oracle.lsqr_ref import SQRT_EPS, lsqr
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e3])
+def test_lsqr_iterates_are_krylov_minimisers(cond):
+    """The k-th LSQR iterate minimises ||b - A x|| over K_k(A^T A, A^T b):
+    against that minimiser (orthonormal basis with full reorthogonalisation)
+    and against scipy's LSQR while the recurrence keeps orthogonality (the
+    normal equations have condition cond^2; both drift alike after that)."""
+    n, K = 30, 8
+    A = spd(n, cond, 18)
+    b = np.random.default_rng(19).standard_normal(n)
+    hist = []
+    lsqr(A, b, K, atol=0.0, btol=0.0, conlim=0.0, history=hist)
+    assert len(hist) == K
+    M, q = A.T @ A, A.T @ b
+    Q = np.zeros((n, 0))
+    q = q / np.linalg.norm(q)
+    for k in range(K):
+        Q = np.column_stack([Q, q])
+        z = M @ q
+        for _ in range(2):
+            z = z - Q @ (Q.T @ z)
+        q = z / np.linalg.norm(z)
+        xe = Q @ np.linalg.lstsq(A @ Q, b, rcond=None)[0]
+        xs = scipy_lsqr(A, b, atol=0.0, btol=0.0, conlim=0.0, iter_lim=k + 1)[0]
+        assert np.linalg.norm(hist[k][0] - xe) <= 1e-11 * np.linalg.norm(xe)
+        assert np.linalg.norm(hist[k][0] - xs) <= 1e-11 * np.linalg.norm(xs)
+        assert abs(hist[k][1]["rnorm"] - np.linalg.norm(b - A @ hist[k][0])) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("cond,seed", [(1e2, 20), (1e4, 21), (1e9, 22)])
+def test_lsqr_stops_where_scipy_stops(cond, seed):
+    """The reference's defaults (atol = btol = sqrt(eps), conlim = 1/sqrt(eps),
+    maxiter = 10 n): the same stopping reason as scipy's LSQR, a step count
+    within 5 % (rounding moves it by a few steps), and the solution."""
+    n = 40
+    A = spd(n, cond, seed)
+    b = np.random.default_rng(seed + 100).standard_normal(n)
+    x, itn, istop, ok = lsqr(A, b, 10 * n)
+    xs, istop_s, itn_s = scipy_lsqr(A, b, atol=SQRT_EPS, btol=SQRT_EPS, conlim=1 / SQRT_EPS, iter_lim=10 * n)[:3]
+    assert istop == istop_s and abs(itn - itn_s) <= 0.05 * itn_s
+    assert np.linalg.norm(x - xs) <= 1e-4 * np.linalg.norm(xs)
+    assert ok == (istop not in (3, 6, 7))
