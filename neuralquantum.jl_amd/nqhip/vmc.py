@@ -3,6 +3,7 @@
 Julia (reference)                                   here
 -------------------------------------------------   ---------------------------------------------
 NDM(Float64, N, αh, αa, af_softplus)  NDM.jl:50     NDM(N, alpha_h, alpha_a)
+RBM(ComplexF64, N, α)                  RBM.jl:37-44  RBM(N, alpha)   (kets, with a Hamiltonian)
 init_random_pars!(net; sigma)          rng.jl:20    net.init_random_pars(rng, sigma)
 MetropolisSampler(LocalRule(), L, passes; burn,     MetropolisSampler(LocalRule(), L, passes,
                   seed)                Metropolis.jl:26                  burn=, seed=)
@@ -30,9 +31,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .operators import KLocalLiouvillian, flatten
+from .operators import KLocalLiouvillian, KLocalOperator, KLocalOperatorSum, flatten
 
-__all__ = ["NDM", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
+__all__ = ["NDM", "RBM", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
            "BatchedSampler", "HIPBatchedSampler", "sample_", "precondition_", "update_",
            "init_random_pars_", "add_observable_", "compute_observables"]
 
@@ -69,6 +70,35 @@ class NDM:
         """`init_random_pars!(net; sigma)`: randn * sqrt(sigma) (rng.jl:20-31)."""
         rng = rng if rng is not None else np.random.default_rng()
         self.flat[:] = rng.standard_normal(self.n_params) * math.sqrt(sigma)
+        return self
+
+
+class RBM:
+    """`RBM(ComplexF64, N, α)` on kets (RBM.jl:37-44, af_softplus): complex
+    a[N], b[nh], W[nh x N] in the flat order a, b, W (W column-major),
+    log psi(s) = a.s + sum softplus(b + W s)."""
+
+    def __init__(self, N: int, alpha: float):
+        self.N = int(N)
+        self.nh = int(alpha * N)
+        if self.nh != alpha * N:
+            raise ValueError("alpha * N must be an integer (convert(Int, α*in), RBM.jl:44)")
+        N, nh = self.N, self.nh
+        self.flat = np.zeros(N + nh + nh * N, dtype=np.complex128)
+        self.a = self.flat[:N]
+        self.b = self.flat[N:N + nh]
+        self.W = self.flat[N + nh:].reshape((nh, N), order="F")
+
+    @property
+    def n_params(self) -> int:
+        return len(self.flat)
+
+    def init_random_pars(self, rng=None, sigma: float = 0.01):
+        """`init_random_pars!(net; sigma)`: complex randn (unit variance) *
+        sqrt(sigma) (rng.jl:20-31)."""
+        rng = rng if rng is not None else np.random.default_rng()
+        z = rng.standard_normal(self.n_params) + 1j * rng.standard_normal(self.n_params)
+        self.flat[:] = z * math.sqrt(sigma / 2)
         return self
 
 
@@ -199,13 +229,25 @@ _PREC = {"fp64": (_lib.PREC_FP64, _lib.PREC_FP64),
 
 
 class HIPBatchedSampler:
-    """`BatchedGradSampler` (BatchedGradSampler.jl:29-75) running on libnqhip."""
+    """`BatchedGradSampler` (BatchedGradSampler.jl:29-75: an NDM and a
+    KLocalLiouvillian, the L†L problem) or `BatchedValSampler`
+    (BatchedValSampler.jl:70-139: a ket RBM and a Hamiltonian operator, the
+    energy problem), as build_Batched.jl:1-14 dispatches, on libnqhip."""
 
-    def __init__(self, net: NDM, sampler: MetropolisSampler, prob: KLocalLiouvillian, algo: SR,
+    def __init__(self, net, sampler: MetropolisSampler, prob, algo: SR,
                  batch_sz: int = 16, device: int = 0, precision: str = "fp64"):
-        if not isinstance(prob, KLocalLiouvillian):
-            raise NotImplementedError("only the KLocalLiouvillian (L†L) problem is implemented")
-        if prob.phys.n_sites != net.N:
+        ket = isinstance(prob, (KLocalOperator, KLocalOperatorSum))
+        if isinstance(prob, KLocalLiouvillian):
+            if not isinstance(net, NDM):
+                raise NotImplementedError("the L†L problem takes an NDM")
+            n_sites = prob.phys.n_sites
+        elif ket:
+            if not isinstance(net, RBM):
+                raise NotImplementedError("a Hamiltonian takes a ket RBM")
+            n_sites = prob.hilb.n_sites if hasattr(prob, "hilb") else net.N
+        else:
+            raise NotImplementedError("prob must be a KLocalLiouvillian or a Hamiltonian operator")
+        if n_sites != net.N:
             raise ValueError("network and problem have different numbers of sites")
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {sorted(_PREC)}")
@@ -214,9 +256,12 @@ class HIPBatchedSampler:
         self.precision = precision
         self.ctx = _lib.Context(device, sampler.seed)
         self.ctx.set_problem(flatten(prob))
-        self.ctx.set_ndm(net.N, net.nh, net.na)
+        if ket:
+            self.ctx.set_rbm(net.N, net.nh)
+        else:
+            self.ctx.set_ndm(net.N, net.nh, net.na)
         self.ctx.set_precision(*_PREC[precision])
-        self.ctx.set_params(net.flat)
+        self.ctx.set_params(net.flat.view(np.float64))
         self.ctx.set_sampler(self.batch_sz, sampler.chain_length, sampler.passes, sampler.burn, sampler.flags)
         self.ctx.set_sr_solver(algo.solver, algo.precision, algo.maxiter)
         self._pending = False
@@ -228,7 +273,7 @@ class HIPBatchedSampler:
         return self.batch_sz * self.sampler.chain_length
 
     def push_params(self):
-        self.ctx.set_params(self.net.flat)
+        self.ctx.set_params(self.net.flat.view(np.float64))
 
     def sample(self):
         st = self.ctx.sample_grad()
@@ -259,7 +304,7 @@ class HIPBatchedSampler:
 
     def update(self, opt: Descent, grad=None):
         self.ctx.update(opt.eta)
-        self.net.flat[:] = self.ctx.get_params()
+        self.net.flat[:] = self.ctx.get_params().view(self.net.flat.dtype)
         self._pending = False
 
     def step(self, opt: Descent):

@@ -102,3 +102,42 @@ def test_fused_step_follows_the_schedule(gpu_ok):
     finally:
         a.close()
         b.close()
+
+
+def test_hamiltonian_energy_as_the_reference_test(gpu_ok):
+    """test/solvers/ising1d_closed.jl: a ket RBM(N, 2) on the closed Ising
+    chain, BatchedSampler(net, sampler, H, SR(); batch_sz=32), one sample!:
+    the sampled energy agrees with the exact <psi|H|psi>/<psi|psi> of the
+    same network (dense H from the operator algebra, psi from the oracle's
+    RBM), here within 4 standard errors of the sampled statistics; then a
+    few SR steps lower the energy."""
+    from nqhip import RBM
+    from oracle import rbm_ref
+    N, g, V = 4, 0.7, 2.0
+    h = hilbert.HomogeneousSpin(N)
+    H = operators.LocalOperator(h)
+    for i in range(1, N + 1):
+        H = H + (g / 2.0) * operators.sigmax(h, i)
+        H = H + (V / 4.0) * (operators.sigmaz(h, i) * operators.sigmaz(h, i % N + 1))
+    net = RBM(N, 2).init_random_pars(np.random.default_rng(6), sigma=0.01)
+    with pytest.warns(UserWarning):
+        sampl = MetropolisSampler(LocalRule(), 300, N, burn=30, seed=11)
+    algo = SR(0.001)
+    is_ = BatchedSampler(net, sampl, H, algo, batch_sz=32)
+    try:
+        res, prec = sample_(is_)
+        keys = np.arange(2 ** N, dtype=np.uint64)
+        psi = np.exp(rbm_ref.ket_logpsi(net.flat, keys, N, net.nh))
+        Hm = operators.to_matrix(H)
+        E_ex = (np.vdot(psi, Hm @ psi) / np.vdot(psi, psi)).real
+        assert 0 < res.error < 0.1
+        assert abs(res.mean - E_ex) < 4 * res.error
+        first = res.mean
+        for it in range(1, 31):
+            grad = precondition_(prec, algo, it)
+            assert np.iscomplexobj(grad) and np.all(np.isfinite(grad))
+            update_(Descent(0.05), is_, grad)
+            res, prec = sample_(is_)
+        assert res.mean < first - 0.1
+    finally:
+        is_.close()
