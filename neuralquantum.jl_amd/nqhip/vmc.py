@@ -4,6 +4,7 @@ Julia (reference)                                   here
 -------------------------------------------------   ---------------------------------------------
 NDM(Float64, N, αh, αa, af_softplus)  NDM.jl:50     NDM(N, alpha_h, alpha_a)
 RBM(ComplexF64, N, α)                  RBM.jl:37-44  RBM(N, alpha)   (kets, with a Hamiltonian)
+RBMSplit(ComplexF64, N, α)             RBMSplit.jl:33-40  RBMSplit(N, alpha)   (density matrices)
 init_random_pars!(net; sigma)          rng.jl:20    net.init_random_pars(rng, sigma)
 MetropolisSampler(LocalRule(), L, passes; burn,     MetropolisSampler(LocalRule(), L, passes,
                   seed)                Metropolis.jl:26                  burn=, seed=)
@@ -33,7 +34,7 @@ import numpy as np
 from . import _lib
 from .operators import KLocalLiouvillian, KLocalOperator, KLocalOperatorSum, flatten
 
-__all__ = ["NDM", "RBM", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
+__all__ = ["NDM", "RBM", "RBMSplit", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
            "BatchedSampler", "HIPBatchedSampler", "sample_", "precondition_", "update_",
            "init_random_pars_", "add_observable_", "compute_observables"]
 
@@ -100,6 +101,25 @@ class RBM:
         z = rng.standard_normal(self.n_params) + 1j * rng.standard_normal(self.n_params)
         self.flat[:] = z * math.sqrt(sigma / 2)
         return self
+
+
+class RBMSplit(RBM):
+    """`RBMSplit(ComplexF64, N, α)` (RBMSplit.jl:33-40): a density matrix
+    rho(s, e) from complex ar[N], ac[N], b[nh], Wr[nh x N], Wc[nh x N] in that
+    flat order (matrices column-major)."""
+
+    def __init__(self, N: int, alpha: float):
+        self.N = int(N)
+        self.nh = int(alpha * N)
+        if self.nh != alpha * N:
+            raise ValueError("alpha * N must be an integer (convert(Int, α*in), RBMSplit.jl:40)")
+        N, nh = self.N, self.nh
+        self.flat = np.zeros(2 * N + nh + 2 * nh * N, dtype=np.complex128)
+        self.ar, self.ac = self.flat[:N], self.flat[N:2 * N]
+        self.b = self.flat[2 * N:2 * N + nh]
+        o = 2 * N + nh
+        self.Wr = self.flat[o:o + nh * N].reshape((nh, N), order="F")
+        self.Wc = self.flat[o + nh * N:].reshape((nh, N), order="F")
 
 
 def init_random_pars_(net: NDM, rng=None, sigma: float = 0.01):
@@ -238,11 +258,11 @@ class HIPBatchedSampler:
                  batch_sz: int = 16, device: int = 0, precision: str = "fp64"):
         ket = isinstance(prob, (KLocalOperator, KLocalOperatorSum))
         if isinstance(prob, KLocalLiouvillian):
-            if not isinstance(net, NDM):
-                raise NotImplementedError("the L†L problem takes an NDM")
+            if not isinstance(net, (NDM, RBMSplit)):
+                raise NotImplementedError("the L†L problem takes an NDM or an RBMSplit")
             n_sites = prob.phys.n_sites
         elif ket:
-            if not isinstance(net, RBM):
+            if not isinstance(net, RBM) or isinstance(net, RBMSplit):
                 raise NotImplementedError("a Hamiltonian takes a ket RBM")
             n_sites = prob.hilb.n_sites if hasattr(prob, "hilb") else net.N
         else:
@@ -258,6 +278,8 @@ class HIPBatchedSampler:
         self.ctx.set_problem(flatten(prob))
         if ket:
             self.ctx.set_rbm(net.N, net.nh)
+        elif isinstance(net, RBMSplit):
+            self.ctx.set_rbmsplit(net.N, net.nh)
         else:
             self.ctx.set_ndm(net.N, net.nh, net.na)
         self.ctx.set_precision(*_PREC[precision])

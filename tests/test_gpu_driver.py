@@ -141,3 +141,39 @@ def test_hamiltonian_energy_as_the_reference_test(gpu_ok):
         assert res.mean < first - 0.1
     finally:
         is_.close()
+
+
+def test_rbmsplit_driver_matches_the_abi(gpu_ok):
+    """RBMSplit (RBMSplit.jl) through BatchedSampler / sample! / precondition! /
+    update! against the same steps through the C ABI (complex SR: the step
+    comes back complex)."""
+    from nqhip import RBMSplit
+    N, batch, L, passes, burn, seed = 4, 16, 16, 5, 10, 21
+    liouv, _ = example_problem(N)
+    net = RBMSplit(N, 1).init_random_pars(np.random.default_rng(8), sigma=0.01)
+    p0 = net.flat.copy()
+    algo = SR(0.01)
+    is_ = BatchedSampler(net, MetropolisSampler(LocalRule(), L, passes, burn=burn, seed=seed), liouv, algo,
+                         batch_sz=batch)
+    ctx = Context(0, seed)
+    ctx.set_problem(operators.flatten(liouv))
+    ctx.set_rbmsplit(N, N)
+    ctx.set_precision(_lib.PREC_FP64, _lib.PREC_FP64)
+    ctx.set_params(p0.view(np.float64))
+    ctx.set_sampler(batch, L, passes, burn, 0)
+    try:
+        for i in range(1, 3):
+            ldata, prec = sample_(is_)
+            grad = precondition_(prec, algo, i)
+            update_(Descent(0.01), is_, grad)
+            st = ctx.sample_grad()
+            d = ctx.precondition(0.01)
+            ctx.update(0.01)
+            assert np.iscomplexobj(grad) and grad.shape == (net.n_params,)
+            assert abs(ldata.mean - st[0]) <= 1e-10 * abs(st[0])
+            np.testing.assert_allclose(grad, d, rtol=0, atol=1e-10 * np.abs(d).max())
+            np.testing.assert_allclose(net.flat, ctx.get_params().view(np.complex128), rtol=0, atol=1e-12)
+        assert not np.array_equal(net.flat, p0)
+    finally:
+        is_.close()
+        ctx.close()
