@@ -165,7 +165,8 @@ class MetropolisSampler:
 
 class SR:
     """`SR(; ϵ=0.001, algorithm=sr_cholesky, precondition_type=sr_shift,
-    precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky" (SRDirect.jl),
+    precision=1e-4)` (SR.jl:37-57).  `algorithm` "cholesky", "diag" or "div"
+    (SRDirect.jl; all three by the Cholesky factorization of the shifted S),
     "cg", "minres", "qlp" or "lsq" (SRIterative.jl with the matrix-free S of
     SR_notfull.jl: S is never formed; `precision` is the solver's tolerance,
     unused by "lsq" as in the reference; `maxiter` 0 means the reference's
@@ -180,9 +181,10 @@ class SR:
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
                  refine: int = 0, precision: float = 1e-4, maxiter: int = 0, lambda0: float = 100.0,
                  b: float = 0.95, lambda_min: float = 1e-4):
-        if algorithm not in ("cholesky", "cg", "minres", "qlp", "lsq"):
-            raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'cg' (sr_cg), "
-                                      "'minres' (sr_minres), 'qlp' (sr_qlp) or 'lsq' (sr_lsq)")
+        if algorithm not in ("cholesky", "diag", "div", "cg", "minres", "qlp", "lsq"):
+            # sr_pivcholesky is declared by the reference but throws "Alg not known" (SRDirect.jl:85-86)
+            raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'diag' (sr_diag), 'div' "
+                                      "(sr_div), 'cg' (sr_cg), 'minres' (sr_minres), 'qlp' (sr_qlp) or 'lsq' (sr_lsq)")
         if precondition_type not in ("none", "shift", "multiplicative", "diag"):
             raise ValueError("precondition_type must be 'none', 'shift', 'multiplicative' or 'diag'")
         self.eps = float(eps)
@@ -205,7 +207,10 @@ class SR:
 
     @property
     def solver(self):
-        return {"cholesky": _lib.SOLVER_CHOLESKY, "cg": _lib.SOLVER_CG, "minres": _lib.SOLVER_MINRES,
+        # sr_diag (pseudo-inverse) and sr_div (Bunch-Kaufman) solve the same shifted,
+        # positive definite S: the Cholesky factorization gives their step
+        return {"cholesky": _lib.SOLVER_CHOLESKY, "diag": _lib.SOLVER_CHOLESKY, "div": _lib.SOLVER_CHOLESKY,
+                "cg": _lib.SOLVER_CG, "minres": _lib.SOLVER_MINRES,
                 "qlp": _lib.SOLVER_MINRES_QLP, "lsq": _lib.SOLVER_LSQR}[self.algorithm]
 
     @property

@@ -70,6 +70,7 @@ def test_sr_mirror_algorithms():
     assert SR(0.01, algorithm="minres").solver == _lib.SOLVER_MINRES
     assert SR(0.01, algorithm="qlp").solver == _lib.SOLVER_MINRES_QLP
     assert SR(0.01, algorithm="lsq").solver == _lib.SOLVER_LSQR
+    assert SR(0.01, algorithm="div").solver == SR(0.01, algorithm="diag").solver == _lib.SOLVER_CHOLESKY
     with pytest.raises(NotImplementedError):
         SR(0.01, algorithm="pivcholesky")
 
