@@ -230,7 +230,8 @@ nqh_status nqh_precondition(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
 #define NQH_SOLVER_MINRES_QLP 3
 #define NQH_SOLVER_LSQR 4
 nqh_status nqh_set_sr_solver(nqh_ctx* ctx, int32_t solver, double tol, int32_t maxiter);
-/* products and final relative residual of the last iterative solve (CG:
+/* steps (one product each, two for LSQR) and final relative residual of the
+ * last iterative solve (CG:
  * ||r|| / ||F|| of the recurrence's r; MINRES: its residual-norm estimate over
  * ||F||; MINRES-QLP: relres = ||r|| / (||A|| ||x|| + ||F||) of its estimates;
  * LSQR: its ||r|| estimate over ||F||) */
