@@ -604,28 +604,38 @@ __global__ void k_split_combine(const double* S, int lds, const double* F, int n
 }
 
 // y1[i] -= sum_{c in [nl, m)} A1[i, c] x0[c] for i < nl (the mu rows' share of
-// the Pi solution; A1's upper triangle holds L[c, i] transposed): 64 rows x 16
-// column groups per workgroup, fixed-order combination
-__global__ __launch_bounds__(1024) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
-                                                    double* y1) {
-    __shared__ double red[16][64];
+// the Pi solution; A1's upper triangle holds L[c, i] transposed), in two
+// fixed-order stages: k_split_upd gives every (64-row block, column chunk) pair
+// a workgroup (64 rows x 4 column groups) and writes part[chunk][i];
+// k_split_upd_fin sums the chunks in order
+constexpr int kUpdChunks = 16;
+
+__global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
+                                                   double* part) {
+    __shared__ double red[4][64];
     const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + r;
     const int ic = i < nl ? i : nl - 1;
+    const int w = (m - nl + kUpdChunks - 1) / kUpdChunks;
+    const int c0 = nl + blockIdx.y * w, c1 = min(m, c0 + w);
     double a0 = 0.0, a1 = 0.0;
-    int c = nl + g;
-    for (; c + 16 < m; c += 32) {
+    int c = c0 + g;
+    for (; c + 4 < c1; c += 8) {
         a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
-        a1 += A1[(int64_t)ic + (int64_t)(c + 16) * lda] * x0[c + 16];
+        a1 += A1[(int64_t)ic + (int64_t)(c + 4) * lda] * x0[c + 4];
     }
-    if (c < m) a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
+    if (c < c1) a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
     red[g][r] = a0 + a1;
     __syncthreads();
-    if (g == 0 && i < nl) {
-        double t = 0.0;
-        for (int q = 0; q < 16; ++q) t += red[q][r];
-        y1[i] -= t;
-    }
+    if (g == 0 && i < nl) part[(int64_t)blockIdx.y * nl + i] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+}
+
+__global__ void k_split_upd_fin(const double* part, int nl, double* y1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    double t = 0.0;
+    for (int q = 0; q < kUpdChunks; ++q) t += part[(int64_t)q * nl + i];
+    y1[i] -= t;
 }
 
 // delta in P order from the two instances' solutions
@@ -812,6 +822,8 @@ hipError_t nq_cholesky_solve(const double* L, int n, int lds, const double* winv
     return nq_cholesky_backsolve(L, n, lds, winv, tmp, y, st);
 }
 
+size_t nq_split_x_doubles(int nl, int npi) { return 2 * (size_t)(nl + npi + 1) + (size_t)kUpdChunks * nl; }
+
 size_t nq_split_winv_doubles(int nl, int npi) {
     return 2 * nq_cholesky_winv_doubles(nl) + nq_cholesky_winv_doubles(npi);
 }
@@ -844,7 +856,9 @@ hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* 
     if ((e = nq_copy_aug_col(A2 + sA, m, lda, y1, st)) != hipSuccess) return e;
     // x_Pi, with the lambda rows of y updated on the way; then the mu rows
     if ((e = nq_cholesky_backsolve_range(A2, nl, m, lda, winvP, y0, x0, 1, 0, 0, 0, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_split_upd, dim3(nblk(nl, 64)), dim3(1024), 0, st, A2 + sA, lda, nl, m, x0, y1);
+    double* part = x2 + 2 * sY;   // kUpdChunks x nl scratch after the two solutions
+    hipLaunchKernelGGL(k_split_upd, dim3(nblk(nl, 64), kUpdChunks), dim3(256), 0, st, A2 + sA, lda, nl, m, x0, part);
+    hipLaunchKernelGGL(k_split_upd_fin, dim3(nblk(nl, 256)), dim3(256), 0, st, part, nl, y1);
     // x_lambda and x_mu together
     if ((e = nq_cholesky_backsolve_range(A2, 0, nl, lda, winv, y0, x0, 2, sA, sW, sY, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_split_gather, dim3(nblk(m + nl, 256)), dim3(256), 0, st, x0, x1, nl, npi, m + nl, delta);

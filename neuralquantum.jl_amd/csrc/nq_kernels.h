@@ -157,9 +157,11 @@ hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* inf
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
                                        double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st);
 // the NDM's shifted SR solve through the lambda / mu split (k_solve.hip):
-// A2 2 (m+1)^2 doubles, winv nq_split_winv_doubles, y2 and x2 2 (m+1), m = nl + npi;
-// delta in P order (np = 2 nl + npi)
+// A2 2 (m+1)^2 doubles, winv nq_split_winv_doubles, y2 2 (m+1), x2
+// nq_split_x_doubles (the two solutions, then scratch), m = nl + npi; delta in
+// P order (np = 2 nl + npi)
 size_t nq_split_winv_doubles(int nl, int npi);
+size_t nq_split_x_doubles(int nl, int npi);
 hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
                            double* A2, double* winv, int* info, hipStream_t st);
 hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,

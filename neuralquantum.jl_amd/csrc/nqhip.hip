@@ -293,7 +293,7 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
             if ((s = ensure(c, c->A2, sizeof(double) * 2 * lda * lda))) return s;
             if ((s = ensure(c, c->winv2, sizeof(double) * nq_split_winv_doubles(c->d.nlam, c->d.npi)))) return s;
             if ((s = ensure(c, c->y2, sizeof(double) * 2 * lda))) return s;
-            if ((s = ensure(c, c->x2, sizeof(double) * 2 * lda))) return s;
+            if ((s = ensure(c, c->x2, sizeof(double) * nq_split_x_doubles(c->d.nlam, c->d.npi)))) return s;
         }
     }
     if ((s = ensure(c, c->fpart2, sizeof(double) * (size_t)NQ_FORCE_GROUPS * nsys(c)))) return s;
