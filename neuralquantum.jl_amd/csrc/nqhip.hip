@@ -1557,6 +1557,7 @@ static nqh_status qlp_loop(nqh_ctx* c, double eps, int kind, double res0, int ma
         q.finish();
         iteration = q.iters;
     }
+    if (!finite) return fail(c, NQH_E_NAN, "non-finite value in the MINRES-QLP recurrence");
     c->cg_iters = q.iters;
     c->cg_resid = q.relres;
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
@@ -1658,10 +1659,11 @@ static nqh_status minres_solve(nqh_ctx* c, double eps, int kind, bool qlp) {
         resnorm = std::fabs(rhs2);
         ++it;
     }
-    const bool ok = finite && resnorm <= tolerance;
+    if (!finite) return fail(c, NQH_E_NAN, "non-finite value in the MINRES recurrence");
+    const bool ok = resnorm <= tolerance;
     c->cg_iters = it - 1;
     c->cg_resid = res0 > 0.0 ? resnorm / res0 : 0.0;
-    if (!ok && finite) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    if (!ok) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
     HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
@@ -1796,10 +1798,11 @@ static nqh_status lsqr_solve(nqh_ctx* c, double eps, int kind) {
             if (istop) break;
         }
     }
-    const bool ok = finite && istop != 3 && istop != 6 && istop != 7;
+    if (!finite) return fail(c, NQH_E_NAN, "non-finite value in the LSQR recurrence");
+    const bool ok = istop != 3 && istop != 6 && istop != 7;
     c->cg_iters = itn;
     c->cg_resid = bnorm > 0.0 ? rnorm / bnorm : 0.0;
-    if (!ok && finite) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
+    if (!ok) HIPCHK(c, nq_launch_fill(x, np, 0.0, st));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->info), 1, 0u, st));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, st));
     HIPCHK(c, nq_launch_check_finite(x, np, P<int>(c->flag), st));
