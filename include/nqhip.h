@@ -106,7 +106,8 @@ const char* nqh_last_error(const nqh_ctx* ctx);
 nqh_status nqh_set_problem(nqh_ctx* ctx, const nqh_optable* table);
 
 /* NDM(Float64, N, alpha_h, alpha_a, af_softplus) with nh = alpha_h N hidden and
- * na = alpha_a N ancilla units (NDM.jl:50-63); act 0 = softplus (the default) */
+ * na = alpha_a N ancilla units (NDM.jl:50-63); act 0 = softplus (the default).
+ * N in 1..16 (doubled keys of 32 bits), nh and na in 1..64 */
 nqh_status nqh_set_ndm(nqh_ctx* ctx, int32_t N, int32_t nh, int32_t na, int32_t act);
 
 /* RBMSplit(T, N, alpha) with nh = alpha N hidden units (RBMSplit.jl:33-40):

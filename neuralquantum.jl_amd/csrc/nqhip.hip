@@ -820,8 +820,9 @@ nqh_status nqh_set_problem(nqh_ctx* c, const nqh_optable* t) {
 nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t act) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
-    if (N < 1 || N > 32) return fail(c, NQH_E_ARG, "N must be in 1..32");
-    if (nh < 1 || na < 1 || nh > 128 || na > 128) return fail(c, NQH_E_ARG, "nh, na must be in 1..128");
+    // the estimator's limits: 2N-bit keys in 32 bits of lane state, two angle registers per lane
+    if (N < 1 || N > 16) return fail(c, NQH_E_ARG, "N must be in 1..16");
+    if (nh < 1 || na < 1 || nh > 64 || na > 64) return fail(c, NQH_E_ARG, "nh, na must be in 1..64");
     if (act != 0) return fail(c, NQH_E_ARG, "only the softplus activation (af_softplus) is implemented");
     c->d = make_ndm_dims(N, nh, na);
     c->net_kind = 0;
