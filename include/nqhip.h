@@ -5,8 +5,8 @@
  * The reference's boundary for this path is Julia method dispatch, not an
  * FFI; every entry point below names the reference method it stands in for
  * (file:line in NeuralQuantum.jl).  A Julia wrapper `ccall`s these symbols
- * (see INTEGRATION.md); in this repository they are exercised from Python
- * through ctypes.
+ * (neuralquantum.jl_amd/julia/NeuralQuantumHIP.jl); in this repository they
+ * are exercised from Python through ctypes.
  *
  * Conventions
  *   - status codes only, nothing is thrown across the ABI; on error the
