@@ -17,7 +17,7 @@ the device.  Two forms are timed:
 `value` is the chosen form's SR iterations/s.
 
 Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per process,
-no collectives in this build).
+libnqhip has no collectives).
 """
 from __future__ import annotations
 
@@ -449,7 +449,7 @@ def main():
                          "scaled by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU per process and has no multi-GPU path; "
+        print("bench.py: libnqhip runs on one GPU per process and has no multi-GPU path; "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

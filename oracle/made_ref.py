@@ -11,7 +11,7 @@ two hidden ReLU layers of width H, sigmoid Bernoulli conditionals
 with the masks of the natural ordering: input degree d + 1, hidden degrees
 m(k) = (k mod (D - 1)) + 1, W1[k, d] kept iff m1(k) >= d + 1, W2[k', k] kept
 iff m2(k') >= m1(k), W3[d, k'] kept iff d + 1 > m2(k').  The reference has no
-such network (it is this build's addition, plugged in through the sampler's
+such network (it is an addition of libnqhip, plugged in through the sampler's
 `log_prob_bias` hook, src/Samplers/Metropolis.jl:54-67,149).
 
 Parity status: pinned by normalisation (sum_x q(x) = 1 over all 2^D
