@@ -1,4 +1,4 @@
-"""The host-side mirror of the reference's driver API, run as the reference's
+"""The host-side mirror of the reference's VMC API, run as the reference's
 example runs it (examples/dissipative_ising1d.jl: BatchedSampler, sample!,
 compute_observables, precondition!, Optimisers.update!), against the same
 steps made directly through the C ABI with the same seeds.  Both paths issue
