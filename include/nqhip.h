@@ -105,9 +105,13 @@ const char* nqh_last_error(const nqh_ctx* ctx);
  * argument of BatchedSampler(net, sampler, prob, algo) (build_Batched.jl:1-14) */
 nqh_status nqh_set_problem(nqh_ctx* ctx, const nqh_optable* table);
 
-/* NDM(Float64, N, alpha_h, alpha_a, af_softplus) with nh = alpha_h N hidden and
- * na = alpha_a N ancilla units (NDM.jl:50-63); act 0 = softplus (the default).
- * N in 1..16 (doubled keys of 32 bits), nh and na in 1..64 */
+/* NDM activation f (src/Networks/activation.jl) */
+#define NQH_ACT_SOFTPLUS 0   /* af_softplus = logℒ, f' = 1/(1+e^-x) (:14-18), the NDM default */
+#define NQH_ACT_LOGCOSH  1   /* af_logcosh = logℒ2, f' = tanh (:5-10, :22-26) */
+
+/* NDM(Float64, N, alpha_h, alpha_a, f) with nh = alpha_h N hidden and
+ * na = alpha_a N ancilla units (NDM.jl:50-63); act = NQH_ACT_SOFTPLUS or
+ * NQH_ACT_LOGCOSH.  N in 1..16 (doubled keys of 32 bits), nh and na in 1..64 */
 nqh_status nqh_set_ndm(nqh_ctx* ctx, int32_t N, int32_t nh, int32_t na, int32_t act);
 
 /* RBMSplit(T, N, alpha) with nh = alpha N hidden units (RBMSplit.jl:33-40):
@@ -181,6 +185,13 @@ nqh_status nqh_set_iteration(nqh_ctx* ctx, uint32_t iter);
 /* one sampling pass (_sample_state!, BaseIterativeSampler.jl:5-17); keys_out
  * (nullable): chain_length x chains; accepted (nullable): accepted proposals */
 nqh_status nqh_sample(nqh_ctx* ctx, uint64_t* keys_out, int64_t* accepted);
+
+/* accepted moves per chain in the last sampling call (local proposals and,
+ * with the learned proposal, global ones), the acceptance counter of
+ * Metropolis.jl:57-59,162-163 split by chain: a chain at 0 has not moved
+ * (stuck).  accepted_out: `chains` entries.  Valid after nqh_sample,
+ * nqh_sample_grad or an SR iteration. */
+nqh_status nqh_get_chain_acceptance(nqh_ctx* ctx, uint32_t* accepted_out, int64_t chains);
 
 /* sample!(is) (BatchedGradSampler.jl:77-123): sample, log rho, O, L_loc,
  * statistics, centring, force and the S build; stats: 5 doubles (nullable) */
@@ -260,9 +271,12 @@ nqh_status nqh_wait(nqh_ctx* ctx, double* stats);
  * is launched once per iteration (members with persistent chains run their
  * first, restart + burn-in, iteration eagerly and concurrently).  Members
  * are fully set-up contexts on one device (problem, network, sampler; no
- * learned proposal); they must not be reconfigured (nqh_set_problem,
- * nqh_set_ndm / rbmsplit / rbm, nqh_set_sampler, nqh_set_precision) nor
- * destroyed while the batch exists; reading parameters or samples between
+ * learned proposal) and must not be destroyed while the batch exists.  A
+ * member reconfigured between batch iterations (nqh_set_problem,
+ * nqh_set_ndm / rbmsplit / rbm, nqh_set_sampler, nqh_set_precision,
+ * nqh_set_iteration, or a buffer regrown by its own calls) makes the next
+ * batch iteration capture the graph again with the member's new buffers,
+ * shapes and iteration number; reading parameters or samples between
  * iterations is fine, and a member's own calls stay ordered with the
  * batch's iterations.  stats: n x 5, member i at 5 i in nqh_sr_iteration's
  * layout; on a member's failure the message names the member.  No reference

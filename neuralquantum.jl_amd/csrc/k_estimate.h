@@ -1,0 +1,772 @@
+// Per-sample kernels of the steady-state loop: log rho(s), the compact
+// log-derivative row O(s), the local estimator L_loc(s) = sum_c mel_c
+// exp(log rho_c - log rho_s) over the Liouvillian connections, and the force
+// sum_s conj(L_loc(s)) dL_loc(s) / Ns contracted on the fly, never
+// materialising the Np-vector dL_loc(s).
+//
+// Reference semantics: NDMBatched.jl:94-280 (log rho, O), AccumulatorObsGrad.jl
+// :39-127 (L_loc, dL_loc = sum_c mel_c exp(..) O(c), no -O(s) term),
+// BatchedGradSampler.jl:99-118 (force = conj(L_loc) dL_loc / Ns), with the
+// NDM's activation f (softplus or logcosh, activation.jl:5-18) a template
+// parameter.
+//
+// Mapping: one wavefront per sample (the table walk is wave-uniform), lanes
+// over hidden units.  Lane l owns real unit u = l + 64 r (lambda for u < nh,
+// mu otherwise, row and column side) and Pi half-unit q = l + 64 r (q < na:
+// row half of Pi unit q, otherwise column half of unit q - na).  The
+// effective angles theta = h + W x of the sample live in registers; a
+// connection flipping k sites costs k FMAs per unit.
+//
+// Three kinds of launch (template parameter KIND):
+//   EK_WALK   log rho, O rows, and pass 1 over the connections: softplus
+//             (or logcosh) of the changed angles + one wave reduction -> W_c
+//             = mel_c exp(log rho_c - log rho_s), L_loc = sum_c W_c; with
+//             rec_mode 1 it writes per sample the changed connections (W_c,
+//             flip masks), their count and the unchanged connections' mel sum.
+//   EK_FORCE  reads those records and L_loc and adds the force
+//             Re(V W_c O_k(c)), V = conj(L_loc)/Ns: per unit a dense part (x_s
+//             times a per-unit scalar) plus a sparse part on the flipped sites
+//             only (x_c = x_s + Delta_c).  Each lane sums its units' entries
+//             of one sample in registers (the flipped site indexes a register
+//             vector with a wave-uniform index: VGPR index mode), then adds
+//             them once per sample into the workgroup's force accumulator in
+//             LDS, held as exact 64-bit fixed-point integers (fx_add), so the
+//             sum is the same whatever order the waves arrive in.  The
+//             per-workgroup partials are then summed in a fixed order.
+//   EK_GRAD   the parity mode: everything for one sample per wavefront and the
+//             per-sample dL_loc(s) written out (the force pass run with V = 1
+//             and V = -i for its real and imaginary parts).
+#pragma once
+#include "nq_device.h"
+#include "nq_table.h"
+#include "nq_kernels.h"
+
+namespace nq_est {
+
+enum { EK_WALK = 0, EK_FORCE = 1, EK_GRAD = 2 };
+
+template <typename T>
+__device__ __forceinline__ T xv(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(1) : T(-1); }
+template <typename T>
+__device__ __forceinline__ T cflip(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(-2) : T(2); }
+
+// P-order index -> reference flat index (the parity mode stores reference order)
+__device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
+    const int N = d.N, nh = d.nh, na = d.na;
+    if (k < N) return d.bla + k;
+    k -= N;
+    if (k < nh) return d.hla + k;
+    k -= nh;
+    if (k < nh * N) return d.wla + k;
+    k -= nh * N;
+    if (k < na) return d.dla + k;
+    k -= na;
+    if (k < na * N) return d.ula + k;
+    k -= na * N;
+    if (k < na * N) return d.umu + k;
+    k -= na * N;
+    if (k < N) return d.bmu + k;
+    k -= N;
+    if (k < nh) return d.hmu + k;
+    k -= nh;
+    return d.wmu + k;
+}
+
+// changed connections evaluated per transposed reduction in pass 1 (fewer
+// in FP64, where the shares take twice the registers); lanes per connection
+// after the reduction
+template <typename T> constexpr int est_batch() { return sizeof(T) == 8 ? 4 : 8; }
+
+// TAB_LDS: the connection table is staged in LDS (a compile-time choice, so
+// every table access is a known LDS or global access, never a flat one)
+template <typename T, int NMAX, int RS, int PS, bool TAB_LDS, int KIND, int ACT>
+__global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD) ? 4 : 2) void k_estimate(est_args a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool WALK = KIND == EK_WALK, FORCE = KIND == EK_FORCE, GRAD = KIND == EK_GRAD;
+    using vN = T __attribute__((ext_vector_type(NMAX)));
+    const ndm_dims d = a.d;
+    const int N = d.N, nh = d.nh, na = d.na;
+    T* sp = reinterpret_cast<T*>(smem);
+    // force accumulator: two 64-bit words per entry (exact fixed point), then the poison word
+    unsigned long long* sF = reinterpret_cast<unsigned long long*>(smem + a.lds_param_bytes);
+    int* poison = reinterpret_cast<int*>(sF + 2 * d.np);
+    const int fbytes = WALK ? 0 : fx_bytes(d.np);
+    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + fbytes);
+    const int wpb0 = blockDim.x >> 6;
+    uint2* sM = reinterpret_cast<uint2*>(sW + wpb0 * a.max_conn);          // flip masks of the cached connections
+    unsigned char* sT = smem + ((reinterpret_cast<unsigned char*>(sM + wpb0 * a.max_conn) - smem + 15) & ~15);
+    const T* gp = reinterpret_cast<const T*>(a.p);
+    const dev_table tab = TAB_LDS ? table_to_lds(a.tab, sT) : a.tab;
+    for (int k = threadIdx.x; k < d.np; k += blockDim.x) sp[k] = gp[k];
+    if constexpr (!WALK) {
+        for (int k = threadIdx.x; k < 2 * d.np; k += blockDim.x) sF[k] = 0ull;
+        if (threadIdx.x == 0) *poison = 0;
+    }
+    __syncthreads();
+
+    const int lane0 = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int64_t gw = (int64_t)blockIdx.x * wpb + wave;
+    const int64_t nw = (int64_t)gridDim.x * wpb;
+    const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
+    cplx<T>* wbuf = sW + wave * a.max_conn;
+    uint2* mbuf = sM + wave * a.max_conn;
+
+    // P-order offsets of the force vector (compact layout, see ndm_dims)
+    const int PB_bla = 0, PB_hla = N, PB_wla = N + nh;
+    const int PB_d = d.nlam, PB_ula = d.nlam + na, PB_umu = d.nlam + na + na * N;
+    const int PB_bmu = d.nlam + d.npi, PB_hmu = PB_bmu + N, PB_wmu = PB_hmu + nh;
+    const int npi = d.npi, R = d.R;
+    const T half = T(0.5);
+
+    for (int64_t s = gw; s < a.Ns; s += nw) {
+        // an opaque copy of the lane index per sample keeps the compiler from
+        // hoisting (and spilling) every lane-dependent address out of the loop
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const uint64_t key = a.keys[s];
+        const uint32_t ks = (uint32_t)key & pmask;
+        const uint32_t ke = (uint32_t)(key >> N) & pmask;
+
+        // ---- effective angles of the sample (the forces need only f')
+        T th_s[RS], th_e[RS], f_s[RS], f_e[RS], a_s[RS], a_e[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int u = lane + 64 * r;
+            th_s[r] = th_e[r] = f_s[r] = f_e[r] = a_s[r] = a_e[r] = T(0);
+            if (u < 2 * nh) {
+                const bool lam = u < nh;
+                const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
+                const T h0 = lam ? sp[d.hla + u] : sp[d.hmu + u - nh];
+                T ts = h0, te = h0;
+#pragma unroll
+                for (int i = 0; i < NMAX; ++i) {
+                    if (i < N) {
+                        const T wi = w[nh * i];
+                        ts += wi * xv<T>(ks, i);
+                        te += wi * xv<T>(ke, i);
+                    }
+                }
+                th_s[r] = ts; th_e[r] = te;
+                if constexpr (FORCE) {
+                    a_s[r] = act_df<ACT>(ts);
+                    a_e[r] = act_df<ACT>(te);
+                } else {
+                    act_f_df<ACT>(ts, &f_s[r], &a_s[r]);
+                    act_f_df<ACT>(te, &f_e[r], &a_e[r]);
+                }
+            }
+        }
+        cplx<T> pi_z[PS], pi_f[PS], pi_p[PS];
+#pragma unroll
+        for (int r = 0; r < PS; ++r) {
+            const int q = lane + 64 * r;
+            pi_z[r] = pi_f[r] = pi_p[r] = cmk(T(0), T(0));
+            if (q < 2 * na) {
+                const int au = q < na ? q : q - na;
+                T zr = sp[d.dla + au], zi = T(0);
+#pragma unroll
+                for (int i = 0; i < NMAX; ++i) {
+                    if (i < N) {
+                        const T xs = xv<T>(ks, i), xe = xv<T>(ke, i);
+                        zr += half * sp[d.ula + au + na * i] * (xs + xe);
+                        zi += half * sp[d.umu + au + na * i] * (xs - xe);
+                    }
+                }
+                pi_z[r] = cmk(zr, zi);
+                if constexpr (FORCE) pi_p[r] = cact_df<ACT>(pi_z[r]);
+                else cact_f_df<ACT>(pi_z[r], &pi_f[r], &pi_p[r]);
+            }
+        }
+
+        if constexpr (!FORCE) {
+            // ---- log rho(s) = Gamma_la + i Gamma_mu + Pi
+            {
+                T re = T(0), im = T(0);
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    const int u = lane + 64 * r;
+                    if (u < nh) re += half * (f_s[r] + f_e[r]);
+                    else if (u < 2 * nh) im += half * (f_s[r] - f_e[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < PS; ++r) {
+                    const int q = lane + 64 * r;
+                    if (q < na) { re += pi_f[r].re; im += pi_f[r].im; }
+                }
+                if (lane < N) {
+                    const T xs = xv<T>(ks, lane), xe = xv<T>(ke, lane);
+                    re += half * sp[d.bla + lane] * (xs + xe);
+                    im += half * sp[d.bmu + lane] * (xs - xe);
+                }
+                re = wave_total(re);
+                im = wave_total(im);
+                if (lane == 0 && a.lp) {
+                    T* lp = reinterpret_cast<T*>(a.lp);
+                    lp[2 * s] = re;
+                    lp[2 * s + 1] = im;
+                }
+            }
+
+            // ---- compact O(s) row
+            if (a.Y) {
+                T* y = reinterpret_cast<T*>(a.Y) + (int64_t)s * R;
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    const int u = lane + 64 * r;
+                    if (u < nh) {
+                        y[PB_hla + u] = half * (a_s[r] + a_e[r]);
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N) y[PB_wla + u + nh * i] = half * (a_s[r] * xv<T>(ks, i) + a_e[r] * xv<T>(ke, i));
+                    } else if (u < 2 * nh) {
+                        const int h = u - nh;
+                        y[npi + PB_hmu + h] = half * (a_s[r] - a_e[r]);
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N) y[npi + PB_wmu + h + nh * i] = half * (a_s[r] * xv<T>(ks, i) - a_e[r] * xv<T>(ke, i));
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PS; ++r) {
+                    const int q = lane + 64 * r;
+                    if (q < na) {
+                        const cplx<T> p = pi_p[r];
+                        y[PB_d + q] = p.re;
+                        y[npi + PB_d + q] = p.im;
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N) {
+                                const T sx = half * (xv<T>(ks, i) + xv<T>(ke, i));
+                                y[PB_ula + q + na * i] = p.re * sx;
+                                y[npi + PB_ula + q + na * i] = p.im * sx;
+                            }
+                    } else if (q < 2 * na) {
+                        const int au = q - na;
+                        const cplx<T> p = pi_p[r];
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N) {
+                                const T dx = half * (xv<T>(ks, i) - xv<T>(ke, i));
+                                y[PB_umu + au + na * i] = -p.im * dx;
+                                y[npi + PB_umu + au + na * i] = p.re * dx;
+                            }
+                    }
+                }
+                if (lane < N) {
+                    const T xs = xv<T>(ks, lane), xe = xv<T>(ke, lane);
+                    y[PB_bla + lane] = half * (xs + xe);
+                    y[npi + PB_bmu + lane] = half * (xs - xe);
+                }
+            }
+        }
+
+        if (!a.do_conn) continue;
+
+        int ncc;
+        cplx<T> Wdiag, Lloc;
+        if constexpr (FORCE) {
+            // the connections of pass 1 from the walk launch
+            const int64_t rb = s * (int64_t)a.max_conn;
+            ncc = a.cnum[s];
+            const T* wd = reinterpret_cast<const T*>(a.cwd) + 2 * s;
+            const T* ll = reinterpret_cast<const T*>(a.Lloc) + 2 * s;
+            Wdiag = cmk(wd[0], wd[1]);
+            Lloc = cmk(ll[0], ll[1]);
+            const cplx<T>* rw = reinterpret_cast<const cplx<T>*>(a.crw) + rb;
+            for (int c = lane0; c < ncc; c += 64) {
+                wbuf[c] = rw[c];
+                mbuf[c] = a.crm[rb + c];
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            // ---- pass 1: W_c and L_loc.  The walk (terms spread over the
+            // lanes) sums the unchanged connections (W = mel) per lane and
+            // caches the changed ones (mel, flip masks) in LDS; those are then
+            // evaluated EB at a time: every lane adds its units' share of
+            // dlog rho for each of them, one transposed reduction sums all
+            // 2 EB shares, and a group of lanes per connection finishes it
+            // (bias terms, e^dlog rho, W = mel e^dlog rho), all EB in
+            // parallel; W replaces mel in the cache for the force pass
+            cplx<T> Lacc = cmk(T(0), T(0));   // this lane's share of L_loc = sum_c W_c
+            cplx<T> Wd = cmk(T(0), T(0));     // and of the unchanged connections' sum
+            const int nc = table_walk_lanes<T>(
+                tab, ks, ke, lane0,
+                [&](cplx<T> mel) {
+                    // unchanged configuration: exp(0) = 1, W = mel exactly (the
+                    // reference evaluates the network anyway, AccumulatorObsGrad.jl:72)
+                    Lacc = cadd(Lacc, mel);
+                    Wd = cadd(Wd, mel);
+                },
+                [&](int slot, cplx<T> mel, uint32_t ms, uint32_t me) {
+                    if (slot < a.max_conn) {
+                        wbuf[slot] = mel;
+                        mbuf[slot] = make_uint2(ms, me);
+                    }
+                });
+            Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
+            __builtin_amdgcn_wave_barrier();
+            ncc = nc < a.max_conn ? nc : a.max_conn;
+            constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
+            // FP32 with one Pi unit per lane (na <= 32): two connections' Pi
+            // units per pass (in FP64 the second set of operands would spill)
+            constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
+            for (int c0 = 0; c0 < ncc; c0 += EB) {
+                T part[2 * EB];
+#pragma unroll
+                for (int j = 0; j < EB; ++j) {
+                    T dre = T(0), dim = T(0);
+                    if (c0 + j < ncc) {
+                        const uint2 mm = mbuf[c0 + j];
+                        const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
+#pragma unroll
+                        for (int r = 0; r < RS; ++r) {
+                            const int u = lane + 64 * r;
+                            if (u < 2 * nh) {
+                                const bool lam = u < nh;
+                                const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
+                                T dfs = T(0), dfe = T(0);
+                                if (ms) {
+                                    T t = th_s[r];
+                                    for (uint32_t m = ms; m; m &= m - 1) {
+                                        const int i = __builtin_ctz(m);
+                                        t += cflip<T>(ks, i) * w[nh * i];
+                                    }
+                                    dfs = act_f<ACT>(t) - f_s[r];
+                                }
+                                if (me) {
+                                    T t = th_e[r];
+                                    for (uint32_t m = me; m; m &= m - 1) {
+                                        const int i = __builtin_ctz(m);
+                                        t += cflip<T>(ke, i) * w[nh * i];
+                                    }
+                                    dfe = act_f<ACT>(t) - f_e[r];
+                                }
+                                if (lam) dre += half * (dfs + dfe);
+                                else dim += half * (dfs - dfe);
+                            }
+                        }
+                        if (!PAIR_PI) {
+#pragma unroll
+                            for (int r = 0; r < PS; ++r) {
+                                const int q = lane + 64 * r;
+                                if (q < na) {
+                                    cplx<T> z = pi_z[r];
+                                    for (uint32_t m = ms; m; m &= m - 1) {
+                                        const int i = __builtin_ctz(m);
+                                        const T c = cflip<T>(ks, i) * half;
+                                        z.re += c * sp[d.ula + q + na * i];
+                                        z.im += c * sp[d.umu + q + na * i];
+                                    }
+                                    for (uint32_t m = me; m; m &= m - 1) {
+                                        const int i = __builtin_ctz(m);
+                                        const T c = cflip<T>(ke, i) * half;
+                                        z.re += c * sp[d.ula + q + na * i];
+                                        z.im -= c * sp[d.umu + q + na * i];
+                                    }
+                                    const cplx<T> f2 = cact_f<ACT>(z);
+                                    dre += f2.re - pi_f[r].re;
+                                    dim += f2.im - pi_f[r].im;
+                                }
+                            }
+                        }
+                    }
+                    part[j] = dre;
+                    part[EB + j] = dim;
+                    if (PAIR_PI && (j & 1)) {
+                        // the Pi units of connections j - 1 and j at once: lanes
+                        // [0, na) take j - 1 and lanes [na, 2 na), which hold the
+                        // same units' angles, take j (flip masks per lane)
+                        const bool hi = lane0 >= na;
+                        const int cj = c0 + (hi ? j : j - 1);
+                        if (lane0 < 2 * na && cj < ncc) {
+                            const int q = hi ? lane0 - na : lane0;
+                            const uint2 mm = mbuf[cj];
+                            cplx<T> z = pi_z[0];
+                            for (uint32_t m = mm.x; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T c = cflip<T>(ks, i) * half;
+                                z.re += c * sp[d.ula + q + na * i];
+                                z.im += c * sp[d.umu + q + na * i];
+                            }
+                            for (uint32_t m = mm.y; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T c = cflip<T>(ke, i) * half;
+                                z.re += c * sp[d.ula + q + na * i];
+                                z.im -= c * sp[d.umu + q + na * i];
+                            }
+                            const cplx<T> f2 = cact_f<ACT>(z);
+                            const T vr = f2.re - pi_f[0].re, vi = f2.im - pi_f[0].im;
+                            part[j - 1] += hi ? T(0) : vr;
+                            part[EB + j - 1] += hi ? T(0) : vi;
+                            part[j] += hi ? vr : T(0);
+                            part[EB + j] += hi ? vi : T(0);
+                        }
+                    }
+                }
+                // lane L < 32: sum of dre of connection c0 + L/LG; lane 32 + L: its dim
+                T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
+                plswap<true>(dre, dim);   // lanes 0-31: dre in `dre`, dim in `dim`
+                const int c = c0 + lane0 / LG;
+                if (lane0 < 32 && c < ncc) {
+                    const uint2 mm = mbuf[c];
+                    for (uint32_t m = mm.x; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        const T cc = cflip<T>(ks, i) * half;
+                        dre += cc * sp[d.bla + i];
+                        dim += cc * sp[d.bmu + i];
+                    }
+                    for (uint32_t m = mm.y; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        const T cc = cflip<T>(ke, i) * half;
+                        dre += cc * sp[d.bla + i];
+                        dim -= cc * sp[d.bmu + i];
+                    }
+                    T sn, cs;
+                    nq_sincos(dim, &sn, &cs);
+                    const T er = nq_exp(dre);
+                    const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
+                    if (lane0 % LG == 0) {
+                        wbuf[c] = W;
+                        Lacc = cadd(Lacc, W);
+                    }
+                }
+            }
+            Lloc = cmk(wave_total(Lacc.re), wave_total(Lacc.im));
+            __builtin_amdgcn_wave_barrier();
+
+            if (lane == 0 && a.Lloc) {
+                T* lo = reinterpret_cast<T*>(a.Lloc);
+                lo[2 * s] = Lloc.re;
+                lo[2 * s + 1] = Lloc.im;
+            }
+            if (WALK && a.rec_mode == 1) {
+                // keep the changed connections (W, flip masks) for the forces launch
+                const int64_t rb = s * (int64_t)a.max_conn;
+                cplx<T>* rw = reinterpret_cast<cplx<T>*>(a.crw) + rb;
+                for (int c = lane0; c < ncc; c += 64) {
+                    rw[c] = wbuf[c];
+                    a.crm[rb + c] = mbuf[c];
+                }
+                if (lane0 == 0) {
+                    a.cnum[s] = ncc;
+                    T* wd = reinterpret_cast<T*>(a.cwd) + 2 * s;
+                    wd[0] = Wdiag.re;
+                    wd[1] = Wdiag.im;
+                }
+            }
+        }
+        if constexpr (!WALK) {
+            // ---- force pass: Re(V W_c O_k(c)) summed over the connections of
+            // this sample in registers, then added to the workgroup's exact
+            // accumulator once
+            constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
+            constexpr int NV = GRAD ? 2 : 1;
+            for (int vi = 0; vi < NV; ++vi) {
+                // Re(1 G) = Re G, Re(-i G) = Im G in the parity mode
+                const cplx<double> Vd = GRAD ? (vi == 0 ? cmk(1.0, 0.0) : cmk(0.0, -1.0))
+                                             : cmk((double)Lloc.re * a.invNs, -(double)Lloc.im * a.invNs);
+                const cplx<T> V = cmk((T)Vd.re, (T)Vd.im);
+                // per unit: weights of the dense part (e_s, e_e, rho1, rho2) and the
+                // flipped-site shares (fw: w entries of the lambda / mu unit; fur,
+                // fui: u_la, u_mu entries of the Pi unit, this half's sites)
+                T e_s[RS], e_e[RS];
+                T rho1[PS], rho2[PS];
+                vN fw[RS], fur[PS], fui[PS];
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    e_s[r] = e_e[r] = T(0);
+                    fw[r] = vN(T(0));
+                }
+#pragma unroll
+                for (int r = 0; r < PS; ++r) {
+                    rho1[r] = rho2[r] = T(0);
+                    fur[r] = fui[r] = vN(T(0));
+                }
+                T fbl = T(0), fbm = T(0);   // b_la, b_mu of site `lane` (lane < N)
+                // unchanged configurations contribute V W = V mel with O(c) = O(s):
+                // only the dense parts, added once below
+                const cplx<T> VWdiag = cmul(V, Wdiag);
+                // one changed connection's terms; p2in: its Pi units' f'(z) when
+                // already known (paired evaluation), else recomputed here
+                auto conn = [&](int c, bool have_p2, cplx<T> p2in) {
+                    const cplx<T> VW = cmul(V, wbuf[c]);
+                    const uint2 mm = mbuf[c];
+                    const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        const int u = lane + 64 * r;
+                        if (u < 2 * nh) {
+                            const bool lam = u < nh;
+                            const int h = lam ? u : u - nh;
+                            const T* w = lam ? sp + d.wla + u : sp + d.wmu + h;
+                            const T gam = lam ? VW.re : -VW.im;
+                            const T seta = lam ? T(1) : T(-1);
+                            T as2 = a_s[r], ae2 = a_e[r];
+                            if (ms) {
+                                T t = th_s[r];
+                                for (uint32_t m = ms; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    t += cflip<T>(ks, i) * w[nh * i];
+                                }
+                                as2 = act_df<ACT>(t);
+                            }
+                            if (me) {
+                                T t = th_e[r];
+                                for (uint32_t m = me; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    t += cflip<T>(ke, i) * w[nh * i];
+                                }
+                                ae2 = act_df<ACT>(t);
+                            }
+                            e_s[r] += gam * as2;
+                            e_e[r] += seta * gam * ae2;
+                            const T gs = half * gam * as2, ge = half * seta * gam * ae2;
+                            for (uint32_t m = ms; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                fw[r][i] += gs * cflip<T>(ks, i);
+                            }
+                            for (uint32_t m = me; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                fw[r][i] += ge * cflip<T>(ke, i);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < PS; ++r) {
+                        const int q = lane + 64 * r;
+                        if (q < 2 * na) {
+                            const bool rowhalf = q < na;
+                            const int au = rowhalf ? q : q - na;
+                            cplx<T> p2 = have_p2 ? p2in : pi_p[r];
+                            if (!have_p2 && (ms | me)) {
+                                cplx<T> z = pi_z[r];
+                                for (uint32_t m = ms; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T cc = cflip<T>(ks, i) * half;
+                                    z.re += cc * sp[d.ula + au + na * i];
+                                    z.im += cc * sp[d.umu + au + na * i];
+                                }
+                                for (uint32_t m = me; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T cc = cflip<T>(ke, i) * half;
+                                    z.re += cc * sp[d.ula + au + na * i];
+                                    z.im -= cc * sp[d.umu + au + na * i];
+                                }
+                                p2 = cact_df<ACT>(z);
+                            }
+                            const cplx<T> vp = cmul(VW, p2);
+                            if (rowhalf) {
+                                rho1[r] += vp.re;
+                                rho2[r] += vp.im;
+                            }
+                            // the row half takes the row flips (u_mu share -cc Im),
+                            // the column half the column flips (+cc Im)
+                            const uint32_t mq = rowhalf ? ms : me, kq = rowhalf ? ks : ke;
+                            const T vim = rowhalf ? -vp.im : vp.im;
+                            for (uint32_t m = ms | me; m; m &= m - 1) {
+                                const int i = __builtin_ctz(m);
+                                const T cc = ((mq >> i) & 1u) ? half * cflip<T>(kq, i) : T(0);
+                                fur[r][i] += cc * vp.re;
+                                fui[r][i] += cc * vim;
+                            }
+                        }
+                    }
+                    if (lane < N) {
+                        if ((ms >> lane) & 1u) {
+                            const T cc = half * cflip<T>(ks, lane);
+                            fbl += cc * VW.re;
+                            fbm -= cc * VW.im;
+                        }
+                        if ((me >> lane) & 1u) {
+                            const T cc = half * cflip<T>(ke, lane);
+                            fbl += cc * VW.re;
+                            fbm += cc * VW.im;
+                        }
+                    }
+                };
+                if (PAIR_PI) {
+                    // connections in pairs: lanes [0, na) compute f'(z) of c's Pi
+                    // units, lanes [na, 2 na) (the same units) that of c + 1, then
+                    // each half takes the other's values by a lane exchange
+                    for (int c = 0; c < ncc; c += 2) {
+                        const bool hi = lane0 >= na;
+                        const int cj = c + (hi ? 1 : 0);
+                        cplx<T> p2 = pi_p[0];
+                        if (lane0 < 2 * na && cj < ncc) {
+                            const uint2 mm = mbuf[cj];
+                            if (mm.x | mm.y) {
+                                const int au = hi ? lane0 - na : lane0;
+                                cplx<T> z = pi_z[0];
+                                for (uint32_t m = mm.x; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T cc = cflip<T>(ks, i) * half;
+                                    z.re += cc * sp[d.ula + au + na * i];
+                                    z.im += cc * sp[d.umu + au + na * i];
+                                }
+                                for (uint32_t m = mm.y; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T cc = cflip<T>(ke, i) * half;
+                                    z.re += cc * sp[d.ula + au + na * i];
+                                    z.im -= cc * sp[d.umu + au + na * i];
+                                }
+                                p2 = cact_df<ACT>(z);
+                            }
+                        }
+                        const int src = lane0 < 2 * na ? (hi ? lane0 - na : lane0 + na) : lane0;
+                        const cplx<T> po = cmk(__shfl(p2.re, src, 64), __shfl(p2.im, src, 64));
+                        conn(c, true, hi ? po : p2);
+                        if (c + 1 < ncc) conn(c + 1, true, hi ? p2 : po);
+                    }
+                } else {
+                    for (int c = 0; c < ncc; ++c) conn(c, false, pi_p[0]);
+                }
+
+                // unchanged configurations contribute O(s) with weight sum VWdiag
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    const int u = lane + 64 * r;
+                    if (u < 2 * nh) {
+                        const bool lam = u < nh;
+                        const T gam = lam ? VWdiag.re : -VWdiag.im;
+                        e_s[r] += gam * a_s[r];
+                        e_e[r] += (lam ? gam : -gam) * a_e[r];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PS; ++r) {
+                    const int q = lane + 64 * r;
+                    if (q < na) {
+                        const cplx<T> vp = cmul(VWdiag, pi_p[r]);
+                        rho1[r] += vp.re;
+                        rho2[r] += vp.im;
+                    }
+                }
+                if (lane < N) {
+                    const cplx<T> VL = cmul(V, Lloc);
+                    const T xs = xv<T>(ks, lane), xe = xv<T>(ke, lane);
+                    fbl += half * VL.re * (xs + xe);
+                    fbm -= half * VL.im * (xs - xe);
+                }
+
+                // this sample's entries: dense parts (x_s times the per-unit
+                // sums) plus the flipped-site shares, into the exact accumulator
+                // (the opaque copy of the unit index keeps the addresses from
+                // being hoisted out of the sample loop and spilled)
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    int u = lane + 64 * r;
+                    asm volatile("" : "+v"(u));
+                    if (u < 2 * nh) {
+                        const bool lam = u < nh;
+                        const int h = lam ? u : u - nh;
+                        fx_add(sF, (lam ? PB_hla : PB_hmu) + h, (double)(half * (e_s[r] + e_e[r])), poison);
+                        const int pb = lam ? PB_wla : PB_wmu;
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N)
+                                fx_add(sF, pb + h + nh * i,
+                                       (double)(fw[r][i] + half * (e_s[r] * xv<T>(ks, i) + e_e[r] * xv<T>(ke, i))), poison);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PS; ++r) {
+                    int q = lane + 64 * r;
+                    asm volatile("" : "+v"(q));
+                    if (q < 2 * na) {
+                        const bool rowhalf = q < na;
+                        const int au = rowhalf ? q : q - na;
+                        if (rowhalf) fx_add(sF, PB_d + au, (double)rho1[r], poison);
+#pragma unroll
+                        for (int i = 0; i < NMAX; ++i)
+                            if (i < N) {
+                                T ur = fur[r][i], ui = fui[r][i];
+                                if (rowhalf) {
+                                    const T xs = xv<T>(ks, i), xe = xv<T>(ke, i);
+                                    ur += half * rho1[r] * (xs + xe);
+                                    ui -= half * rho2[r] * (xs - xe);
+                                }
+                                fx_add(sF, PB_ula + au + na * i, (double)ur, poison);
+                                fx_add(sF, PB_umu + au + na * i, (double)ui, poison);
+                            }
+                    }
+                }
+                if (lane < N) {
+                    fx_add(sF, PB_bla + lane, (double)fbl, poison);
+                    fx_add(sF, PB_bmu + lane, (double)fbm, poison);
+                }
+                if constexpr (GRAD) {   // one wave per block in this mode: sF belongs to this sample
+                    __syncthreads();
+                    double* g = reinterpret_cast<double*>(a.G) + (int64_t)s * 2 * d.np;
+                    const bool bad = *poison != 0;
+                    for (int k = lane; k < d.np; k += 64) {
+                        g[2 * p2ref(k, d) + vi] = bad ? __builtin_nan("") : fx_value(sF, k);
+                        sF[2 * k] = 0ull;
+                        sF[2 * k + 1] = 0ull;
+                    }
+                    __syncthreads();
+                    if (lane == 0) *poison = 0;
+                    __syncthreads();
+                }
+            }
+        }
+    }
+
+    if constexpr (FORCE) {
+        __syncthreads();
+        double* out = a.Fpart + (int64_t)blockIdx.x * d.np;
+        const bool bad = *poison != 0;
+        for (int k = threadIdx.x; k < d.np; k += blockDim.x) out[k] = bad ? __builtin_nan("") : fx_value(sF, k);
+    }
+}
+
+template <typename T, int NMAX, int RS, int PS, int ACT>
+hipError_t launch_t(const est_args& a0, int grid, hipStream_t st) {
+    est_args a = a0;
+    const int kind = a.G ? EK_GRAD : (a.rec_mode == 2 ? EK_FORCE : EK_WALK);
+    if (kind == EK_WALK && a.Fpart) return hipErrorInvalidValue;   // forces take a launch of their own
+    if (kind == EK_FORCE) a.tab_lds = 0;                             // the records replace the walk
+    // 8 waves per block share one copy of the parameters, the force
+    // accumulator and the table (two blocks per CU while that fits in half
+    // the LDS: 4 waves per SIMD); otherwise 4.  The parity mode (per-sample
+    // gradients) runs one wave per block.
+    int wpb = a.G ? 1 : 8;
+    if (!a.G && est_lds_bytes(a, sizeof(T), wpb) > 80 * 1024) wpb = 4;
+    const size_t lds = est_lds_bytes(a, sizeof(T), wpb);
+    if (kind == EK_FORCE)
+        hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_FORCE, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+    else if (kind == EK_GRAD) {
+        if (a.tab_lds)
+            hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, true, EK_GRAD, ACT>), dim3(grid), dim3(64), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_GRAD, ACT>), dim3(grid), dim3(64), lds, st, a);
+    } else {
+        if (a.tab_lds)
+            hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, true, EK_WALK, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_WALK, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int NMAX, int ACT>
+hipError_t launch_n(const est_args& a, int grid, hipStream_t st) {
+    const int rs = (2 * a.d.nh + 63) / 64, ps = (2 * a.d.na + 63) / 64;
+    if (rs <= 1 && ps <= 1) return launch_t<T, NMAX, 1, 1, ACT>(a, grid, st);
+    if (rs <= 2 && ps <= 2) return launch_t<T, NMAX, 2, 2, ACT>(a, grid, st);
+    return hipErrorInvalidValue;
+}
+
+// every launch of one precision
+template <typename T>
+hipError_t launch_prec(const est_args& a, int grid, hipStream_t st) {
+    if (a.d.act == NQ_ACT_LOGCOSH) {
+        if (a.d.N <= 8) return launch_n<T, 8, NQ_ACT_LOGCOSH>(a, grid, st);
+        return launch_n<T, 16, NQ_ACT_LOGCOSH>(a, grid, st);
+    }
+    if (a.d.N <= 8) return launch_n<T, 8, NQ_ACT_SOFTPLUS>(a, grid, st);
+    return launch_n<T, 16, NQ_ACT_SOFTPLUS>(a, grid, st);
+}
+
+}  // namespace nq_est

@@ -294,7 +294,7 @@ __global__ void k_made_sum_parts(const float* gpart, int nparts, int P, float* g
 template <typename T>
 __global__ void k_made_accept(const uint64_t* keys, const T* lp, const float* lq, int64_t n, uint64_t seed,
                               uint32_t iter, const uint32_t* iter_dev, uint32_t slice, uint64_t* chain_state,
-                              unsigned long long* accepted) {
+                              unsigned long long* accepted, uint32_t* chain_acc) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool acc = false;
     if (c < n) {
@@ -304,7 +304,10 @@ __global__ void k_made_accept(const uint64_t* keys, const T* lp, const float* lq
         philox_out o = philox4x32_10((uint32_t)c, slice, it, 4u, (uint32_t)seed, (uint32_t)(seed >> 32));
         const double u = u01_53(o.x[1], o.x[2]);
         acc = !(u - exp(dlp + dlq) >= 0.0);
-        if (acc) chain_state[c] = keys[n + c];
+        if (acc) {
+            chain_state[c] = keys[n + c];
+            if (chain_acc) chain_acc[c] += 1u;
+        }
     }
     const unsigned long long m = __ballot(acc);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(accepted, (unsigned long long)__popcll(m));
@@ -405,14 +408,15 @@ hipError_t nq_launch_made_sum_parts(const float* gpart, int nparts, int D, float
 
 hipError_t nq_launch_made_accept(const uint64_t* keys, const void* lp, bool lp_fp64, const float* lq, int64_t n,
                                  uint64_t seed, uint32_t iter, const uint32_t* iter_dev, uint32_t slice,
-                                 uint64_t* chain_state, unsigned long long* accepted, hipStream_t st) {
+                                 uint64_t* chain_state, unsigned long long* accepted, uint32_t* chain_acc,
+                                 hipStream_t st) {
     const unsigned g = (unsigned)((n + 255) / 256);
     if (lp_fp64)
         hipLaunchKernelGGL(k_made_accept<double>, dim3(g), dim3(256), 0, st, keys, (const double*)lp, lq, n, seed,
-                           iter, iter_dev, slice, chain_state, accepted);
+                           iter, iter_dev, slice, chain_state, accepted, chain_acc);
     else
         hipLaunchKernelGGL(k_made_accept<float>, dim3(g), dim3(256), 0, st, keys, (const float*)lp, lq, n, seed, iter,
-                           iter_dev, slice, chain_state, accepted);
+                           iter_dev, slice, chain_state, accepted, chain_acc);
     return hipGetLastError();
 }
 

@@ -17,8 +17,9 @@
 //   k_rbm_estimate   one wavefront per sample, lane h owns hidden unit h: log
 //                    rho, the O row [Re O | Im O], L_loc, and the force
 //                    sum_s conj(L_loc)/Ns sum_c W_c O(c) accumulated per unit
-//                    (dense part x_s e_h plus sparse flipped-site terms) in FP64
-//                    LDS; or per-sample dL_loc (parity mode, one wave per block)
+//                    (dense part x_s e_h plus sparse flipped-site terms) in an
+//                    exact fixed-point LDS accumulator (order independent);
+//                    or per-sample dL_loc (parity mode, one wave per block)
 //   k_rbm_metropolis LocalRule Metropolis on |rho|^2, the same random streams
 //                    and schedule as the NDM sampler; a group of lanes per chain
 //                    keeps theta in registers
@@ -49,19 +50,24 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, nh = a.nh, npc = a.npc;
     cplx<T>* sp = reinterpret_cast<cplx<T>*>(smem);
-    double* sF = reinterpret_cast<double*>(smem + a.lds_param_bytes);   // [Re (npc) | Im (npc)]
+    // exact fixed-point force accumulator (fx_add): entries [Re (npc) | Im (npc)], then the poison word
+    unsigned long long* sF = reinterpret_cast<unsigned long long*>(smem + a.lds_param_bytes);
+    int* poison = reinterpret_cast<int*>(sF + 4 * npc);
     const bool useF = a.Fpart != nullptr || a.G != nullptr;
-    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + (useF ? 2 * npc * (int)sizeof(double) : 0));
+    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + a.lds_param_bytes + (useF ? fx_bytes(2 * npc) : 0));
     unsigned char* sT = smem + ((reinterpret_cast<unsigned char*>(sW + (blockDim.x >> 6) * a.max_conn) - smem + 15) & ~15);
     const cplx<T>* gp = reinterpret_cast<const cplx<T>*>(a.p);
     for (int k = threadIdx.x; k < npc; k += blockDim.x) sp[k] = gp[k];
-    if (useF)
-        for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) sF[k] = 0.0;
+    if (useF) {
+        for (int k = threadIdx.x; k < 4 * npc; k += blockDim.x) sF[k] = 0ull;
+        if (threadIdx.x == 0) *poison = 0;
+    }
     const dev_table tab = a.tab_lds ? table_to_lds(a.tab, sT) : a.tab;
     __syncthreads();
+    // integer sums: the same total whatever order the waves add in
     auto fadd = [&](int k, cplx<T> v) {
-        atomicAdd(&sF[k], (double)v.re);
-        atomicAdd(&sF[npc + k], (double)v.im);
+        fx_add(sF, k, (double)v.re, poison);
+        fx_add(sF, npc + k, (double)v.im, poison);
     };
     // ket (RBM.jl): a[N] b[nh] W[nh x N] on the row configuration only
     const bool ket = a.ket != 0;
@@ -272,19 +278,23 @@ __global__ __launch_bounds__(256, 2) void k_rbm_estimate(rbm_args a) {
         if (a.G) {   // one wave per block: sF belongs to this sample
             __syncthreads();
             double* g = a.G + (int64_t)s * 2 * npc;
+            const bool bad = *poison != 0;
             for (int k = lane; k < npc; k += 64) {
-                g[2 * k] = sF[k];
-                g[2 * k + 1] = sF[npc + k];
-                sF[k] = 0.0;
-                sF[npc + k] = 0.0;
+                g[2 * k] = bad ? __builtin_nan("") : fx_value(sF, k);
+                g[2 * k + 1] = bad ? __builtin_nan("") : fx_value(sF, npc + k);
+                sF[2 * k] = sF[2 * k + 1] = 0ull;
+                sF[2 * (npc + k)] = sF[2 * (npc + k) + 1] = 0ull;
             }
+            __syncthreads();
+            if (lane == 0) *poison = 0;
             __syncthreads();
         }
     }
     if (a.Fpart && !a.G) {
         __syncthreads();
         double* out = a.Fpart + (int64_t)blockIdx.x * 2 * npc;
-        for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) out[k] = sF[k];
+        const bool bad = *poison != 0;
+        for (int k = threadIdx.x; k < 2 * npc; k += blockDim.x) out[k] = bad ? __builtin_nan("") : fx_value(sF, k);
     }
 }
 
@@ -409,6 +419,7 @@ __global__ __launch_bounds__(256) void k_rbm_metropolis(rbm_samp_args a) {
         if (a.compat_stale_last_slice && a.samples)
             a.samples[chain + (int64_t)(a.slice0 + a.stored) * a.chains] = 0ull;
         if (a.accepted) atomicAdd(a.accepted, (unsigned long long)acc);
+        if (a.chain_acc) a.chain_acc[chain] += (uint32_t)acc;   // one lane per chain: no race
     }
 }
 
@@ -524,9 +535,11 @@ __global__ void k_force_all(const double* part2, int ngroups, const double* mu, 
 }
 
 // p[k] -= lr (delta[k] + i delta[npc + k]) on the interleaved complex parameters
-__global__ void k_update_complex(double* p64, float* p32, const double* delta, int npc, double lr) {
+__global__ void k_update_complex(double* p64, float* p32, const double* delta, int npc, double lr, nq_step_guard g) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npc) return;
+    // no step after a failed solve, a non-finite step or a non-finite loss
+    if ((g.info && *g.info != 0) || (g.flag && *g.flag != 0) || (g.stats && !isfinite(g.stats[0]))) return;
     const double re = p64[2 * k] - lr * delta[k];
     const double im = p64[2 * k + 1] - lr * delta[npc + k];
     p64[2 * k] = re;
@@ -562,7 +575,7 @@ hipError_t samp_t(const rbm_samp_args& a, hipStream_t st) {
 }  // namespace
 
 size_t rbm_est_lds_bytes(const rbm_args& a, size_t elt) {
-    const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? 2 * sizeof(double) * (size_t)a.npc : 0) +
+    const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? (size_t)fx_bytes(2 * a.npc) : 0) +
                      4 * (size_t)a.max_conn * 2 * elt;
     return ((b + 15) & ~(size_t)15) + (a.tab_lds ? (size_t)a.tab.bytes : 0);
 }
@@ -626,8 +639,8 @@ hipError_t nq_launch_energy_force(const void* Y, const void* E, int fp64, const 
 }
 
 hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,
-                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_update_complex, dim3((npc + 255) / 256), dim3(256), 0, st, p64, p32, delta, npc, lr);
+                                    nq_step_guard g, hipStream_t st) {
+    hipLaunchKernelGGL(k_update_complex, dim3((npc + 255) / 256), dim3(256), 0, st, p64, p32, delta, npc, lr, g);
     return hipGetLastError();
 }
 

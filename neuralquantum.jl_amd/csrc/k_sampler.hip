@@ -11,8 +11,9 @@
 // per wavefront.  Lane g of a group owns lambda units h = g + lpc r (both
 // sides) and Pi units a = g + lpc r; only the lambda and Pi blocks enter
 // Re log rho, the mu block does not.  The effective angles stay in registers;
-// a proposal costs one FMA + one softplus per lambda unit of the flipped side
-// and one complex update + Re softplus per Pi unit, then a group reduction.
+// a proposal costs one FMA + one f (softplus or logcosh, the NDM's activation)
+// per lambda unit of the flipped side and one complex update + Re f per Pi
+// unit, then a group reduction.
 // Angles are recomputed from the configuration at the start of every call so
 // incremental rounding never accumulates over more than `passes` updates.
 // Random numbers: Philox4x32-10, counter (t, iteration, chain, 1) for proposal
@@ -26,7 +27,7 @@ namespace {
 template <typename T>
 __device__ __forceinline__ T xv(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(1) : T(-1); }
 
-template <typename T, int US>
+template <typename T, int US, int ACT>
 __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ndm_dims d = a.d;
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                     te += w * xv<T>(ke, i);
                 }
                 th_s[r] = ts; th_e[r] = te;
-                f_s[r] = softplus(ts); f_e[r] = softplus(te);
+                f_s[r] = act_f<ACT>(ts); f_e[r] = act_f<ACT>(te);
             }
             if (h < na) {
                 T zr = sp[d.dla + h], zi = T(0);
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                     zi += half * sp[d.umu + h + na * i] * (xs - xe);
                 }
                 z[r] = cmk(zr, zi);
-                fpi[r] = csoftplus_re(z[r]);
+                fpi[r] = cact_re<ACT>(z[r]);
             }
         }
         for (int q = 0; q < a.passes; ++q, ++t) {
@@ -130,19 +131,19 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
                 if (h < nh) {
                     const T th = col ? th_e[r] : th_s[r];
                     nth[r] = th + c * sp[d.wla + h + nh * j];
-                    nf[r] = softplus(nth[r]);
+                    nf[r] = act_f<ACT>(nth[r]);
                     delta += nf[r] - (col ? f_e[r] : f_s[r]);
                 }
                 if (h < na) {
                     if (diag) {
                         nz[r].re += c * sp[d.ula + h + na * j];
-                        nfpi[r] = csoftplus_re(nz[r]);
+                        nfpi[r] = cact_re<ACT>(nz[r]);
                         delta += nfpi[r] - fpi[r];
                     } else {
                         const T cu = c * half;
                         nz[r].re += cu * sp[d.ula + h + na * j];
                         nz[r].im += (col ? -cu : cu) * sp[d.umu + h + na * j];
-                        nfpi[r] = csoftplus_re(nz[r]);
+                        nfpi[r] = cact_re<ACT>(nz[r]);
                         delta += T(2) * (nfpi[r] - fpi[r]);
                     }
                 }
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
         if (a.compat_stale_last_slice && a.samples)
             a.samples[chain + (int64_t)(a.stored) * a.chains] = 0ull;   // defect D1: slice L stays at the fill
         if (a.accepted) atomicAdd(a.accepted, (unsigned long long)acc);
+        if (a.chain_acc) a.chain_acc[chain] += (uint32_t)acc;   // one lane per chain: no race
     }
 }
 
@@ -193,7 +195,10 @@ hipError_t launch_t(const samp_args& a, hipStream_t st) {
     const int wpb = 4;
     const int64_t grid = (waves + wpb - 1) / wpb;
     size_t lds = sizeof(T) * (size_t)a.d.np;
-    hipLaunchKernelGGL((k_metropolis<T, US>), dim3((unsigned)grid), dim3(64 * wpb), lds, st, a);
+    if (a.d.act == NQ_ACT_LOGCOSH)
+        hipLaunchKernelGGL((k_metropolis<T, US, NQ_ACT_LOGCOSH>), dim3((unsigned)grid), dim3(64 * wpb), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_metropolis<T, US, NQ_ACT_SOFTPLUS>), dim3((unsigned)grid), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }
 

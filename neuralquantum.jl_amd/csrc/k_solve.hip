@@ -695,9 +695,16 @@ __device__ __forceinline__ int p2ref(int k, const ndm_dims& d) {
     return d.wmu + k;
 }
 
-__global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr) {
+// the step is applied only when the solve succeeded: no failed pivot (info,
+// PosDefException in SRDirect.jl:79 is thrown before any update), a finite
+// step (flag) and a finite loss; otherwise the parameters stay as they were
+__device__ __forceinline__ bool step_ok(const nq_step_guard& g) {
+    return (!g.info || *g.info == 0) && (!g.flag || *g.flag == 0) && (!g.stats || isfinite(g.stats[0]));
+}
+
+__global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr, nq_step_guard g) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.np) return;
+    if (k >= d.np || !step_ok(g)) return;
     const int r = p2ref(k, d);
     const double v = p64[r] - lr * delta[k];
     p64[r] = v;
@@ -882,8 +889,8 @@ hipError_t nq_launch_axpy(double* x, const double* y, int n, hipStream_t st) {
 }
 
 hipError_t nq_launch_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr,
-                            hipStream_t st) {
-    hipLaunchKernelGGL(k_update, dim3(nblk(d.np, 256)), dim3(256), 0, st, p64, p32, delta, d, lr);
+                            nq_step_guard g, hipStream_t st) {
+    hipLaunchKernelGGL(k_update, dim3(nblk(d.np, 256)), dim3(256), 0, st, p64, p32, delta, d, lr, g);
     return hipGetLastError();
 }
 

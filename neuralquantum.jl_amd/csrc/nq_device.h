@@ -61,6 +61,8 @@ __device__ __forceinline__ float nq_atan2(float y, float x) {
 }
 __device__ __forceinline__ void nq_sincos(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ void nq_sincos(float x, float* s, float* c) { *s = __sinf(x); *c = __cosf(x); }
+__device__ __forceinline__ double nq_sin_only(double x) { return sin(x); }
+__device__ __forceinline__ float nq_sin_only(float x) { return __sinf(x); }
 
 // softplus f(x) = log(1 + e^x) and its derivative 1/(1 + e^-x)
 // (src/Networks/activation.jl:14-18), sharing one exponential.
@@ -106,6 +108,143 @@ __device__ __forceinline__ T csoftplus_re(cplx<T> z) {
     nq_sincos(z.im, &s, &c);
     return T(0.5) * nq_log1p(ex * (T(2) * c + ex));
 }
+
+// ---------------------------------------------------------------- NDM activations
+// The NDM's f (src/Networks/activation.jl): NQ_ACT_SOFTPLUS is logℒ(x) =
+// log1p(e^x) with f' = 1/(1 + e^-x) (:14-18, af_softplus); NQ_ACT_LOGCOSH is
+// logℒ2(x) = log cosh x with f' = tanh x (:5-10, af_logcosh).  Complex
+// arguments (the ancilla units) follow :9-10:
+//   logℒ2(a + ib) = logℒ2(a) + log(cos b + i tanh(a) sin b),
+// evaluated here as, with e = e^-2|a| (no overflow for any a),
+//   log cosh a = |a| + log1p(e) - log 2,
+//   log|cos b + i tanh a sin b| = 1/2 log1p(-sin^2 b sech^2 a), sech^2 a = 4e / (1 + e)^2,
+//   arg = atan2(tanh(a) sin b, cos b),
+//   tanh(a + ib) = (sgn(a) (1 - e^2) + i 2 e sin 2b) / (1 + 2 e cos 2b + e^2).
+enum { NQ_ACT_SOFTPLUS = 0, NQ_ACT_LOGCOSH = 1 };
+
+template <typename T>
+__device__ __forceinline__ T nq_ln2() { return T(0.69314718055994530942); }
+
+template <typename T>
+__device__ __forceinline__ void logcosh_and_tanh(T x, T* f, T* df) {
+    const T ax = fabs(x), e = nq_exp(T(-2) * ax);
+    *f = ax + nq_log1p(e) - nq_ln2<T>();
+    *df = copysign((T(1) - e) * nq_rcp(T(1) + e), x);
+}
+template <typename T>
+__device__ __forceinline__ void clogcosh_and_tanh(cplx<T> z, cplx<T>* f, cplx<T>* df) {
+    const T ax = fabs(z.re), e = nq_exp(T(-2) * ax);
+    T s, c;
+    nq_sincos(z.im, &s, &c);
+    const T ope = T(1) + e, rope = nq_rcp(ope);
+    const T th = (T(1) - e) * rope;                       // tanh |a|
+    const T sech2 = T(4) * e * rope * rope;
+    f->re = ax + nq_log1p(e) - nq_ln2<T>() + T(0.5) * nq_log1p(-s * s * sech2);
+    f->im = nq_atan2(copysign(th, z.re) * s, c);
+    const T iD = nq_rcp(T(1) + e * (e + T(2) * (T(2) * c * c - T(1))));
+    df->re = copysign((T(1) - e) * ope * iD, z.re);
+    df->im = T(4) * e * s * c * iD;
+}
+
+template <int ACT, typename T>
+__device__ __forceinline__ void act_f_df(T x, T* f, T* df) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) logcosh_and_tanh(x, f, df);
+    else softplus_and_sigmoid(x, f, df);
+}
+template <int ACT, typename T>
+__device__ __forceinline__ T act_f(T x) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) {
+        const T ax = fabs(x);
+        return ax + nq_log1p(nq_exp(T(-2) * ax)) - nq_ln2<T>();
+    } else {
+        return softplus(x);
+    }
+}
+template <int ACT, typename T>
+__device__ __forceinline__ T act_df(T x) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) {
+        const T e = nq_exp(T(-2) * fabs(x));
+        return copysign((T(1) - e) * nq_rcp(T(1) + e), x);
+    } else {
+        return sigmoid(x);
+    }
+}
+template <int ACT, typename T>
+__device__ __forceinline__ void cact_f_df(cplx<T> z, cplx<T>* f, cplx<T>* df) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) clogcosh_and_tanh(z, f, df);
+    else csoftplus_and_sigmoid(z, f, df);
+}
+template <int ACT, typename T>
+__device__ __forceinline__ cplx<T> cact_f(cplx<T> z) {
+    cplx<T> f, df;
+    cact_f_df<ACT>(z, &f, &df);
+    return f;
+}
+template <int ACT, typename T>
+__device__ __forceinline__ cplx<T> cact_df(cplx<T> z) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) {
+        const T e = nq_exp(T(-2) * fabs(z.re));
+        T s, c;
+        nq_sincos(z.im, &s, &c);
+        const T iD = nq_rcp(T(1) + e * (e + T(2) * (T(2) * c * c - T(1))));
+        return cmk(copysign((T(1) - e) * (T(1) + e) * iD, z.re), T(4) * e * s * c * iD);
+    } else {
+        // f'(z) = w / (1 + w), w = e^z
+        T s, c;
+        nq_sincos(z.im, &s, &c);
+        const T ex = nq_exp(z.re);
+        const T wr = ex * c, wi = ex * s, dr = T(1) + wr;
+        const T den = nq_rcp(dr * dr + wi * wi);
+        return cmk((wr * dr + wi * wi) * den, wi * den);
+    }
+}
+template <int ACT, typename T>
+__device__ __forceinline__ T cact_re(cplx<T> z) {
+    if constexpr (ACT == NQ_ACT_LOGCOSH) {
+        const T ax = fabs(z.re), e = nq_exp(T(-2) * ax);
+        const T s = nq_sin_only(z.im), rope = nq_rcp(T(1) + e);
+        return ax + nq_log1p(e) - nq_ln2<T>() + T(0.5) * nq_log1p(-s * s * T(4) * e * rope * rope);
+    } else {
+        return csoftplus_re(z);
+    }
+}
+
+// ---------------------------------------------------------------- exact sums
+// Order-independent accumulation of doubles (the force partials of a
+// workgroup, added by all its waves through LDS atomics).  Each addend v is
+// split into an integer multiple of 2^-20 (hi) and a multiple of 2^-64 (lo),
+// both rounded to nearest, and the two parts are summed as 64-bit integers.
+// Integer addition is associative, so the total does not depend on the order
+// in which lanes and waves add, and it is exact up to the 2^-65 rounding of
+// each addend; intermediate wrap-around is harmless as long as the final hi
+// sum stays below 2^63 (|total| < 2^43) and the lo sum below 2^63 (fewer than
+// 2^20 addends per entry).  Addends must satisfy |v| < 2^30: a larger or
+// non-finite addend sets *poison, and the entries read as NaN.
+#define NQ_FX_MAGIC 6755399441055744.0   // 1.5 * 2^52: y + MAGIC rounds |y| < 2^51 to an integer
+__device__ __forceinline__ void fx_split(double v, long long* hi, long long* lo) {
+    const double t = v * 1048576.0;                              // v 2^20, exact
+    const double d1 = t + NQ_FX_MAGIC;
+    *hi = __double_as_longlong(d1) - __double_as_longlong(NQ_FX_MAGIC);
+    const double r = t - (d1 - NQ_FX_MAGIC);                     // exact, |r| <= 1/2
+    const double d2 = fma(r, 17592186044416.0, NQ_FX_MAGIC);     // r 2^44 rounded
+    *lo = __double_as_longlong(d2) - __double_as_longlong(NQ_FX_MAGIC);
+}
+// acc: 2 words per entry; entry k at acc[2k] (hi), acc[2k + 1] (lo)
+__device__ __forceinline__ void fx_add(unsigned long long* acc, int k, double v, int* poison) {
+    if (!(fabs(v) < 1073741824.0)) {
+        atomicOr(poison, 1);
+        return;
+    }
+    long long hi, lo;
+    fx_split(v, &hi, &lo);
+    atomicAdd(acc + 2 * k, (unsigned long long)hi);
+    atomicAdd(acc + 2 * k + 1, (unsigned long long)lo);
+}
+__device__ __forceinline__ double fx_value(const unsigned long long* acc, int k) {
+    return (double)(long long)acc[2 * k] * 0x1p-20 + (double)(long long)acc[2 * k + 1] * 0x1p-64;
+}
+// LDS bytes of an accumulator of n entries plus its poison word (16-aligned)
+__host__ __device__ inline int fx_bytes(int n) { return 16 * n + 16; }
 
 // ---------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).
@@ -300,6 +439,7 @@ __device__ __forceinline__ T wave_totals(T (&v)[K], int lane) {
 // (matrices column-major, element (h, i) at h + nh*i).
 struct ndm_dims {
     int N, nh, na;
+    int act;   // NQ_ACT_SOFTPLUS or NQ_ACT_LOGCOSH
     int bmu, hmu, wmu, umu, bla, hla, dla, wla, ula, np;
     // compact gradient layout (one row per sample, "P order"):
     //   lambda block (real):  b_la h_la w_la            -> cols [0, nlam)
@@ -312,6 +452,7 @@ struct ndm_dims {
 static inline ndm_dims make_ndm_dims(int N, int nh, int na) {
     ndm_dims d;
     d.N = N; d.nh = nh; d.na = na;
+    d.act = 0;
     int p = 0;
     d.bmu = p; p += N;
     d.hmu = p; p += nh;

@@ -55,6 +55,7 @@ struct samp_args {
     uint64_t* chain_state;     // chains, in/out (nullable when restart)
     uint64_t* samples;         // (stored [+1]) x chains, chain fastest
     unsigned long long* accepted;
+    uint32_t* chain_acc;       // chains: accepted proposals per chain, added to (nullable)
 };
 int nq_sampler_lanes_per_chain(const ndm_dims& d);
 hipError_t nq_launch_metropolis(const samp_args& a, bool fp64, hipStream_t st);
@@ -183,8 +184,16 @@ hipError_t nq_residual(const double* S, int n, int lds, double eps, int kind, co
 // x += y
 hipError_t nq_launch_axpy(double* x, const double* y, int n, hipStream_t st);
 // params[perm_ref(k)] -= lr * delta[k] (delta in P order); refresh the FP32 copy
+// device status words an update checks before it applies the step (each nullable):
+// info = failed pivot column of the factorization, flag = non-finite step,
+// stats[0] = the loss of the gradient's samples
+struct nq_step_guard {
+    const int* info;
+    const int* flag;
+    const double* stats;
+};
 hipError_t nq_launch_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr,
-                            hipStream_t st);
+                            nq_step_guard g, hipStream_t st);
 // FP32 copy of the parameters
 hipError_t nq_launch_to_f32(const double* p64, float* p32, int64_t n, hipStream_t st);
 // NaN / Inf check: *flag |= 1 if any non-finite in x
@@ -218,7 +227,8 @@ hipError_t nq_launch_made_sample(const float* th, int D, int64_t n, uint64_t see
 // independence Metropolis-Hastings step of n chains (keys[0,n) states, keys[n,2n) proposals)
 hipError_t nq_launch_made_accept(const uint64_t* keys, const void* lp, bool lp_fp64, const float* lq, int64_t n,
                                  uint64_t seed, uint32_t iter, const uint32_t* iter_dev, uint32_t slice,
-                                 uint64_t* chain_state, unsigned long long* accepted, hipStream_t st);
+                                 uint64_t* chain_state, unsigned long long* accepted, uint32_t* chain_acc,
+                                 hipStream_t st);
 hipError_t nq_launch_copy_keys(const uint64_t* src, int64_t n, uint64_t* dst, hipStream_t st);
 // append the last min(n, cap) keys of src to the ring; pos = {write position, fill}
 hipError_t nq_launch_ring_push(const uint64_t* src, int64_t n, uint64_t* ring, int64_t cap, int64_t* pos,
@@ -265,6 +275,7 @@ struct rbm_samp_args {
     uint64_t* chain_state;
     uint64_t* samples;
     unsigned long long* accepted;
+    uint32_t* chain_acc;       // chains: accepted proposals per chain, added to (nullable)
 };
 hipError_t nq_launch_rbm_metropolis(const rbm_samp_args& a, bool fp64, hipStream_t st);
 // in place: [[G_rr, G_ri], [G_ir, G_ii]] -> [[A, -B], [B, A]] (A = G_rr + G_ii, B = G_ri - G_ir)
@@ -274,7 +285,7 @@ hipError_t nq_launch_force_all(const double* Fpart, int nblk, double* part2, con
                                int n, double* F, hipStream_t st);
 // p -= lr (delta[0:npc] + i delta[npc:2 npc]); refresh the FP32 copy
 hipError_t nq_launch_update_complex(double* p64, float* p32, const double* delta, int npc, double lr,
-                                    hipStream_t st);
+                                    nq_step_guard g, hipStream_t st);
 // energy path: statistics of complex values, out = [Re mean, err, var, tau, R, Im mean]
 hipError_t nq_launch_stats_cplx(const void* E, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
 // F = [Re | Im] of sum_s E_s conj(Y_s - mu) / Ns (Y rows [Re O | Im O]); part: chunks x 2 npc scratch

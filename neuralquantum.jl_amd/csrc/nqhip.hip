@@ -56,6 +56,7 @@ struct nqh_ctx {
     uint32_t iter = 0;
     bool chains_initialised = false;
     dbuf chain_state, samples, acc;
+    dbuf chain_acc;            // accepted proposals per chain in the last sampling call
 
     // per-sample buffers (capacity Ns_cap)
     int64_t ns_cap = 0;
@@ -78,6 +79,12 @@ struct nqh_ctx {
     dbuf mr_vw;   // MINRES: three Lanczos vectors and three directions, Np doubles each
     int64_t last_ns = 0;
     bool grad_ready = false;
+    bool solve_ok = false;    // the last precondition / iteration solved (its step may be applied)
+    // configuration generation: bumped by every reallocation of a device
+    // buffer, every dropped graph (which every reconfiguration call does) and
+    // nqh_set_iteration; a batch's captured graph is valid only for the
+    // generations it was captured at
+    uint64_t gen = 0;
 
     // graph capture of the SR iteration
     bool use_graph = false;
@@ -115,7 +122,7 @@ struct nqh_ctx {
 namespace {
 
 constexpr int kColChunks = 256;
-constexpr int64_t kSplitSamples = 4096;   // batches from this size run the estimator as two launches
+constexpr int64_t kSplitSamples = 4096;   // batches from this size run the forces launch on a side stream
 constexpr int kTableLdsMax = 32 * 1024;   // connection tables up to this size are staged in LDS
 
 nqh_status fail(nqh_ctx* c, nqh_status s, const std::string& msg) {
@@ -164,6 +171,7 @@ nqh_status ensure(nqh_ctx* c, dbuf& b, size_t bytes) {
     b.p = nullptr;
     b.bytes = 0;
     if (bytes == 0) return NQH_OK;
+    c->gen++;
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) return fail(c, NQH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     b.bytes = bytes;
@@ -261,14 +269,16 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
     if ((s = ensure(c, c->lp, 2 * e * Ns))) return s;
     if ((s = ensure(c, c->Y, e * (size_t)ycols(c) * Ns))) return s;
     if ((s = ensure(c, c->Lloc, 2 * e * Ns))) return s;
-    if (!is_rbm(c) && Ns >= kSplitSamples) {
-        // the changed connections between the two estimator launches, and the
-        // side stream (created on first use, so that contexts that never fork
-        // do not take a hardware queue from the others)
+    if (!is_rbm(c)) {
+        // the changed connections between the two estimator launches
         const size_t nrec = (size_t)Ns * (size_t)std::max(c->max_conn, 1);
         if ((s = ensure(c, c->crw, nrec * 2 * e)) || (s = ensure(c, c->crm, nrec * sizeof(uint2))) ||
             (s = ensure(c, c->cnum, (size_t)Ns * sizeof(int))) || (s = ensure(c, c->cwd, (size_t)Ns * 2 * e)))
             return s;
+    }
+    if (!is_rbm(c) && Ns >= kSplitSamples) {
+        // the side stream (created on first use, so that contexts that never
+        // fork do not take a hardware queue from the others)
         if (!c->side) {
             HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
             HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -359,11 +369,14 @@ hipError_t launch_logpsi(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns, void*
     return nq_launch_estimate(a, fp64_net(c), est_grid_for(Ns), c->stream);
 }
 
+// the update reads the solve's status words on the device and applies no
+// step after a failed factorization, a non-finite step or a non-finite loss
 hipError_t launch_update(nqh_ctx* c, double lr) {
+    const nq_step_guard g{P<int>(c->info), P<int>(c->flag), P<double>(c->stats)};
     if (is_rbm(c))
         return nq_launch_update_complex(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->rbm_npc, lr,
-                                        c->stream);
-    return nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, c->stream);
+                                        g, c->stream);
+    return nq_launch_update(P<double>(c->p64), P<float>(c->p32), P<double>(c->delta), c->d, lr, g, c->stream);
 }
 
 hipError_t launch_metropolis(nqh_ctx* c, const samp_args& a) {
@@ -393,6 +406,7 @@ hipError_t launch_metropolis(nqh_ctx* c, const samp_args& a) {
     r.chain_state = a.chain_state;
     r.samples = a.samples;
     r.accepted = a.accepted;
+    r.chain_acc = a.chain_acc;
     return nq_launch_rbm_metropolis(r, fp64_net(c), c->stream);
 }
 
@@ -455,6 +469,7 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
     phase_end(c, PH_SBUILD);
     c->last_ns = Ns;
     c->grad_ready = true;
+    c->solve_ok = false;
     return NQH_OK;
 }
 
@@ -482,24 +497,16 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     a1.Y = c->Y.p;
     a2.rec_mode = 2;
     a2.Fpart = P<double>(c->Fpart);
-    // Large batches: two launches, the forces beside the S build (in order
-    // when per-phase times are recorded, or when S is not built (CG)).
-    // Below kSplitSamples the kernels are short and one launch does it all:
-    // a second launch and stream cost more than the overlap gains (e.g. many
-    // small problems each on their own stream)
-    const bool split = Ns >= kSplitSamples;
-    const bool beside = split && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY;
+    // Two launches: the walk (log rho, O rows, L_loc, changed connections),
+    // then the forces.  Large batches run the forces beside the S build (in
+    // order when per-phase times are recorded, or when S is not built (CG));
+    // below kSplitSamples both run in order on the context's stream (a second
+    // stream costs more than the overlap gains, e.g. for many small problems
+    // each on their own stream)
+    const bool beside = Ns >= kSplitSamples && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY;
     phase_begin(c, PH_ESTIMATE);
-    if (split) {
-        HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
-        if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
-    } else {
-        est_args a0 = a;
-        a0.lp = c->lp.p;
-        a0.Y = c->Y.p;
-        a0.Fpart = P<double>(c->Fpart);
-        HIPCHK(c, nq_launch_estimate(a0, f64, c->est_grid, c->stream));
-    }
+    HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
+    if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
     phase_end(c, PH_ESTIMATE);
     phase_begin(c, PH_POST);
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
@@ -522,6 +529,7 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     if (s) return s;
     c->last_ns = Ns;
     c->grad_ready = true;
+    c->solve_ok = false;
     return NQH_OK;
 }
 
@@ -530,6 +538,7 @@ static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
     if (c->solver != NQH_SOLVER_CHOLESKY) {   // matrix-free: the solve works on Y and mu directly
         c->last_ns = Ns;
         c->grad_ready = true;
+        c->solve_ok = false;
         return NQH_OK;
     }
     // S in P order: A-gram at [0, nA), B-gram at [nlam, np)
@@ -550,6 +559,7 @@ static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
         phase_end(c, PH_SBUILD);
         c->last_ns = Ns;
         c->grad_ready = true;
+        c->solve_ok = false;
         return NQH_OK;
     }
     HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
@@ -574,6 +584,7 @@ static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
     phase_end(c, PH_SBUILD);
     c->last_ns = Ns;
     c->grad_ready = true;
+    c->solve_ok = false;
     return NQH_OK;
 }
 
@@ -656,7 +667,7 @@ void nqh_destroy(nqh_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->keys_in, &c->lp,
+    dbuf* all[] = {&c->table_mem, &c->p64, &c->p32, &c->chain_state, &c->samples, &c->acc, &c->chain_acc, &c->keys_in, &c->lp,
                    &c->Y, &c->Lloc, &c->G, &c->Yt, &c->Fpart, &c->colpart, &c->mu, &c->stats, &c->F, &c->S, &c->S0,
                    &c->delta, &c->resid, &c->info, &c->flag, &c->fpart2, &c->winv, &c->tmp, &c->gpart, &c->A2,
                    &c->winv2, &c->y2, &c->x2, &c->cg_w,
@@ -823,8 +834,10 @@ nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t ac
     // the estimator's limits: 2N-bit keys in 32 bits of lane state, two angle registers per lane
     if (N < 1 || N > 16) return fail(c, NQH_E_ARG, "N must be in 1..16");
     if (nh < 1 || na < 1 || nh > 64 || na > 64) return fail(c, NQH_E_ARG, "nh, na must be in 1..64");
-    if (act != 0) return fail(c, NQH_E_ARG, "only the softplus activation (af_softplus) is implemented");
+    if (act != NQH_ACT_SOFTPLUS && act != NQH_ACT_LOGCOSH)
+        return fail(c, NQH_E_ARG, "act must be NQH_ACT_SOFTPLUS (af_softplus) or NQH_ACT_LOGCOSH (af_logcosh)");
     c->d = make_ndm_dims(N, nh, na);
+    c->d.act = act;
     c->net_kind = 0;
     drop_graph(c);
     nqh_status s;
@@ -1156,6 +1169,7 @@ nqh_status nqh_set_sampler(nqh_ctx* c, int64_t chains, int32_t chain_length, int
     if ((s = ensure(c, c->chain_state, sizeof(uint64_t) * chains))) return s;
     if ((s = ensure(c, c->samples, sizeof(uint64_t) * chains * chain_length))) return s;
     if ((s = ensure(c, c->acc, sizeof(unsigned long long) * 2))) return s;
+    if ((s = ensure(c, c->chain_acc, sizeof(uint32_t) * chains))) return s;
     c->chains_initialised = false;
     c->has_sampler = true;
     drop_graph(c);
@@ -1180,7 +1194,7 @@ static nqh_status made_global_move(nqh_ctx* c, uint32_t slice, const uint32_t* i
     HIPCHK(c, launch_logpsi(c, kk, 2 * n, c->made_lp.p, nullptr));
     HIPCHK(c, nq_launch_made_accept(kk, c->made_lp.p, fp64_net(c), P<float>(c->made_lq), n, c->seed, c->iter,
                                     iter_dev, slice, P<uint64_t>(c->chain_state),
-                                    P<unsigned long long>(c->acc) + 1, c->stream));
+                                    P<unsigned long long>(c->acc) + 1, P<uint32_t>(c->chain_acc), c->stream));
     return NQH_OK;
 }
 
@@ -1217,9 +1231,11 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.chain_state = P<uint64_t>(c->chain_state);
     a.samples = P<uint64_t>(c->samples);
     a.accepted = P<unsigned long long>(c->acc);
+    a.chain_acc = P<uint32_t>(c->chain_acc);
     a.iter_dev = c->use_graph ? P<uint32_t>(c->iter_dev) : nullptr;
     phase_begin(c, PH_SAMPLER);
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 4, 0u, c->stream));
+    HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->chain_acc), c->chains, 0u, c->stream));
     if (!mixed) {
         HIPCHK(c, launch_metropolis(c, a));
     } else {
@@ -1273,6 +1289,16 @@ nqh_status nqh_sample(nqh_ctx* c, uint64_t* keys_out, int64_t* accepted) {
         HIPCHK(c, hipMemcpy(&v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
         *accepted = (int64_t)v;
     }
+    return NQH_OK;
+}
+
+nqh_status nqh_get_chain_acceptance(nqh_ctx* c, uint32_t* accepted_out, int64_t chains) {
+    if (!c || !accepted_out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->has_sampler || !c->chain_acc.p) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+    if (chains != c->chains) return fail(c, NQH_E_ARG, "chains must equal the sampler's chain count");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(accepted_out, c->chain_acc.p, sizeof(uint32_t) * (size_t)chains, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -1870,21 +1896,31 @@ static nqh_status check_solve(nqh_ctx* c) {
     HIPCHK(c, hipMemcpy(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (info != 0) {
         char buf[128];
-        std::snprintf(buf, sizeof(buf), "S + shift is not positive definite (column %d)", info);
+        std::snprintf(buf, sizeof(buf), "S + shift is not positive definite (column %d): no step applied", info);
         return fail(c, NQH_E_NOTPD, buf);
     }
-    if (flag) return fail(c, NQH_E_NAN, "non-finite SR update");
+    if (flag) return fail(c, NQH_E_NAN, "non-finite SR update: no step applied");
     return NQH_OK;
 }
 
 nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double* delta_out) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
+    c->solve_ok = false;
+    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
+    {
+        // a non-finite loss (NaN in L_loc or in the network) is reported before any solve
+        double loss = 0.0;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(&loss, c->stats.p, sizeof(double), hipMemcpyDeviceToHost));
+        if (!std::isfinite(loss)) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no solve");
+    }
     nqh_status s = precondition_async(c, eps, shift_kind, refine);
     if (s) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
     if ((s = check_solve(c))) return s;
+    c->solve_ok = true;
     if (delta_out && is_rbm(c)) {
         const int npc = c->rbm_npc;
         std::vector<double> dv(2 * npc);
@@ -1902,7 +1938,8 @@ nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
 nqh_status nqh_update(nqh_ctx* c, double lr) {
     if (!c) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
-    if (!c->grad_ready) return fail(c, NQH_E_STATE, "no preconditioned gradient");
+    if (!c->grad_ready || !c->solve_ok)
+        return fail(c, NQH_E_STATE, "no successful nqh_precondition for the current gradient: no step applied");
     phase_begin(c, PH_UPDATE);
     HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
@@ -1912,6 +1949,7 @@ nqh_status nqh_update(nqh_ctx* c, double lr) {
 }
 
 static void drop_graph(nqh_ctx* c) {
+    c->gen++;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
@@ -1975,6 +2013,7 @@ static nqh_status graph_iteration(nqh_ctx* c, double eps, int kind, int refine, 
     c->iter += 1;
     c->chains_initialised = true;
     c->grad_ready = true;
+    c->solve_ok = false;
     c->last_ns = c->chains * c->chain_len;
     return NQH_OK;
 }
@@ -2032,15 +2071,19 @@ static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t
 }
 
 // wait for the enqueued iteration; statistics and solve checks
+// (the update kernel has already skipped the step when the loss, the
+// factorization or the step failed: the parameters are then unchanged)
 static nqh_status sr_finish(nqh_ctx* c, double* stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
-    nqh_status s;
-    if ((s = check_solve(c))) return s;
+    c->solve_ok = false;
     double st[5];
     HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
     if (stats) std::memcpy(stats, st, sizeof(st));
-    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
+    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no step applied");
+    nqh_status s;
+    if ((s = check_solve(c))) return s;
+    c->solve_ok = true;
     return NQH_OK;
 }
 
@@ -2077,6 +2120,7 @@ struct nqh_batch {
     bool valid = false;
     double eps = 0.0, lr = 0.0;
     int kind = -1, refine = -1;
+    std::vector<uint64_t> gens;   // members' configuration generations at capture
     std::string err;
 };
 
@@ -2155,7 +2199,18 @@ nqh_status batch_build(nqh_batch* b, double eps, int kind, int refine, double lr
     BHIPCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->valid = true;
     b->eps = eps; b->kind = kind; b->refine = refine; b->lr = lr;
+    b->gens.resize(n);
+    for (size_t i = 0; i < n; ++i) b->gens[i] = b->m[i]->gen;
     return NQH_OK;
+}
+
+// a member was reconfigured, reallocated or given a new iteration number
+// since the capture: the graph holds stale buffers, shapes or counters
+bool batch_stale(const nqh_batch* b) {
+    if (b->gens.size() != b->m.size()) return true;
+    for (size_t i = 0; i < b->m.size(); ++i)
+        if (b->m[i]->gen != b->gens[i]) return true;
+    return false;
 }
 
 }  // namespace
@@ -2238,7 +2293,7 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
             if (s) return bmember(b, i, s);
         }
     } else {
-        if (!b->valid || b->eps != eps || b->kind != kind || b->refine != refine || b->lr != lr) {
+        if (!b->valid || batch_stale(b) || b->eps != eps || b->kind != kind || b->refine != refine || b->lr != lr) {
             nqh_status s = batch_build(b, eps, kind, refine, lr);
             if (s) return s;
         }
@@ -2256,6 +2311,7 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
             c->iter += 1;
             c->chains_initialised = true;
             c->grad_ready = true;
+            c->solve_ok = false;
             c->last_ns = c->chains * c->chain_len;
         }
     }

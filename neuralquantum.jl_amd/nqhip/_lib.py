@@ -23,6 +23,7 @@ NQH_E_NOMEM = -6
 NQH_E_CAPACITY = -7
 
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+ACT_SOFTPLUS, ACT_LOGCOSH = 0, 1
 SHIFT_ADD, SHIFT_DIAG = 0, 1
 SOLVER_CHOLESKY, SOLVER_CG, SOLVER_MINRES, SOLVER_MINRES_QLP, SOLVER_LSQR = 0, 1, 2, 3, 4
 SAMPLER_COMPAT_STALE_LAST_SLICE = 1
@@ -78,6 +79,7 @@ SIGNATURES = {
     "nqh_set_sampler": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "nqh_set_iteration": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "nqh_sample": (C.c_int32, [C.c_void_p, _u64p, _i64p]),
+    "nqh_get_chain_acceptance": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int64]),
     "nqh_sample_grad": (C.c_int32, [C.c_void_p, _dp]),
     "nqh_precondition": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, _dp]),
     "nqh_update": (C.c_int32, [C.c_void_p, C.c_double]),
@@ -319,6 +321,12 @@ class Context:
         acc = C.c_int64()
         self._chk(self.lib.nqh_sample(self.h, _p(keys, _u64p), C.byref(acc)))
         return keys, int(acc.value)
+
+    def chain_acceptance(self):
+        """Accepted moves per chain in the last sampling call (0: a stuck chain)."""
+        out = np.empty(self.chains, dtype=np.uint32)
+        self._chk(self.lib.nqh_get_chain_acceptance(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)), self.chains))
+        return out
 
     def sample_grad(self):
         st = np.empty(5)

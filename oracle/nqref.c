@@ -72,6 +72,25 @@ static inline cplx sp_c(cplx z) {
 }
 static inline cplx dsp_c(cplx z) { return 1.0 / (1.0 + cexp(-z)); }
 
+/* logL2 = log cosh (af_logcosh), as activation.jl:5-10 writes it:
+ * log(cosh x) for |x| <= 12, |x| - log 2 beyond; complex: logL2(Re z) +
+ * log(cos(Im z) + i tanh(Re z) sin(Im z)); derivative tanh (:6, :26) */
+static inline double lc_r(double x) { return fabs(x) <= 12.0 ? log(cosh(x)) : fabs(x) - log(2.0); }
+static inline double dlc_r(double x) { return tanh(x); }
+static inline cplx lc_c(cplx z) {
+    double a = creal(z), b = cimag(z);
+    return lc_r(a) + clog(cos(b) + I * tanh(a) * sin(b));
+}
+static inline cplx dlc_c(cplx z) { return ctanh(z); }
+
+/* the NDM's activation f (NDM.jl:50): 0 = softplus (default), 1 = logcosh */
+static int g_act = 0;
+void nqref_set_activation(int act) { g_act = act; }
+static inline double f_r(double x) { return g_act ? lc_r(x) : sp_r(x); }
+static inline double df_r(double x) { return g_act ? dlc_r(x) : dsp_r(x); }
+static inline cplx f_c(cplx z) { return g_act ? lc_c(z) : sp_c(z); }
+static inline cplx df_c(cplx z) { return g_act ? dlc_c(z) : dsp_c(z); }
+
 /* ------------------------------------------------------------ NDM layout */
 typedef struct { int N, nh, na; int64_t bmu, hmu, wmu, umu, bla, hla, dla, wla, ula, np; } ndm_off;
 
@@ -116,10 +135,10 @@ static cplx ndm_eval(const ndm_off *L, const double *p, const double *s, const d
             tls += wl * s[i]; tle += wl * e[i];
             tms += wm * s[i]; tme += wm * e[i];
         }
-        sl_s += sp_r(tls); sl_e += sp_r(tle);
-        sm_s += sp_r(tms); sm_e += sp_r(tme);
+        sl_s += f_r(tls); sl_e += f_r(tle);
+        sm_s += f_r(tms); sm_e += f_r(tme);
         if (O) {
-            double als = dsp_r(tls), ale = dsp_r(tle), ams = dsp_r(tms), ame = dsp_r(tme);
+            double als = df_r(tls), ale = df_r(tle), ams = df_r(tms), ame = df_r(tme);
             O[L->hla + h] = 0.5 * (als + ale);
             O[L->hmu + h] = 0.5 * I * (ams - ame);
             for (int i = 0; i < N; ++i) {
@@ -136,9 +155,9 @@ static cplx ndm_eval(const ndm_off *L, const double *p, const double *s, const d
             um += p[L->umu + a + (int64_t)na * i] * (s[i] - e[i]);
         }
         cplx arg = (0.5 * ul + p[L->dla + a]) + I * (0.5 * um);
-        pi += sp_c(arg);
+        pi += f_c(arg);
         if (O) {
-            cplx d = dsp_c(arg);
+            cplx d = df_c(arg);
             O[L->dla + a] = d;
             for (int i = 0; i < N; ++i) {
                 O[L->ula + a + (int64_t)na * i] = 0.5 * d * (s[i] + e[i]);

@@ -69,10 +69,16 @@ class Table:
 
 
 class Ref:
-    def __init__(self):
+    """act: the NDM's activation f (NDM.jl:50), 0 = softplus (af_softplus,
+    the default), 1 = logcosh (af_logcosh, activation.jl:5-10); set before
+    every network evaluation of this object."""
+
+    def __init__(self, act=0):
         if not os.path.exists(_LIB):
             build()
+        self.act = int(act)
         L = C.CDLL(_LIB)
+        L.nqref_set_activation.argtypes = [C.c_int]
         L.nqref_ndm_np.restype = C.c_int64
         L.nqref_ndm_np.argtypes = [C.c_int, C.c_int, C.c_int]
         L.nqref_ndm_logpsi.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int64, _u64p, _dp]
@@ -97,6 +103,7 @@ class Ref:
         p = np.ascontiguousarray(p, dtype=np.float64)
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.empty(2 * len(keys))
+        self.L.nqref_set_activation(self.act)
         self.L.nqref_ndm_logpsi(N, nh, na, _ptr(p, _dp), len(keys), _ptr(keys, _u64p), _ptr(out, _dp))
         return out.view(np.complex128)
 
@@ -106,6 +113,7 @@ class Ref:
         np_ = self.ndm_np(N, nh, na)
         out = np.empty(2 * len(keys))
         O = np.empty(2 * len(keys) * np_)
+        self.L.nqref_set_activation(self.act)
         self.L.nqref_ndm_logpsi_grad(N, nh, na, _ptr(p, _dp), len(keys), _ptr(keys, _u64p),
                                      _ptr(out, _dp), _ptr(O, _dp))
         return out.view(np.complex128), O.view(np.complex128).reshape(len(keys), np_)
@@ -127,6 +135,7 @@ class Ref:
         np_ = self.ndm_np(N, nh, na)
         Lloc = np.empty(2 * len(keys))
         dL = np.empty(2 * len(keys) * np_) if grad else None
+        self.L.nqref_set_activation(self.act)
         self.L.nqref_local_estimator(C.byref(table.s), nh, na, _ptr(p, _dp), len(keys),
                                      _ptr(keys, _u64p), _ptr(Lloc, _dp),
                                      _ptr(dL, _dp) if grad else None)
@@ -164,6 +173,7 @@ class Ref:
     def metropolis(self, N, nh, na, p, chains, chain_len, passes, burn, seed, it=0, compat=False):
         p = np.ascontiguousarray(p, dtype=np.float64)
         out = np.empty(chains * chain_len, dtype=np.uint64)
+        self.L.nqref_set_activation(self.act)
         acc = self.L.nqref_metropolis(N, nh, na, _ptr(p, _dp), chains, chain_len, passes, burn,
                                       C.c_uint64(seed), C.c_uint32(it), int(compat), _ptr(out, _u64p))
         return out, int(acc)
@@ -188,9 +198,11 @@ def ndm_layout(N, nh, na):
     return off, p
 
 
-def ndm_closed_form(N, nh, na, p, s, e):
+def ndm_closed_form(N, nh, na, p, s, e, act=0):
     """log rho(s, e) by the NDM formula with numpy, vectorised over columns of
-    s, e (N x B)."""
+    s, e (N x B); act 1 takes f = log cosh (numpy's complex log of cosh: the
+    principal value, which the reference's logL2(a) + log(cos b + i tanh a
+    sin b) equals since cosh a > 0)."""
     off, _ = ndm_layout(N, nh, na)
 
     def blk(name, shape=None):
@@ -199,7 +211,7 @@ def ndm_closed_form(N, nh, na, p, s, e):
         return v.reshape(shape, order="F") if shape else v
 
     def sp(x):
-        return np.log1p(np.exp(x))
+        return np.log(np.cosh(x)) if act else np.log1p(np.exp(x))
     wl, wm = blk("w_la", (nh, N)), blk("w_mu", (nh, N))
     ul, um = blk("u_la", (na, N)), blk("u_mu", (na, N))
     hl, hm, dl = blk("h_la")[:, None], blk("h_mu")[:, None], blk("d_la")[:, None]
@@ -207,7 +219,7 @@ def ndm_closed_form(N, nh, na, p, s, e):
     gl = 0.5 * (sp(wl @ s + hl).sum(0) + sp(wl @ e + hl).sum(0) + bl @ (s + e))
     gm = 0.5 * (sp(wm @ s + hm).sum(0) - sp(wm @ e + hm).sum(0) + bm @ (s - e))
     arg = 0.5 * (ul @ (s + e)) + dl + 0.5j * (um @ (s - e))
-    return gl + 1j * gm + np.log1p(np.exp(arg)).sum(0)
+    return gl + 1j * gm + sp(arg).sum(0)
 
 
 def all_keys(N):

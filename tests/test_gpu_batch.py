@@ -83,3 +83,36 @@ def test_batch_create_rejects_duplicates(gpu_ok):
     with pytest.raises(NQHError):
         Batch([c, c])
     c.close()
+
+
+def test_batch_recaptures_after_a_member_is_reconfigured(gpu_ok, ref):
+    """A member given a new sampler (other chain count: its buffers are
+    regrown), new parameters and a new iteration number between batch
+    iterations: the next batch iteration captures the graph again and the
+    member matches the same context run alone."""
+    prec = (_lib.PREC_FP64, _lib.PREC_FP64)
+    alone = [member(ref, N, g, 300 + i, 0, prec) for i, (N, g) in enumerate(SIZES[:3])]
+    inb = [member(ref, N, g, 300 + i, 0, prec) for i, (N, g) in enumerate(SIZES[:3])]
+    batch = Batch(inb)
+
+    def step(it):
+        st = batch.sr_iteration(0.01, 0.01)
+        for i, c in enumerate(alone):
+            sa = c.sr_iteration(0.01, 0.01)
+            assert rel_err(st[i], sa) < 1e-12, (it, i)
+            assert np.array_equal(inb[i].get_samples(), c.get_samples()), (it, i)
+
+    step(0)
+    step(1)
+    for c in (alone[1], inb[1]):
+        N = SIZES[1][0]
+        c.set_sampler(96, 6, 2 * N + 1, 10, 0)
+        c.set_params(params(ref.ndm_np(N, N, N), seed=77, sigma=0.05))
+        c.set_iteration(40)
+    step(2)
+    step(3)
+    for a, b in zip(alone, inb):
+        assert rel_err(b.get_params(), a.get_params()) < 1e-12
+    batch.close()
+    for c in alone + inb:
+        c.close()

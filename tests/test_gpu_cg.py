@@ -1,9 +1,12 @@
-"""Matrix-free SR (conjugate gradients or MINRES on S v = Re(Oc^H Oc v)/Ns +
-shift, S never formed) against the Cholesky solve of the same S and F, which
-is itself pinned to LAPACK elsewhere, and MINRES against the MINRES oracle run
-on the explicit S.  The solvers stop at ||r|| <= 1e-12 ||F||, so the
-solutions agree to about cond(S + shift) x 1e-12 (a few 1e-9 with the
-eps diag(S) shift at N = 6)."""
+"""Matrix-free SR (conjugate gradients, MINRES, MINRES-QLP or LSQR on
+S v = Re(Oc^H Oc v)/Ns + shift, S never formed) against the Cholesky solve
+of the same S and F, which is itself pinned to LAPACK elsewhere, and each
+Krylov solver against its oracle run on the explicit S.  The solvers stop at
+||r|| <= tol ||F||, so the solutions agree to about cond(S + shift) x tol.
+Where Lanczos loses orthogonality (the eps diag(S) shift, cond ~4e5 at
+N = 6) rounding-level changes of S or F move the iterates, so those cases
+are held to bounds rounding cannot move (residuals, distance to the LAPACK
+solve) and to count windows measured with the oracle on the same inputs."""
 import numpy as np
 import pytest
 
@@ -84,22 +87,35 @@ def test_cg_does_not_form_s(gpu_ok):
     b.close()
 
 
-def perturbed(A, seed):
-    """A + a symmetric perturbation at the level of the rounding of the
-    device's products (1e-15 of the largest entry)."""
-    E = np.random.default_rng(seed).standard_normal(A.shape) * 1e-15 * np.abs(A).max()
-    return A + 0.5 * (E + E.T)
+def perturbed(A, F, seed):
+    """A and F with perturbations at the level of the rounding of the device's
+    products: a symmetric 1e-15 of the largest entry of A, 1e-15 relative on F."""
+    r = np.random.default_rng(seed)
+    E = r.standard_normal(A.shape) * 1e-15 * np.abs(A).max()
+    return A + 0.5 * (E + E.T), F * (1.0 + 1e-15 * r.standard_normal(F.shape))
+
+
+def error_bound(A, F, tol):
+    """||x - A^-1 F|| <= ||A^-1|| ||F - A x|| <= tol ||F|| / lambda_min(A) for any
+    x with a relative residual below tol, relative to max |A^-1 F|."""
+    x0 = np.linalg.solve(A, F)
+    return tol * np.linalg.norm(F) / np.linalg.eigvalsh(A)[0] / np.abs(x0).max(), x0
 
 
 @pytest.mark.parametrize("kind", [_lib.SHIFT_ADD, _lib.SHIFT_DIAG])
 def test_minres_follows_the_oracle(gpu_ok, kind):
     """The device MINRES against the oracle's MINRES on the explicit S + shift
-    and F of the same samples.  Lanczos runs part by rounding once
-    orthogonality is lost (the eps diag(S) shift needs ~Np products here), so
-    the product count is held to the spread the oracle itself shows under a
-    rounding-level perturbation of S (at least +-1), and the iterate is
-    compared with the oracle's after the same number of products, within 10x
-    that spread."""
+    and F of the same samples.  What rounding cannot move is held exactly:
+    the true residual, and the distance to the LAPACK solve, which a relative
+    residual below tol bounds by tol ||F|| / lambda_min.  With the additive
+    shift (cond ~2e2 here) Lanczos keeps its orthogonality: the oracle takes
+    19 products under every 1e-15 perturbation of S and F, so the count must
+    match and the iterate agrees within 1e-9 (the perturbations move it by
+    1.5e-12).  With eps diag(S) (cond ~4e5) orthogonality is lost and a
+    rounding-level change of F alone moves the count by a few products
+    (151..154 over eight perturbations, measured with the oracle on these
+    inputs) and the iterate by ~1e-2, so the count is held to a window of
+    8 products and the iterate to the residual bounds only."""
     N, eps, tol = 6, 0.01, 1e-6
     keys = random_keys(N, 128 * 8, seed=34)
     A, F = explicit_system(N, 8, kind, eps, keys)
@@ -110,12 +126,16 @@ def test_minres_follows_the_oracle(gpu_ok, kind):
     n = len(F)
     xr, itr, rr, ok = minres_oracle(A, F, tol, 10 * n)
     assert ok and 0 < it and res <= tol
-    counts = [itr] + [minres_oracle(perturbed(A, s), F, tol, 10 * n)[1] for s in range(4)]
-    assert abs(it - itr) <= max(1, 2 * (max(counts) - min(counts)))
-    xk = minres_oracle(A, F, 0.0, it)[0]
-    spread = max(rel_err(minres_oracle(perturbed(A, s), F, 0.0, it)[0], xk) for s in range(3))
-    assert rel_err(x, xk) <= 10 * spread + 1e-12
-    assert np.linalg.norm(F - A @ x) <= 1.01 * tol * np.linalg.norm(F)
+    assert np.linalg.norm(F - A @ x) <= 1.05 * tol * np.linalg.norm(F)
+    bound, x0 = error_bound(A, F, tol)
+    assert rel_err(x, x0) <= 1.05 * bound
+    if kind == _lib.SHIFT_ADD:
+        counts = {minres_oracle(*perturbed(A, F, s), tol, 10 * n)[1] for s in range(4)}
+        assert counts == {itr} and abs(it - itr) <= 1
+        if it == itr:
+            assert rel_err(x, xr) <= 1e-9
+    else:
+        assert abs(it - itr) <= 8
     b.close()
 
 
@@ -130,27 +150,33 @@ def explicit_system(N, seed, kind, eps, keys):
 def test_minres_qlp_follows_the_reference_file(gpu_ok, kind):
     """sr_qlp against the oracle's restatement of minresqlp.jl (its QLP-branch
     updates, rotations, norm estimates and stopping flags) on the explicit
-    S + shift, with the product count and the iterate held to the oracle's
-    own spread under a rounding-level perturbation of S, as for MINRES.  The test's relres is relative to
-    ||A|| ||x|| + ||F||, so the solution is held to that bound, not to the
-    Cholesky solve."""
+    S + shift.  The oracle's product count is the same under every 1e-15
+    perturbation of S and F for both shifts here (16 and 41), so the device
+    must take it too (within one product).  As minresqlp.jl is written,
+    reaching maxiter keeps the iterate, so the device's iterate after 10
+    products is compared with the oracle's within 1e-9 (rounding-level
+    perturbations move it by < 1e-12).  The test's relres is relative to
+    ||A|| ||x|| + ||F||, so the final solution is held to that bound."""
     N, eps, tol = 6, 0.01, 1e-6
     keys = random_keys(N, 128 * 8, seed=35)
     A, F = explicit_system(N, 9, kind, eps, keys)
+    n = len(F)
+    xr, sc = minresqlp_oracle(A, F, tol, 10 * n)
+    counts = {minresqlp_oracle(*perturbed(A, F, s), tol, 10 * n)[1].iters for s in range(4)}
+    assert sc.flag == 1 and counts == {sc.iters}
     b = ndm(N, 1, 9, solver=_lib.SOLVER_MINRES_QLP, tol=tol)
     b.grad_from_samples(keys, 128, 8, want_S=False)
     x = b.precondition(eps, kind)
     it, relres = b.sr_solver_info()
-    n = len(F)
-    xr, sc = minresqlp_oracle(A, F, tol, 10 * n)
-    assert sc.flag == 1 and 0 < it and relres <= tol
-    counts = [sc.iters] + [minresqlp_oracle(perturbed(A, s), F, tol, 10 * n)[1].iters for s in range(4)]
-    assert abs(it - sc.iters) <= max(1, 2 * (max(counts) - min(counts)))
-    xk, _ = minresqlp_oracle(A, F, 0.0, it)
-    spread = max(rel_err(minresqlp_oracle(perturbed(A, s), F, 0.0, it)[0], xk) for s in range(3))
-    assert rel_err(x, xk) <= 10 * spread + 1e-12
+    assert 0 < it and relres <= tol and abs(it - sc.iters) <= 1
     # relres bounds the true residual through the estimates of ||A|| and ||x||
     assert np.linalg.norm(F - A @ x) <= 1.01 * tol * (sc.anorm * np.linalg.norm(x) + np.linalg.norm(F))
+    b.close()
+    b = ndm(N, 1, 9, solver=_lib.SOLVER_MINRES_QLP, tol=1e-14, maxiter=10)
+    b.grad_from_samples(keys, 128, 8, want_S=False)
+    x10 = b.precondition(eps, kind)
+    assert b.sr_solver_info()[0] == 10
+    assert rel_err(x10, minresqlp_oracle(A, F, 1e-14, 10)[0]) <= 1e-9
     b.close()
 
 
@@ -173,11 +199,16 @@ def test_minres_qlp_maxiter_keeps_the_iterate(gpu_ok):
 @pytest.mark.parametrize("kind", [_lib.SHIFT_ADD, _lib.SHIFT_DIAG])
 def test_lsqr_follows_the_oracle(gpu_ok, kind):
     """sr_lsq against the LSQR oracle on the explicit S + shift with the
-    reference's default tolerances: the same stopping reason's outcome
-    (converged), a step count within the oracle's own spread under a
-    rounding-level perturbation of S (LSQR works on the normal equations,
-    so runs part by rounding early), and the iterate after the same number of
-    steps within 10x that spread."""
+    reference's default tolerances (atol = btol = sqrt(eps), conlim =
+    1/sqrt(eps)): a converged stopping reason, and a step count and a true
+    residual ||F - A x|| within what the oracle itself shows under 1e-15
+    perturbations of S and F (LSQR's residual estimate drifts from the true
+    residual over ~1000 steps, so the true residual is held to 3x the
+    oracle's worst rather than to the stopping rule).  Measured with the
+    oracle on these inputs: 36..37 steps with the additive shift, final
+    iterates within 1.2e-6 of each other (the device's is held to 1e-4 of the
+    oracle's); 979..991 steps with eps diag(S), iterates ~1e-2 apart (count
+    window 30, residual only)."""
     N, eps = 6, 0.01
     keys = random_keys(N, 128 * 8, seed=37)
     A, F = explicit_system(N, 11, kind, eps, keys)
@@ -188,10 +219,12 @@ def test_lsqr_follows_the_oracle(gpu_ok, kind):
     n = len(F)
     xr, itr, istop, ok = lsqr_oracle(A, F, 10 * n)
     assert ok and np.any(x)
-    counts = [itr] + [lsqr_oracle(perturbed(A, s), F, 10 * n)[1] for s in range(4)]
-    assert abs(it - itr) <= max(2, 2 * (max(counts) - min(counts)))
-    xk = lsqr_oracle(A, F, it, atol=0.0, btol=0.0, conlim=0.0)[0]
-    spread = max(rel_err(lsqr_oracle(perturbed(A, s), F, it, atol=0.0, btol=0.0, conlim=0.0)[0], xk)
-                 for s in range(3))
-    assert rel_err(x, xk) <= 10 * spread + 1e-10
+    resid = lambda y: np.linalg.norm(F - A @ y)
+    worst = max([resid(xr)] + [resid(lsqr_oracle(*perturbed(A, F, s), 10 * n)[0]) for s in range(4)])
+    assert resid(x) <= 3.0 * worst
+    if kind == _lib.SHIFT_ADD:
+        assert abs(it - itr) <= 2
+        assert rel_err(x, xr) <= 1e-4
+    else:
+        assert abs(it - itr) <= 30
     b.close()

@@ -109,6 +109,89 @@ def test_steady_state_trace_preserving():
 
 
 # ---------------------------------------------------------------- estimator
+# ---------------------------------------------------------------- NDM with af_logcosh
+def test_logcosh_activation_known_values():
+    """The oracle's logcosh restates activation.jl:5-10 (a branch at |x| = 12,
+    a complex form through tanh); numpy's log(cosh z) is a different
+    construction of the same function.  Known values: log cosh 1 =
+    0.43378083048302..., log cosh(i pi/4) = log cos(pi/4) = -log(2)/2, and
+    the |x| > 12 branch |x| - log 2, which differs from log cosh x by
+    log1p(e^-2|x|) (< 3.8e-11), is taken there and only there."""
+    assert abs(math.log(math.cosh(1.0)) - 0.4337808304830271) < 1e-15
+    r = nqref.Ref(act=1)
+    N, nh, na = 1, 1, 1
+    off, npar = nqref.ndm_layout(N, nh, na)
+    # a single Pi unit with argument d_la + i u_mu (s - e)/2: s = +1, e = -1 gives z = d + i u
+    for d, u in [(0.0, math.pi / 4), (1.0, 0.3), (13.5, -0.7), (-14.0, 2.0)]:
+        p = np.zeros(npar)
+        p[off["d_la"][0]] = d
+        p[off["u_mu"][0]] = u
+        key = np.array([1], dtype=np.uint64)          # s = +1, e = -1
+        base = 2 * math.log(math.cosh(0.0))            # the lambda / mu units at zero angles
+        z = complex(d, u)
+        want = complex(np.log(np.cosh(z))) + base
+        got = r.logpsi(N, nh, na, p, key)[0]
+        # beyond |x| = 12 the reference's branch drops log1p(e^-2|x|) < 3.8e-11
+        branch = math.log1p(math.exp(-2 * abs(d))) if abs(d) > 12 else 0.0
+        assert abs(got - want) < 1e-13 * max(1.0, abs(want)) + 1.01 * branch
+        assert branch == 0.0 or abs(got - want) > 0.5 * branch
+    assert abs(r.logpsi(1, 1, 1, np.eye(1, npar, off["u_mu"][0]).ravel() * (math.pi / 4),
+                        np.array([1], dtype=np.uint64))[0] - (-0.5 * math.log(2.0))) < 1e-15
+
+
+@pytest.mark.parametrize("N,nh,na,sigma", [(2, 2, 2, 0.4), (4, 8, 4, 0.4), (3, 3, 6, 6.0)])
+def test_ndm_logcosh_vs_numpy_closed_form(N, nh, na, sigma):
+    """sigma = 6 puts many angles beyond the |x| = 12 branch point."""
+    r = nqref.Ref(act=1)
+    p = params(r.ndm_np(N, nh, na), seed=N + nh + na + 1, sigma=sigma)
+    keys = nqref.all_keys(N)
+    s = ((keys[None, :] >> np.arange(N, dtype=np.uint64)[:, None]) & np.uint64(1)).astype(float) * 2 - 1
+    e = ((keys[None, :] >> (np.arange(N, dtype=np.uint64) + np.uint64(N))[:, None]) & np.uint64(1)).astype(float) * 2 - 1
+    cf = nqref.ndm_closed_form(N, nh, na, p, s, e, act=1)
+    # the imaginary part is defined mod 2 pi per Pi unit: compare exp(i Im)
+    got = r.logpsi(N, nh, na, p, keys)
+    # the reference's branch beyond |x| = 12 drops log1p(e^-2|x|) < 3.8e-11 per unit
+    units = 2 * 2 * nh + na
+    assert np.max(np.abs(got.real - cf.real)) < 1e-13 * np.max(np.abs(cf.real)) + units * 3.8e-11
+    # (Gamma_mu, a sum of real f, is part of Im log rho: the same branch bound)
+    assert np.max(np.abs(np.exp(1j * got.imag) - np.exp(1j * cf.imag))) < 1e-12 + units * 3.8e-11
+    # and it differs from softplus
+    assert rel_err(nqref.Ref(act=0).logpsi(N, nh, na, p, keys).real, cf.real) > 1e-3
+
+
+def test_ndm_logcosh_gradient_vs_finite_differences():
+    """O = d log rho / dp with f' = tanh (fwd_der(logL2) = dlogL2, activation.jl:26)."""
+    r = nqref.Ref(act=1)
+    N, nh, na = 3, 4, 3
+    p = params(r.ndm_np(N, nh, na), seed=6, sigma=0.5)
+    keys = nqref.all_keys(N)
+    _, O = r.logpsi_grad(N, nh, na, p, keys)
+    h = 1e-6
+    for k in range(len(p)):
+        dp = np.zeros_like(p)
+        dp[k] = h
+        fd = (r.logpsi(N, nh, na, p + dp, keys) - r.logpsi(N, nh, na, p - dp, keys)) / (2 * h)
+        assert np.max(np.abs(fd - O[:, k])) < 1e-8
+
+
+def test_ndm_logcosh_hermitian_and_sampled(ref):
+    """rho(e, s) = conj(rho(s, e)) with logcosh too, and the oracle's Metropolis
+    samples |rho|^2 of NDM(2, 1, 1, af_logcosh) (TV against the exact pdf)."""
+    r = nqref.Ref(act=1)
+    N, nh, na = 2, 2, 2
+    p = params(r.ndm_np(N, nh, na), seed=41, sigma=0.8)
+    keys = nqref.all_keys(N)
+    mask = np.uint64(2 ** N - 1)
+    swapped = (keys >> np.uint64(N)) | ((keys & mask) << np.uint64(N))
+    lp = r.logpsi(N, nh, na, p, keys)
+    assert np.max(np.abs(np.exp(r.logpsi(N, nh, na, p, swapped)) - np.conj(np.exp(lp)))) < 1e-13
+    w = np.exp(2 * lp.real)
+    exact = w / w.sum()
+    out, _ = r.metropolis(N, nh, na, p, 64, 400, 5, 20, seed=98)
+    hist = np.bincount(out.astype(np.int64), minlength=16) / len(out)
+    assert 0.5 * np.abs(hist - exact).sum() < 0.02
+
+
 @pytest.mark.parametrize("model", ["ising3", "xyz22"])
 def test_local_estimator_is_L_rho_over_rho(ref, model):
     if model == "ising3":
