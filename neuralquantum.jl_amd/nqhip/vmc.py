@@ -2,7 +2,8 @@
 
 Julia (reference)                                   here
 -------------------------------------------------   ---------------------------------------------
-NDM(Float64, N, αh, αa, af_softplus)  NDM.jl:50     NDM(N, alpha_h, alpha_a)
+NDM(Float64, N, αh, αa, af_softplus)  NDM.jl:50     NDM(N, alpha_h, alpha_a, act="softplus")
+  (af_logcosh, activation.jl:5-10)                    NDM(N, alpha_h, alpha_a, act="logcosh")
 RBM(ComplexF64, N, α)                  RBM.jl:37-44  RBM(N, alpha)   (kets, with a Hamiltonian)
 RBMSplit(ComplexF64, N, α)             RBMSplit.jl:33-40  RBMSplit(N, alpha)   (density matrices)
 init_random_pars!(net; sigma)          rng.jl:20    net.init_random_pars(rng, sigma)
@@ -21,7 +22,12 @@ compute_observables(is)                BatchedObsDMSampler.jl:59    compute_obse
 
 The loss statistics come back as `Measurement` (stats.jl:1-13).  Device state
 (samples, O, S, the factor, the update) stays on the GPU; `net.flat` is the
-canonical host copy and is refreshed after every update.
+canonical host copy: it is uploaded at every sample_ (so host-side changes
+to the network take effect, as in the reference) and refreshed after every
+update_.  precondition_ returns a `Gradient` (the step, reference order,
+owned by the cache and overwritten by the next call as SRDirect.jl:89 does)
+that knows its sampler, so update_(opt, net, grad) takes the reference's
+arguments.
 """
 from __future__ import annotations
 
@@ -34,7 +40,7 @@ import numpy as np
 from . import _lib
 from .operators import KLocalLiouvillian, KLocalOperator, KLocalOperatorSum, flatten
 
-__all__ = ["NDM", "RBM", "RBMSplit", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement",
+__all__ = ["NDM", "RBM", "RBMSplit", "LocalRule", "MetropolisSampler", "SR", "Descent", "Measurement", "Gradient",
            "BatchedSampler", "HIPBatchedSampler", "sample_", "precondition_", "update_",
            "init_random_pars_", "add_observable_", "compute_observables"]
 
@@ -46,9 +52,12 @@ class NDM:
 
     FIELDS = ("b_mu", "h_mu", "w_mu", "u_mu", "b_la", "h_la", "d_la", "w_la", "u_la")
 
+    ACTS = {"softplus": _lib.ACT_SOFTPLUS, "logcosh": _lib.ACT_LOGCOSH}
+
     def __init__(self, N: int, alpha_h: float, alpha_a: float, act: str = "softplus"):
-        if act != "softplus":
-            raise NotImplementedError("only af_softplus (the default) is implemented")
+        if act not in self.ACTS:
+            raise ValueError("act must be 'softplus' (af_softplus) or 'logcosh' (af_logcosh)")
+        self.act = act
         self.N = int(N)
         self.nh = int(alpha_h * N)
         self.na = int(alpha_a * N)
@@ -174,19 +183,27 @@ class SR:
     (SR.jl:4, SRDirect.jl:56-70): "none" solves S itself; "shift" adds eps*I;
     "multiplicative" adds lambda*diag(S) with lambda = max(lambda0 * b^n,
     lambda_min) at iteration n (the reference's Cholesky path drops this
-    shift, SRDirect.jl:67-78; here every solver applies it); "diag" adds a
-    fixed eps*diag(S).  `refine` extra FP64 refinement steps of the Cholesky
+    shift, SRDirect.jl:67-78; here every solver applies it, unless `as_run`
+    is set: then "multiplicative" with "cholesky" or "div" factors the
+    unshifted S as the reference does as written); "diag" adds a fixed
+    eps*diag(S).  "diag" (pinv of the preconditioned S) with "none" is
+    refused: on a singular S the pseudo-inverse step differs from any
+    factorization.  `refine` extra FP64 refinement steps of the Cholesky
     solve."""
 
     def __init__(self, eps: float = 0.001, algorithm: str = "cholesky", precondition_type: str = "shift",
                  refine: int = 0, precision: float = 1e-4, maxiter: int = 0, lambda0: float = 100.0,
-                 b: float = 0.95, lambda_min: float = 1e-4):
+                 b: float = 0.95, lambda_min: float = 1e-4, as_run: bool = False):
         if algorithm not in ("cholesky", "diag", "div", "cg", "minres", "qlp", "lsq"):
             # sr_pivcholesky is declared by the reference but throws "Alg not known" (SRDirect.jl:85-86)
             raise NotImplementedError("algorithm must be 'cholesky' (sr_cholesky), 'diag' (sr_diag), 'div' "
                                       "(sr_div), 'cg' (sr_cg), 'minres' (sr_minres), 'qlp' (sr_qlp) or 'lsq' (sr_lsq)")
         if precondition_type not in ("none", "shift", "multiplicative", "diag"):
             raise ValueError("precondition_type must be 'none', 'shift', 'multiplicative' or 'diag'")
+        if algorithm == "diag" and precondition_type == "none":
+            raise NotImplementedError("sr_diag with sr_none is the pseudo-inverse of the unshifted, possibly "
+                                      "singular S (SRDirect.jl:73-75): not provided; use a shift")
+        self.as_run = bool(as_run)
         self.eps = float(eps)
         self.algorithm = algorithm
         self.precondition_type = precondition_type
@@ -202,6 +219,9 @@ class SR:
         if self.precondition_type == "shift":
             return self.eps, _lib.SHIFT_ADD
         if self.precondition_type == "multiplicative":
+            if self.as_run and self.algorithm in ("cholesky", "div"):
+                # defect D4 as run: cholesky!(Hermitian(S)) / ldiv!(Hermitian(S)) take S, not Sprecond
+                return 0.0, _lib.SHIFT_ADD
             return max(self.lambda0 * self.b ** it, self.lambda_min), _lib.SHIFT_DIAG
         return self.eps, _lib.SHIFT_DIAG
 
@@ -216,6 +236,20 @@ class SR:
     @property
     def shift_kind(self):
         return self.shift(0)[1]
+
+
+class Gradient(np.ndarray):
+    """The preconditioned gradient precondition_ returns (reference order):
+    an array that also knows the sampler whose device state holds it, so
+    update_(opt, net, grad) applies it on the device (apply.jl:9)."""
+
+    def __new__(cls, values, owner):
+        obj = np.asarray(values).view(cls)
+        obj.owner = owner
+        return obj
+
+    def __array_finalize__(self, obj):
+        self.owner = getattr(obj, "owner", None)
 
 
 class Descent:
@@ -286,7 +320,7 @@ class HIPBatchedSampler:
         elif isinstance(net, RBMSplit):
             self.ctx.set_rbmsplit(net.N, net.nh)
         else:
-            self.ctx.set_ndm(net.N, net.nh, net.na)
+            self.ctx.set_ndm(net.N, net.nh, net.na, NDM.ACTS[net.act])
         self.ctx.set_precision(*_PREC[precision])
         self.ctx.set_params(net.flat.view(np.float64))
         self.ctx.set_sampler(self.batch_sz, sampler.chain_length, sampler.passes, sampler.burn, sampler.flags)
@@ -303,6 +337,7 @@ class HIPBatchedSampler:
         self.ctx.set_params(self.net.flat.view(np.float64))
 
     def sample(self):
+        self.push_params()         # the host network is canonical
         st = self.ctx.sample_grad()
         self._pending = True
         return Measurement.from_array(st), self
@@ -311,7 +346,7 @@ class HIPBatchedSampler:
         if not self._pending:
             raise RuntimeError("call sample_(is) before precondition_")
         eps, kind = algo.shift(it)
-        return self.ctx.precondition(eps, kind, algo.refine)
+        return Gradient(self.ctx.precondition(eps, kind, algo.refine), self)
 
     def add_observable(self, name: str, op):
         """`add_observable!(is, name, Ô)` (BatchedGradSampler.jl:129-131): a
@@ -367,13 +402,20 @@ def precondition_(cache: HIPBatchedSampler, algo: SR, it: int = 0):
     return cache.precondition(algo, it)
 
 
-def update_(opt: Descent, is_or_net, grad=None):
-    """`Optimisers.update!(opt, net, grad)`: applies p -= eta * grad on the
-    device and refreshes the host parameters."""
-    if isinstance(is_or_net, HIPBatchedSampler):
-        is_or_net.update(opt, grad)
-        return is_or_net.net
-    raise TypeError("pass the HIPBatchedSampler that owns the network")
+def update_(opt: Descent, net, grad=None):
+    """`Optimisers.update!(opt, net, grad)` (apply.jl:9): p -= eta * grad,
+    applied on the device by the sampler that produced `grad`, and the host
+    network refreshed.  The sampler itself may stand in for `net`."""
+    if isinstance(net, HIPBatchedSampler):
+        net.update(opt, grad)
+        return net.net
+    owner = getattr(grad, "owner", None)
+    if owner is None:
+        raise TypeError("grad must come from precondition_ (or pass the HIPBatchedSampler)")
+    if owner.net is not net:
+        raise ValueError("grad was computed for another network")
+    owner.update(opt, grad)
+    return net
 
 
 def add_observable_(is_: HIPBatchedSampler, name: str, op):
