@@ -314,6 +314,16 @@ nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
  * At most 2^30 configurations. */
 nqh_status nqh_exact(nqh_ctx* ctx, double* out, double* logp_out);
 
+/* nqh_exact without any device allocation: the same sums (out[0..3], logp_out
+ * nullable), computed in chunks of 2^16 configurations with every buffer
+ * beyond the caller's outputs in a file-scope device array of 3,801,344
+ * bytes that each GPU holds from first use until the process ends; at most
+ * 2^30 configurations (NQH_E_ARG beyond).  Calls that overlap on one GPU
+ * share that array and race.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc). */
+nqh_status nqh_exact_noalloc(nqh_ctx* ctx, double* out, double* logp_out);
+
 /* compute_observables(is) for one observable (BatchedObsDMSampler.jl:59-105):
  * a Metropolis chain on the diagonal (sigma, sigma) of the density matrix,
  * restarted at every call and drawn from its own random stream (counter word

@@ -699,6 +699,51 @@ hipError_t nq_launch_iota_keys(uint64_t* keys, int64_t n, uint64_t start, hipStr
     return hipGetLastError();
 }
 
+namespace {
+// log p = 2 Re log rho of n configurations, as doubles
+template <typename T>
+__global__ void k_logp_of(const T* lp, int64_t n, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 2.0 * (double)lp[2 * i];
+}
+
+// the chunks' partial sums (max log p, sum w, sum w |L|^2, sum w Re L,
+// sum w Im L, each relative to its chunk's max) combined in chunk order:
+// out = [sum p |L|^2, Re sum p L, Im sum p L, log Z]
+__global__ void k_exact_combine(const double* part, int64_t nch, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double M = -INFINITY;
+    for (int64_t ci = 0; ci < nch; ++ci) M = fmax(M, part[5 * ci]);
+    double Z = 0.0, ZC = 0.0, ZR = 0.0, ZI = 0.0;
+    for (int64_t ci = 0; ci < nch; ++ci) {
+        const double f = exp(part[5 * ci] - M);
+        Z += f * part[5 * ci + 1];
+        ZC += f * part[5 * ci + 2];
+        ZR += f * part[5 * ci + 3];
+        ZI += f * part[5 * ci + 4];
+    }
+    out[0] = ZC / Z;
+    out[1] = ZR / Z;
+    out[2] = ZI / Z;
+    out[3] = M + log(Z);
+}
+}  // namespace
+
+hipError_t nq_launch_logp_of(const void* lp, int fp64, int64_t n, double* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL(k_logp_of<double>, dim3(g), dim3(256), 0, st, (const double*)lp, n, out);
+    else
+        hipLaunchKernelGGL(k_logp_of<float>, dim3(g), dim3(256), 0, st, (const float*)lp, n, out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_exact_combine(const double* part, int64_t nch, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_exact_combine, dim3(1), dim3(64), 0, st, part, nch, out);
+    return hipGetLastError();
+}
+
 hipError_t nq_launch_exact_partial(const void* lp, const void* L, int fp64, int64_t n, double* out, hipStream_t st) {
     if (fp64)
         hipLaunchKernelGGL(k_exact_partial<double>, dim3(1), dim3(1024), 0, st, (const double*)lp, (const double*)L, n,

@@ -295,3 +295,7 @@ hipError_t nq_launch_energy_force(const void* Y, const void* E, int fp64, const 
 hipError_t nq_launch_iota_keys(uint64_t* keys, int64_t n, uint64_t start, hipStream_t st);
 // out[0..4] = {max log p, sum w, sum w |L|^2, sum w Re L, sum w Im L}, w = exp(2 Re lp - max)
 hipError_t nq_launch_exact_partial(const void* lp, const void* L, int fp64, int64_t n, double* out, hipStream_t st);
+// log p = 2 Re log rho (complex, network precision) of n configurations, as doubles
+hipError_t nq_launch_logp_of(const void* lp, int fp64, int64_t n, double* out, hipStream_t st);
+// combine nch chunk partials of k_exact_partial: out = [sum p |L|^2, Re sum p L, Im sum p L, log Z]
+hipError_t nq_launch_exact_combine(const double* part, int64_t nch, double* out, hipStream_t st);

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/nqhip.h"
@@ -645,6 +646,8 @@ static nqh_status made_maybe_retrain(nqh_ctx* c) {
 
 int32_t nqh_abi_version(void) { return NQH_ABI_VERSION; }
 
+static nqh_status scratch_setup(nqh_ctx* c);
+
 nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
     if (!out) return NQH_E_ARG;
     *out = nullptr;
@@ -659,6 +662,9 @@ nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
         delete c;
         return NQH_E_HIP;
     }
+    // the allocation-free entry point's scratch address on this device (a
+    // failure here is reported by that entry point, which tries again)
+    (void)scratch_setup(c);
     *out = c;
     return NQH_OK;
 }
@@ -2433,6 +2439,98 @@ nqh_status nqh_exact(nqh_ctx* c, double* out, double* logp_out) {
     out[3] = M + std::log(Z);
     if (logp_out)
         for (int64_t i = 0; i < total; ++i) logp_out[i] -= out[3];
+    if (!std::isfinite(out[0])) return fail(c, NQH_E_NAN, "non-finite exact sum");
+    return NQH_OK;
+}
+
+// ---- exact sums without device allocation.  Every buffer the kernels use
+// beyond the caller's outputs (the keys of a chunk, its log rho and L_loc,
+// its log p, the chunks' partial sums and the combined result) lives in one
+// file-scope device array, `scratch`; each GPU holds its own copy at its own
+// address, taken once per device with hipGetSymbolAddress and kept in
+// g_scratch.  Calls on one GPU share that copy: calls that overlap on one GPU
+// race.  Nothing in it needs clearing (every region is written before it is
+// read within a call).
+namespace {
+constexpr int64_t kXChunk = (int64_t)1 << 16;            // configurations per chunk
+constexpr int64_t kXMaxChunks = (int64_t)1 << 14;        // 2^30 configurations
+constexpr size_t kXKeysOff = 0;
+constexpr size_t kXLpOff = kXKeysOff + sizeof(uint64_t) * kXChunk;
+constexpr size_t kXLOff = kXLpOff + 2 * sizeof(double) * kXChunk;
+constexpr size_t kXLogpOff = kXLOff + 2 * sizeof(double) * kXChunk;
+constexpr size_t kXPartOff = kXLogpOff + sizeof(double) * kXChunk;
+constexpr size_t kXOutOff = kXPartOff + 5 * sizeof(double) * kXMaxChunks;
+constexpr size_t kXBytes = kXOutOff + 256;               // 3,801,344 bytes
+constexpr int kXMaxDevices = 64;
+void* g_scratch[kXMaxDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kXBytes];
+
+// the device address of `scratch` on the context's device (which must be
+// current), taken once per device
+static nqh_status scratch_setup(nqh_ctx* c) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device < 0 || c->device >= kXMaxDevices) return fail(c, NQH_E_ARG, "device index beyond 63");
+    if (g_scratch[c->device]) return NQH_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch[c->device] = ptr;
+    return NQH_OK;
+}
+
+nqh_status nqh_exact_noalloc(nqh_ctx* c, double* out, double* logp_out) {
+    if (!c || !out) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    nqh_status s;
+    if ((s = need_net(c)) || (s = need_problem(c))) return s;
+    const int bits = is_ket(c) ? net_N(c) : 2 * net_N(c);
+    if (bits > 30)
+        return fail(c, NQH_E_ARG, "nqh_exact_noalloc: at most 2^30 configurations (2^16 per chunk, 2^14 chunks "
+                                  "in its scratch memory)");
+    if ((s = scratch_setup(c))) return s;
+    unsigned char* x = static_cast<unsigned char*>(g_scratch[c->device]);
+    uint64_t* kb = reinterpret_cast<uint64_t*>(x + kXKeysOff);
+    void* lpb = x + kXLpOff;
+    void* Lb = x + kXLOff;
+    double* logp = reinterpret_cast<double*>(x + kXLogpOff);
+    double* part = reinterpret_cast<double*>(x + kXPartOff);
+    double* res = reinterpret_cast<double*>(x + kXOutOff);
+    const int64_t total = (int64_t)1 << bits;
+    const int64_t chunk = std::min<int64_t>(total, kXChunk);
+    const int64_t nch = (total + chunk - 1) / chunk;
+    for (int64_t ci = 0; ci < nch; ++ci) {
+        const int64_t start = ci * chunk, n = std::min(chunk, total - start);
+        HIPCHK(c, nq_launch_iota_keys(kb, n, (uint64_t)start, c->stream));
+        if (is_rbm(c)) {
+            rbm_args a = make_rbm(c, kb, n);
+            a.do_conn = 1;
+            a.lp = lpb;
+            a.Lloc = Lb;
+            HIPCHK(c, nq_launch_rbm_estimate(a, fp64_net(c), est_grid_for(n), c->stream));
+        } else {
+            est_args a = make_est(c, kb, n);
+            a.do_conn = 1;
+            a.lp = lpb;
+            a.Lloc = Lb;
+            HIPCHK(c, nq_launch_estimate(a, fp64_net(c), est_grid_for(n), c->stream));
+        }
+        HIPCHK(c, nq_launch_exact_partial(lpb, Lb, fp64_net(c), n, part + 5 * ci, c->stream));
+        if (logp_out) {
+            HIPCHK(c, nq_launch_logp_of(lpb, fp64_net(c), n, logp, c->stream));
+            HIPCHK(c, hipMemcpyAsync(logp_out + start, logp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));   // the chunk's buffers are reused by the next chunk
+        }
+    }
+    HIPCHK(c, nq_launch_exact_combine(part, nch, res, c->stream));
+    double r[4];
+    HIPCHK(c, hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) out[k] = r[k];
+    if (logp_out)
+        for (int64_t i = 0; i < total; ++i) logp_out[i] -= r[3];
     if (!std::isfinite(out[0])) return fail(c, NQH_E_NAN, "non-finite exact sum");
     return NQH_OK;
 }

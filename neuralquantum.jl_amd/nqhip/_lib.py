@@ -65,6 +65,7 @@ SIGNATURES = {
     "nqh_set_rbm": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "nqh_get_stats": (C.c_int32, [C.c_void_p, _dp, C.c_int32]),
     "nqh_exact": (C.c_int32, [C.c_void_p, _dp, _dp]),
+    "nqh_exact_noalloc": (C.c_int32, [C.c_void_p, _dp, _dp]),
     "nqh_observable": (C.c_int32, [C.c_void_p, C.POINTER(OpTable), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                    _dp]),
     "nqh_get_observable_samples": (C.c_int32, [C.c_void_p, _u64p, _dp, C.c_int64]),
