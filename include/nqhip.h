@@ -314,6 +314,21 @@ nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
  * At most 2^30 configurations. */
 nqh_status nqh_exact(nqh_ctx* ctx, double* out, double* logp_out);
 
+/* Checkpoint and resume of the state this backend keeps beyond the
+ * parameters (which nqh_get_params / nqh_set_params cover, the reference's
+ * load_from_dict! hook, src/utils/loading.jl:8-14): the persistent chains'
+ * configurations, the iteration index that keys the Philox streams, and with
+ * the learned proposal its weights, Adam moments and step, its retraining
+ * counter and the ring of recent samples it trains on.  The reference keeps
+ * none of it (it restarts its chains every iteration, Metropolis.jl:106).
+ * nqh_save_state writes nqh_state_bytes(ctx) bytes into buf; nqh_load_state
+ * restores them into a context set up the same way (problem, network,
+ * sampler and nqh_made_setup), after which its iterations continue those of
+ * the saved context bit for bit. */
+int64_t nqh_state_bytes(const nqh_ctx* ctx);
+nqh_status nqh_save_state(nqh_ctx* ctx, void* buf, int64_t bytes);
+nqh_status nqh_load_state(nqh_ctx* ctx, const void* buf, int64_t bytes);
+
 /* nqh_exact without any device allocation: the same sums (out[0..3], logp_out
  * nullable), computed in chunks of 2^16 configurations with every buffer
  * beyond the caller's outputs in a file-scope device array of 3,801,344

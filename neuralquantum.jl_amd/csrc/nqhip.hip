@@ -2443,6 +2443,93 @@ nqh_status nqh_exact(nqh_ctx* c, double* out, double* logp_out) {
     return NQH_OK;
 }
 
+// ---- checkpoint of the state the reference does not keep (it restarts its
+// chains every iteration, Metropolis.jl:106, and has no learned proposal):
+// a little-endian blob
+//   header: int64 [magic "NQHSTAT1", chains, chains_initialised, iter,
+//                  made_P, made_t, made_calls, ring_cap, ring_fill, ring_pos]
+//   chains x uint64 chain configurations
+//   learned proposal (made_P > 0): made_P x float weights, Adam m, Adam v,
+//                  ring_cap x uint64 ring of recent samples
+namespace {
+constexpr int64_t kStateMagic = 0x315441545348514eLL;   // "NQHSTAT1"
+constexpr int kStateHdr = 10;
+}
+
+static int64_t state_bytes(const nqh_ctx* c) {
+    int64_t b = kStateHdr * 8 + (c->has_sampler ? c->chains * 8 : 0);
+    if (c->made_ready) b += 3 * (int64_t)c->made_P * 4 + c->ring_cap * 8;
+    return b;
+}
+
+int64_t nqh_state_bytes(const nqh_ctx* c) { return c ? state_bytes(c) : -1; }
+
+nqh_status nqh_save_state(nqh_ctx* c, void* buf, int64_t bytes) {
+    if (!c || !buf) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->has_sampler || !c->chain_state.p) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+    if (bytes != state_bytes(c)) return fail(c, NQH_E_ARG, "buffer size must be nqh_state_bytes(ctx)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t* h = static_cast<int64_t*>(buf);
+    const int64_t hdr[kStateHdr] = {kStateMagic, c->chains, c->chains_initialised ? 1 : 0, (int64_t)c->iter,
+                                    c->made_ready ? c->made_P : 0, c->made_t, c->made_calls,
+                                    c->made_ready ? c->ring_cap : 0, c->ring_fill, c->ring_pos};
+    std::memcpy(h, hdr, sizeof(hdr));
+    unsigned char* q = static_cast<unsigned char*>(buf) + sizeof(hdr);
+    HIPCHK(c, hipMemcpy(q, c->chain_state.p, sizeof(uint64_t) * c->chains, hipMemcpyDeviceToHost));
+    q += sizeof(uint64_t) * c->chains;
+    if (c->made_ready) {
+        const size_t fb = sizeof(float) * c->made_P;
+        HIPCHK(c, hipMemcpy(q, c->made_th.p, fb, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(q + fb, c->made_m.p, fb, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(q + 2 * fb, c->made_v.p, fb, hipMemcpyDeviceToHost));
+        q += 3 * fb;
+        if (c->ring_cap > 0) HIPCHK(c, hipMemcpy(q, c->ring.p, sizeof(uint64_t) * c->ring_cap, hipMemcpyDeviceToHost));
+    }
+    return NQH_OK;
+}
+
+nqh_status nqh_load_state(nqh_ctx* c, const void* buf, int64_t bytes) {
+    if (!c || !buf) return NQH_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->has_sampler || !c->chain_state.p) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
+    int64_t hdr[kStateHdr];
+    if (bytes < (int64_t)sizeof(hdr)) return fail(c, NQH_E_ARG, "state buffer too short");
+    std::memcpy(hdr, buf, sizeof(hdr));
+    if (hdr[0] != kStateMagic) return fail(c, NQH_E_ARG, "not a libnqhip state (bad magic)");
+    if (hdr[1] != c->chains) return fail(c, NQH_E_ARG, "the state holds another chain count");
+    const int64_t P = hdr[4];
+    if (P != (c->made_ready ? c->made_P : 0) || (P > 0 && hdr[7] != c->ring_cap))
+        return fail(c, NQH_E_ARG, "the state's learned proposal does not match this context's (nqh_made_setup)");
+    if (bytes != state_bytes(c)) return fail(c, NQH_E_ARG, "buffer size must be nqh_state_bytes(ctx)");
+    if (hdr[8] < 0 || hdr[8] > hdr[7] || hdr[9] < 0 || (hdr[7] > 0 && hdr[9] >= hdr[7]))
+        return fail(c, NQH_E_ARG, "inconsistent ring position in the state");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    const unsigned char* q = static_cast<const unsigned char*>(buf) + sizeof(hdr);
+    HIPCHK(c, hipMemcpy(c->chain_state.p, q, sizeof(uint64_t) * c->chains, hipMemcpyHostToDevice));
+    q += sizeof(uint64_t) * c->chains;
+    c->chains_initialised = hdr[2] != 0;
+    c->iter = (uint32_t)hdr[3];
+    if (P > 0) {
+        const size_t fb = sizeof(float) * P;
+        HIPCHK(c, hipMemcpy(c->made_th.p, q, fb, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->made_m.p, q + fb, fb, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->made_v.p, q + 2 * fb, fb, hipMemcpyHostToDevice));
+        q += 3 * fb;
+        c->made_t = (int)hdr[5];
+        c->made_calls = hdr[6];
+        c->ring_fill = hdr[8];
+        c->ring_pos = hdr[9];
+        if (c->ring_cap > 0) {
+            HIPCHK(c, hipMemcpy(c->ring.p, q, sizeof(uint64_t) * c->ring_cap, hipMemcpyHostToDevice));
+            const int64_t pos[2] = {c->ring_pos, c->ring_fill};
+            HIPCHK(c, hipMemcpy(c->ring_posd.p, pos, sizeof(pos), hipMemcpyHostToDevice));
+        }
+    }
+    return NQH_OK;
+}
+
 // ---- exact sums without device allocation.  Every buffer the kernels use
 // beyond the caller's outputs (the keys of a chunk, its log rho and L_loc,
 // its log p, the chunks' partial sums and the combined result) lives in one

@@ -66,6 +66,9 @@ SIGNATURES = {
     "nqh_get_stats": (C.c_int32, [C.c_void_p, _dp, C.c_int32]),
     "nqh_exact": (C.c_int32, [C.c_void_p, _dp, _dp]),
     "nqh_exact_noalloc": (C.c_int32, [C.c_void_p, _dp, _dp]),
+    "nqh_state_bytes": (C.c_int64, [C.c_void_p]),
+    "nqh_save_state": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "nqh_load_state": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
     "nqh_observable": (C.c_int32, [C.c_void_p, C.POINTER(OpTable), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                    _dp]),
     "nqh_get_observable_samples": (C.c_int32, [C.c_void_p, _u64p, _dp, C.c_int64]),
@@ -322,6 +325,18 @@ class Context:
         acc = C.c_int64()
         self._chk(self.lib.nqh_sample(self.h, _p(keys, _u64p), C.byref(acc)))
         return keys, int(acc.value)
+
+    def save_state(self):
+        """The chains, the iteration index and the learned proposal's state
+        (nqh_save_state) as bytes."""
+        n = int(self.lib.nqh_state_bytes(self.h))
+        buf = np.empty(n, dtype=np.uint8)
+        self._chk(self.lib.nqh_save_state(self.h, buf.ctypes.data_as(C.c_void_p), n))
+        return buf.tobytes()
+
+    def load_state(self, blob):
+        buf = np.frombuffer(blob, dtype=np.uint8).copy()
+        self._chk(self.lib.nqh_load_state(self.h, buf.ctypes.data_as(C.c_void_p), len(buf)))
 
     def chain_acceptance(self):
         """Accepted moves per chain in the last sampling call (0: a stuck chain)."""
