@@ -65,6 +65,9 @@ def build_ctx(form: str, cfg=CFG):
     if form == "fp64":
         ctx.set_precision(_lib.PREC_FP64, _lib.PREC_FP64)
         flags = 0
+    elif form == "fp64_persistent":      # the precision of the plain form, the chosen form's schedule
+        ctx.set_precision(_lib.PREC_FP64, _lib.PREC_FP64)
+        flags = _lib.SAMPLER_PERSISTENT
     else:
         ctx.set_precision(_lib.PREC_FP32, _lib.PREC_BF16)
         flags = _lib.SAMPLER_PERSISTENT
@@ -202,6 +205,13 @@ def learned_proposal(steps, cfg=CFG, c4=C4):
             out[name]["global_moves_last_call"] = {"proposed": prop, "accepted": acc, "buffer": fill}
             theta = ctx.made_get_params()
             p = ctx.get_params()
+        # the loop's own effective sample rate: tau_int on its own chains and
+        # schedule (33 local proposals per stored sample, and for the learned
+        # run a global move before half of them) at the parameters reached
+        t_lp, t_L = measure_tau(ctx, "chosen", cfg)
+        out[name]["tau_int_own_schedule"] = {"logp": round(t_lp, 3), "abs2_Lloc": round(t_L, 3)}
+        out[name]["eff_samples_per_s"] = round(cfg["chains"] * cfg["chain_length"] / (2 * max(t_lp, t_L))
+                                               * steps / dt, 1)
         ctx.close()
     N = cfg["lx"] * cfg["ly"]
     taus = {}
@@ -225,18 +235,47 @@ def learned_proposal(steps, cfg=CFG, c4=C4):
     return out
 
 
+def evaluate_loss(p, cfg=CFG, repeats=8):
+    """<L^dag L> of parameters p from `repeats` independent FP64 gradient
+    passes (fresh chains, 50 burn-in calls, Ns samples each): (mean,
+    standard error of the mean over the passes)."""
+    ctx, _ = build_ctx("fp64", cfg)
+    ctx.set_params(p)
+    vals = []
+    for r in range(repeats):
+        ctx.set_iteration(100000 + r)
+        vals.append(ctx.sample_grad()[0])
+    ctx.close()
+    v = np.array(vals)
+    return float(v.mean()), float(v.std(ddof=1) / math.sqrt(len(v)))
+
+
 def equal_loss(iters, cfg=CFG):
-    """Both forms from the same parameters and seeds for `iters` SR
-    iterations: the mean loss of the last 20 and their relative difference."""
-    res = {}
-    for form in ("fp64", "chosen"):
+    """Every form from the same parameters and seeds for `iters` SR
+    iterations.  Per form: the loss of its final parameters evaluated with
+    fresh FP64 samples (mean and standard error), the mean sampled loss of
+    the last 100 iterations against the 100 before (how far from converged),
+    and the final parameters and context (left open: tau_int is measured on
+    each form's own chains at these parameters)."""
+    res, keep = {}, {}
+    for form in ("fp64", "fp64_persistent", "chosen"):
         ctx, _ = build_ctx(form, cfg)
         ctx.set_graph(True)
-        losses = [ctx.sr_iteration(cfg["eps"], cfg["lr"])[0] for _ in range(iters)]
-        res[form] = float(np.mean(losses[-20:]))
-        ctx.close()
-    return {"iterations": iters, "loss_mean_last20": res,
-            "rel_diff": abs(res["chosen"] - res["fp64"]) / abs(res["fp64"])}
+        losses = np.array([ctx.sr_iteration(cfg["eps"], cfg["lr"])[0] for _ in range(iters)])
+        p = ctx.get_params()
+        m, e = evaluate_loss(p, cfg)
+        last, prev = float(losses[-100:].mean()), float(losses[-200:-100].mean())
+        res[form] = {"loss_final_params": m, "loss_final_params_err": e,
+                     "sampled_loss_last100": last, "drift_last100_vs_previous100": (last - prev) / prev}
+        keep[form] = (ctx, p)
+    ref_m, ref_e = res["fp64"]["loss_final_params"], res["fp64"]["loss_final_params_err"]
+    for form in ("fp64_persistent", "chosen"):
+        m, e = res[form]["loss_final_params"], res[form]["loss_final_params_err"]
+        res[form]["rel_diff_vs_fp64"] = (m - ref_m) / ref_m
+        res[form]["rel_diff_err"] = math.hypot(e, ref_e * m / ref_m) / ref_m
+    return {"iterations": iters, "forms": res,
+            "evaluation": "8 independent FP64 gradient passes of 4096 x 4 fresh samples (50 burn-in calls) at "
+                          "each form's final parameters: mean and standard error"}, keep
 
 
 def tau_int(series):
@@ -257,18 +296,22 @@ def tau_int(series):
     return max(tau, 0.5)
 
 
-def measure_tau(form, cfg=CFG, slices=48):
-    """tau_int of Re log p and |L_loc|^2 along the chains of the form's sampler
-    at the initial parameters."""
-    c2 = dict(cfg)
-    ctx, _ = build_ctx(form, c2)
-    chains = 1024
-    flags = 0
-    ctx.set_sampler(chains, slices, cfg["passes"], cfg["burn"], flags)
+def measure_tau(ctx, form, cfg=CFG, slices=48):
+    """tau_int of log p and |L_loc|^2 along the chains of `ctx` on the form's
+    own schedule, at its current parameters, in stored samples (passes
+    proposals apart): the plain form restarts its chains and burns them in
+    (50 calls) as it does every iteration; the persistent forms continue the
+    chains their training left (the checkpoint of the chains is restored
+    after the sampler is resized to `slices` slices)."""
+    chains = cfg["chains"]
+    persistent = form != "fp64"
+    blob = ctx.save_state() if persistent else None
+    ctx.set_sampler(chains, slices, cfg["passes"], cfg["burn"], _lib.SAMPLER_PERSISTENT if persistent else 0)
+    if persistent:
+        ctx.load_state(blob)
     keys, _ = ctx.sample()
     lp = ctx.logpsi(keys).real.reshape(slices, chains).T
     L = ctx.local_estimator(keys).reshape(slices, chains).T
-    ctx.close()
     return tau_int(2 * lp), tau_int(np.abs(L) ** 2)
 
 
@@ -313,10 +356,20 @@ def kernel_rooflines(phases, form, d, pairs):
     t = per["sbuild"] * 1e-3
     fl = 4.0 * np_ * np_ * Ns
     sp = "fp64" if form == "fp64" else "bf16"
+    # the S build computes only what S needs: the lower triangle of the real
+    # lambda and imaginary mu blocks (Ns rows each), of the complex Pi block
+    # (2 Ns rows) and of the Pi-lambda / Pi-mu blocks, and none of the zero
+    # lambda-mu block: 2 Ns (Np - nl)^2 flop.  The figure above counts the
+    # full real GEMM, so its fraction can exceed 1
+    nl = N + nh + nh * N
+    fs = 2.0 * Ns * (np_ - nl) ** 2
     out["sbuild"] = dict(kernel="k_gram" if form == "fp64" else "k_center_t + k_gram_bf16_part + k_gram_finish",
                          bound="mfma",
                          achieved=fl / t / 1e12, peak=PEAK_TFLOPS[sp], unit="TFLOP/s",
-                         algorithmic=f"4 Np^2 Ns = {fl:.4g} flop", ms=per["sbuild"])
+                         algorithmic=f"4 Np^2 Ns = {fl:.4g} flop", ms=per["sbuild"],
+                         structural_flop=fs, structural_frac=fs / t / 1e12 / PEAK_TFLOPS[sp],
+                         frac_note="frac on 4 Np^2 Ns (the full real GEMM); structural_frac on "
+                                   "2 Ns (Np - nl)^2, the non-zero lower-triangle blocks actually needed")
     # the factorization, timed on its own, against Np^3/3 flop (the dense
     # Cholesky of S); the whole solve phase (shift, factor, substitutions,
     # checks) is reported beside it.  S's lambda-mu block is zero, so the
@@ -335,10 +388,21 @@ def kernel_rooflines(phases, form, d, pairs):
     by = 2.0 * Ns * R * elt
     out["post"] = dict(kernel="k_colsum + k_stats + k_force", bound="hbm", achieved=by / t / 1e9,
                        peak=HBM_PEAK_GBS, unit="GB/s", algorithmic=f"2 Ns R p = {by:.4g} B", ms=per["post"])
-    out["sampler"] = dict(kernel="k_metropolis", ms=per["sampler"])
+    # sampler: (24 nh + 32 na + 20) VALU lane-slots per chain-step; chain-steps
+    # per iteration: chains x (burn + L) x passes when the chains restart
+    # every iteration, chains x L x passes when they persist
+    t = per["sampler"] * 1e-3
+    calls = (d["burn"] + d["L"]) if form == "fp64" else d["L"]
+    steps = d["chains"] * calls * d["passes"]
+    slots = steps * (24 * nh + 32 * na + 20)
+    out["sampler"] = dict(kernel="k_metropolis", bound="valu", achieved=slots / t / 1e12,
+                          peak=VALU_LANE_SLOTS_PER_S / 1e12, unit="T lane-slots/s",
+                          algorithmic=f"(24 nh + 32 na + 20) x {steps} chain-steps = {slots:.4g} lane-slots",
+                          ms=per["sampler"])
     for v in out.values():
         if "peak" in v:
             v["frac"] = v["achieved"] / v["peak"]
+    out["sampler"]["frac_note"] = "VALU lane-slots (not in the roofline JSON's hbm/mfma choice)"
     return out, per
 
 
@@ -381,17 +445,31 @@ def copy_calibration(nbytes=256 << 20):
     del src, dst
 
 
-def cpu_baseline():
+def cpu_threads():
+    """Host threads for the multi-thread CPU baseline: the cores this process
+    may run on, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:   # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(threads=1, full=False):
     """The oracle in reference-structured mode (full network recompute per
     proposal, full Np gradient per connection, complex S = Oc Oc^H, LAPACK
-    Cholesky) on a bounded sample of one SR iteration, scaled to the full
-    workload; one thread."""
+    Cholesky) for one SR iteration of the workload, on `threads` host threads
+    (OpenMP over chains and samples, the BLAS of the S build and the solve
+    limited to the same count).  full=False times a bounded sample (512 of
+    the chains, 2048 of the samples) and scales each part linearly;
+    full=True runs the whole iteration."""
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:   # pragma: no cover
         threadpool_limits = None
     from oracle import nqref, ops_ref
     ref = nqref.Ref()
+    ref.set_threads(threads)
     cfg = CFG
     N = cfg["lx"] * cfg["ly"]
     nh, na = cfg["alpha_h"] * N, cfg["alpha_a"] * N
@@ -399,38 +477,44 @@ def cpu_baseline():
     tab = nqref.Table(ops_ref.table_from_rules(ops_ref.xyz_model(cfg["lx"], cfg["ly"])))
     p = np.random.default_rng(cfg["param_seed"]).standard_normal(np_) * math.sqrt(cfg["sigma"])
     chains, Lc, Ns = cfg["chains"], cfg["chain_length"], cfg["chains"] * cfg["chain_length"]
-    ctxm = threadpool_limits(limits=1) if threadpool_limits else None
+    ctxm = threadpool_limits(limits=threads) if threadpool_limits else None
     try:
+        ch_s = chains if full else 512
         t0 = time.perf_counter()
-        ch_s = 512
         keys, _ = ref.metropolis(N, nh, na, p, ch_s, Lc, cfg["passes"], cfg["burn"], cfg["sampler_seed"],
                                  compat=True)
         t_samp = (time.perf_counter() - t0) * chains / ch_s
-        nsamp = 2048
-        ks = np.random.default_rng(1).integers(0, 2 ** (2 * N), size=nsamp, dtype=np.uint64)
+        nsamp = Ns if full else 2048
+        ks = keys if full else np.random.default_rng(1).integers(0, 2 ** (2 * N), size=nsamp, dtype=np.uint64)
         t0 = time.perf_counter()
-        ref.logpsi_grad(N, nh, na, p, ks)
-        ref.local_estimator(tab, nh, na, p, ks, grad=True)
+        _, O = ref.logpsi_grad(N, nh, na, p, ks)
+        _, dL = ref.local_estimator(tab, nh, na, p, ks, grad=True)
         t_est = (time.perf_counter() - t0) * Ns / nsamp
-        ns_s = Ns
-        X = (np.random.default_rng(2).standard_normal((np_, ns_s)) +
-             1j * np.random.default_rng(3).standard_normal((np_, ns_s)))
+        del O, dL
+        X = (np.random.default_rng(2).standard_normal((np_, Ns)) +
+             1j * np.random.default_rng(3).standard_normal((np_, Ns)))
         t0 = time.perf_counter()
         Sc = X @ X.conj().T
-        t_s = (time.perf_counter() - t0) * Ns / ns_s
-        S = Sc.real / ns_s + 0.01 * np.eye(np_)
+        t_s = time.perf_counter() - t0
+        S = Sc.real / Ns + 0.01 * np.eye(np_)
         t0 = time.perf_counter()
         nqref.sr_solve(S, np.ones(np_), 0.0)
         t_chol = time.perf_counter() - t0
     finally:
         if ctxm is not None:
             ctxm.__exit__(None, None, None)
+        ref.set_threads(1)
     total = t_samp + t_est + t_s + t_chol
-    return {"value": 1.0 / total, "unit": "SR iters/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle/nqref.c reference-structured, 1 thread: Metropolis on {ch_s} of {chains} "
-                       f"chains, estimator with full per-connection gradients on {nsamp} of {Ns} samples, "
-                       f"complex S = Oc Oc^H (BLAS) on {ns_s} of {Ns} samples, full Cholesky; each part "
-                       f"scaled linearly to the full iteration"),
+    if full:
+        sample = (f"oracle/nqref.c reference-structured, {threads} threads, one whole iteration: Metropolis on all "
+                  f"{chains} chains (restart + {cfg['burn']} burn-in calls), estimator with full per-connection "
+                  f"gradients on all {Ns} samples, complex S = Oc Oc^H (BLAS) on {Ns} samples, full Cholesky")
+    else:
+        sample = (f"oracle/nqref.c reference-structured, {threads} thread(s): Metropolis on {ch_s} of {chains} "
+                  f"chains, estimator with full per-connection gradients on {nsamp} of {Ns} samples, "
+                  f"complex S = Oc Oc^H (BLAS) on all {Ns} samples, full Cholesky; the sampler and the "
+                  f"estimator scaled linearly to the full iteration")
+    return {"value": 1.0 / total, "unit": "SR iters/s", "cores": threads, "kind": "port", "sample": sample,
             "seconds_per_iteration": {"sampler": t_samp, "estimator": t_est, "sbuild": t_s, "solve": t_chol}}
 
 
@@ -443,7 +527,9 @@ def main():
     ap.add_argument("--no-tau", action="store_true")
     ap.add_argument("--no-instances", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal measurement")
-    ap.add_argument("--no-equal-loss", action="store_true", help="skip the equal-loss run of both forms")
+    ap.add_argument("--no-equal-loss", action="store_true",
+                    help="skip the equal-loss run of the forms (and the tau_int measured at its parameters)")
+    ap.add_argument("--equal-loss-iters", type=int, default=600)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (the known-size kernel the --pmc byte counters are "
                          "scaled by)")
@@ -454,9 +540,11 @@ def main():
         sys.exit(2)
 
     N = CFG["lx"] * CFG["ly"]
-    d = dict(N=N, nh=CFG["alpha_h"] * N, na=CFG["alpha_a"] * N, Ns=CFG["chains"] * CFG["chain_length"])
+    d = dict(N=N, nh=CFG["alpha_h"] * N, na=CFG["alpha_a"] * N, Ns=CFG["chains"] * CFG["chain_length"],
+             chains=CFG["chains"], L=CFG["chain_length"], burn=CFG["burn"], passes=CFG["passes"])
+    forms = ("fp64", "fp64_persistent", "chosen")
     res = {}
-    for form in ("fp64", "chosen"):
+    for form in forms:
         ctx, dt, phases, losses = time_form(form, args.steps, args.warmup)
         pairs = count_pairs(ctx)
         roofs, per = kernel_rooflines(phases, form, d, pairs)
@@ -465,10 +553,6 @@ def main():
         ctx.close()
     if args.calibrate_copy:
         copy_calibration()
-    taus = {}
-    if not args.no_tau:
-        for form in ("fp64", "chosen"):
-            taus[form] = measure_tau(form)
     ch = res["chosen"]
     out = {
         "metric": "SR iters/s + eff. samples/s, 4x4 NDM, mixed vs FP64 GPU at equal <L^dag L>",
@@ -485,27 +569,28 @@ def main():
         "data": "synthetic: random-init NDM, samples drawn by the sampler itself",
         "config": {"workload": "dissipative XYZ 4x4 periodic, NDM alpha_h=alpha_a=2 (Np=2176), "
                                "4096 chains x 4 samples, 33 passes, SR shift 0.01, lr 0.005",
-                   "forms": {"fp64": "FP64 network + FP64 S build, restart + 50 burn-in calls per iteration",
+                   "forms": {"fp64": "FP64 network + FP64 S build, restart + 50 burn-in calls per iteration "
+                                     "(the reference's schedule)",
+                             "fp64_persistent": "FP64 network + FP64 S build, persistent chains (the chosen "
+                                                "schedule at the plain precision)",
                              "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"}},
-        "fp64_form": {"it_s": round(res["fp64"]["it_s"], 3), "ms_per_step": round(res["fp64"]["ms"], 4),
-                      "phase_ms": {k: round(v, 4) for k, v in res["fp64"]["per"].items()},
-                      "loss_last": res["fp64"]["loss_last"]},
-        "chosen_form": {"phase_ms": {k: round(v, 4) for k, v in ch["per"].items()},
-                        "pairs_per_iteration": ch["pairs"], "loss_last": ch["loss_last"]},
-        "speedup_chosen_vs_fp64": round(ch["it_s"] / res["fp64"]["it_s"], 3),
     }
-    if taus:
-        Ns = d["Ns"]
-        out["tau_int"] = {k: {"logp": round(v[0], 3), "abs2_Lloc": round(v[1], 3)} for k, v in taus.items()}
-        out["eff_samples_per_s"] = {
-            k: round(Ns / (2 * max(taus[k])) * res[k]["it_s"], 1) for k in ("fp64", "chosen")}
+    for form in ("fp64", "fp64_persistent"):
+        out[form + "_form"] = {"it_s": round(res[form]["it_s"], 3), "ms_per_step": round(res[form]["ms"], 4),
+                               "phase_ms": {k: round(v, 4) for k, v in res[form]["per"].items()},
+                               "loss_last": res[form]["loss_last"]}
+    out["chosen_form"] = {"phase_ms": {k: round(v, 4) for k, v in ch["per"].items()},
+                          "pairs_per_iteration": ch["pairs"], "loss_last": ch["loss_last"]}
+    out["speedup_chosen_vs_fp64"] = round(ch["it_s"] / res["fp64"]["it_s"], 3)
+    out["speedup_schedule_fp64_persistent_vs_fp64"] = round(res["fp64_persistent"]["it_s"] / res["fp64"]["it_s"], 3)
+    out["speedup_precision_chosen_vs_fp64_persistent"] = round(ch["it_s"] / res["fp64_persistent"]["it_s"], 3)
     # the dominant kernel of the chosen form: the kernel with the most device
     # time per SR iteration.  Every phase but the solve is one main kernel;
     # the solve phase is ~110 launches of several kernels, of which the
     # factorization (k_chol_step, timed on its own) is the largest, so the
     # solve competes with its factorization time
     roofs = ch["roofs"]
-    single = {k: roofs[k]["ms"] for k in roofs if "frac" in roofs[k] and k != "solve"}
+    single = {k: roofs[k]["ms"] for k in roofs if "frac" in roofs[k] and k not in ("solve", "sampler")}
     single["solve"] = ch["per"].get("factor", roofs["solve"]["ms"])
     dom = max(single, key=single.get)
     r = roofs[dom]
@@ -518,14 +603,32 @@ def main():
         out["roofline"]["hbm_algorithmic_gbs"] = round(r["hbm_gbs"], 2)
     out["kernels"] = {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in roofs.items()}
+    out["kernels_fp64_form"] = {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                for k, v in res["fp64"]["roofs"].items() if k in ("sampler", "estimator", "sbuild")}
     if not args.no_equal_loss:
-        out["equal_loss"] = equal_loss(150)
+        eq, trained = equal_loss(args.equal_loss_iters)
+        out["equal_loss"] = eq
+        if not args.no_tau:
+            Ns = d["Ns"]
+            taus = {form: measure_tau(ctx, form) for form, (ctx, _) in trained.items()}
+            out["tau_int"] = {k: {"logp": round(v[0], 3), "abs2_Lloc": round(v[1], 3)} for k, v in taus.items()}
+            out["tau_int"]["note"] = ("in stored samples (33 proposals apart), each form on its own schedule at the "
+                                      "parameters its equal-loss run reached; 4096 chains x 48 slices")
+            out["eff_samples_per_s"] = {
+                k: round(Ns / (2 * max(taus[k])) * res[k]["it_s"], 1) for k in forms}
+            out["eff_samples_per_s"]["speedup_chosen_vs_fp64"] = round(
+                out["eff_samples_per_s"]["chosen"] / out["eff_samples_per_s"]["fp64"], 3)
+        for ctx, _ in trained.values():
+            ctx.close()
     if not args.no_learned:
         out["learned_proposal"] = learned_proposal(20)
     if not args.no_instances:
         out["batched_instances"] = time_instances(max(2, args.steps // 2), 2)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        th = cpu_threads()
+        base = cpu_baseline(threads=th, full=th > 1)
+        base["single_thread"] = cpu_baseline(threads=1, full=False)
+        out["cpu_baseline"] = base
     print(json.dumps(out))
 
 

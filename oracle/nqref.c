@@ -23,6 +23,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -176,10 +177,15 @@ static cplx ndm_eval(const ndm_off *L, const double *p, const double *s, const d
     return gl + I * gm + pi;
 }
 
+/* threads of the loops over samples and chains below (OpenMP; the
+ * reference's @MT mode runs one sampler per thread, threads_wrappers.jl:14-31) */
+void nqref_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 void nqref_ndm_logpsi(int N, int nh, int na, const double *p, int64_t B, const uint64_t *keys, double *out) {
     ndm_off L = ndm_layout(N, nh, na);
-    double s[64], e[64];
+#pragma omp parallel for schedule(static)
     for (int64_t b = 0; b < B; ++b) {
+        double s[64], e[64];
         unpack(keys[b], N, s, e);
         cplx v = ndm_eval(&L, p, s, e, NULL);
         out[2 * b] = creal(v); out[2 * b + 1] = cimag(v);
@@ -190,8 +196,9 @@ void nqref_ndm_logpsi(int N, int nh, int na, const double *p, int64_t B, const u
 void nqref_ndm_logpsi_grad(int N, int nh, int na, const double *p, int64_t B, const uint64_t *keys,
                            double *out, double *O) {
     ndm_off L = ndm_layout(N, nh, na);
-    double s[64], e[64];
+#pragma omp parallel for schedule(static)
     for (int64_t b = 0; b < B; ++b) {
+        double s[64], e[64];
         unpack(keys[b], N, s, e);
         cplx v = ndm_eval(&L, p, s, e, (cplx *)(O + 2 * b * L.np));
         out[2 * b] = creal(v); out[2 * b + 1] = cimag(v);
@@ -299,20 +306,24 @@ void nqref_local_estimator(const nqref_table *t, int nh, int na, const double *p
                            const uint64_t *keys, double *Lloc, double *dLloc) {
     const int N = t->n_sites;
     ndm_off L = ndm_layout(N, nh, na);
-    cplx *Obuf = (cplx *)malloc(sizeof(cplx) * L.np);
-    est_ctx x;
-    x.L = &L; x.p = p; x.N = N; x.Obuf = Obuf;
-    for (int64_t b = 0; b < B; ++b) {
-        double s[64], e[64];
-        unpack(keys[b], N, s, e);
-        x.logpsi_s = ndm_eval(&L, p, s, e, NULL);
-        x.res = 0;
-        x.dres = dLloc ? (cplx *)(dLloc + 2 * b * L.np) : NULL;
-        if (x.dres) memset(x.dres, 0, sizeof(cplx) * L.np);
-        walk(t, keys[b], 1, est_cb, &x);
-        Lloc[2 * b] = creal(x.res); Lloc[2 * b + 1] = cimag(x.res);
+#pragma omp parallel
+    {
+        cplx *Obuf = (cplx *)malloc(sizeof(cplx) * L.np);
+        est_ctx x;
+        x.L = &L; x.p = p; x.N = N; x.Obuf = Obuf;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t b = 0; b < B; ++b) {
+            double s[64], e[64];
+            unpack(keys[b], N, s, e);
+            x.logpsi_s = ndm_eval(&L, p, s, e, NULL);
+            x.res = 0;
+            x.dres = dLloc ? (cplx *)(dLloc + 2 * b * L.np) : NULL;
+            if (x.dres) memset(x.dres, 0, sizeof(cplx) * L.np);
+            walk(t, keys[b], 1, est_cb, &x);
+            Lloc[2 * b] = creal(x.res); Lloc[2 * b + 1] = cimag(x.res);
+        }
+        free(Obuf);
     }
-    free(Obuf);
 }
 
 /* ----------------------------------------------- loss and gradient (SR) */
@@ -413,6 +424,7 @@ int64_t nqref_metropolis(int N, int nh, int na, const double *p, int64_t B, int6
     const uint64_t mask2 = (2 * N >= 64) ? ~0ull : ((1ull << (2 * N)) - 1);
     int64_t acc = 0;
     const int64_t stored = compat_stale_last_slice ? Lc - 1 : Lc;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : acc)
     for (int64_t j = 0; j < B; ++j) {
         uint32_t o[4];
         draw(seed, 0, iter, (uint32_t)j, 0, o);

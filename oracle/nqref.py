@@ -79,6 +79,7 @@ class Ref:
         self.act = int(act)
         L = C.CDLL(_LIB)
         L.nqref_set_activation.argtypes = [C.c_int]
+        L.nqref_set_threads.argtypes = [C.c_int]
         L.nqref_ndm_np.restype = C.c_int64
         L.nqref_ndm_np.argtypes = [C.c_int, C.c_int, C.c_int]
         L.nqref_ndm_logpsi.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int64, _u64p, _dp]
@@ -94,6 +95,11 @@ class Ref:
                                        C.c_int, C.c_uint64, C.c_uint32, C.c_int, _u64p]
         L.nqref_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         self.L = L
+        L.nqref_set_threads(1)
+
+    def set_threads(self, n):
+        """Threads of the loops over samples and chains (OpenMP); 1 by default."""
+        self.L.nqref_set_threads(int(n))
 
     # -- network -----------------------------------------------------------
     def ndm_np(self, N, nh, na):
