@@ -9,11 +9,15 @@ Metropolis with 2N+1 = 33 passes, SR additive shift 0.01, lr 0.005.
 
 One "step" = one SR iteration: sampling, log rho / O / L_loc / force, loss
 statistics, S build, shifted Cholesky solve and the parameter update, all on
-the device.  Two forms are timed:
-  fp64    the plain form: FP64 network and S build, the reference's schedule
-          (chains restarted and burned in (50 calls) every iteration);
-  chosen  FP32 network kernels, BF16 S build on MFMA with FP32 accumulation,
-          FP64 solve, persistent chains (burn-in only once).
+the device.  Three forms are timed:
+  fp64             the plain form: FP64 network and S build, the reference's
+                   schedule (chains restarted and burned in (50 calls) every
+                   iteration);
+  fp64_persistent  the plain precision on the chosen schedule (persistent
+                   chains), so schedule and precision gains stand apart;
+  chosen           FP32 network kernels, BF16 S build on MFMA with FP32
+                   accumulation, FP64 solve, persistent chains (burn-in only
+                   once).
 `value` is the chosen form's SR iterations/s.
 
 Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per process,

@@ -130,8 +130,8 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
 //   * winv[b] = L_bb^-T (32 x 32, column-major) of every diagonal block.
 // Step k (one launch, k_chol_step): every trailing tile (I >= J, 64 x 64)
 // stages its rows of the panel A_ik, transforms them to L_ik = A_ik L_kk^-T
-// on the FP64 matrix cores, and applies A_IJ -= L_Ik L_Jk^T; the diagonal
-// tiles write L_Ik out (upper triangle); tile (0, 0), once updated, factors
+// on the FP64 matrix cores, and applies A_IJ -= L_Ik L_Jk^T; the first tile
+// of each row writes L_Ik out (upper triangle); tile (0, 0), once updated, factors
 // the next diagonal block in one wavefront (look-ahead), so the serial
 // column chain of block k + 1 runs while the other tiles of step k update.
 // The right-hand side row takes part as an ordinary trailing row, which is
@@ -256,28 +256,37 @@ __global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, i
     factor_diag32<false>(A, lda, k, kb, n, winv, info, true, Ls, nullptr);
 }
 
-__device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
-    int j = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
-    while ((j + 1) * (j + 2) / 2 <= b) ++j;
-    while (j * (j + 1) / 2 > b) --j;
+// tiles of a step, column tile by column tile: column tile J (of TJ) holds
+// row tiles J .. T - 1, so block b = 0 is tile (0, 0)
+__host__ __device__ __forceinline__ int step_tiles(int T, int TJ) { return TJ * T - TJ * (TJ - 1) / 2; }
+
+__device__ __forceinline__ void step_decode(int b, int T, int TJ, int* I, int* J) {
+    int j = 0;
+    while (j + 1 < TJ && step_tiles(T, j + 1) <= b) ++j;
     *J = j;
-    *I = b - j * (j + 1) / 2;
+    *I = j + (b - step_tiles(T, j));
 }
 
 // step k: panel transform + trailing update of the lower tiles (I >= J) of
-// the rows/columns >= o = k + kb; n = columns of the matrix, nr = rows
-// columns [k0, n) are factored (n <= nr: a partial factorization leaves the
-// Schur complement of the trailing rows / columns [n, nr)); winv block
-// (k - k0) / 32 holds panel k; blockIdx.y = instance (A, winv at strides)
-__global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int lda, int k, int kb, double* winv,
-                                                   int* info, int k0, int64_t sA, int64_t sW) {
+// the rows >= o = k + kb and the columns [o, ncu); n = columns of the
+// matrix, nr = rows; columns [k0, n) are factored (n <= nr: a partial
+// factorization leaves the Schur complement of the trailing rows / columns
+// [n, nr), updated here when ncu = nr; with ncu = n those columns are left
+// alone and only the panels' rows below are transformed, for one rank-n
+// update afterwards); winv block (k - k0) / 32 holds panel k; blockIdx.y =
+// instance (A, winv at strides)
+__global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int ncu, int lda, int k, int kb,
+                                                   double* winv, int* info, int k0, int64_t sA, int64_t sW) {
     __shared__ double Xi[CB][65];               // staged A_Ik: [m][row], then L_Ik
     __shared__ double Xj[CB][65];               // the same for A_Jk (off-diagonal tiles only)
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ __align__(16) double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
     int tI, tJ;
-    tri_decode(blockIdx.x, &tJ, &tI);           // tJ <= tI
+    {
+        const int T = (nr - k - kb + 63) / 64, TJ = max(1, (ncu - k - kb + 63) / 64);
+        step_decode(blockIdx.x, T, TJ, &tI, &tJ);   // tJ <= tI
+    }
     A += blockIdx.y * sA;
     winv += blockIdx.y * sW;
     const int o = k + kb;
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
                     const double val = old[a][b][v] - acc[a][b][v];
                     Ld[lr][lc] = val;
                     const int r = o + lr, c = o + lc;
-                    if (lr >= kbn && r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
+                    if (lr >= kbn && r < nr && c < ncu && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
                 }
         __builtin_amdgcn_wave_barrier();
         factor_diag32<true>(A, lda, o, kbn, n, winv + (size_t)((o - k0) / CB) * CB * CB, info, true, Ls, Ld);
@@ -413,13 +422,14 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
             for (int v = 0; v < 4; ++v) {
                 const int r = i0 + wr * 32 + a * 16 + lk + 4 * v;
                 const int c = j0 + wc * 32 + b * 16 + li;
-                if (r < nr && c < nr && r >= c) A[(int64_t)r + (int64_t)c * lda] = old[a][b][v] - acc[a][b][v];
+                if (r < nr && c < ncu && r >= c) A[(int64_t)r + (int64_t)c * lda] = old[a][b][v] - acc[a][b][v];
             }
-    if (dg) {
-        // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c]
-        // by the waves still running: 0, 2 and 3, or 2 and 3 in the look-ahead tile
-        const int t0 = la ? (int)threadIdx.x - 128 : (wave == 0 ? lane : (int)threadIdx.x - 64);
-        const int nt = la ? 128 : 192;
+    if (tJ == 0) {
+        // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c],
+        // by the first tile of the row of tiles: all four waves, or those still
+        // running in a diagonal tile (0, 2 and 3) and in the look-ahead tile (2 and 3)
+        const int t0 = la ? (int)threadIdx.x - 128 : (dg ? (wave == 0 ? lane : (int)threadIdx.x - 64) : (int)threadIdx.x);
+        const int nt = la ? 128 : (dg ? 192 : 256);
         for (int e = t0; e < 64 * CB; e += nt) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && i0 + r < nr) A[(int64_t)(k + c) + (int64_t)(i0 + r) * lda] = Pi[c][r];
@@ -557,22 +567,28 @@ __global__ __launch_bounds__(256) void k_fwd_step(const double* A, int n, int ld
 // [Spl, Spp, Spm], [0, Smp, Smm]] (lambda, Pi, mu; the lambda-mu block is
 // zero: O_lambda is real and O_mu imaginary).  Instance 0 = the [lambda | Pi]
 // block and instance 1 = the [mu | Pi] block (mu first), each with its own
-// right-hand side row; partially factoring the first nl columns of both (in
-// the same launches) leaves two Schur complements of the Pi block whose sum
-// minus the original is the Pi block's Schur complement in the whole matrix.
-// So the serial pivot chain is nl + npi long instead of 2 nl + npi.
+// right-hand side row.  Both factor their first nl columns in the same
+// launches, with updates to their lambda / mu columns only: that gives the
+// lambda and mu rows of L for every Pi column (and the right-hand side),
+// U_s = L_s[Pi, 0:nl]^T, stored transposed in the upper triangles.  The Pi
+// block's Schur complement in the whole matrix is then S_PiPi - U_0^T U_0 -
+// U_1^T U_1, one rank-2nl update on the matrix cores (k_split_schur) in
+// place of nl / 32 rank-32 updates of both instances' Pi blocks, and
+// instance 0 factors it.  The serial pivot chain is nl + npi long instead of
+// 2 nl + npi.
 __device__ __forceinline__ int split_p(int inst, int i, int nl, int npi) {
     return (inst == 0 || i >= nl) ? i : nl + npi + i;
 }
 
 // lower triangle (with the shift on the diagonal) and the right-hand side row
-// of both instances (m = nl + npi, lda = m + 1); grid (rows, m, 2)
+// of both instances (m = nl + npi, lda = m + 1; instance 1 only its mu
+// columns: its Pi columns are never read); grid (rows, m, 2)
 __global__ void k_split_copy(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
                              double* A2, int lda) {
     const int m = nl + npi;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, inst = blockIdx.z;
-    if (i > m || i < j) return;
+    if (i > m || i < j || (inst == 1 && j >= nl)) return;
     double* A = A2 + (int64_t)inst * lda * lda;
     const int pj = split_p(inst, j, nl, npi);
     double v;
@@ -585,22 +601,73 @@ __global__ void k_split_copy(const double* S, int lds, const double* F, int nl, 
     A[(int64_t)i + (int64_t)j * lda] = v;
 }
 
-// instance 0's Pi block and right-hand side entries += instance 1's - original
-__global__ void k_split_combine(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
-                                double* A2, int lda) {
-    const int m = nl + npi;
-    const int i = nl + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = nl + blockIdx.y;
-    if (i > m || i < j) return;
-    double orig;
-    if (i == m) {
-        orig = F[j];
-    } else {
-        orig = S[(int64_t)i + (int64_t)j * lds];
-        if (i == j) orig += (kind == 0) ? eps : eps * orig;
+// instance 0's Pi rows and columns [nl, m] (the right-hand side row m
+// included) -= U_0^T U_0 + U_1^T U_1, U_s = A_s[0:nl, nl:m+1] (row c of U_s:
+// L_s[., c] of the Pi rows, from the upper triangle).  One 32 x 32 tile of
+// the lower triangle per workgroup (tiles numbered as the factorization
+// steps number theirs), each wave one 16 x 16 quarter over all 2 nl terms
+// on the FP64 matrix cores.  The terms come in chunks of KC rows c: each
+// chunk's two 32-column slices of U are read coalesced (64 consecutive rows
+// of a column per wave) into registers PF chunks ahead, and staged in LDS
+// for the products, so PF memory round trips are in flight while a chunk is
+// multiplied.  f64 MFMA operands: A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k; output D[lane/16 + 4v][lane%16]
+constexpr int KC = 64, PF = 4;
+__global__ __launch_bounds__(256) void k_split_schur(double* A2, int lda, int nl, int npi) {
+    __shared__ double Xa[32][KC + 1], Xb[32][KC + 1];   // [tile column][chunk row]
+    const int m = nl + npi, T = (npi + 1 + 31) / 32;
+    int tI, tJ;
+    step_decode(blockIdx.x, T, T, &tI, &tJ);   // tJ <= tI
+    const int i0 = nl + 32 * tI, j0 = nl + 32 * tJ;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, li = lane & 15, lk = lane >> 4;
+    const int rt = wave >> 1, ct = wave & 1;
+    const int cpi = (nl + KC - 1) / KC, nch = 2 * cpi;   // chunks per instance / in all
+    const int r = t & (KC - 1), c4 = t >> 6;             // this thread's chunk row; columns c4 + 4 q
+    // raw loads from clamped addresses; the rows past nl (and the chunks
+    // past nch) are zeroed when staged, so no select waits on a load
+    double va[PF][8], vb[PF][8];
+    bool ok[PF];
+    auto fetch = [&](int ch, double* a, double* b, bool& okc) {
+        const int s = ch >= cpi ? 1 : 0;
+        const int c = (ch - s * cpi) * KC + r;
+        const double* U = A2 + (int64_t)s * lda * lda + min(c, nl - 1);
+        okc = c < nl && ch < nch;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a[q] = U[(int64_t)min(i0 + c4 + 4 * q, m) * lda];
+            b[q] = U[(int64_t)min(j0 + c4 + 4 * q, m) * lda];
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) fetch(u, va[u], vb[u], ok[u]);
+    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // whole rounds of PF chunks (the chunks past nch are zero), so that no
+    // branch separates a chunk's loads from its use
+    for (int ch0 = 0; ch0 < nch; ch0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            __syncthreads();   // every wave is done with the previous chunk
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                Xa[c4 + 4 * q][r] = ok[u] ? va[u][q] : 0.0;
+                Xb[c4 + 4 * q][r] = ok[u] ? vb[u][q] : 0.0;
+            }
+            __syncthreads();
+            fetch(ch0 + u + PF, va[u], vb[u], ok[u]);   // in flight during the next PF chunks
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 8) {
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Xa[rt * 16 + li][kk + lk], Xb[ct * 16 + li][kk + lk], acc0,
+                                                            0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Xa[rt * 16 + li][kk + 4 + lk],
+                                                            Xb[ct * 16 + li][kk + 4 + lk], acc1, 0, 0, 0);
+            }
+        }
     }
-    const int64_t e = (int64_t)i + (int64_t)j * lda;
-    A2[e] += A2[(int64_t)lda * lda + e] - orig;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = i0 + rt * 16 + lk + 4 * v, j = j0 + ct * 16 + li;
+        if (i <= m && j <= m && i >= j) A2[(int64_t)i + (int64_t)j * lda] -= acc0[v] + acc1[v];
+    }
 }
 
 // y1[i] -= sum_{c in [nl, m)} A1[i, c] x0[c] for i < nl (the mu rows' share of
@@ -773,7 +840,7 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 
 size_t nq_cholesky_winv_doubles(int n) { return (size_t)((n + CB - 1) / CB) * CB * CB; }
 
-hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* info, double* winv, int ninst,
+hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda, int* info, double* winv, int ninst,
                               int64_t sA, int64_t sW, hipStream_t st) {
     if (k0 >= n) return hipSuccess;
     hipLaunchKernelGGL(k_chol_diag32, dim3(1, ninst), dim3(64), 0, st, A, n, lda, k0, (n - k0 < CB ? n - k0 : CB),
@@ -782,16 +849,19 @@ hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* inf
         const int kb = (n - k < CB) ? n - k : CB;
         const int rest = nr - k - kb;
         if (rest <= 0) break;
-        const int T = (rest + 63) / 64;
-        hipLaunchKernelGGL(k_chol_step, dim3(T * (T + 1) / 2, ninst), dim3(256), 0, st, A, n, nr, lda, k, kb, winv,
-                           info, k0, sA, sW);
+        // row tiles of the trailing rows; column tiles of the updated columns
+        // (at least one, whose tiles still transform the panel's rows)
+        const int T = (rest + 63) / 64, TJ = std::max(1, (ncu - k - kb + 63) / 64);
+        hipLaunchKernelGGL(k_chol_step, dim3(step_tiles(T, TJ), ninst), dim3(256), 0, st, A, n, nr, ncu, lda, k, kb,
+                           winv, info, k0, sA, sW);
     }
     return hipGetLastError();
 }
 
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st) {
     // rows >= n (at most one extra row, the right-hand side) are carried along
-    return nq_cholesky_range(S, 0, n, (lds > n) ? n + 1 : n, lds, info, winv, 1, 0, 0, st);
+    const int nr = (lds > n) ? n + 1 : n;
+    return nq_cholesky_range(S, 0, n, nr, nr, lds, info, winv, 1, 0, 0, st);
 }
 
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
@@ -842,13 +912,14 @@ hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, in
     const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
     hipLaunchKernelGGL(k_split_copy, dim3(nblk(m + 1, 256), m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind,
                        A2, lda);
-    // both [lambda | Pi] and [mu | Pi] factored over their first nl columns together
-    hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, lda, info, winv, 2, sA, sW, st);
+    // both [lambda | Pi] and [mu | Pi] factored over their first nl columns
+    // together, their Pi columns left as they are
+    hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, nl, lda, info, winv, 2, sA, sW, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_split_combine, dim3(nblk(npi + 1, 256), npi), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind,
-                       A2, lda);
+    const int T = (npi + 1 + 31) / 32;
+    hipLaunchKernelGGL(k_split_schur, dim3(step_tiles(T, T)), dim3(256), 0, st, A2, lda, nl, npi);
     // the Pi block of instance 0 now holds the whole matrix's Schur complement
-    return nq_cholesky_range(A2, nl, m, m + 1, lda, info, winv + 2 * sW, 1, 0, 0, st);
+    return nq_cholesky_range(A2, nl, m, m + 1, m + 1, lda, info, winv + 2 * sW, 1, 0, 0, st);
 }
 
 hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,

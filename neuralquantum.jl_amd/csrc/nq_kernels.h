@@ -151,9 +151,11 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 // nq_copy_aug_col).
 size_t nq_cholesky_winv_doubles(int n);
 // factor columns [k0, n) of the nr x nr lower matrix A (rows past n are carried
-// along: a partial factorization leaves their Schur complement); winv block
-// (k - k0) / 32 holds panel k; ninst instances at strides sA (A), sW (winv)
-hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int lda, int* info, double* winv, int ninst,
+// along: a partial factorization leaves their Schur complement in columns
+// [n, nr) when ncu = nr, and leaves those columns untouched when ncu = n);
+// winv block (k - k0) / 32 holds panel k; ninst instances at strides sA (A),
+// sW (winv)
+hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda, int* info, double* winv, int ninst,
                               int64_t sA, int64_t sW, hipStream_t st);
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
                                        double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st);
