@@ -131,8 +131,9 @@ __global__ void k_shift(double* S, int n, int lds, double eps, int kind) {
 // Step k (one launch, k_chol_step): every trailing tile (I >= J, 64 x 64)
 // stages its rows of the panel A_ik, transforms them to L_ik = A_ik L_kk^-T
 // on the FP64 matrix cores, and applies A_IJ -= L_Ik L_Jk^T; the first tile
-// of each row writes L_Ik out (upper triangle); tile (0, 0), once updated, factors
-// the next diagonal block in one wavefront (look-ahead), so the serial
+// of each row writes L_Ik out (upper triangle); tile (0, 0) transforms and
+// updates the next diagonal block first, one 16 x 16 piece per wave, and
+// factors it in one wavefront (look-ahead, chol_lookahead), so the serial
 // column chain of block k + 1 runs while the other tiles of step k update.
 // The right-hand side row takes part as an ordinary trailing row, which is
 // the forward substitution L y = b; y ends in column n of the upper triangle.
@@ -267,6 +268,144 @@ __device__ __forceinline__ void step_decode(int b, int T, int TJ, int* I, int* J
     *I = j + (b - step_tiles(T, j));
 }
 
+// The look-ahead tile (0, 0) of step k: rows and columns [o, o + 64),
+// o = k + kb, whose first 32 rows and columns D are the next diagonal block.
+// Its critical path is load -> L_D = A_Dk W (32 x 32) -> D -= L_D L_D^T ->
+// factor D, so that part is spread over all four waves, one 16 x 16 tile
+// each, with the MFMA operands loaded from memory in their lane layout (no
+// LDS staging before the first product).  Then wave 0 factors D; waves 2
+// and 3 store the panel rows of D; wave 1 loads, transforms and stores the
+// other 32 rows (panel rows, the block below D, the lower half of the block
+// right of it), all while the pivot chain runs.
+// Operand layouts (f64 16x16x4 MFMA): A[i][k] from lane i + 16 k, B[k][j]
+// from lane j + 16 k; output D[lane/16 + 4v][lane%16].
+__device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu, int lda, int k, int kb,
+                                               const double* w, double* wnext, int* info, double (*Lr)[CB + 1],
+                                               double (*Lr2)[CB + 1], double (*Ld)[CB + 1], double (*Ls)[64]) {
+    const int o = k + kb, kbn = min(CB, n - o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    // this wave's transform tile (rows rt, columns ct of L_D) and update tile
+    // (ra, ca) of D: waves 0, 1, 2 own (0, 0), (1, 0), (1, 1); wave 3's
+    // update tile (0, 1) is above the diagonal
+    const int rt = wave >> 1, ct = wave & 1;
+    const int ra = (wave == 1 || wave == 2) ? 1 : 0, ca = (wave >= 2) ? 1 : 0;
+    double xa[CB / 4], wb[CB / 4], od[4];
+    {
+        const int row = o + rt * 16 + li;
+        const int64_t rc = min(row, nr - 1);
+#pragma unroll
+        for (int s = 0; s < CB / 4; ++s) {
+            const int m = 4 * s + lk;
+            xa[s] = A[rc + (int64_t)(k + min(m, kb - 1)) * lda];
+            wb[s] = w[m + CB * (ct * 16 + li)];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            od[v] = A[(int64_t)min(o + ra * 16 + lk + 4 * v, nr - 1) + (int64_t)min(o + ca * 16 + li, nr - 1) * lda];
+        d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < CB / 4; ++s) {
+            const double a = (4 * s + lk < kb && row < nr) ? xa[s] : 0.0;
+            t = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wb[s], t, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Lr[rt * 16 + lk + 4 * v][ct * 16 + li] = t[v];
+    }
+    __syncthreads();
+    if (wave < 3) {
+        d4 u = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < CB / 4; ++s)
+            u = __builtin_amdgcn_mfma_f64_16x16x4f64(Lr[ra * 16 + li][4 * s + lk], Lr[ca * 16 + li][4 * s + lk], u, 0,
+                                                     0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int lr = ra * 16 + lk + 4 * v, lc = ca * 16 + li;
+            const double val = od[v] - u[v];
+            Ld[lr][lc] = val;
+            // rows past the diagonal block (the right-hand side row, or rows
+            // past n of a partial factorization) are ordinary trailing rows
+            const int r = o + lr, c = o + lc;
+            if (lr >= kbn && r < nr && c < ncu && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        factor_diag32<true>(A, lda, o, kbn, n, wnext, info, true, Ls, Ld);
+        return;
+    }
+    if (wave >= 2) {
+        // L_D out, transposed into the upper triangle
+        for (int e = (int)threadIdx.x - 128; e < CB * CB; e += 128) {
+            const int c = e & (CB - 1), r = e >> 5;
+            if (c < kb && o + r < nr) A[(int64_t)(k + c) + (int64_t)(o + r) * lda] = Lr[r][c];
+        }
+        return;
+    }
+    // wave 1: rows R = [o + 32, o + 64)
+    const int rb = o + CB;
+    if (rb >= nr) return;
+    double xb[2][CB / 4], wc2[2][CB / 4], q10[2][2][4], q11[2][2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int s = 0; s < CB / 4; ++s) {
+            const int m = 4 * s + lk;
+            xb[h][s] = A[(int64_t)min(rb + h * 16 + li, nr - 1) + (int64_t)(k + min(m, kb - 1)) * lda];
+            wc2[h][s] = w[m + CB * (h * 16 + li)];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t r = min(rb + a * 16 + lk + 4 * v, nr - 1);
+                q10[a][b][v] = A[r + (int64_t)min(o + b * 16 + li, nr - 1) * lda];
+                q11[a][b][v] = A[r + (int64_t)min(rb + b * 16 + li, nr - 1) * lda];
+            }
+    // L_R = A_Rk W into Lr2 (this wave's own LDS), then its panel rows out
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < CB / 4; ++s) {
+                const double x = (4 * s + lk < kb && rb + a * 16 + li < nr) ? xb[a][s] : 0.0;
+                t = __builtin_amdgcn_mfma_f64_16x16x4f64(x, wc2[b][s], t, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Lr2[a * 16 + lk + 4 * v][b * 16 + li] = t[v];
+        }
+    __builtin_amdgcn_wave_barrier();
+    for (int e = lane; e < CB * CB; e += 64) {
+        const int c = e & (CB - 1), r = e >> 5;
+        if (c < kb && rb + r < nr) A[(int64_t)(k + c) + (int64_t)(rb + r) * lda] = Lr2[r][c];
+    }
+    // the block below D (rows R, columns [o, o + 32)) and the lower half of
+    // the block R x R
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            d4 u = {0.0, 0.0, 0.0, 0.0}, x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < CB / 4; ++s) {
+                const double ar = Lr2[a * 16 + li][4 * s + lk];
+                u = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, Lr[b * 16 + li][4 * s + lk], u, 0, 0, 0);
+                if (b <= a) x = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, Lr2[b * 16 + li][4 * s + lk], x, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rb + a * 16 + lk + 4 * v;
+                const int c0 = o + b * 16 + li, c1 = rb + b * 16 + li;
+                if (r < nr && c0 < ncu) A[(int64_t)r + (int64_t)c0 * lda] = q10[a][b][v] - u[v];
+                if (b <= a && r < nr && c1 < ncu && r >= c1) A[(int64_t)r + (int64_t)c1 * lda] = q11[a][b][v] - x[v];
+            }
+        }
+}
+
 // step k: panel transform + trailing update of the lower tiles (I >= J) of
 // the rows >= o = k + kb and the columns [o, ncu); n = columns of the
 // matrix, nr = rows; columns [k0, n) are factored (n <= nr: a partial
@@ -281,18 +420,25 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     __shared__ double Xj[CB][65];               // the same for A_Jk (off-diagonal tiles only)
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ __align__(16) double Ls[CB][64];
-    __shared__ double Ld[CB][CB + 1];           // the next diagonal block, from wave 0's update
+    __shared__ double Ld[CB][CB + 1];           // the next diagonal block (look-ahead tile)
+    A += blockIdx.y * sA;
+    winv += blockIdx.y * sW;
+    const int o = k + kb;
+    const double* w = winv + (size_t)((k - k0) / CB) * CB * CB;
+    if (blockIdx.x == 0 && o < n) {
+        // tile (0, 0): the look-ahead tile (Xi and Xj serve as its LDS)
+        chol_lookahead(A, n, nr, ncu, lda, k, kb, w, winv + (size_t)((o - k0) / CB) * CB * CB, info,
+                       reinterpret_cast<double (*)[CB + 1]>(&Xi[0][0]), reinterpret_cast<double (*)[CB + 1]>(&Xj[0][0]),
+                       Ld, Ls);
+        return;
+    }
     int tI, tJ;
     {
         const int T = (nr - k - kb + 63) / 64, TJ = max(1, (ncu - k - kb + 63) / 64);
         step_decode(blockIdx.x, T, TJ, &tI, &tJ);   // tJ <= tI
     }
-    A += blockIdx.y * sA;
-    winv += blockIdx.y * sW;
-    const int o = k + kb;
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
     const bool dg = tI == tJ;                   // diagonal tile: A_Jk = A_Ik
-    const double* w = winv + (size_t)((k - k0) / CB) * CB * CB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int wr = wave >> 1, wc = wave & 1;
@@ -366,7 +512,6 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     __syncthreads();
     double (*Pi)[65] = Xi;
     double (*Pj)[65] = dg ? Xi : Xj;
-    const bool la = blockIdx.x == 0 && o < n;
     // the quarter above the diagonal of a diagonal tile is never stored: its
     // wave has no update to make
     if (dg && wr == 0 && wc == 1) return;
@@ -391,29 +536,6 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    // look-ahead: in tile (0, 0) wave 0 owns rows and columns [o, o + 32), the
-    // next diagonal block; it keeps its update in LDS and factors it at once,
-    // while waves 2 and 3 store the rest of the tile and the panel
-    if (la && wave == 0) {
-        // rows of the quarter past the diagonal block (only the right-hand
-        // side row n, when fewer than 32 columns remain) still go to A
-        const int kbn = min(CB, n - o);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int lr = a * 16 + lk + 4 * v, lc = b * 16 + li;
-                    const double val = old[a][b][v] - acc[a][b][v];
-                    Ld[lr][lc] = val;
-                    const int r = o + lr, c = o + lc;
-                    if (lr >= kbn && r < nr && c < ncu && r >= c) A[(int64_t)r + (int64_t)c * lda] = val;
-                }
-        __builtin_amdgcn_wave_barrier();
-        factor_diag32<true>(A, lda, o, kbn, n, winv + (size_t)((o - k0) / CB) * CB * CB, info, true, Ls, Ld);
-        return;
-    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -427,9 +549,9 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     if (tJ == 0) {
         // L_Ik out, transposed into the upper triangle: A[k + c, i] = L[i, k + c],
         // by the first tile of the row of tiles: all four waves, or those still
-        // running in a diagonal tile (0, 2 and 3) and in the look-ahead tile (2 and 3)
-        const int t0 = la ? (int)threadIdx.x - 128 : (dg ? (wave == 0 ? lane : (int)threadIdx.x - 64) : (int)threadIdx.x);
-        const int nt = la ? 128 : (dg ? 192 : 256);
+        // running in a diagonal tile (0, 2 and 3)
+        const int t0 = dg ? (wave == 0 ? lane : (int)threadIdx.x - 64) : (int)threadIdx.x;
+        const int nt = dg ? 192 : 256;
         for (int e = t0; e < 64 * CB; e += nt) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && i0 + r < nr) A[(int64_t)(k + c) + (int64_t)(i0 + r) * lda] = Pi[c][r];
