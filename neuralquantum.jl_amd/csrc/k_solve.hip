@@ -156,11 +156,14 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
 // L_kk turns the identity rows into I L_kk^-T = L_kk^-T (the bordering rule
 // L_21 = A_21 L_11^-T with A_21 = I).
 // The sweep runs in sub-panels of 8 columns: inside a sub-panel each column
-// updates only the sub-panel's later columns (values broadcast from their
-// lanes by v_readlane), so the serial chain from one pivot to the next stays
-// short; the rest of the block then takes one rank-8 update, its broadcast
-// operands read from LDS (Lc[column][lane]).  Writes L_kk (lower triangle)
-// and L_kk^-T to `w`.
+// updates only the sub-panel's later columns, so the serial chain from one
+// pivot to the next stays short: the next column at once (its multiplier
+// broadcast from its lane by v_readlane), the others one pivot later (their
+// multipliers broadcast from LDS, Lc[column][lane], written as each column
+// is finished); the rest of the block then takes one rank-8 update from the
+// same LDS copy.  Lane j's own entry of column j is v, so r[j] rsqrt(v) is
+// L[j][j] = sqrt(v) there and needs no special case.  Writes L_kk (lower
+// triangle) and L_kk^-T to `w`.
 // The block comes from A (global) or, when `src` is given, from LDS
 // (src[row][column], rows and columns of the block).
 constexpr int SUBP = 8;
@@ -192,17 +195,15 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
             const double rp = rsqrt_nr(v);
             if (j > p0) {
 #pragma unroll
-                for (int m = j + 1; m < p0 + SUBP; ++m) r[m] -= lprev * rl_d(lprev, m);
+                for (int m = j + 1; m < p0 + SUBP; ++m) r[m] -= lprev * Lc[j - 1][m];
             }
-            const double piv = v * rp;
-            const double lj = (mrow && tr == j) ? piv : r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
+            const double lj = r[j] * rp;   // L[tr][j] (identity rows: (L^-T)[tr][j])
             r[j] = lj;
+            Lc[j][t] = lj;
             if (j + 1 < p0 + SUBP) r[j + 1] -= lj * rl_d(lj, j + 1);
             lprev = lj;
         }
         if (p0 + SUBP < CB) {
-#pragma unroll
-            for (int q = p0; q < p0 + SUBP; ++q) Lc[q][t] = r[q];
             __builtin_amdgcn_wave_barrier();
             // column pairs, each pinned in place (otherwise the compiler sinks
             // every column's update to its first use, or hoists all the LDS
