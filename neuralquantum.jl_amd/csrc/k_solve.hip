@@ -570,10 +570,10 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 // registers): for p = last .. 0, x_p = L_pp^-T y_p, then y_q -= L_pq^T x_p for
 // the group's earlier panels q < p (the blocks A[q rows, p columns] of the
 // upper triangle, staged once); then rows i < k take y_i -= sum_m
-// L[k + m, i] x_m over all the group's columns (64 rows per workgroup, four
-// lanes per row with 32 columns each), their factor entries loaded before the
-// group solve so that the two overlap.  One launch instead of BG.
-constexpr int BG = 4;
+// L[k + m, i] x_m over all the group's columns (BR rows per workgroup, LPR
+// lanes per row with BG CB / LPR columns each), their factor entries loaded
+// before the group solve so that the two overlap.  One launch instead of BG.
+constexpr int BG = 6, LPR = 8, BR = 256 / LPR, BCW = BG * CB / LPR;
 __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, int k, int kend, int n,
                                                     const double* winv, double* y, double* x, int k0, int64_t sA,
                                                     int64_t sW, int64_t sY) {
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
     const int t = threadIdx.x;
     const int ng = kend - k;                       // columns of the group (<= BG * CB)
     const int npan = (ng + CB - 1) / CB;
-    const int i = blockIdx.x * 64 + (t >> 2), h = t & 3;   // row below, its column quarter
+    const int i = blockIdx.x * BR + t / LPR, h = t % LPR;   // row below, its share of the columns
     // every load is issued before the first use: the group's upper-triangle
     // blocks (q < p), this thread's row of L_pp^-T (threads 32p .. 32p + 31),
     // y of the group, then the factor entries A[i, k + m] of the rows below
@@ -609,11 +609,11 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
 #pragma unroll
         for (int m = 0; m < CB; ++m) wv[m] = w[c + CB * m];
     }
-    const double yv = y[k + min(t & (BG * CB - 1), ng - 1)];
+    const double yv = y[k + min(t, ng - 1)];
     const int64_t ic = (int64_t)min(i, max(k - 1, 0));
-    double a[CB];
+    double a[BCW];
 #pragma unroll
-    for (int m = 0; m < CB; ++m) a[m] = A[ic + (int64_t)min(k + h * CB + m, n - 1) * lda];
+    for (int m = 0; m < BCW; ++m) a[m] = A[ic + (int64_t)min(k + h * BCW + m, n - 1) * lda];
 #pragma unroll
     for (int p = 1; p < BG; ++p)
 #pragma unroll
@@ -645,9 +645,9 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
     if (blockIdx.x == 0 && t < ng) x[k + t] = xs[t];
     double acc = 0.0;
 #pragma unroll
-    for (int m = 0; m < CB; ++m) acc += a[m] * (h * CB + m < ng ? xs[h * CB + m] : 0.0);
-    acc += __shfl_xor(acc, 1, 64);
-    acc += __shfl_xor(acc, 2, 64);
+    for (int m = 0; m < BCW; ++m) acc += a[m] * (h * BCW + m < ng ? xs[h * BCW + m] : 0.0);
+#pragma unroll
+    for (int d = 1; d < LPR; d <<= 1) acc += __shfl_xor(acc, d, 64);
     if (h == 0 && i < k) y[i] -= acc;
 }
 
@@ -994,7 +994,7 @@ hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, 
     const int groups = (n - k0 + BG * CB - 1) / (BG * CB);
     for (int g = groups - 1; g >= 0; --g) {
         const int k = k0 + g * BG * CB, kend = (n - k < BG * CB) ? n : k + BG * CB;
-        hipLaunchKernelGGL(k_back_group, dim3(k > 0 ? nblk(k, 64) : 1, ninst), dim3(256), 0, st, L, lds, k, kend, n,
+        hipLaunchKernelGGL(k_back_group, dim3(k > 0 ? nblk(k, BR) : 1, ninst), dim3(256), 0, st, L, lds, k, kend, n,
                            winv, y, x, k0, sA, sW, sY);
     }
     return hipGetLastError();
