@@ -141,6 +141,19 @@ nqh_status hip_fail(nqh_ctx* c, hipError_t e, const char* where) {
         if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
     } while (0)
 
+// Host/device copies and clears are queued on the context's stream and waited for there. That
+// stream is non-blocking, so a plain hipMemcpy or hipMemset (null stream) would not be ordered
+// against its kernels: a clear could land after a later upload or conversion.
+static hipError_t copy_sync(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+static hipError_t zero_sync(hipStream_t st, void* dst, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
 enum { PH_SAMPLER = 0, PH_ESTIMATE, PH_POST, PH_SBUILD, PH_SOLVE, PH_UPDATE, PH_FACTOR };
 
 void phase_begin(nqh_ctx* c, int ph) {
@@ -796,7 +809,7 @@ static nqh_status upload_table(nqh_ctx* c, const nqh_optable* t, bool ket_as_row
     }
     nqh_status s = ensure(c, mem, host.size());
     if (s) return s;
-    HIPCHK(c, hipMemcpy(mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c->stream, mem.p, host.data(), host.size(), hipMemcpyHostToDevice));
     unsigned char* base = reinterpret_cast<unsigned char*>(mem.p);
     const int32_t* di = reinterpret_cast<const int32_t*>(base);
     tab.n_sites = N;
@@ -849,8 +862,8 @@ nqh_status nqh_set_ndm(nqh_ctx* c, int32_t N, int32_t nh, int32_t na, int32_t ac
     nqh_status s;
     if ((s = ensure(c, c->p64, sizeof(double) * c->d.np))) return s;
     if ((s = ensure(c, c->p32, sizeof(float) * c->d.np))) return s;
-    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * c->d.np));
-    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * c->d.np));
+    HIPCHK(c, zero_sync(c->stream, c->p64.p, sizeof(double) * c->d.np));
+    HIPCHK(c, zero_sync(c->stream, c->p32.p, sizeof(float) * c->d.np));
     c->has_net = true;
     c->ns_cap = 0;
     c->grad_ready = false;
@@ -873,8 +886,8 @@ nqh_status nqh_set_rbmsplit(nqh_ctx* c, int32_t N, int32_t nh) {
     nqh_status s;
     if ((s = ensure(c, c->p64, sizeof(double) * n))) return s;
     if ((s = ensure(c, c->p32, sizeof(float) * n))) return s;
-    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * n));
-    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * n));
+    HIPCHK(c, zero_sync(c->stream, c->p64.p, sizeof(double) * n));
+    HIPCHK(c, zero_sync(c->stream, c->p32.p, sizeof(float) * n));
     c->has_net = true;
     c->ns_cap = 0;
     c->grad_ready = false;
@@ -898,8 +911,8 @@ nqh_status nqh_set_rbm(nqh_ctx* c, int32_t N, int32_t nh) {
     nqh_status s;
     if ((s = ensure(c, c->p64, sizeof(double) * n))) return s;
     if ((s = ensure(c, c->p32, sizeof(float) * n))) return s;
-    HIPCHK(c, hipMemset(c->p64.p, 0, sizeof(double) * n));
-    HIPCHK(c, hipMemset(c->p32.p, 0, sizeof(float) * n));
+    HIPCHK(c, zero_sync(c->stream, c->p64.p, sizeof(double) * n));
+    HIPCHK(c, zero_sync(c->stream, c->p32.p, sizeof(float) * n));
     c->has_net = true;
     c->ns_cap = 0;
     c->grad_ready = false;
@@ -1020,10 +1033,10 @@ nqh_status nqh_logpsi_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, double* 
         const int npc = c->rbm_npc;
         std::vector<double> y((size_t)2 * npc * B);
         if (fp64_net(c)) {
-            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
         } else {
             std::vector<float> yf(y.size());
-            HIPCHK(c, hipMemcpy(yf.data(), c->Y.p, sizeof(float) * yf.size(), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, yf.data(), c->Y.p, sizeof(float) * yf.size(), hipMemcpyDeviceToHost));
             for (size_t k = 0; k < y.size(); ++k) y[k] = yf[k];
         }
         for (int64_t b = 0; b < B; ++b)
@@ -1035,11 +1048,11 @@ nqh_status nqh_logpsi_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, double* 
         const ndm_dims& d = c->d;
         if (fp64_net(c)) {
             std::vector<double> y((size_t)d.R * B);
-            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, y.data(), c->Y.p, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
             for (int64_t b = 0; b < B; ++b) expand_row(y.data() + (size_t)b * d.R, d, O + (size_t)2 * b * d.np);
         } else {
             std::vector<float> y((size_t)d.R * B);
-            HIPCHK(c, hipMemcpy(y.data(), c->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, y.data(), c->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
             for (int64_t b = 0; b < B; ++b) expand_row(y.data() + (size_t)b * d.R, d, O + (size_t)2 * b * d.np);
         }
     }
@@ -1065,14 +1078,14 @@ nqh_status nqh_connections(nqh_ctx* c, const uint64_t* keys, int64_t B, int32_t 
         cleanup();
         return s;
     }
-    hipError_t e = hipMemcpy(kin.p, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice);
+    hipError_t e = copy_sync(c->stream, kin.p, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = nq_launch_connections(c->tab, N, P<uint64_t>(kin), B, cap, P<int>(cnt), P<uint64_t>(ko), P<double>(mo),
                                   c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(counts, cnt.p, sizeof(int) * B, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && keys_out && cap > 0) e = hipMemcpy(keys_out, ko.p, sizeof(uint64_t) * B * cap, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && mel_out && cap > 0) e = hipMemcpy(mel_out, mo.p, 2 * sizeof(double) * B * cap, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_sync(c->stream, counts, cnt.p, sizeof(int) * B, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && keys_out && cap > 0) e = copy_sync(c->stream, keys_out, ko.p, sizeof(uint64_t) * B * cap, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mel_out && cap > 0) e = copy_sync(c->stream, mel_out, mo.p, 2 * sizeof(double) * B * cap, hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return hip_fail(c, e, "nqh_connections");
     for (int64_t b = 0; b < B; ++b)
@@ -1090,7 +1103,7 @@ nqh_status nqh_local_estimator(nqh_ctx* c, const uint64_t* keys, int64_t B, doub
     if ((s = eval_host_keys(c, keys, B, false, true, dLloc != nullptr))) return s;
     if ((s = download_cplx(c, c->Lloc, Lloc, 2 * B))) return s;
     if (dLloc) {
-        HIPCHK(c, hipMemcpy(dLloc, c->G.p, sizeof(double) * 2 * B * nderiv(c), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, dLloc, c->G.p, sizeof(double) * 2 * B * nderiv(c), hipMemcpyDeviceToHost));
     }
     return NQH_OK;
 }
@@ -1111,25 +1124,25 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
     const ndm_dims& d = c->d;
-    if (stats) HIPCHK(c, hipMemcpy(stats, c->stats.p, sizeof(double) * 5, hipMemcpyDeviceToHost));
+    if (stats) HIPCHK(c, copy_sync(c->stream, stats, c->stats.p, sizeof(double) * 5, hipMemcpyDeviceToHost));
     if (is_rbm(c)) {
         // F and Obar interleaved complex (reference order); S as the real
         // embedding [[Re S, -Im S], [Im S, Re S]] (2 npc x 2 npc, column-major)
         const int npc = c->rbm_npc, n = 2 * npc;
         if (F) {
             std::vector<double> f(n);
-            HIPCHK(c, hipMemcpy(f.data(), c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, f.data(), c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost));
             for (int k = 0; k < npc; ++k) { F[2 * k] = f[k]; F[2 * k + 1] = f[npc + k]; }
         }
         if (Obar) {
             std::vector<double> mu(n);
-            HIPCHK(c, hipMemcpy(mu.data(), c->mu.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, mu.data(), c->mu.p, sizeof(double) * n, hipMemcpyDeviceToHost));
             for (int k = 0; k < npc; ++k) { Obar[2 * k] = mu[k]; Obar[2 * k + 1] = mu[npc + k]; }
         }
         if (S) {
             const size_t ldS = (size_t)n + 1;
             std::vector<double> sp(ldS * ldS);
-            HIPCHK(c, hipMemcpy(sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c->stream, sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
             for (int l = 0; l < n; ++l)
                 for (int k = 0; k < n; ++k) S[(size_t)k + (size_t)l * n] = sp[(size_t)k + (size_t)l * ldS];
         }
@@ -1138,18 +1151,18 @@ nqh_status nqh_grad_from_samples(nqh_ctx* c, const uint64_t* keys, int64_t chain
     if (F) {
 
         std::vector<double> f(d.np);
-        HIPCHK(c, hipMemcpy(f.data(), c->F.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, f.data(), c->F.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
         for (int k = 0; k < d.np; ++k) F[p2ref_host(k, d)] = f[k];
     }
     if (Obar) {
         std::vector<double> mu(d.R);
-        HIPCHK(c, hipMemcpy(mu.data(), c->mu.p, sizeof(double) * d.R, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, mu.data(), c->mu.p, sizeof(double) * d.R, hipMemcpyDeviceToHost));
         expand_row(mu.data(), d, Obar);
     }
     if (S) {
         const size_t ldS = (size_t)d.np + 1;
         std::vector<double> sp(ldS * ldS);
-        HIPCHK(c, hipMemcpy(sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, sp.data(), c->S.p, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
         std::vector<int> r(d.np);
         for (int k = 0; k < d.np; ++k) r[k] = p2ref_host(k, d);
         for (int l = 0; l < d.np; ++l)
@@ -1289,10 +1302,10 @@ nqh_status nqh_sample(nqh_ctx* c, uint64_t* keys_out, int64_t* accepted) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
     if (keys_out)
-        HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * c->chains * c->chain_len, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, keys_out, c->samples.p, sizeof(uint64_t) * c->chains * c->chain_len, hipMemcpyDeviceToHost));
     if (accepted) {
         unsigned long long v = 0;
-        HIPCHK(c, hipMemcpy(&v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, &v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
         *accepted = (int64_t)v;
     }
     return NQH_OK;
@@ -1304,7 +1317,7 @@ nqh_status nqh_get_chain_acceptance(nqh_ctx* c, uint32_t* accepted_out, int64_t 
     if (!c->has_sampler || !c->chain_acc.p) return fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)");
     if (chains != c->chains) return fail(c, NQH_E_ARG, "chains must equal the sampler's chain count");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(accepted_out, c->chain_acc.p, sizeof(uint32_t) * (size_t)chains, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, accepted_out, c->chain_acc.p, sizeof(uint32_t) * (size_t)chains, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -1328,7 +1341,7 @@ nqh_status nqh_sample_grad(nqh_ctx* c, double* stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
     double st[5];
-    HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
     if (stats) std::memcpy(stats, st, sizeof(st));
     if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>");
     return NQH_OK;
@@ -1898,8 +1911,8 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
 
 static nqh_status check_solve(nqh_ctx* c) {
     int info = 0, flag = 0;
-    HIPCHK(c, hipMemcpy(&info, c->info.p, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, &info, c->info.p, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, &flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (info != 0) {
         char buf[128];
         std::snprintf(buf, sizeof(buf), "S + shift is not positive definite (column %d): no step applied", info);
@@ -1918,7 +1931,7 @@ nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
         // a non-finite loss (NaN in L_loc or in the network) is reported before any solve
         double loss = 0.0;
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(&loss, c->stats.p, sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, &loss, c->stats.p, sizeof(double), hipMemcpyDeviceToHost));
         if (!std::isfinite(loss)) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no solve");
     }
     nqh_status s = precondition_async(c, eps, shift_kind, refine);
@@ -1930,12 +1943,12 @@ nqh_status nqh_precondition(nqh_ctx* c, double eps, int32_t shift_kind, int32_t 
     if (delta_out && is_rbm(c)) {
         const int npc = c->rbm_npc;
         std::vector<double> dv(2 * npc);
-        HIPCHK(c, hipMemcpy(dv.data(), c->delta.p, sizeof(double) * 2 * npc, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, dv.data(), c->delta.p, sizeof(double) * 2 * npc, hipMemcpyDeviceToHost));
         for (int k = 0; k < npc; ++k) { delta_out[2 * k] = dv[k]; delta_out[2 * k + 1] = dv[npc + k]; }
     } else if (delta_out) {
         const ndm_dims& d = c->d;
         std::vector<double> dv(d.np);
-        HIPCHK(c, hipMemcpy(dv.data(), c->delta.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, dv.data(), c->delta.p, sizeof(double) * d.np, hipMemcpyDeviceToHost));
         for (int k = 0; k < d.np; ++k) delta_out[p2ref_host(k, d)] = dv[k];
     }
     return NQH_OK;
@@ -2084,7 +2097,7 @@ static nqh_status sr_finish(nqh_ctx* c, double* stats) {
     phase_collect(c);
     c->solve_ok = false;
     double st[5];
-    HIPCHK(c, hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
     if (stats) std::memcpy(stats, st, sizeof(st));
     if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no step applied");
     nqh_status s;
@@ -2334,7 +2347,7 @@ nqh_status nqh_get_samples(nqh_ctx* c, uint64_t* keys_out, int64_t n) {
     (void)hipSetDevice(c->device);
     if (!c->has_sampler) return fail(c, NQH_E_STATE, "sampler not set");
     if (n != c->chains * c->chain_len) return fail(c, NQH_E_ARG, "n must be chains * chain_length");
-    HIPCHK(c, hipMemcpy(keys_out, c->samples.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, keys_out, c->samples.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -2420,7 +2433,7 @@ nqh_status nqh_exact(nqh_ctx* c, double* out, double* logp_out) {
     }
     std::vector<double> part(5 * nch);
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
-    if (err == hipSuccess) err = hipMemcpy(part.data(), pb.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = copy_sync(c->stream, part.data(), pb.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost);
     cleanup();
     if (err != hipSuccess) return hip_fail(c, err, "nqh_exact");
     double M = -INFINITY;
@@ -2476,15 +2489,15 @@ nqh_status nqh_save_state(nqh_ctx* c, void* buf, int64_t bytes) {
                                     c->made_ready ? c->ring_cap : 0, c->ring_fill, c->ring_pos};
     std::memcpy(h, hdr, sizeof(hdr));
     unsigned char* q = static_cast<unsigned char*>(buf) + sizeof(hdr);
-    HIPCHK(c, hipMemcpy(q, c->chain_state.p, sizeof(uint64_t) * c->chains, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, q, c->chain_state.p, sizeof(uint64_t) * c->chains, hipMemcpyDeviceToHost));
     q += sizeof(uint64_t) * c->chains;
     if (c->made_ready) {
         const size_t fb = sizeof(float) * c->made_P;
-        HIPCHK(c, hipMemcpy(q, c->made_th.p, fb, hipMemcpyDeviceToHost));
-        HIPCHK(c, hipMemcpy(q + fb, c->made_m.p, fb, hipMemcpyDeviceToHost));
-        HIPCHK(c, hipMemcpy(q + 2 * fb, c->made_v.p, fb, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, q, c->made_th.p, fb, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, q + fb, c->made_m.p, fb, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, q + 2 * fb, c->made_v.p, fb, hipMemcpyDeviceToHost));
         q += 3 * fb;
-        if (c->ring_cap > 0) HIPCHK(c, hipMemcpy(q, c->ring.p, sizeof(uint64_t) * c->ring_cap, hipMemcpyDeviceToHost));
+        if (c->ring_cap > 0) HIPCHK(c, copy_sync(c->stream, q, c->ring.p, sizeof(uint64_t) * c->ring_cap, hipMemcpyDeviceToHost));
     }
     return NQH_OK;
 }
@@ -2507,24 +2520,24 @@ nqh_status nqh_load_state(nqh_ctx* c, const void* buf, int64_t bytes) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graph(c);
     const unsigned char* q = static_cast<const unsigned char*>(buf) + sizeof(hdr);
-    HIPCHK(c, hipMemcpy(c->chain_state.p, q, sizeof(uint64_t) * c->chains, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c->stream, c->chain_state.p, q, sizeof(uint64_t) * c->chains, hipMemcpyHostToDevice));
     q += sizeof(uint64_t) * c->chains;
     c->chains_initialised = hdr[2] != 0;
     c->iter = (uint32_t)hdr[3];
     if (P > 0) {
         const size_t fb = sizeof(float) * P;
-        HIPCHK(c, hipMemcpy(c->made_th.p, q, fb, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->made_m.p, q + fb, fb, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->made_v.p, q + 2 * fb, fb, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c->stream, c->made_th.p, q, fb, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c->stream, c->made_m.p, q + fb, fb, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c->stream, c->made_v.p, q + 2 * fb, fb, hipMemcpyHostToDevice));
         q += 3 * fb;
         c->made_t = (int)hdr[5];
         c->made_calls = hdr[6];
         c->ring_fill = hdr[8];
         c->ring_pos = hdr[9];
         if (c->ring_cap > 0) {
-            HIPCHK(c, hipMemcpy(c->ring.p, q, sizeof(uint64_t) * c->ring_cap, hipMemcpyHostToDevice));
+            HIPCHK(c, copy_sync(c->stream, c->ring.p, q, sizeof(uint64_t) * c->ring_cap, hipMemcpyHostToDevice));
             const int64_t pos[2] = {c->ring_pos, c->ring_fill};
-            HIPCHK(c, hipMemcpy(c->ring_posd.p, pos, sizeof(pos), hipMemcpyHostToDevice));
+            HIPCHK(c, copy_sync(c->stream, c->ring_posd.p, pos, sizeof(pos), hipMemcpyHostToDevice));
         }
     }
     return NQH_OK;
@@ -2683,7 +2696,7 @@ nqh_status nqh_observable(nqh_ctx* c, const nqh_optable* op, int64_t chains, int
     }
     HIPCHK(c, nq_launch_stats_cplx(c->obs_L.p, fp64_net(c), chains, chain_length, P<double>(c->obs_stats), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(stats_out, c->obs_stats.p, sizeof(double) * 6, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, stats_out, c->obs_stats.p, sizeof(double) * 6, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -2692,7 +2705,7 @@ nqh_status nqh_get_observable_samples(nqh_ctx* c, uint64_t* keys_out, double* lo
     if (!c || n < 0) return NQH_E_ARG;
     (void)hipSetDevice(c->device);
     if ((size_t)n * sizeof(uint64_t) > c->obs_keys.bytes) return fail(c, NQH_E_ARG, "n exceeds the last observable's samples");
-    if (keys_out) HIPCHK(c, hipMemcpy(keys_out, c->obs_keys.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    if (keys_out) HIPCHK(c, copy_sync(c->stream, keys_out, c->obs_keys.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
     if (local_out) return download_cplx(c, c->obs_L, local_out, 2 * n);
     return NQH_OK;
 }
@@ -2702,7 +2715,7 @@ nqh_status nqh_get_stats(nqh_ctx* c, double* out, int32_t n) {
     (void)hipSetDevice(c->device);
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no statistics yet");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, c->stats.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, out, c->stats.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -2790,10 +2803,10 @@ nqh_status nqh_made_setup(nqh_ctx* c, double mix, int32_t retrain_every, int32_t
     if (buffer_cap > 0 && (s = ensure(c, c->ring, sizeof(uint64_t) * buffer_cap))) return s;
     std::vector<float> th;
     made_init_host(D, c->seed, th);
-    HIPCHK(c, hipMemcpy(c->made_th.p, th.data(), sizeof(float) * P, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->made_m.p, 0, sizeof(float) * P));
-    HIPCHK(c, hipMemset(c->made_v.p, 0, sizeof(float) * P));
-    HIPCHK(c, hipMemset(c->ring_posd.p, 0, sizeof(int64_t) * 2));
+    HIPCHK(c, copy_sync(c->stream, c->made_th.p, th.data(), sizeof(float) * P, hipMemcpyHostToDevice));
+    HIPCHK(c, zero_sync(c->stream, c->made_m.p, sizeof(float) * P));
+    HIPCHK(c, zero_sync(c->stream, c->made_v.p, sizeof(float) * P));
+    HIPCHK(c, zero_sync(c->stream, c->ring_posd.p, sizeof(int64_t) * 2));
     HIPCHK(c, nq_made_prepare());
     c->made_D = D;
     c->made_P = P;
@@ -2817,9 +2830,9 @@ nqh_status nqh_made_set_params(nqh_ctx* c, const float* theta, int64_t n) {
     if ((s = need_made(c))) return s;
     if (n != c->made_P) return fail(c, NQH_E_ARG, "parameter count mismatch (nqh_made_num_params)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->made_th.p, theta, sizeof(float) * n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->made_m.p, 0, sizeof(float) * n));
-    HIPCHK(c, hipMemset(c->made_v.p, 0, sizeof(float) * n));
+    HIPCHK(c, copy_sync(c->stream, c->made_th.p, theta, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, zero_sync(c->stream, c->made_m.p, sizeof(float) * n));
+    HIPCHK(c, zero_sync(c->stream, c->made_v.p, sizeof(float) * n));
     c->made_t = 0;
     return NQH_OK;
 }
@@ -2831,7 +2844,7 @@ nqh_status nqh_made_get_params(nqh_ctx* c, float* theta, int64_t n) {
     if ((s = need_made(c))) return s;
     if (n != c->made_P) return fail(c, NQH_E_ARG, "parameter count mismatch (nqh_made_num_params)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(theta, c->made_th.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, theta, c->made_th.p, sizeof(float) * n, hipMemcpyDeviceToHost));
     return NQH_OK;
 }
 
@@ -2863,10 +2876,10 @@ nqh_status nqh_made_grad(nqh_ctx* c, const uint64_t* keys, int64_t B, float* gra
     HIPCHK(c, nq_launch_made_sum_parts(P<float>(c->made_gpart), (int)parts, c->made_D, P<float>(c->made_g),
                                        P<float>(c->made_lpart), P<float>(c->made_nll), B, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(grad, c->made_g.p, sizeof(float) * c->made_P, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c->stream, grad, c->made_g.p, sizeof(float) * c->made_P, hipMemcpyDeviceToHost));
     if (nll) {
         float v = 0.f;
-        HIPCHK(c, hipMemcpy(&v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, &v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
         *nll = v;
     }
     return NQH_OK;
@@ -2914,7 +2927,7 @@ nqh_status nqh_made_train(nqh_ctx* c, const uint64_t* keys, int64_t n, int32_t s
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (nll_last) {
         float v = 0.f;
-        HIPCHK(c, hipMemcpy(&v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, &v, c->made_nll.p, sizeof(float), hipMemcpyDeviceToHost));
         *nll_last = v;
     }
     return NQH_OK;
@@ -2932,7 +2945,7 @@ nqh_status nqh_made_logq(nqh_ctx* c, const uint64_t* keys, int64_t n, float* log
     hipError_t e = nq_launch_made_logq(P<float>(c->made_th), c->made_D, P<uint64_t>(c->made_tmp), n, P<float>(out),
                                        c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(logq, out.p, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_sync(c->stream, logq, out.p, sizeof(float) * n, hipMemcpyDeviceToHost);
     release(out);
     if (e != hipSuccess) return hip_fail(c, e, "nqh_made_logq");
     return NQH_OK;
@@ -2950,8 +2963,8 @@ nqh_status nqh_made_sample(nqh_ctx* c, int64_t n, uint32_t iter, uint32_t tag, u
     hipError_t e = nq_launch_made_sample(P<float>(c->made_th), c->made_D, n, c->seed, iter, nullptr, tag,
                                          P<uint64_t>(kd), P<float>(lq), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(keys, kd.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && logq) e = hipMemcpy(logq, lq.p, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_sync(c->stream, keys, kd.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && logq) e = copy_sync(c->stream, logq, lq.p, sizeof(float) * n, hipMemcpyDeviceToHost);
     release(kd);
     release(lq);
     if (e != hipSuccess) return hip_fail(c, e, "nqh_made_sample");
@@ -2965,7 +2978,7 @@ nqh_status nqh_made_stats(nqh_ctx* c, int64_t* proposed, int64_t* accepted, int6
     if (proposed) *proposed = c->made_proposed;
     if (accepted) {
         unsigned long long v[2] = {0, 0};
-        if (c->acc.p) HIPCHK(c, hipMemcpy(v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
+        if (c->acc.p) HIPCHK(c, copy_sync(c->stream, v, c->acc.p, sizeof(v), hipMemcpyDeviceToHost));
         *accepted = (int64_t)v[1];
     }
     if (buffer_fill) *buffer_fill = c->ring_fill;
