@@ -216,46 +216,57 @@ __global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_
     const __bf16* yb0 = Y + (int64_t)min(q0 + row0, n - 1) * nsp + ch * 8;
     const __bf16* yb1 = Y + (int64_t)min(q0 + row0 + 64, n - 1) * nsp + ch * 8;
     const u4 z4 = u4{0u, 0u, 0u, 0u};
-    u4 pa0 = z4, pa1 = z4, pb0 = z4, pb1 = z4;
-    auto load = [&](int64_t k0) {   // unconditional (clamped rows), zeroed when out of range
-        const u4 a0 = *reinterpret_cast<const u4*>(ya0 + k0), a1 = *reinterpret_cast<const u4*>(ya1 + k0);
-        const u4 b0 = *reinterpret_cast<const u4*>(yb0 + k0), b1 = *reinterpret_cast<const u4*>(yb1 + k0);
-        pa0 = va0 ? a0 : z4;
-        pa1 = va1 ? a1 : z4;
-        pb0 = vb0 ? b0 : z4;
-        pb1 = vb1 ? b1 : z4;
+    // k-steps are loaded PFD steps ahead into a register ring: raw loads from
+    // clamped rows and steps (no select on a loaded value, which would wait
+    // for it at once), zeroed for rows past the block when staged in LDS
+    constexpr int PFD = 4;
+    u4 ra0[PFD], ra1[PFD], rb0[PFD], rb1[PFD];
+    auto load = [&](int step, int slot) {
+        const int64_t k0 = kbeg + (int64_t)min(step, max(nsteps - 1, 0)) * KS;
+        ra0[slot] = *reinterpret_cast<const u4*>(ya0 + k0);
+        ra1[slot] = *reinterpret_cast<const u4*>(ya1 + k0);
+        rb0[slot] = *reinterpret_cast<const u4*>(yb0 + k0);
+        rb1[slot] = *reinterpret_cast<const u4*>(yb1 + k0);
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, int slot) {
         __bf16* A = As + buf * BUF;
         __bf16* B = Bs + buf * BUF;
-        *reinterpret_cast<u4*>(&A[row0 * PITCH + ch * 8]) = pa0;
-        *reinterpret_cast<u4*>(&A[(row0 + 64) * PITCH + ch * 8]) = pa1;
-        *reinterpret_cast<u4*>(&B[row0 * PITCH + ch * 8]) = pb0;
-        *reinterpret_cast<u4*>(&B[(row0 + 64) * PITCH + ch * 8]) = pb1;
+        *reinterpret_cast<u4*>(&A[row0 * PITCH + ch * 8]) = va0 ? ra0[slot] : z4;
+        *reinterpret_cast<u4*>(&A[(row0 + 64) * PITCH + ch * 8]) = va1 ? ra1[slot] : z4;
+        *reinterpret_cast<u4*>(&B[row0 * PITCH + ch * 8]) = vb0 ? rb0[slot] : z4;
+        *reinterpret_cast<u4*>(&B[(row0 + 64) * PITCH + ch * 8]) = vb1 ? rb1[slot] : z4;
     };
     if (nsteps > 0) {
-        load(kbeg);
-        store(0);
-        if (nsteps > 1) load(kbeg + KS);
+#pragma unroll
+        for (int u = 0; u < PFD; ++u) load(u, u);
+        store(0, 0);
         __syncthreads();
-        for (int i = 0; i < nsteps; ++i) {
-            const __bf16* A = As + (i & 1) * BUF;
-            const __bf16* B = Bs + (i & 1) * BUF;
-            bf8 av[4], bv[4];
+        // whole rounds of PFD steps (LDS buffer i & 1 = u & 1, ring slot i %
+        // PFD = u); the products of steps past nsteps are skipped
+        for (int i0 = 0; i0 < nsteps; i0 += PFD) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-                av[a] = *reinterpret_cast<const bf8*>(&A[(wr * 64 + a * 16 + li) * PITCH + lk * 8]);
+            for (int u = 0; u < PFD; ++u) {
+                const int i = i0 + u;
+                const __bf16* A = As + (u & 1) * BUF;
+                const __bf16* B = Bs + (u & 1) * BUF;
+                bf8 av[4], bv[4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
-                bv[b] = *reinterpret_cast<const bf8*>(&B[(wc * 64 + b * 16 + li) * PITCH + lk * 8]);
-            if (i + 1 < nsteps) store((i + 1) & 1);       // step i + 1, loaded one step ago
-            if (i + 2 < nsteps) load(kbeg + (int64_t)(i + 2) * KS);
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < 4; ++a)
+                    av[a] = *reinterpret_cast<const bf8*>(&A[(wr * 64 + a * 16 + li) * PITCH + lk * 8]);
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
-            __syncthreads();
+                    bv[b] = *reinterpret_cast<const bf8*>(&B[(wc * 64 + b * 16 + li) * PITCH + lk * 8]);
+                store((u + 1) & 1, (u + 1) % PFD);   // step i + 1, loaded PFD - 1 steps ago
+                load(i + PFD, u);                    // slot u (step i) was staged one step ago
+                if (i < nsteps) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+                }
+                __syncthreads();
+            }
         }
     }
     float* P = part + ((int64_t)blockIdx.y * tiles + tile) * (128 * 128);
