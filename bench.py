@@ -128,16 +128,20 @@ def build_instances(cfg=C5):
 
 
 def time_instances(steps, warmup, cfg=C5):
-    """Aggregate SR iterations/s of the batched instances, three ways: one
+    """Aggregate SR iterations/s of the batched instances, four ways: one
     after another (each member replaying its own graph), all members queued
-    on their own streams then awaited, and one batch graph per iteration."""
+    on their own streams then awaited, one graph per iteration holding every
+    member's own launches, and the packed form (one launch per kernel for all
+    members, one graph per iteration)."""
     eps, lr = cfg["eps"], cfg["lr"]
     res, losses = {}, {}
-    for mode in ("sequential", "streams", "batch_graph"):
+    launches = None
+    for mode in ("sequential", "streams", "batch_graph", "batch_packed"):
         members, ns = build_instances(cfg)
         batch = None
-        if mode == "batch_graph":
+        if mode.startswith("batch"):
             batch = _lib.Batch(members)
+            batch.set_mode(_lib.BATCH_GRAPH if mode == "batch_graph" else _lib.BATCH_PACKED)
         else:
             for c in members:
                 c.set_graph(True)
@@ -161,11 +165,14 @@ def time_instances(steps, warmup, cfg=C5):
         dt = time.perf_counter() - t0
         res[mode] = cfg["instances"] * steps / dt
         losses[mode] = st[:, 0].copy()
+        if mode == "batch_packed":
+            launches = batch.launches()
         if batch is not None:
             batch.close()
         for c in members:
             c.close()
-    d = float(np.max(np.abs(losses["batch_graph"] - losses["sequential"]) / np.abs(losses["sequential"])))
+    d = max(float(np.max(np.abs(losses[m] - losses["sequential"]) / np.abs(losses["sequential"])))
+            for m in ("batch_graph", "batch_packed"))
     np_total = int(sum(5 * n + 4 * n * n for n in ns))
     return {"workload": (f"{cfg['instances']} dissipative Ising chains (periodic, V=2, gamma=1), N~U{{6..16}}, "
                          f"g~U[0.5,2] (seed {cfg['seed']}), NDM alpha=1 (sum Np = {np_total}), "
@@ -174,6 +181,7 @@ def time_instances(steps, warmup, cfg=C5):
             "unit": "SR iters/s summed over instances",
             "aggregate_it_s": {k: round(v, 1) for k, v in res.items()},
             "steps": steps, "warmup": warmup,
+            "packed_launches_per_iteration": launches,
             "max_rel_loss_diff_batch_vs_sequential": d}
 
 

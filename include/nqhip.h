@@ -288,6 +288,23 @@ const char* nqh_batch_last_error(const nqh_batch* batch);
 nqh_status nqh_batch_sr_iteration(nqh_batch* batch, double eps, int32_t shift_kind, int32_t refine, double lr,
                                   double* stats);
 
+/* How a batch runs its iterations.  NQH_BATCH_PACKED (the default): when
+ * every member qualifies (NDM network, Cholesky solver with refine = 0, BF16
+ * S build, nh <= 32 and na <= 32, no learned proposal), each kernel of the
+ * iteration is launched ONCE for all members: one grid holds every member's
+ * workgroups, each running exactly the blocks of the member's own launch
+ * with the member's arguments from a device descriptor, so a member's
+ * results are bit-identical to its own run; the launches (about 30 plus the
+ * longest member's Cholesky panels and substitution groups) form the
+ * batch's graph.  Otherwise, and with NQH_BATCH_GRAPH, every member's own
+ * launches are captured.  nqh_batch_launches: kernel launches per iteration
+ * of the packed graph, 0 for the per-member graph, -1 before the first
+ * captured iteration.  No reference counterpart. */
+#define NQH_BATCH_PACKED 0
+#define NQH_BATCH_GRAPH 1
+nqh_status nqh_batch_set_mode(nqh_batch* batch, int32_t mode);
+int32_t nqh_batch_launches(const nqh_batch* batch);
+
 /* device time per phase, measured with HIP events on the context's stream
  * while profiling is on (accumulated, in ms; counts = timed occurrences):
  * [0] sampler  [1] estimator (log rho, O, L_loc, force)  [2] means, statistics,

@@ -79,8 +79,9 @@ template <typename T> constexpr int est_batch() { return sizeof(T) == 8 ? 4 : 8;
 
 // TAB_LDS: the connection table is staged in LDS (a compile-time choice, so
 // every table access is a known LDS or global access, never a flat one)
+// the block body; bx = the workgroup's number in its problem's grid of gx
 template <typename T, int NMAX, int RS, int PS, bool TAB_LDS, int KIND, int ACT>
-__global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD) ? 4 : 2) void k_estimate(est_args a) {
+__device__ __forceinline__ void estimate_body(const est_args& a, int bx, int gx) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool WALK = KIND == EK_WALK, FORCE = KIND == EK_FORCE, GRAD = KIND == EK_GRAD;
     using vN = T __attribute__((ext_vector_type(NMAX)));
@@ -107,8 +108,8 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD)
     const int lane0 = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    const int64_t gw = (int64_t)blockIdx.x * wpb + wave;
-    const int64_t nw = (int64_t)gridDim.x * wpb;
+    const int64_t gw = (int64_t)bx * wpb + wave;
+    const int64_t nw = (int64_t)gx * wpb;
     const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
     cplx<T>* wbuf = sW + wave * a.max_conn;
     uint2* mbuf = sM + wave * a.max_conn;
@@ -715,10 +716,36 @@ __global__ __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD)
 
     if constexpr (FORCE) {
         __syncthreads();
-        double* out = a.Fpart + (int64_t)blockIdx.x * d.np;
+        double* out = a.Fpart + (int64_t)bx * d.np;
         const bool bad = *poison != 0;
         for (int k = threadIdx.x; k < d.np; k += blockDim.x) out[k] = bad ? __builtin_nan("") : fx_value(sF, k);
     }
+}
+
+#define NQ_EST_BOUNDS __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD) ? 4 : 2)
+
+template <typename T, int NMAX, int RS, int PS, bool TAB_LDS, int KIND, int ACT>
+__global__ NQ_EST_BOUNDS void k_estimate(est_args a) {
+    estimate_body<T, NMAX, RS, PS, TAB_LDS, KIND, ACT>(a, blockIdx.x, gridDim.x);
+}
+
+// packed: the members' walk (KIND = EK_WALK) or force launches in one grid
+template <typename T, int NMAX, int RS, int PS, bool TAB_LDS, int KIND, int ACT>
+__global__ NQ_EST_BOUNDS void k_estimate_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = M[b.x & 0xffff];
+    const est_args a = KIND == EK_FORCE ? m.ef : m.ew;
+    estimate_body<T, NMAX, RS, PS, TAB_LDS, KIND, ACT>(a, (int)(b.y & 0xffff), m.est_grid);
+}
+#undef NQ_EST_BOUNDS
+
+// waves per workgroup of a launch: 8 share one copy of the parameters, the
+// force accumulator and the table (two blocks per CU while that fits in half
+// the LDS: 4 waves per SIMD); otherwise 4.  The parity mode (per-sample
+// gradients) runs one wave per block.
+inline int est_waves(const est_args& a, size_t elt) {
+    if (a.G) return 1;
+    return est_lds_bytes(a, elt, 8) > 80 * 1024 ? 4 : 8;
 }
 
 template <typename T, int NMAX, int RS, int PS, int ACT>
@@ -727,12 +754,7 @@ hipError_t launch_t(const est_args& a0, int grid, hipStream_t st) {
     const int kind = a.G ? EK_GRAD : (a.rec_mode == 2 ? EK_FORCE : EK_WALK);
     if (kind == EK_WALK && a.Fpart) return hipErrorInvalidValue;   // forces take a launch of their own
     if (kind == EK_FORCE) a.tab_lds = 0;                             // the records replace the walk
-    // 8 waves per block share one copy of the parameters, the force
-    // accumulator and the table (two blocks per CU while that fits in half
-    // the LDS: 4 waves per SIMD); otherwise 4.  The parity mode (per-sample
-    // gradients) runs one wave per block.
-    int wpb = a.G ? 1 : 8;
-    if (!a.G && est_lds_bytes(a, sizeof(T), wpb) > 80 * 1024) wpb = 4;
+    const int wpb = est_waves(a, sizeof(T));
     const size_t lds = est_lds_bytes(a, sizeof(T), wpb);
     if (kind == EK_FORCE)
         hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_FORCE, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
@@ -767,6 +789,30 @@ hipError_t launch_prec(const est_args& a, int grid, hipStream_t st) {
     }
     if (a.d.N <= 8) return launch_n<T, 8, NQ_ACT_SOFTPLUS>(a, grid, st);
     return launch_n<T, 16, NQ_ACT_SOFTPLUS>(a, grid, st);
+}
+
+// packed launches of one precision; variant bits: 0 force (else walk), 1 N > 8
+// (NMAX 16), 2 table in LDS, 3 logcosh (the walk and force kinds, one unit
+// row and one Pi row per lane only)
+template <typename T, int NMAX, int ACT>
+hipError_t pk_launch_n(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    const dim3 g((unsigned)L.nblk), b((unsigned)L.threads);
+    const bool force = (L.variant & 1) != 0, tl = (L.variant & 4) != 0;
+    if (force)
+        hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, false, EK_FORCE, ACT>), g, b, L.lds, st, M, L.blk);
+    else if (tl)
+        hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, true, EK_WALK, ACT>), g, b, L.lds, st, M, L.blk);
+    else
+        hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, false, EK_WALK, ACT>), g, b, L.lds, st, M, L.blk);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t pk_launch_prec(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    if (L.nblk <= 0) return hipSuccess;
+    const bool n16 = (L.variant & 2) != 0, lc = (L.variant & 8) != 0;
+    if (lc) return n16 ? pk_launch_n<T, 16, NQ_ACT_LOGCOSH>(M, L, st) : pk_launch_n<T, 8, NQ_ACT_LOGCOSH>(M, L, st);
+    return n16 ? pk_launch_n<T, 16, NQ_ACT_SOFTPLUS>(M, L, st) : pk_launch_n<T, 8, NQ_ACT_SOFTPLUS>(M, L, st);
 }
 
 }  // namespace nq_est

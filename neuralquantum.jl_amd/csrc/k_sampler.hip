@@ -27,8 +27,9 @@ namespace {
 template <typename T>
 __device__ __forceinline__ T xv(uint32_t bits, int i) { return ((bits >> i) & 1u) ? T(1) : T(-1); }
 
+// the block body; bx = the workgroup's number in its problem's grid
 template <typename T, int US, int ACT>
-__global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
+__device__ __forceinline__ void metropolis_body(const samp_args& a, int bx) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ndm_dims d = a.d;
     const int N = d.N, nh = d.nh, na = d.na;
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     const int g = lane & (lpc - 1);
     const int gbase = lane - g;
     const int cpw = 64 / lpc;
-    const int64_t wave_glob = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t wave_glob = (int64_t)bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t chain = wave_glob * cpw + lane / lpc;
     const bool live = chain < a.chains;
     const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
@@ -188,12 +189,31 @@ __global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
     }
 }
 
-template <typename T, int US>
-hipError_t launch_t(const samp_args& a, hipStream_t st) {
+template <typename T, int US, int ACT>
+__global__ __launch_bounds__(256) void k_metropolis(samp_args a) {
+    metropolis_body<T, US, ACT>(a, blockIdx.x);
+}
+
+// packed: the members' sampler launches in one grid (block records as in nq_pk_member)
+template <typename T, int US, int ACT>
+__global__ __launch_bounds__(256) void k_metropolis_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const samp_args a = M[b.x & 0xffff].sa;
+    metropolis_body<T, US, ACT>(a, (int)(b.y & 0xffff));
+}
+
+constexpr int kSampWaves = 4;   // wavefronts per workgroup
+
+int64_t samp_grid(const samp_args& a) {
     const int cpw = 64 / a.lpc;
     const int64_t waves = (a.chains + cpw - 1) / cpw;
-    const int wpb = 4;
-    const int64_t grid = (waves + wpb - 1) / wpb;
+    return (waves + kSampWaves - 1) / kSampWaves;
+}
+
+template <typename T, int US>
+hipError_t launch_t(const samp_args& a, hipStream_t st) {
+    const int wpb = kSampWaves;
+    const int64_t grid = samp_grid(a);
     size_t lds = sizeof(T) * (size_t)a.d.np;
     if (a.d.act == NQ_ACT_LOGCOSH)
         hipLaunchKernelGGL((k_metropolis<T, US, NQ_ACT_LOGCOSH>), dim3((unsigned)grid), dim3(64 * wpb), lds, st, a);
@@ -225,4 +245,30 @@ hipError_t nq_launch_metropolis(const samp_args& a, bool fp64, hipStream_t st) {
         if (us <= 4) return launch_t<float, 4>(a, st);
     }
     return hipErrorInvalidValue;
+}
+
+// packed launches: one lane group per chain covers every unit (lanes per
+// chain >= max(nh, na), so one unit slot per lane); variant = fp64 | logcosh << 1
+int nq_sampler_variant(const samp_args& a, bool fp64, int* threads, size_t* lds, int* grid) {
+    const int m = a.d.nh > a.d.na ? a.d.nh : a.d.na;
+    if (a.d.N > 32 || a.lpc < 1 || a.lpc > 64 || (a.lpc & (a.lpc - 1)) || m > a.lpc) return -1;
+    const int64_t g = samp_grid(a);
+    if (g > 0xffff) return -1;
+    *threads = 64 * kSampWaves;
+    *lds = (fp64 ? sizeof(double) : sizeof(float)) * (size_t)a.d.np;
+    *grid = (int)g;
+    return (fp64 ? 1 : 0) | (a.d.act == NQ_ACT_LOGCOSH ? 2 : 0);
+}
+
+hipError_t nq_pk_sampler(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    if (L.nblk <= 0) return hipSuccess;
+    const dim3 g((unsigned)L.nblk), b((unsigned)L.threads);
+    switch (L.variant) {
+        case 0: hipLaunchKernelGGL((k_metropolis_pk<float, 1, NQ_ACT_SOFTPLUS>), g, b, L.lds, st, M, L.blk); break;
+        case 1: hipLaunchKernelGGL((k_metropolis_pk<double, 1, NQ_ACT_SOFTPLUS>), g, b, L.lds, st, M, L.blk); break;
+        case 2: hipLaunchKernelGGL((k_metropolis_pk<float, 1, NQ_ACT_LOGCOSH>), g, b, L.lds, st, M, L.blk); break;
+        case 3: hipLaunchKernelGGL((k_metropolis_pk<double, 1, NQ_ACT_LOGCOSH>), g, b, L.lds, st, M, L.blk); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }

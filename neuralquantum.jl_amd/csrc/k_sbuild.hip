@@ -34,7 +34,7 @@ __device__ __forceinline__ void tri_decode(int b, int* I, int* J) {
 }
 
 template <int MODE, typename YT>
-__global__ __launch_bounds__(256) void k_gram(gram_args a) {
+__device__ __forceinline__ void gram_body(const gram_args& a, int bx) {
     typedef typename gram_traits<MODE>::st ST;
     typedef typename gram_traits<MODE>::acc AT;
     constexpr int KC = gram_traits<MODE>::KC;
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
     __shared__ ST Bs[2 * TILE];
 
     int I, J;
-    tri_decode(blockIdx.x, &I, &J);
+    tri_decode(bx, &I, &J);
     const int r0 = I * 64, q0 = J * 64;
     const YT* Y = reinterpret_cast<const YT*>(a.Y);
     const int tid = threadIdx.x;
@@ -140,12 +140,12 @@ __global__ __launch_bounds__(256) void k_gram(gram_args a) {
 // ---- BF16 path: centred, transposed copy Yt[c][s] = bf16(Y[s][c0 + c] - mu),
 // samples padded with zeros to a multiple of 64, then a 128x128-tile Gram.
 template <typename YT>
-__global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int ldy, int c0, int n,
-                                                  const double* mu, __bf16* Yt, int64_t nsp) {
+__device__ __forceinline__ void center_t_body(const YT* Y, int64_t Ns, int ldy, int c0, int n, const double* mu,
+                                              __bf16* Yt, int64_t nsp, int bx, int by) {
     __shared__ float tile[64][65];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int cb = blockIdx.x * 64;
-    const int64_t sb = (int64_t)blockIdx.y * 64;
+    const int cb = bx * 64;
+    const int64_t sb = (int64_t)by * 64;
     const int c = cb + tx;
     const double m = (c < n && mu) ? mu[c0 + c] : 0.0;
 #pragma unroll
@@ -176,13 +176,13 @@ __global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int l
 // S[offB, offB + nB) (adding where they overlap), writes every element of
 // the (np + 1)^2 matrix (zeros outside the blocks) and mirrors the lower
 // triangle through an LDS transpose so both writes are coalesced.
-__global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_t nsp, int nA, int TA, int nB,
-                                                        int64_t kchunk, int tiles, float* part) {
+__device__ __forceinline__ void gram_bf16_part_body(const __bf16* Yt, int64_t nsp, int nA, int TA, int nB,
+                                                    int64_t kchunk, int tiles, float* part, int bx, int by) {
     constexpr int KS = 32, PITCH = 40, BUF = 128 * PITCH;
     __shared__ __align__(16) __bf16 As[2 * BUF];
     __shared__ __align__(16) __bf16 Bs[2 * BUF];
     const int tilesA = TA * (TA + 1) / 2;
-    const int tile = blockIdx.x;
+    const int tile = bx;
     const __bf16* Y = Yt;
     int n = nA, tl = tile;
     if (tile >= tilesA) {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_
     int I, J;
     tri_decode(tl, &I, &J);
     const int r0 = I * 128, q0 = J * 128;
-    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+    const int64_t kbeg = (int64_t)by * kchunk;
     const int64_t kend = (kbeg + kchunk < nsp) ? kbeg + kchunk : nsp;
     const int nsteps = kend > kbeg ? (int)((kend - kbeg) / KS) : 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_
             }
         }
     }
-    float* P = part + ((int64_t)blockIdx.y * tiles + tile) * (128 * 128);
+    float* P = part + ((int64_t)by * tiles + tile) * (128 * 128);
     const int lj = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -292,11 +292,11 @@ __device__ __forceinline__ int gram_part_off(int toff, int rr, int cc) {
     return (toff + J * (J + 1) / 2 + I) * (128 * 128) + x * 128 + y;
 }
 
-__global__ __launch_bounds__(256) void k_gram_finish(const float* part, int splits, int tiles, int nA, int TA, int nB,
-                                                     int offB, double scale, double* S, int np, int lds) {
+__device__ __forceinline__ void gram_finish_body(const float* part, int splits, int tiles, int nA, int TA, int nB,
+                                                 int offB, double scale, double* S, int np, int lds, int bx) {
     __shared__ double T[64][65];
     int bi, bj;
-    tri_decode(blockIdx.x, &bi, &bj);   // bi <= bj: 64 x 64 tiles of the (np + 1)^2 matrix
+    tri_decode(bx, &bi, &bj);   // bi <= bj: 64 x 64 tiles of the (np + 1)^2 matrix
     const int y = threadIdx.x & 63, x0 = threadIdx.x >> 6;
     const int c = bj * 64 + y;
     const int tilesA = TA * (TA + 1) / 2;
@@ -354,9 +354,9 @@ __global__ __launch_bounds__(256) void k_gram_finish(const float* part, int spli
 }
 
 template <typename YT>
-__global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, double* part, int chunks) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int chunk = blockIdx.y;
+__device__ __forceinline__ void colsum_body(const YT* Y, int64_t Ns, int R, double* part, int chunks, int bx, int by) {
+    const int c = bx * blockDim.x + threadIdx.x;
+    const int chunk = by;
     if (c >= R) return;
     const int64_t per = (Ns + chunks - 1) / chunks;
     const int64_t s0 = chunk * per, s1 = (s0 + per < Ns) ? s0 + per : Ns;
@@ -376,11 +376,11 @@ __global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, 
 
 // mu[c] = scale * sum_k part[k][c]: 64 columns per workgroup, the chunks split
 // over 4 thread groups (coalesced along the columns), combined in a fixed order
-__global__ __launch_bounds__(256) void k_colsum_finish(const double* part, int chunks, int R, double scale,
-                                                       double* mu) {
+__device__ __forceinline__ void colsum_finish_body(const double* part, int chunks, int R, double scale, double* mu,
+                                                   int bx) {
     __shared__ double red[4][64];
     const int x = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + x;
+    const int c = bx * 64 + x;
     const int cc = c < R ? c : R - 1;
     double a0 = 0.0, a1 = 0.0;
     int k = g;
@@ -392,6 +392,97 @@ __global__ __launch_bounds__(256) void k_colsum_finish(const double* part, int c
     red[g][x] = a0 + a1;
     __syncthreads();
     if (g == 0 && c < R) mu[c] = ((red[0][x] + red[1][x]) + (red[2][x] + red[3][x])) * scale;
+}
+
+template <int MODE, typename YT>
+__global__ __launch_bounds__(256) void k_gram(gram_args a) {
+    gram_body<MODE, YT>(a, blockIdx.x);
+}
+
+template <typename YT>
+__global__ __launch_bounds__(256) void k_center_t(const YT* Y, int64_t Ns, int ldy, int c0, int n,
+                                                  const double* mu, __bf16* Yt, int64_t nsp) {
+    center_t_body<YT>(Y, Ns, ldy, c0, n, mu, Yt, nsp, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_gram_bf16_part(const __bf16* Yt, int64_t nsp, int nA, int TA, int nB,
+                                                        int64_t kchunk, int tiles, float* part) {
+    gram_bf16_part_body(Yt, nsp, nA, TA, nB, kchunk, tiles, part, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish(const float* part, int splits, int tiles, int nA, int TA, int nB,
+                                                     int offB, double scale, double* S, int np, int lds) {
+    gram_finish_body(part, splits, tiles, nA, TA, nB, offB, scale, S, np, lds, blockIdx.x);
+}
+
+template <typename YT>
+__global__ __launch_bounds__(256) void k_colsum(const YT* Y, int64_t Ns, int R, double* part, int chunks) {
+    colsum_body<YT>(Y, Ns, R, part, chunks, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_colsum_finish(const double* part, int chunks, int R, double scale,
+                                                       double* mu) {
+    colsum_finish_body(part, chunks, R, scale, mu, blockIdx.x);
+}
+
+// ---- packed forms (block records and descriptors as in nq_pk_member)
+__device__ __forceinline__ const nq_pk_member& pk_member(const nq_pk_member* M, uint2 b) { return M[b.x & 0xffff]; }
+
+// the FP64 / FP32 Gram block `which` of a member (build_s: block A then B, adding into S)
+__host__ __device__ inline gram_args pk_gram_args(const nq_pk_member& m, int which) {
+    gram_args g{};
+    g.Y = m.Y;
+    g.y_fp64 = m.fp64;
+    g.Ns = m.Ns;
+    g.ldy = m.R;
+    g.mu = m.mu;
+    g.S = m.S;
+    g.lds = m.ldS;
+    g.scale = 1.0 / (double)m.Ns;
+    g.c0 = which == 0 ? 0 : m.nA;
+    g.n = which == 0 ? m.nA : m.nB;
+    g.o = which == 0 ? 0 : m.offB;
+    return g;
+}
+
+template <int MODE, typename YT>
+__global__ __launch_bounds__(256) void k_gram_pk(const nq_pk_member* M, const uint2* blk, int which) {
+    const uint2 b = blk[blockIdx.x];
+    const gram_args a = pk_gram_args(pk_member(M, b), which);
+    gram_body<MODE, YT>(a, b.y & 0xffff);
+}
+
+template <typename YT>
+__global__ __launch_bounds__(256) void k_colsum_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_member(M, b);
+    colsum_body<YT>((const YT*)m.Y, m.Ns, m.R, m.colpart, m.col_chunks, b.y & 0xffff, b.y >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_colsum_finish_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_member(M, b);
+    colsum_finish_body(m.colpart, m.col_chunks, m.R, 1.0 / (double)m.Ns, m.mu, b.y & 0xffff);
+}
+
+template <typename YT>
+__global__ __launch_bounds__(256) void k_center_t_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_member(M, b);
+    center_t_body<YT>((const YT*)m.Y, m.Ns, m.R, 0, m.R, m.mu, m.Yt, m.nsp, b.y & 0xffff, b.y >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_gram_bf16_part_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_member(M, b);
+    gram_bf16_part_body(m.Yt, m.nsp, m.nA, m.TA, m.nB, m.kchunk, m.tiles, m.gpart, b.y & 0xffff, b.y >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_member(M, b);
+    gram_finish_body(m.gpart, m.splits, m.tiles, m.nA, m.TA, m.nB, m.offB, 1.0 / (double)m.Ns, m.S, m.np, m.ldS,
+                     b.y & 0xffff);
 }
 
 template <int MODE>
@@ -430,7 +521,13 @@ int nq_gram_split_plan(int nA, int nB, int64_t nsp, int64_t* kchunk) {
     const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
     const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
     int s = tiles > 0 ? (3 * 256 + tiles - 1) / tiles : 1;   // about three tiles in flight per CU
-    const int maxs = nsp >= 64 ? (int)(nsp / 64) : 1;
+    int maxs = nsp >= 64 ? (int)(nsp / 64) : 1;
+    // the partial sums (64 KB per tile and chunk) stay within twice the BF16
+    // input, so that small problems (many of them sharing the device) do not
+    // turn into partial-sum traffic
+    const int64_t in_bytes = 2 * nsp * (int64_t)(nA + nB);
+    const int64_t cap = tiles > 0 ? 2 * in_bytes / ((int64_t)tiles * 128 * 128 * 4) : 1;
+    if (cap < maxs) maxs = cap > 1 ? (int)cap : 1;
     s = s < maxs ? s : maxs;
     s = s < 64 ? s : 64;
     s = s > 1 ? s : 1;
@@ -478,4 +575,47 @@ hipError_t nq_launch_colsum_finish(const double* part, int chunks, int R, double
                                    hipStream_t st) {
     hipLaunchKernelGGL(k_colsum_finish, dim3((R + 63) / 64), dim3(256), 0, st, part, chunks, R, scale, mu);
     return hipGetLastError();
+}
+
+void nq_gram_shape(int nA, int nB, int64_t nsp, int* TA, int* tiles, int* splits, int64_t* kchunk) {
+    *splits = nq_gram_split_plan(nA, nB, nsp, kchunk);
+    *TA = (nA + 127) / 128;
+    const int TB = (nB + 127) / 128;
+    *tiles = *TA * (*TA + 1) / 2 + TB * (TB + 1) / 2;
+}
+
+hipError_t nq_pk_sbuild(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    if (L.nblk <= 0) return hipSuccess;
+    const dim3 g((unsigned)L.nblk), b((unsigned)L.threads);
+    const bool f64 = L.variant != 0;
+    switch (L.kernel) {
+        case NQ_PK_COLSUM:
+            if (f64) hipLaunchKernelGGL((k_colsum_pk<double>), g, b, 0, st, M, L.blk);
+            else hipLaunchKernelGGL((k_colsum_pk<float>), g, b, 0, st, M, L.blk);
+            break;
+        case NQ_PK_COLSUM_FIN: hipLaunchKernelGGL(k_colsum_finish_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_CENTER:
+            if (f64) hipLaunchKernelGGL((k_center_t_pk<double>), g, b, 0, st, M, L.blk);
+            else hipLaunchKernelGGL((k_center_t_pk<float>), g, b, 0, st, M, L.blk);
+            break;
+        case NQ_PK_GRAM_PART: hipLaunchKernelGGL(k_gram_bf16_part_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_GRAM_FIN: hipLaunchKernelGGL(k_gram_finish_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_GRAM:
+            // variant: S-build mode | FP64 rows << 1
+            switch (L.variant) {
+                case 0: hipLaunchKernelGGL((k_gram_pk<NQ_GRAM_F64, float>), g, b, 0, st, M, L.blk, L.param); break;
+                case 1: hipLaunchKernelGGL((k_gram_pk<NQ_GRAM_F32, float>), g, b, 0, st, M, L.blk, L.param); break;
+                case 2: hipLaunchKernelGGL((k_gram_pk<NQ_GRAM_F64, double>), g, b, 0, st, M, L.blk, L.param); break;
+                case 3: hipLaunchKernelGGL((k_gram_pk<NQ_GRAM_F32, double>), g, b, 0, st, M, L.blk, L.param); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int nq_pk_gram_blocks(const nq_pk_member& m, int which) {
+    const int T = (pk_gram_args(m, which).n + 63) / 64;
+    return T * (T + 1) / 2;
 }

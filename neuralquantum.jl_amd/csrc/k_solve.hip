@@ -48,7 +48,7 @@ __device__ double block_sum(double v, double* red) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_stats(const T* L, int64_t B, int64_t Lc, double* out) {
+__device__ __forceinline__ void stats_body(const T* L, int64_t B, int64_t Lc, double* out) {
     __shared__ double red[16];
     const double invL = 1.0 / (double)Lc;
     // pass 1: mean of chain means (= overall mean, chains have equal length)
@@ -90,9 +90,10 @@ __global__ __launch_bounds__(1024) void k_stats(const T* L, int64_t B, int64_t L
 }
 
 // stage 1: part2[g][k] = sum of Fpart rows g*per .. (g+1)*per - 1 (grid: k chunks x groups)
-__global__ void k_force_part(const double* Fpart, int nblk, int per, int np, double* part2) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = blockIdx.y;
+__device__ __forceinline__ void force_part_body(const double* Fpart, int nblk, int per, int np, double* part2, int bx,
+                                                int by) {
+    const int k = bx * blockDim.x + threadIdx.x;
+    const int g = by;
     if (k >= np) return;
     const int b0 = g * per, b1 = (b0 + per < nblk) ? b0 + per : nblk;
     double acc = 0.0;
@@ -101,9 +102,9 @@ __global__ void k_force_part(const double* Fpart, int nblk, int per, int np, dou
 }
 
 // stage 2: F[k] = sum_g part2[g][k] - C Re(Obar_k)
-__global__ void k_force_finish(const double* part2, int ngroups, const double* mu, const double* stats,
-                               ndm_dims d, double* F) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void force_finish_body(const double* part2, int ngroups, const double* mu,
+                                                  const double* stats, const ndm_dims& d, double* F, int bx) {
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k >= d.np) return;
     double acc = 0.0;
     for (int g = 0; g < ngroups; ++g) acc += part2[(int64_t)g * d.np + k];
@@ -250,11 +251,11 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
 
 // the first diagonal block of a column range [k0, ...): winv block 0;
 // blockIdx.y = instance (A and winv at the given strides)
-__global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info,
-                                                    int64_t sA, int64_t sW) {
+__device__ __forceinline__ void chol_diag32_body(double* A, int n, int lda, int k, int kb, double* winv, int* info,
+                                                 int64_t sA, int64_t sW, int by) {
     __shared__ __align__(16) double Ls[CB][64];
-    A += blockIdx.y * sA;
-    winv += blockIdx.y * sW;
+    A += by * sA;
+    winv += by * sW;
     factor_diag32<false>(A, lda, k, kb, n, winv, info, true, Ls, nullptr);
 }
 
@@ -415,18 +416,18 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
 // alone and only the panels' rows below are transformed, for one rank-n
 // update afterwards); winv block (k - k0) / 32 holds panel k; blockIdx.y =
 // instance (A, winv at strides)
-__global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int ncu, int lda, int k, int kb,
-                                                   double* winv, int* info, int k0, int64_t sA, int64_t sW) {
+__device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu, int lda, int k, int kb, double* winv,
+                                               int* info, int k0, int64_t sA, int64_t sW, int bx, int by) {
     __shared__ double Xi[CB][65];               // staged A_Ik: [m][row], then L_Ik
     __shared__ double Xj[CB][65];               // the same for A_Jk (off-diagonal tiles only)
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ __align__(16) double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block (look-ahead tile)
-    A += blockIdx.y * sA;
-    winv += blockIdx.y * sW;
+    A += by * sA;
+    winv += by * sW;
     const int o = k + kb;
     const double* w = winv + (size_t)((k - k0) / CB) * CB * CB;
-    if (blockIdx.x == 0 && o < n) {
+    if (bx == 0 && o < n) {
         // tile (0, 0): the look-ahead tile (Xi and Xj serve as its LDS)
         chol_lookahead(A, n, nr, ncu, lda, k, kb, w, winv + (size_t)((o - k0) / CB) * CB * CB, info,
                        reinterpret_cast<double (*)[CB + 1]>(&Xi[0][0]), reinterpret_cast<double (*)[CB + 1]>(&Xj[0][0]),
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
     int tI, tJ;
     {
         const int T = (nr - k - kb + 63) / 64, TJ = max(1, (ncu - k - kb + 63) / 64);
-        step_decode(blockIdx.x, T, TJ, &tI, &tJ);   // tJ <= tI
+        step_decode(bx, T, TJ, &tI, &tJ);   // tJ <= tI
     }
     const int i0 = o + tI * 64, j0 = o + tJ * 64;
     const bool dg = tI == tJ;                   // diagonal tile: A_Jk = A_Ik
@@ -574,18 +575,18 @@ __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int
 // lanes per row with BG CB / LPR columns each), their factor entries loaded
 // before the group solve so that the two overlap.  One launch instead of BG.
 constexpr int BG = 6, LPR = 8, BR = 256 / LPR, BCW = BG * CB / LPR;
-__global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, int k, int kend, int n,
-                                                    const double* winv, double* y, double* x, int k0, int64_t sA,
-                                                    int64_t sW, int64_t sY) {
+__device__ __forceinline__ void back_group_body(const double* A, int lda, int k, int kend, int n, const double* winv,
+                                                double* y, double* x, int k0, int64_t sA, int64_t sW, int64_t sY,
+                                                int bx, int by) {
     __shared__ double Lq[BG * (BG - 1) / 2][CB][CB + 1];   // (p, q) block pairs, q < p: [row in q][column in p]
     __shared__ double ys[BG * CB], xs[BG * CB];
-    A += blockIdx.y * sA;
-    y += blockIdx.y * sY;
-    x += blockIdx.y * sY;
+    A += by * sA;
+    y += by * sY;
+    x += by * sY;
     const int t = threadIdx.x;
     const int ng = kend - k;                       // columns of the group (<= BG * CB)
     const int npan = (ng + CB - 1) / CB;
-    const int i = blockIdx.x * BR + t / LPR, h = t % LPR;   // row below, its share of the columns
+    const int i = bx * BR + t / LPR, h = t % LPR;   // row below, its share of the columns
     // every load is issued before the first use: the group's upper-triangle
     // blocks (q < p), this thread's row of L_pp^-T (threads 32p .. 32p + 31),
     // y of the group, then the factor entries A[i, k + m] of the rows below
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
     const int pt = t / CB, c = t & (CB - 1);
     double wv[CB];
     if (pt < npan) {
-        const double* w = winv + blockIdx.y * sW + (size_t)((k + pt * CB - k0) / CB) * CB * CB;
+        const double* w = winv + by * sW + (size_t)((k + pt * CB - k0) / CB) * CB * CB;
 #pragma unroll
         for (int m = 0; m < CB; ++m) wv[m] = w[c + CB * m];
     }
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
         }
         __syncthreads();
     }
-    if (blockIdx.x == 0 && t < ng) x[k + t] = xs[t];
+    if (bx == 0 && t < ng) x[k + t] = xs[t];
     double acc = 0.0;
 #pragma unroll
     for (int m = 0; m < BCW; ++m) acc += a[m] * (h * BCW + m < ng ? xs[h * BCW + m] : 0.0);
@@ -705,23 +706,24 @@ __device__ __forceinline__ int split_p(int inst, int i, int nl, int npi) {
 
 // lower triangle (with the shift on the diagonal) and the right-hand side row
 // of both instances (m = nl + npi, lda = m + 1; instance 1 only its mu
-// columns: its Pi columns are never read); grid (rows, m, 2)
-__global__ void k_split_copy(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
-                             double* A2, int lda) {
+// columns: its Pi columns are never read); grid (m columns, 2 instances)
+// one workgroup per column j of an instance (rows j .. m)
+__device__ __forceinline__ void split_copy_body(const double* S, int lds, const double* F, int nl, int npi, double eps,
+                                                int kind, double* A2, int lda, int j, int inst) {
     const int m = nl + npi;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, inst = blockIdx.z;
-    if (i > m || i < j || (inst == 1 && j >= nl)) return;
+    if (inst == 1 && j >= nl) return;
     double* A = A2 + (int64_t)inst * lda * lda;
     const int pj = split_p(inst, j, nl, npi);
-    double v;
-    if (i == m) {
-        v = F[pj];
-    } else {
-        v = S[(int64_t)split_p(inst, i, nl, npi) + (int64_t)pj * lds];
-        if (i == j) v += (kind == 0) ? eps : eps * v;
+    for (int i = j + (int)threadIdx.x; i <= m; i += blockDim.x) {
+        double v;
+        if (i == m) {
+            v = F[pj];
+        } else {
+            v = S[(int64_t)split_p(inst, i, nl, npi) + (int64_t)pj * lds];
+            if (i == j) v += (kind == 0) ? eps : eps * v;
+        }
+        A[(int64_t)i + (int64_t)j * lda] = v;
     }
-    A[(int64_t)i + (int64_t)j * lda] = v;
 }
 
 // instance 0's Pi rows and columns [nl, m] (the right-hand side row m
@@ -736,11 +738,11 @@ __global__ void k_split_copy(const double* S, int lds, const double* F, int nl, 
 // multiplied.  f64 MFMA operands: A[i][k] from lane i + 16 k, B[k][j] from
 // lane j + 16 k; output D[lane/16 + 4v][lane%16]
 constexpr int KC = 64, PF = 4;
-__global__ __launch_bounds__(256) void k_split_schur(double* A2, int lda, int nl, int npi) {
+__device__ __forceinline__ void split_schur_body(double* A2, int lda, int nl, int npi, int bx) {
     __shared__ double Xa[32][KC + 1], Xb[32][KC + 1];   // [tile column][chunk row]
     const int m = nl + npi, T = (npi + 1 + 31) / 32;
     int tI, tJ;
-    step_decode(blockIdx.x, T, T, &tI, &tJ);   // tJ <= tI
+    step_decode(bx, T, T, &tI, &tJ);   // tJ <= tI
     const int i0 = nl + 32 * tI, j0 = nl + 32 * tJ;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, li = lane & 15, lk = lane >> 4;
     const int rt = wave >> 1, ct = wave & 1;
@@ -800,14 +802,14 @@ __global__ __launch_bounds__(256) void k_split_schur(double* A2, int lda, int nl
 // k_split_upd_fin sums the chunks in order
 constexpr int kUpdChunks = 16;
 
-__global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
-                                                   double* part) {
+__device__ __forceinline__ void split_upd_body(const double* A1, int lda, int nl, int m, const double* x0, double* part,
+                                               int bx, int by) {
     __shared__ double red[4][64];
     const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + r;
+    const int i = bx * 64 + r;
     const int ic = i < nl ? i : nl - 1;
     const int w = (m - nl + kUpdChunks - 1) / kUpdChunks;
-    const int c0 = nl + blockIdx.y * w, c1 = min(m, c0 + w);
+    const int c0 = nl + by * w, c1 = min(m, c0 + w);
     double a0 = 0.0, a1 = 0.0;
     int c = c0 + g;
     for (; c + 4 < c1; c += 8) {
@@ -817,11 +819,11 @@ __global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, in
     if (c < c1) a0 += A1[(int64_t)ic + (int64_t)c * lda] * x0[c];
     red[g][r] = a0 + a1;
     __syncthreads();
-    if (g == 0 && i < nl) part[(int64_t)blockIdx.y * nl + i] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+    if (g == 0 && i < nl) part[(int64_t)by * nl + i] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
 }
 
-__global__ void k_split_upd_fin(const double* part, int nl, double* y1) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_upd_fin_body(const double* part, int nl, double* y1, int bx) {
+    const int i = bx * blockDim.x + threadIdx.x;
     if (i >= nl) return;
     double t = 0.0;
     for (int q = 0; q < kUpdChunks; ++q) t += part[(int64_t)q * nl + i];
@@ -829,8 +831,9 @@ __global__ void k_split_upd_fin(const double* part, int nl, double* y1) {
 }
 
 // delta in P order from the two instances' solutions
-__global__ void k_split_gather(const double* x0, const double* x1, int nl, int npi, int np, double* delta) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_gather_body(const double* x0, const double* x1, int nl, int npi, int np,
+                                                  double* delta, int bx) {
+    const int p = bx * blockDim.x + threadIdx.x;
     if (p >= np) return;
     delta[p] = (p < nl + npi) ? x0[p] : x1[p - nl - npi];
 }
@@ -842,8 +845,8 @@ __global__ void k_set_row(double* A, int n, int lda, const double* b) {
 }
 
 // y[j] = A[j, n]: the forward-substituted right-hand side (upper column n)
-__global__ void k_copy_col(const double* A, int n, int lda, double* y) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void copy_col_body(const double* A, int n, int lda, double* y, int bx) {
+    const int j = bx * blockDim.x + threadIdx.x;
     if (j < n) y[j] = A[(int64_t)j + (int64_t)n * lda];
 }
 
@@ -892,8 +895,9 @@ __device__ __forceinline__ bool step_ok(const nq_step_guard& g) {
     return (!g.info || *g.info == 0) && (!g.flag || *g.flag == 0) && (!g.stats || isfinite(g.stats[0]));
 }
 
-__global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr, nq_step_guard g) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void update_body(double* p64, float* p32, const double* delta, const ndm_dims& d, double lr,
+                                            const nq_step_guard& g, int bx) {
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k >= d.np || !step_ok(g)) return;
     const int r = p2ref(k, d);
     const double v = p64[r] - lr * delta[k];
@@ -906,12 +910,298 @@ __global__ void k_to_f32(const double* p64, float* p32, int64_t n) {
     if (i < n) p32[i] = (float)p64[i];
 }
 
-__global__ void k_check_finite(const double* x, int64_t n, int* flag) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void check_finite_body(const double* x, int64_t n, int* flag, int bx) {
+    int64_t i = (int64_t)bx * blockDim.x + threadIdx.x;
     if (i < n && !isfinite(x[i])) atomicOr(flag, 1);
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- the kernels (one problem per launch)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_stats(const T* L, int64_t B, int64_t Lc, double* out) {
+    stats_body<T>(L, B, Lc, out);
+}
+
+__global__ void k_force_part(const double* Fpart, int nblk, int per, int np, double* part2) {
+    force_part_body(Fpart, nblk, per, np, part2, blockIdx.x, blockIdx.y);
+}
+
+__global__ void k_force_finish(const double* part2, int ngroups, const double* mu, const double* stats,
+                               ndm_dims d, double* F) {
+    force_finish_body(part2, ngroups, mu, stats, d, F, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, int k, int kb, double* winv, int* info,
+                                                    int64_t sA, int64_t sW) {
+    chol_diag32_body(A, n, lda, k, kb, winv, info, sA, sW, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int ncu, int lda, int k, int kb,
+                                                   double* winv, int* info, int k0, int64_t sA, int64_t sW) {
+    chol_step_body(A, n, nr, ncu, lda, k, kb, winv, info, k0, sA, sW, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, int k, int kend, int n,
+                                                    const double* winv, double* y, double* x, int k0, int64_t sA,
+                                                    int64_t sW, int64_t sY) {
+    back_group_body(A, lda, k, kend, n, winv, y, x, k0, sA, sW, sY, blockIdx.x, blockIdx.y);
+}
+
+__global__ void k_split_copy(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
+                             double* A2, int lda) {
+    split_copy_body(S, lds, F, nl, npi, eps, kind, A2, lda, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_split_schur(double* A2, int lda, int nl, int npi) {
+    split_schur_body(A2, lda, nl, npi, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
+                                                   double* part) {
+    split_upd_body(A1, lda, nl, m, x0, part, blockIdx.x, blockIdx.y);
+}
+
+__global__ void k_split_upd_fin(const double* part, int nl, double* y1) { split_upd_fin_body(part, nl, y1, blockIdx.x); }
+
+__global__ void k_split_gather(const double* x0, const double* x1, int nl, int npi, int np, double* delta) {
+    split_gather_body(x0, x1, nl, npi, np, delta, blockIdx.x);
+}
+
+__global__ void k_copy_col(const double* A, int n, int lda, double* y) { copy_col_body(A, n, lda, y, blockIdx.x); }
+
+__global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr, nq_step_guard g) {
+    update_body(p64, p32, delta, d, lr, g, blockIdx.x);
+}
+
+__global__ void k_check_finite(const double* x, int64_t n, int* flag) { check_finite_body(x, n, flag, blockIdx.x); }
+
+// ---- packed forms: a member's part of the split solve, its arguments
+// derived from its descriptor the way nq_split_factor / nq_split_substitute
+// derive them (the host lays out the blocks with the same functions)
+struct chol_desc {
+    double* A;
+    int n, nr, ncu, lda, k, kb;
+    double* winv;
+    int* info;
+    int k0;
+    int64_t sA, sW;
+    int gx, gy;
+};
+
+__host__ __device__ inline int64_t winv_doubles(int n) { return (int64_t)((n + CB - 1) / CB) * CB * CB; }
+
+__host__ __device__ inline int pk_panels(int nl, int npi, int phase) {
+    return phase == 0 ? (nl + CB - 1) / CB : (npi + CB - 1) / CB;
+}
+
+__host__ __device__ inline int pk_groups(int nl, int npi, int phase) {
+    return phase == 0 ? (npi + BG * CB - 1) / (BG * CB) : (nl + BG * CB - 1) / (BG * CB);
+}
+
+// phase 0: [lambda | Pi] and [mu | Pi] over their first nl columns (2 instances);
+// phase 1: the Pi block; panel j of the phase (the diagonal block alone when j < 0)
+__host__ __device__ inline chol_desc pk_chol(const nq_pk_member& m, int phase, int j) {
+    chol_desc c;
+    const int mm = m.nl + m.npi;
+    c.A = m.A2;
+    c.lda = mm + 1;
+    c.nr = mm + 1;
+    c.info = m.info;
+    if (phase == 0) {
+        c.n = m.nl;
+        c.ncu = m.nl;
+        c.k0 = 0;
+        c.winv = m.winv;
+        c.sA = (int64_t)c.lda * c.lda;
+        c.sW = winv_doubles(m.nl);
+        c.gy = 2;
+    } else {
+        c.n = mm;
+        c.ncu = mm + 1;
+        c.k0 = m.nl;
+        c.winv = m.winv + 2 * winv_doubles(m.nl);
+        c.sA = 0;
+        c.sW = 0;
+        c.gy = 1;
+    }
+    c.k = c.k0 + CB * (j < 0 ? 0 : j);
+    c.kb = (c.n - c.k < CB) ? c.n - c.k : CB;
+    if (j < 0) {
+        c.gx = 1;
+    } else {
+        const int rest = c.nr - c.k - c.kb;
+        const int T = (rest + 63) / 64, TJ = ((c.ncu - c.k - c.kb + 63) / 64 > 1) ? (c.ncu - c.k - c.kb + 63) / 64 : 1;
+        c.gx = rest > 0 ? step_tiles(T, TJ) : 0;
+    }
+    return c;
+}
+
+struct back_desc {
+    int k, kend, n, k0;
+    const double* winv;
+    int64_t sA, sW, sY;
+    int gx, gy;
+};
+
+// phase 0: the Pi block's groups; phase 1: lambda and mu together; group j of
+// the phase counts from the last group of panels
+__host__ __device__ inline back_desc pk_back(const nq_pk_member& m, int phase, int j) {
+    back_desc b;
+    const int mm = m.nl + m.npi, lda = mm + 1, groups = pk_groups(m.nl, m.npi, phase);
+    const int g = groups - 1 - j;
+    if (phase == 0) {
+        b.k0 = m.nl;
+        b.n = mm;
+        b.winv = m.winv + 2 * winv_doubles(m.nl);
+        b.sA = b.sW = b.sY = 0;
+        b.gy = 1;
+    } else {
+        b.k0 = 0;
+        b.n = m.nl;
+        b.winv = m.winv;
+        b.sA = (int64_t)lda * lda;
+        b.sW = winv_doubles(m.nl);
+        b.sY = lda;
+        b.gy = 2;
+    }
+    b.k = b.k0 + g * BG * CB;
+    b.kend = (b.n - b.k < BG * CB) ? b.n : b.k + BG * CB;
+    b.gx = g < 0 ? 0 : (b.k > 0 ? (b.k + BR - 1) / BR : 1);
+    return b;
+}
+
+__device__ __forceinline__ const nq_pk_member& pk_m(const nq_pk_member* M, uint2 b) { return M[b.x & 0xffff]; }
+__device__ __forceinline__ int pk_x(uint2 b) { return (int)(b.y & 0xffff); }
+__device__ __forceinline__ int pk_y(uint2 b) { return (int)(b.y >> 16); }
+__device__ __forceinline__ int pk_z(uint2 b) { return (int)(b.x >> 16); }
+
+// the counters a member's iteration starts from: acceptance counts, info, flag
+__global__ __launch_bounds__(256) void k_prep_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int64_t i = (int64_t)pk_x(b) * blockDim.x + threadIdx.x;
+    if (i < m.chains) m.chain_acc[i] = 0u;
+    if (i == 0) {
+        m.acc[0] = 0ull;
+        m.acc[1] = 0ull;
+        *m.info = 0;
+        *m.flag = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sfill_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int64_t i = (int64_t)pk_x(b) * blockDim.x + threadIdx.x;
+    if (i < (int64_t)m.ldS * m.ldS) m.S[i] = 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_stats_pk(const nq_pk_member* M, const uint2* blk) {
+    const nq_pk_member& m = pk_m(M, blk[blockIdx.x]);
+    stats_body<T>((const T*)m.Lloc, m.chains, m.Lc, m.stats);
+}
+
+__global__ __launch_bounds__(256) void k_force_part_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int per = (m.est_grid + NQ_FORCE_GROUPS - 1) / NQ_FORCE_GROUPS;
+    force_part_body(m.Fpart, m.est_grid, per, m.np, m.fpart2, pk_x(b), pk_y(b));
+}
+
+__global__ __launch_bounds__(256) void k_force_finish_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    force_finish_body(m.fpart2, NQ_FORCE_GROUPS, m.mu, m.stats, m.sa.d, m.F, pk_x(b));
+}
+
+__global__ __launch_bounds__(256) void k_split_copy_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    split_copy_body(m.S, m.ldS, m.F, m.nl, m.npi, m.eps, m.kind, m.A2, m.nl + m.npi + 1, pk_x(b), pk_y(b));
+}
+
+__global__ __launch_bounds__(64) void k_chol_diag32_pk(const nq_pk_member* M, const uint2* blk, int phase) {
+    const uint2 b = blk[blockIdx.x];
+    const chol_desc c = pk_chol(pk_m(M, b), phase, -1);
+    chol_diag32_body(c.A, c.n, c.lda, c.k, c.kb, c.winv, c.info, c.sA, c.sW, pk_y(b));
+}
+
+__global__ __launch_bounds__(256) void k_chol_step_pk(const nq_pk_member* M, const uint2* blk, int phase, int j) {
+    const uint2 b = blk[blockIdx.x];
+    const chol_desc c = pk_chol(pk_m(M, b), phase, j);
+    chol_step_body(c.A, c.n, c.nr, c.ncu, c.lda, c.k, c.kb, c.winv, c.info, c.k0, c.sA, c.sW, pk_x(b), pk_y(b));
+}
+
+__global__ __launch_bounds__(256) void k_split_schur_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    split_schur_body(m.A2, m.nl + m.npi + 1, m.nl, m.npi, pk_x(b));
+}
+
+// the forward-substituted right-hand sides of both instances (z = instance)
+__global__ __launch_bounds__(256) void k_copy_cols_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int mm = m.nl + m.npi, lda = mm + 1, z = pk_z(b);
+    copy_col_body(m.A2 + (int64_t)z * lda * lda, mm, lda, m.y2 + (int64_t)z * lda, pk_x(b));
+}
+
+__global__ __launch_bounds__(256) void k_back_group_pk(const nq_pk_member* M, const uint2* blk, int phase, int j) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const back_desc d = pk_back(m, phase, j);
+    back_group_body(m.A2, m.nl + m.npi + 1, d.k, d.kend, d.n, d.winv, m.y2, m.x2, d.k0, d.sA, d.sW, d.sY, pk_x(b),
+                    pk_y(b));
+}
+
+__global__ __launch_bounds__(256) void k_split_upd_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int mm = m.nl + m.npi, lda = mm + 1;
+    split_upd_body(m.A2 + (int64_t)lda * lda, lda, m.nl, mm, m.x2, m.x2 + 2 * (int64_t)lda, pk_x(b), pk_y(b));
+}
+
+__global__ __launch_bounds__(256) void k_split_upd_fin_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int lda = m.nl + m.npi + 1;
+    split_upd_fin_body(m.x2 + 2 * (int64_t)lda, m.nl, m.y2 + lda, pk_x(b));
+}
+
+__global__ __launch_bounds__(256) void k_split_gather_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const int lda = m.nl + m.npi + 1;
+    split_gather_body(m.x2, m.x2 + lda, m.nl, m.npi, m.np, m.delta, pk_x(b));
+}
+
+__global__ __launch_bounds__(256) void k_check_finite_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    check_finite_body(m.delta, m.np, m.flag, pk_x(b));
+}
+
+__global__ __launch_bounds__(256) void k_update_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = pk_m(M, b);
+    const nq_step_guard g{m.info, m.flag, m.stats};
+    update_body(m.p64, m.p32, m.delta, m.sa.d, m.lr, g, pk_x(b));
+}
+
+__global__ void k_bump_pk(const nq_pk_member* M, const uint2* blk) {
+    const nq_pk_member& m = pk_m(M, blk[blockIdx.x]);
+    if (threadIdx.x == 0) m.iter_dev[0] += 1u;
+}
+
+__global__ void k_collect_pk(const nq_pk_member* M, const uint2* blk) {
+    const nq_pk_member& m = pk_m(M, blk[blockIdx.x]);
+    const int t = threadIdx.x;
+    if (t < 5) m.status[t] = m.stats[t];
+    if (t == 5) m.status[5] = (double)*m.info;
+    if (t == 6) m.status[6] = (double)*m.flag;
+}
 
 }  // namespace
 
@@ -1033,8 +1323,7 @@ hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, in
     const int m = nl + npi, lda = m + 1;
     const int64_t sA = (int64_t)lda * lda;
     const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
-    hipLaunchKernelGGL(k_split_copy, dim3(nblk(m + 1, 256), m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind,
-                       A2, lda);
+    hipLaunchKernelGGL(k_split_copy, dim3(m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind, A2, lda);
     // both [lambda | Pi] and [mu | Pi] factored over their first nl columns
     // together, their Pi columns left as they are
     hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, nl, lda, info, winv, 2, sA, sW, st);
@@ -1096,4 +1385,168 @@ hipError_t nq_launch_to_f32(const double* p64, float* p32, int64_t n, hipStream_
 hipError_t nq_launch_check_finite(const double* x, int64_t n, int* flag, hipStream_t st) {
     hipLaunchKernelGGL(k_check_finite, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, flag);
     return hipGetLastError();
+}
+
+int nq_pk_split_panels(int nl, int npi, int phase) { return pk_panels(nl, npi, phase); }
+int nq_pk_split_groups(int nl, int npi, int phase) { return pk_groups(nl, npi, phase); }
+
+hipError_t nq_pk_plan(const nq_pk_member& m, int kernel, int param, nq_pk_shape* o) {
+    *o = nq_pk_shape{0, 1, 1, 0, 256, 0};
+    const int mm = m.nl + m.npi;
+    const int phase = param >> 16, j = param & 0xffff;
+    switch (kernel) {
+        case NQ_PK_PREP: o->gx = (int)nblk(m.chains > 0 ? m.chains : 1, 256); break;
+        case NQ_PK_METROPOLIS: {
+            int grid = 0;
+            o->variant = nq_sampler_variant(m.sa, m.fp64 != 0, &o->threads, &o->lds, &grid);
+            if (o->variant < 0) return hipErrorInvalidValue;
+            o->gx = grid;
+            break;
+        }
+        case NQ_PK_EST_WALK:
+        case NQ_PK_EST_FORCE:
+            o->variant = nq_est_variant(kernel == NQ_PK_EST_WALK ? m.ew : m.ef, m.fp64 != 0, &o->threads, &o->lds);
+            if (o->variant < 0) return hipErrorInvalidValue;
+            o->gx = m.est_grid;
+            break;
+        case NQ_PK_COLSUM:
+            o->variant = m.fp64;
+            o->gx = (m.R + 255) / 256;
+            o->gy = m.col_chunks;
+            break;
+        case NQ_PK_COLSUM_FIN: o->gx = (m.R + 63) / 64; break;
+        case NQ_PK_STATS:
+            o->variant = m.fp64;
+            o->threads = 1024;
+            o->gx = 1;
+            break;
+        case NQ_PK_FORCE_PART:
+            o->gx = (int)nblk(m.np, 256);
+            o->gy = NQ_FORCE_GROUPS;
+            break;
+        case NQ_PK_FORCE_FIN: o->gx = (int)nblk(m.np, 256); break;
+        case NQ_PK_SFILL:
+            if (m.s_mode >= 0) o->gx = (int)nblk((int64_t)m.ldS * m.ldS, 256);
+            break;
+        case NQ_PK_GRAM:
+            if (m.s_mode >= 0) {
+                o->variant = m.s_mode | (m.fp64 << 1);
+                o->gx = nq_pk_gram_blocks(m, param);
+            }
+            break;
+        case NQ_PK_CENTER:
+            if (m.s_mode >= 0) break;
+            o->variant = m.fp64;
+            o->gx = (m.R + 63) / 64;
+            o->gy = (int)((m.nsp + 63) / 64);
+            break;
+        case NQ_PK_GRAM_PART:
+            if (m.s_mode >= 0) break;
+            o->gx = m.tiles;
+            o->gy = m.splits;
+            break;
+        case NQ_PK_GRAM_FIN: {
+            if (m.s_mode >= 0) break;
+            const int TS = (m.np + 1 + 63) / 64;
+            o->gx = TS * (TS + 1) / 2;
+            break;
+        }
+        case NQ_PK_SPLIT_COPY:
+            o->gx = mm;
+            o->gy = 2;
+            break;
+        case NQ_PK_CHOL_DIAG: {
+            const chol_desc c = pk_chol(m, param, -1);
+            o->threads = 64;
+            o->gx = c.k0 < c.n ? 1 : 0;
+            o->gy = c.gy;
+            break;
+        }
+        case NQ_PK_CHOL_STEP: {
+            if (j >= pk_panels(m.nl, m.npi, phase)) break;
+            const chol_desc c = pk_chol(m, phase, j);
+            o->gx = c.gx;
+            o->gy = c.gy;
+            break;
+        }
+        case NQ_PK_SCHUR: {
+            const int T = (m.npi + 1 + 31) / 32;
+            o->gx = step_tiles(T, T);
+            break;
+        }
+        case NQ_PK_COPY_COLS:
+            o->gx = (int)nblk(mm, 256);
+            o->gy = 1;
+            o->gz = 2;
+            break;
+        case NQ_PK_BACK: {
+            if (j >= pk_groups(m.nl, m.npi, phase)) break;
+            const back_desc d = pk_back(m, phase, j);
+            o->gx = d.gx;
+            o->gy = d.gy;
+            break;
+        }
+        case NQ_PK_SPLIT_UPD:
+            o->gx = (int)nblk(m.nl, 64);
+            o->gy = kUpdChunks;
+            break;
+        case NQ_PK_UPD_FIN: o->gx = (int)nblk(m.nl, 256); break;
+        case NQ_PK_GATHER: o->gx = (int)nblk(m.np, 256); break;
+        case NQ_PK_CHECK: o->gx = (int)nblk(m.np, 256); break;
+        case NQ_PK_UPDATE: o->gx = (int)nblk(m.np, 256); break;
+        case NQ_PK_BUMP:
+        case NQ_PK_COLLECT:
+            o->threads = 64;
+            o->gx = 1;
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    if (o->gx > 0xffff || o->gy > 0xffff || o->gz > 0xffff) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+hipError_t nq_pk_solve(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    if (L.nblk <= 0) return hipSuccess;
+    const dim3 g((unsigned)L.nblk), b((unsigned)L.threads);
+    const int phase = L.param >> 16, j = L.param & 0xffff;
+    switch (L.kernel) {
+        case NQ_PK_PREP: hipLaunchKernelGGL(k_prep_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_STATS:
+            if (L.variant) hipLaunchKernelGGL((k_stats_pk<double>), g, b, 0, st, M, L.blk);
+            else hipLaunchKernelGGL((k_stats_pk<float>), g, b, 0, st, M, L.blk);
+            break;
+        case NQ_PK_FORCE_PART: hipLaunchKernelGGL(k_force_part_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_FORCE_FIN: hipLaunchKernelGGL(k_force_finish_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_SFILL: hipLaunchKernelGGL(k_sfill_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_SPLIT_COPY: hipLaunchKernelGGL(k_split_copy_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_CHOL_DIAG: hipLaunchKernelGGL(k_chol_diag32_pk, g, b, 0, st, M, L.blk, L.param); break;
+        case NQ_PK_CHOL_STEP: hipLaunchKernelGGL(k_chol_step_pk, g, b, 0, st, M, L.blk, phase, j); break;
+        case NQ_PK_SCHUR: hipLaunchKernelGGL(k_split_schur_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_COPY_COLS: hipLaunchKernelGGL(k_copy_cols_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_BACK: hipLaunchKernelGGL(k_back_group_pk, g, b, 0, st, M, L.blk, phase, j); break;
+        case NQ_PK_SPLIT_UPD: hipLaunchKernelGGL(k_split_upd_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_UPD_FIN: hipLaunchKernelGGL(k_split_upd_fin_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_GATHER: hipLaunchKernelGGL(k_split_gather_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_CHECK: hipLaunchKernelGGL(k_check_finite_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_UPDATE: hipLaunchKernelGGL(k_update_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_BUMP: hipLaunchKernelGGL(k_bump_pk, g, b, 0, st, M, L.blk); break;
+        case NQ_PK_COLLECT: hipLaunchKernelGGL(k_collect_pk, g, b, 0, st, M, L.blk); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t nq_pk_run(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
+    switch (L.kernel) {
+        case NQ_PK_METROPOLIS: return nq_pk_sampler(M, L, st);
+        case NQ_PK_EST_WALK:
+        case NQ_PK_EST_FORCE: return nq_pk_estimate(M, L, st);
+        case NQ_PK_COLSUM:
+        case NQ_PK_COLSUM_FIN:
+        case NQ_PK_CENTER:
+        case NQ_PK_GRAM_PART:
+        case NQ_PK_GRAM_FIN:
+        case NQ_PK_GRAM: return nq_pk_sbuild(M, L, st);
+        default: return nq_pk_solve(M, L, st);
+    }
 }

@@ -301,3 +301,121 @@ hipError_t nq_launch_exact_partial(const void* lp, const void* L, int fp64, int6
 hipError_t nq_launch_logp_of(const void* lp, int fp64, int64_t n, double* out, hipStream_t st);
 // combine nch chunk partials of k_exact_partial: out = [sum p |L|^2, Re sum p L, Im sum p L, log Z]
 hipError_t nq_launch_exact_combine(const double* part, int64_t nch, double* out, hipStream_t st);
+
+// ---- packed launches: the workgroups of many independent problems (batch
+// members) in one grid.  Block b runs the block body of member blk[b].x &
+// 0xffff, instance / z index blk[b].x >> 16, at (x, y) = (blk[b].y & 0xffff,
+// blk[b].y >> 16) of that member's own grid, with the member's arguments from
+// a device array of descriptors: each block does exactly what it does in the
+// member's own launch, so a member's results do not depend on the batch.
+// One SR iteration (the split Cholesky solve of an NDM, BF16 S build) in the
+// order enqueue_iteration runs it; the host lays the blocks out per launch.
+struct nq_pk_member {
+    samp_args sa;              // the sampler launch (iteration number from sa.iter_dev)
+    est_args ew, ef;           // estimator: walk (rec_mode 1), forces (rec_mode 2)
+    int est_grid;              // workgroups of each estimator launch
+    int fp64;                  // network precision of Y, L_loc
+    const void* Y;             // Ns x R compact O rows
+    const void* Lloc;
+    int64_t Ns, chains, Lc;
+    int R, np;
+    double* colpart;           // kColChunks x R column-sum partials
+    int col_chunks;
+    double* mu;
+    double* stats;
+    double* Fpart;             // est_grid x np force partials
+    double* fpart2;            // NQ_FORCE_GROUPS x np
+    double* F;
+    int s_mode;                // S build: NQ_GRAM_F64 / NQ_GRAM_F32 (fill + two k_gram), -1: BF16 split-K
+    // BF16 S build
+    __bf16* Yt;
+    int64_t nsp, kchunk;
+    int nA, nB, offB, TA, tiles, splits, ldS;
+    float* gpart;
+    double* S;
+    // split Cholesky solve (nq_split_factor, nq_split_substitute)
+    double* A2;
+    double* winv;
+    int* info;
+    int* flag;
+    int nl, npi;
+    double eps;
+    int kind;
+    double* y2;
+    double* x2;
+    double* delta;
+    // update
+    double* p64;
+    float* p32;
+    double lr;
+    uint32_t* iter_dev;
+    unsigned long long* acc;
+    uint32_t* chain_acc;
+    double* status;            // 8 doubles: the statistics, info, flag (read back once for the whole batch)
+};
+
+enum {
+    NQ_PK_PREP = 0,      // zero the acceptance counters, info and flag
+    NQ_PK_METROPOLIS,
+    NQ_PK_EST_WALK,
+    NQ_PK_EST_FORCE,
+    NQ_PK_COLSUM,
+    NQ_PK_COLSUM_FIN,
+    NQ_PK_STATS,
+    NQ_PK_FORCE_PART,
+    NQ_PK_FORCE_FIN,
+    NQ_PK_CENTER,
+    NQ_PK_GRAM_PART,
+    NQ_PK_GRAM_FIN,
+    NQ_PK_SFILL,         // S = 0 before the FP64 / FP32 Gram blocks
+    NQ_PK_GRAM,          // param: Gram block (0: A, 1: B)
+    NQ_PK_SPLIT_COPY,
+    NQ_PK_CHOL_DIAG,     // param: phase (0: lambda / mu columns, 1: Pi columns)
+    NQ_PK_CHOL_STEP,     // param: phase << 16 | panel number within the phase
+    NQ_PK_SCHUR,
+    NQ_PK_COPY_COLS,
+    NQ_PK_BACK,          // param: phase << 16 | group number within the phase
+    NQ_PK_SPLIT_UPD,
+    NQ_PK_UPD_FIN,
+    NQ_PK_GATHER,
+    NQ_PK_CHECK,
+    NQ_PK_UPDATE,
+    NQ_PK_BUMP,
+    NQ_PK_COLLECT,       // statistics and status words into the member's status slot
+    NQ_PK_COUNT
+};
+
+// one packed launch
+struct nq_pk_launch {
+    int kernel;                // NQ_PK_*
+    int param;                 // kernel-specific (see the enum)
+    int variant;               // instantiation (nq_pk_plan)
+    int threads;               // threads per workgroup
+    size_t lds;                // dynamic LDS bytes
+    const uint2* blk;          // nblk block records (device)
+    int nblk;
+};
+
+// a member's grid for one launch kind (x, y, z; 0 blocks: the member sits it
+// out), its instantiation and block shape: the same choices its own launch
+// makes.  Host side, from the host copy of the descriptor.
+struct nq_pk_shape {
+    int gx, gy, gz;
+    int variant, threads;
+    size_t lds;
+};
+hipError_t nq_pk_plan(const nq_pk_member& m, int kernel, int param, nq_pk_shape* out);
+// panels / back-substitution groups of each phase of the split solve
+int nq_pk_split_panels(int nl, int npi, int phase);
+int nq_pk_split_groups(int nl, int npi, int phase);
+hipError_t nq_pk_run(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+// per file (nq_pk_run dispatches)
+hipError_t nq_pk_sampler(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+hipError_t nq_pk_estimate(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+hipError_t nq_pk_estimate_f32(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+hipError_t nq_pk_sbuild(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+hipError_t nq_pk_solve(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
+int nq_est_variant(const est_args& a, bool fp64, int* threads, size_t* lds);
+int nq_sampler_variant(const samp_args& a, bool fp64, int* threads, size_t* lds, int* grid);
+void nq_gram_shape(int nA, int nB, int64_t nsp, int* TA, int* tiles, int* splits, int64_t* kchunk);
+int nq_pk_gram_blocks(const nq_pk_member& m, int which);

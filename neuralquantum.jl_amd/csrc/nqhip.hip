@@ -249,11 +249,23 @@ nqh_status need_problem(nqh_ctx* c) {
     return NQH_OK;
 }
 
+// estimator workgroups for Ns samples: two samples per wave at 8 waves per
+// workgroup, at most 1024 workgroups (every workgroup stages the parameters
+// and, for the forces, writes np partials: few enough that this stays small
+// next to the samples' work, also when many problems share the device)
 int est_grid_for(int64_t Ns) {
-    int64_t g = (Ns + 3) / 4;
+    int64_t g = (Ns + 15) / 16;
     if (g > 1024) g = 1024;
     if (g < 1) g = 1;
     return (int)g;
+}
+
+// column-sum chunks of the NDM's O rows: 64 samples or more per chunk, at most kColChunks
+int col_chunks_for(int64_t Ns) {
+    int64_t k = Ns / 64;
+    if (k > kColChunks) k = kColChunks;
+    if (k < 1) k = 1;
+    return (int)k;
 }
 
 // the Gram blocks of the S build: block A = columns [0, nA) of the compact
@@ -523,8 +535,8 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
     phase_end(c, PH_ESTIMATE);
     phase_begin(c, PH_POST);
-    HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), kColChunks, c->stream));
-    HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), kColChunks, c->d.R, 1.0 / (double)Ns,
+    HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), col_chunks_for(Ns), c->stream));
+    HIPCHK(c, nq_launch_colsum_finish(P<double>(c->colpart), col_chunks_for(Ns), c->d.R, 1.0 / (double)Ns,
                                       P<double>(c->mu), c->stream));
     hipStream_t fs = c->stream;
     if (beside) {
@@ -2092,18 +2104,32 @@ static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t
 // wait for the enqueued iteration; statistics and solve checks
 // (the update kernel has already skipped the step when the loss, the
 // factorization or the step failed: the parameters are then unchanged)
+// an iteration's outcome from its statistics and the solve's status words
+static nqh_status finish_status(nqh_ctx* c, const double* st, int info, int flag, double* stats) {
+    c->solve_ok = false;
+    if (stats) std::memcpy(stats, st, 5 * sizeof(double));
+    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no step applied");
+    if (info != 0) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "S + shift is not positive definite (column %d): no step applied", info);
+        return fail(c, NQH_E_NOTPD, buf);
+    }
+    if (flag) return fail(c, NQH_E_NAN, "non-finite SR update: no step applied");
+    c->solve_ok = true;
+    return NQH_OK;
+}
+
 static nqh_status sr_finish(nqh_ctx* c, double* stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
-    c->solve_ok = false;
     double st[5];
+    int info = 0, flag = 0;
     HIPCHK(c, copy_sync(c->stream, st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
-    if (stats) std::memcpy(stats, st, sizeof(st));
-    if (!std::isfinite(st[0])) return fail(c, NQH_E_NAN, "non-finite loss <L^dag L>: no step applied");
-    nqh_status s;
-    if ((s = check_solve(c))) return s;
-    c->solve_ok = true;
-    return NQH_OK;
+    if (std::isfinite(st[0])) {
+        HIPCHK(c, copy_sync(c->stream, &info, c->info.p, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c->stream, &flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    return finish_status(c, st, info, flag, stats);
 }
 
 nqh_status nqh_sr_iteration(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr, double* stats) {
@@ -2141,6 +2167,12 @@ struct nqh_batch {
     int kind = -1, refine = -1;
     std::vector<uint64_t> gens;   // members' configuration generations at capture
     std::string err;
+    int mode = NQH_BATCH_PACKED;
+    bool packed = false;          // the current graph is the packed form
+    void* pk_dev = nullptr;       // its member descriptors and block records
+    double* pk_status = nullptr;  // in pk_dev: 8 doubles per member (statistics, info, flag)
+    std::vector<double> pk_host;
+    int pk_launches = 0;
 };
 
 namespace {
@@ -2160,6 +2192,15 @@ void batch_drop_graph(nqh_batch* b) {
     b->gexec = nullptr;
     b->graph = nullptr;
     b->valid = false;
+    if (b->pk_dev) {
+        // no launch of the packed graph is in flight: every batch iteration
+        // ends with each member's statistics read back
+        (void)hipFree(b->pk_dev);
+        b->pk_dev = nullptr;
+        b->pk_status = nullptr;
+    }
+    b->packed = false;
+    b->pk_launches = 0;
 }
 
 #define BHIPCHK(b, expr)                                                                           \
@@ -2218,6 +2259,233 @@ nqh_status batch_build(nqh_batch* b, double eps, int kind, int refine, double lr
     BHIPCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->valid = true;
     b->eps = eps; b->kind = kind; b->refine = refine; b->lr = lr;
+    b->gens.resize(n);
+    for (size_t i = 0; i < n; ++i) b->gens[i] = b->m[i]->gen;
+    return NQH_OK;
+}
+
+// ---- the packed form: every kernel of the iteration launched once for all
+// members.  A member qualifies when its iteration is the NDM's split Cholesky
+// solve without refinement, with the BF16 S build, one hidden and one ancilla
+// row per lane (nh, na <= 32) and no learned proposal; the descriptors hold
+// exactly the arguments enqueue_iteration passes in graph form.
+bool pk_qualifies(nqh_ctx* c, int refine) {
+    if (is_rbm(c) || c->solver != NQH_SOLVER_CHOLESKY || refine != 0 || !use_split(c)) return false;
+    if (c->made_ready && c->made_mix > 0.0) return false;
+    if (c->d.nh > 32 || c->d.na > 32) return false;
+    return true;
+}
+
+nq_pk_member pk_describe(nqh_ctx* c, double eps, int kind, double lr) {
+    nq_pk_member m{};
+    const ndm_dims& d = c->d;
+    const int64_t Ns = c->chains * c->chain_len;
+    const bool f64 = fp64_net(c);
+    // the sampler (run_sampler in graph form: iteration number from iter_dev)
+    samp_args& a = m.sa;
+    a.d = d;
+    a.p = f64 ? c->p64.p : c->p32.p;
+    a.chains = c->chains;
+    a.passes = c->passes;
+    const bool compat = (c->flags & NQH_SAMPLER_COMPAT_STALE_LAST_SLICE) != 0;
+    const bool persistent = (c->flags & NQH_SAMPLER_PERSISTENT) != 0;
+    a.restart = (!persistent || !c->chains_initialised) ? 1 : 0;
+    a.burn = a.restart ? c->burn : 0;
+    a.stored = compat ? c->chain_len - 1 : c->chain_len;
+    a.compat_stale_last_slice = compat ? 1 : 0;
+    a.lpc = nq_sampler_lanes_per_chain(d);
+    a.seed = c->seed;
+    a.iter = c->iter;
+    a.iter_dev = P<uint32_t>(c->iter_dev);
+    a.chain_state = P<uint64_t>(c->chain_state);
+    a.samples = P<uint64_t>(c->samples);
+    a.accepted = P<unsigned long long>(c->acc);
+    a.chain_acc = P<uint32_t>(c->chain_acc);
+    // the estimator's two launches (run_grad_pipeline)
+    est_args e = make_est(c, P<uint64_t>(c->samples), Ns);
+    e.do_conn = 1;
+    e.Lloc = c->Lloc.p;
+    e.crw = c->crw.p;
+    e.crm = P<uint2>(c->crm);
+    e.cnum = P<int>(c->cnum);
+    e.cwd = c->cwd.p;
+    m.ew = e;
+    m.ew.rec_mode = 1;
+    m.ew.lp = c->lp.p;
+    m.ew.Y = c->Y.p;
+    m.ef = e;
+    m.ef.rec_mode = 2;
+    m.ef.tab_lds = 0;   // the records replace the walk
+    m.ef.Fpart = P<double>(c->Fpart);
+    m.est_grid = c->est_grid;
+    m.fp64 = f64 ? 1 : 0;
+    m.Y = c->Y.p;
+    m.Lloc = c->Lloc.p;
+    m.Ns = Ns;
+    m.chains = c->chains;
+    m.Lc = c->chain_len;
+    m.R = d.R;
+    m.np = d.np;
+    m.colpart = P<double>(c->colpart);
+    m.col_chunks = col_chunks_for(Ns);
+    m.mu = P<double>(c->mu);
+    m.stats = P<double>(c->stats);
+    m.Fpart = P<double>(c->Fpart);
+    m.fpart2 = P<double>(c->fpart2);
+    m.F = P<double>(c->F);
+    // the S build (build_s): BF16 split-K, or FP64 / FP32 Gram blocks
+    m.s_mode = c->s_prec == NQH_PREC_BF16 ? -1 : (c->s_prec == NQH_PREC_FP64 ? NQ_GRAM_F64 : NQ_GRAM_F32);
+    m.nsp = (Ns + 63) / 64 * 64;
+    m.Yt = reinterpret_cast<__bf16*>(c->Yt.p);
+    gram_blocks(c, &m.nA, &m.nB, &m.offB);
+    nq_gram_shape(m.nA, m.nB, m.nsp, &m.TA, &m.tiles, &m.splits, &m.kchunk);
+    m.gpart = P<float>(c->gpart);
+    m.S = P<double>(c->S);
+    m.ldS = d.np + 1;
+    // the split solve (precondition_async) and the update
+    m.A2 = P<double>(c->A2);
+    m.winv = P<double>(c->winv2);
+    m.info = P<int>(c->info);
+    m.flag = P<int>(c->flag);
+    m.nl = d.nlam;
+    m.npi = d.npi;
+    m.eps = eps;
+    m.kind = kind;
+    m.y2 = P<double>(c->y2);
+    m.x2 = P<double>(c->x2);
+    m.delta = P<double>(c->delta);
+    m.p64 = P<double>(c->p64);
+    m.p32 = P<float>(c->p32);
+    m.lr = lr;
+    m.iter_dev = P<uint32_t>(c->iter_dev);
+    m.acc = P<unsigned long long>(c->acc);
+    m.chain_acc = P<uint32_t>(c->chain_acc);
+    return m;
+}
+
+// the launches of one packed iteration, in the order enqueue_iteration runs a member's
+std::vector<std::pair<int, int>> pk_sequence(const std::vector<nq_pk_member>& M) {
+    int pa = 0, pb = 0, ga = 0, gb = 0;
+    for (const nq_pk_member& m : M) {
+        pa = std::max(pa, nq_pk_split_panels(m.nl, m.npi, 0));
+        pb = std::max(pb, nq_pk_split_panels(m.nl, m.npi, 1));
+        ga = std::max(ga, nq_pk_split_groups(m.nl, m.npi, 0));
+        gb = std::max(gb, nq_pk_split_groups(m.nl, m.npi, 1));
+    }
+    std::vector<std::pair<int, int>> q;
+    for (int k : {NQ_PK_PREP, NQ_PK_METROPOLIS, NQ_PK_EST_WALK, NQ_PK_EST_FORCE, NQ_PK_COLSUM, NQ_PK_COLSUM_FIN,
+                  NQ_PK_STATS, NQ_PK_FORCE_PART, NQ_PK_FORCE_FIN, NQ_PK_CENTER, NQ_PK_GRAM_PART, NQ_PK_GRAM_FIN,
+                  NQ_PK_SFILL})
+        q.push_back({k, 0});
+    q.push_back({NQ_PK_GRAM, 0});
+    q.push_back({NQ_PK_GRAM, 1});
+    q.push_back({NQ_PK_SPLIT_COPY, 0});
+    q.push_back({NQ_PK_CHOL_DIAG, 0});
+    for (int j = 0; j < pa; ++j) q.push_back({NQ_PK_CHOL_STEP, j});
+    q.push_back({NQ_PK_SCHUR, 0});
+    q.push_back({NQ_PK_CHOL_DIAG, 1});
+    for (int j = 0; j < pb; ++j) q.push_back({NQ_PK_CHOL_STEP, (1 << 16) | j});
+    q.push_back({NQ_PK_COPY_COLS, 0});
+    for (int j = 0; j < ga; ++j) q.push_back({NQ_PK_BACK, j});
+    q.push_back({NQ_PK_SPLIT_UPD, 0});
+    q.push_back({NQ_PK_UPD_FIN, 0});
+    for (int j = 0; j < gb; ++j) q.push_back({NQ_PK_BACK, (1 << 16) | j});
+    for (int k : {NQ_PK_GATHER, NQ_PK_CHECK, NQ_PK_UPDATE, NQ_PK_BUMP, NQ_PK_COLLECT}) q.push_back({k, 0});
+    return q;
+}
+
+nqh_status batch_build_packed(nqh_batch* b, double eps, int kind, double lr) {
+    batch_drop_graph(b);
+    const size_t n = b->m.size();
+    if (n > 0xffff) return bfail(b, NQH_E_ARG, "at most 65535 members in a packed batch");
+    std::vector<nq_pk_member> M(n);
+    for (size_t i = 0; i < n; ++i) {
+        nqh_ctx* c = b->m[i];
+        nqh_status s;
+        if ((s = need_net(c)) || (s = need_problem(c))) return bmember(b, i, s);
+        if (!c->has_sampler) return bmember(b, i, fail(c, NQH_E_STATE, "sampler not set (nqh_set_sampler)"));
+        if ((s = ensure_samples(c, c->chains * c->chain_len))) return bmember(b, i, s);
+        if ((s = ensure(c, c->iter_dev, sizeof(uint32_t) * 4))) return bmember(b, i, s);
+        BHIPCHK(b, hipMemcpyAsync(c->iter_dev.p, &c->iter, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        BHIPCHK(b, hipStreamSynchronize(c->stream));
+        M[i] = pk_describe(c, eps, kind, lr);
+    }
+    // block records per launch: members with more parameters first (their
+    // blocks run longest), instances / z, then y, then x, as in their own grids
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return M[x].np > M[y].np; });
+    struct plan { nq_pk_launch L; size_t off; };
+    std::vector<plan> launches;
+    std::vector<uint2> rec;
+    for (const auto& kp : pk_sequence(M)) {
+        // members grouped by instantiation and block shape: one launch each
+        std::vector<std::pair<nq_pk_shape, size_t>> sh;
+        for (size_t i : order) {
+            nq_pk_shape o;
+            const hipError_t e = nq_pk_plan(M[i], kp.first, kp.second, &o);
+            if (e != hipSuccess)
+                return bfail(b, NQH_E_STATE, "member " + std::to_string(i) + ": no packed form of launch " +
+                                                 std::to_string(kp.first));
+            if ((int64_t)o.gx * o.gy * o.gz > 0) sh.push_back({o, i});
+        }
+        std::vector<char> done(sh.size(), 0);
+        for (size_t a = 0; a < sh.size(); ++a) {
+            if (done[a]) continue;
+            plan p{};
+            p.L.kernel = kp.first;
+            p.L.param = kp.second;
+            p.L.variant = sh[a].first.variant;
+            p.L.threads = sh[a].first.threads;
+            p.L.lds = 0;
+            p.off = rec.size();
+            for (size_t q = a; q < sh.size(); ++q) {
+                const nq_pk_shape& o = sh[q].first;
+                if (done[q] || o.variant != p.L.variant || o.threads != p.L.threads) continue;
+                done[q] = 1;
+                p.L.lds = std::max(p.L.lds, o.lds);
+                for (int z = 0; z < o.gz; ++z)
+                    for (int y = 0; y < o.gy; ++y)
+                        for (int x = 0; x < o.gx; ++x)
+                            rec.push_back(make_uint2((uint32_t)sh[q].second | ((uint32_t)z << 16),
+                                                     (uint32_t)x | ((uint32_t)y << 16)));
+            }
+            p.L.nblk = (int)(rec.size() - p.off);
+            launches.push_back(p);
+        }
+    }
+    // device layout: status slots (8 doubles per member) | descriptors | block records
+    const size_t sbytes = (sizeof(double) * 8 * n + 255) / 256 * 256;
+    const size_t mbytes = (sizeof(nq_pk_member) * n + 255) / 256 * 256;
+    const size_t bytes = sbytes + mbytes + sizeof(uint2) * rec.size();
+    BHIPCHK(b, hipMalloc(&b->pk_dev, bytes));
+    b->pk_status = static_cast<double*>(b->pk_dev);
+    for (size_t i = 0; i < n; ++i) M[i].status = b->pk_status + 8 * i;
+    unsigned char* base = static_cast<unsigned char*>(b->pk_dev) + sbytes;
+    const nq_pk_member* Md = reinterpret_cast<const nq_pk_member*>(base);
+    const uint2* Rd = reinterpret_cast<const uint2*>(base + mbytes);
+    BHIPCHK(b, hipMemcpyAsync(base, M.data(), sizeof(nq_pk_member) * n, hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipMemcpyAsync(base + mbytes, rec.data(), sizeof(uint2) * rec.size(), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bfail(b, NQH_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    for (size_t k = 0; k < launches.size() && e == hipSuccess; ++k) {
+        nq_pk_launch L = launches[k].L;
+        L.blk = Rd + launches[k].off;
+        e = nq_pk_run(Md, L, b->stream);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return bfail(b, NQH_E_HIP, std::string("packed batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    BHIPCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->valid = true;
+    b->packed = true;
+    b->pk_launches = (int)launches.size();
+    b->eps = eps; b->kind = kind; b->refine = 0; b->lr = lr;
     b->gens.resize(n);
     for (size_t i = 0; i < n; ++i) b->gens[i] = b->m[i]->gen;
     return NQH_OK;
@@ -2286,6 +2554,23 @@ void nqh_batch_destroy(nqh_batch* b) {
 
 const char* nqh_batch_last_error(const nqh_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
+nqh_status nqh_batch_set_mode(nqh_batch* b, int32_t mode) {
+    if (!b) return NQH_E_ARG;
+    if (mode != NQH_BATCH_PACKED && mode != NQH_BATCH_GRAPH) return bfail(b, NQH_E_ARG, "unknown batch mode");
+    if (mode != b->mode) {
+        (void)hipSetDevice(b->device);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        batch_drop_graph(b);
+        b->mode = mode;
+    }
+    return NQH_OK;
+}
+
+int32_t nqh_batch_launches(const nqh_batch* b) {
+    if (!b || !b->valid) return -1;
+    return b->packed ? b->pk_launches : 0;
+}
+
 nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_t refine, double lr, double* stats) {
     if (!b) return NQH_E_ARG;
     (void)hipSetDevice(b->device);
@@ -2312,8 +2597,11 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
             if (s) return bmember(b, i, s);
         }
     } else {
-        if (!b->valid || batch_stale(b) || b->eps != eps || b->kind != kind || b->refine != refine || b->lr != lr) {
-            nqh_status s = batch_build(b, eps, kind, refine, lr);
+        bool pk = b->mode == NQH_BATCH_PACKED;
+        for (size_t i = 0; i < n && pk; ++i) pk = pk_qualifies(b->m[i], refine);
+        if (!b->valid || batch_stale(b) || b->packed != pk || b->eps != eps || b->kind != kind ||
+            b->refine != refine || b->lr != lr) {
+            nqh_status s = pk ? batch_build_packed(b, eps, kind, lr) : batch_build(b, eps, kind, refine, lr);
             if (s) return s;
         }
         // work already queued on a member's stream precedes the batch's graph,
@@ -2335,6 +2623,18 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
         }
     }
     nqh_status first = NQH_OK;
+    if (!eager && b->packed) {
+        // every member's statistics and status words in one read-back
+        b->pk_host.resize(8 * n);
+        BHIPCHK(b, copy_sync(b->stream, b->pk_host.data(), b->pk_status, sizeof(double) * 8 * n,
+                             hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            const double* q = b->pk_host.data() + 8 * i;
+            nqh_status s = finish_status(b->m[i], q, (int)q[5], (int)q[6], stats ? stats + 5 * i : nullptr);
+            if (s && !first) first = bmember(b, i, s);
+        }
+        return first;
+    }
     for (size_t i = 0; i < n; ++i) {
         nqh_status s = sr_finish(b->m[i], stats ? stats + 5 * i : nullptr);
         if (s && !first) first = bmember(b, i, s);

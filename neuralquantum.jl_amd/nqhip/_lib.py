@@ -28,6 +28,7 @@ SHIFT_ADD, SHIFT_DIAG = 0, 1
 SOLVER_CHOLESKY, SOLVER_CG, SOLVER_MINRES, SOLVER_MINRES_QLP, SOLVER_LSQR = 0, 1, 2, 3, 4
 SAMPLER_COMPAT_STALE_LAST_SLICE = 1
 SAMPLER_PERSISTENT = 2
+BATCH_PACKED, BATCH_GRAPH = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _u64p = C.POINTER(C.c_uint64)
@@ -101,6 +102,8 @@ SIGNATURES = {
     "nqh_batch_destroy": (None, [C.c_void_p]),
     "nqh_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "nqh_batch_sr_iteration": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double, _dp]),
+    "nqh_batch_set_mode": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "nqh_batch_launches": (C.c_int32, [C.c_void_p]),
     "nqh_made_num_params": (C.c_int64, [C.c_int32]),
     "nqh_made_setup": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                    C.c_int64]),
@@ -477,6 +480,18 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_mode(self, mode):
+        """BATCH_PACKED (default: one launch per kernel for all qualifying
+        members) or BATCH_GRAPH (every member's own launches in the graph)."""
+        r = self.lib.nqh_batch_set_mode(self.h, mode)
+        if r != NQH_OK:
+            raise NQHError(r, self.lib.nqh_batch_last_error(self.h).decode())
+
+    def launches(self):
+        """Kernel launches per iteration of the packed graph (0: per-member
+        graph, -1: nothing captured yet)."""
+        return int(self.lib.nqh_batch_launches(self.h))
 
     def sr_iteration(self, eps, lr, shift_kind=SHIFT_ADD, refine=0):
         """One SR iteration of every member; returns an (n, 5) array of statistics."""

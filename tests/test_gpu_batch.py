@@ -116,3 +116,64 @@ def test_batch_recaptures_after_a_member_is_reconfigured(gpu_ok, ref):
     batch.close()
     for c in alone + inb:
         c.close()
+
+
+UNEVEN = [(16, 1.3), (6, 0.7), (12, 1.9), (9, 0.5), (14, 1.1), (7, 1.6), (11, 0.8), (8, 1.2)]
+
+
+@pytest.mark.parametrize("flags", [0, _lib.SAMPLER_PERSISTENT])
+@pytest.mark.parametrize("prec", [(_lib.PREC_FP64, _lib.PREC_FP64), (_lib.PREC_FP32, _lib.PREC_BF16),
+                                  (_lib.PREC_FP32, _lib.PREC_FP32)])
+def test_packed_batch_matches_the_per_member_graph(gpu_ok, ref, flags, prec):
+    """Members of uneven size (N = 6 .. 16) in the packed form, every kernel
+    launched once for all members, against the same members in the
+    per-member graph: bit-identical statistics, samples and parameters over
+    four iterations."""
+    pk = [member(ref, N, g, 500 + i, flags, prec) for i, (N, g) in enumerate(UNEVEN)]
+    gr = [member(ref, N, g, 500 + i, flags, prec) for i, (N, g) in enumerate(UNEVEN)]
+    bp, bg = Batch(pk), Batch(gr)
+    bg.set_mode(_lib.BATCH_GRAPH)
+    try:
+        for it in range(4):
+            sp = bp.sr_iteration(0.01, 0.01)
+            sg = bg.sr_iteration(0.01, 0.01)
+            assert np.array_equal(sp, sg), it
+            for i in range(len(UNEVEN)):
+                assert np.array_equal(pk[i].get_samples(), gr[i].get_samples()), (it, i)
+        if flags & _lib.SAMPLER_PERSISTENT:
+            assert bp.launches() > 0 and bg.launches() == 0
+        for a, b in zip(pk, gr):
+            assert np.array_equal(a.get_params(), b.get_params())
+    finally:
+        bp.close()
+        bg.close()
+        for c in pk + gr:
+            c.close()
+
+
+def test_packed_batch_falls_back_for_members_it_does_not_cover(gpu_ok, ref):
+    """A member with more than 32 hidden units (two unit rows per lane) keeps
+    the batch on the per-member graph (no packed form), and the batch still
+    matches members run alone."""
+    prec = (_lib.PREC_FP64, _lib.PREC_FP64)
+
+    def wide(seed):
+        ctx = member(ref, 6, 1.0, seed, 0, prec)
+        ctx.set_ndm(6, 40, 6)
+        ctx.set_params(params(ref.ndm_np(6, 40, 6), seed=seed, sigma=0.05))
+        ctx.set_sampler(64, 8, 13, 10, 0)
+        return ctx
+
+    alone = [member(ref, 7, 1.7, 700, 0, prec), wide(701)]
+    inb = [member(ref, 7, 1.7, 700, 0, prec), wide(701)]
+    batch = Batch(inb)
+    try:
+        for it in range(2):
+            st = batch.sr_iteration(0.01, 0.01)
+            for i, c in enumerate(alone):
+                assert np.array_equal(st[i], c.sr_iteration(0.01, 0.01)), (it, i)
+        assert batch.launches() == 0
+    finally:
+        batch.close()
+        for c in alone + inb:
+            c.close()
