@@ -39,7 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "neuralquantum.jl_amd"))
 sys.path.insert(0, ROOT)
 
-from nqhip import _lib, operators, problems  # noqa: E402
+from nqhip import _lib, autotune, operators, problems  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
@@ -76,6 +76,9 @@ def build_ctx(form: str, cfg=CFG):
         ctx.set_precision(_lib.PREC_FP32, _lib.PREC_BF16)
         flags = _lib.SAMPLER_PERSISTENT
     ctx.set_sampler(cfg["chains"], cfg["chain_length"], cfg["passes"], cfg["burn"], flags)
+    shape = autotune.load(TUNED["chosen"]) if form == "chosen" else None
+    if shape:
+        ctx.set_launch_shape(*shape)       # the autotuner's stored choice for this workload
     return ctx, p
 
 
@@ -101,6 +104,9 @@ def time_form(form, steps, warmup, cfg=CFG):
     return ctx, dt, phases, losses
 
 
+# stored launch shapes (nqhip/launch_shapes.json, written by --autotune)
+TUNED = {"chosen": "xyz44_chosen", "instances": "ising64_packed"}
+
 C5 = dict(instances=64, nmin=6, nmax=16, gmin=0.5, gmax=2.0, seed=2024, chains=256, chain_length=8, burn=50,
           eps=0.01, lr=0.01, sigma=1e-4)
 
@@ -123,6 +129,9 @@ def build_instances(cfg=C5):
         ctx.set_params(prng.standard_normal(ctx.np_) * math.sqrt(cfg["sigma"]))
         ctx.set_precision(_lib.PREC_FP32, _lib.PREC_BF16)
         ctx.set_sampler(cfg["chains"], cfg["chain_length"], 2 * N + 1, cfg["burn"], _lib.SAMPLER_PERSISTENT)
+        shape = autotune.load(TUNED["instances"])
+        if shape:
+            ctx.set_launch_shape(*shape)
         members.append(ctx)
     return members, ns
 
@@ -182,6 +191,7 @@ def time_instances(steps, warmup, cfg=C5):
             "aggregate_it_s": {k: round(v, 1) for k, v in res.items()},
             "steps": steps, "warmup": warmup,
             "packed_launches_per_iteration": launches,
+            "launch_shape": autotune.load(TUNED["instances"]),
             "max_rel_loss_diff_batch_vs_sequential": d}
 
 
@@ -530,6 +540,31 @@ def cpu_baseline(threads=1, full=False):
             "seconds_per_iteration": {"sampler": t_samp, "estimator": t_est, "sbuild": t_s, "solve": t_chol}}
 
 
+def run_autotune(path=autotune.PATH):
+    """The autotuner on the two tuned workloads; results to `path` (by default
+    the package's launch_shapes.json)."""
+    ctx, _ = build_ctx("chosen")
+    ctx.set_launch_shape(0, 0, 0, 0)
+    for _ in range(3):                   # the restart + burn-in iteration, then persistent chains
+        ctx.sr_iteration(CFG["eps"], CFG["lr"])
+    r = autotune.tune_context(ctx, CFG["eps"], CFG["lr"], iters=20, warmup=3)
+    autotune.save(TUNED["chosen"], r, "4x4 XYZ NDM alpha 2, 4096 x 4 samples, chosen form, graph replay", path)
+    print(json.dumps({"workload": TUNED["chosen"], **{k: r[k] for k in ("shape", "ms", "default_ms")}}), flush=True)
+    ctx.close()
+    members, _ = build_instances()
+    for c in members:
+        c.set_launch_shape(0, 0, 0, 0)
+    batch = _lib.Batch(members)
+    for _ in range(3):
+        batch.sr_iteration(C5["eps"], C5["lr"])
+    r = autotune.tune_batch(batch, members, C5["eps"], C5["lr"], iters=10, warmup=2)
+    autotune.save(TUNED["instances"], r, "64 Ising chains N 6..16, 256 x 8 samples, packed batch", path)
+    print(json.dumps({"workload": TUNED["instances"], **{k: r[k] for k in ("shape", "ms", "default_ms")}}), flush=True)
+    batch.close()
+    for c in members:
+        c.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -545,7 +580,14 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (the known-size kernel the --pmc byte counters are "
                          "scaled by)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch shapes of the chosen form and of the packed batched instances, store them "
+                         "in nqhip/launch_shapes.json and exit")
+    ap.add_argument("--autotune-out", default=autotune.PATH, help="where --autotune writes its results")
     args = ap.parse_args()
+    if args.autotune:
+        run_autotune(args.autotune_out)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: libnqhip runs on one GPU per process and has no multi-GPU path; "
               "use --gpus 1", file=sys.stderr)
@@ -585,7 +627,8 @@ def main():
                                      "(the reference's schedule)",
                              "fp64_persistent": "FP64 network + FP64 S build, persistent chains (the chosen "
                                                 "schedule at the plain precision)",
-                             "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"}},
+                             "chosen": "FP32 network, BF16 S build (FP32 acc), FP64 solve, persistent chains"},
+                   "chosen_launch_shape": autotune.load(TUNED["chosen"])},
     }
     for form in ("fp64", "fp64_persistent"):
         out[form + "_form"] = {"it_s": round(res[form]["it_s"], 3), "ms_per_step": round(res[form]["ms"], 4),

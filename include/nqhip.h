@@ -322,6 +322,20 @@ nqh_status nqh_get_phase_times(nqh_ctx* ctx, double* ms_out, int64_t* counts_out
  * update!). */
 nqh_status nqh_set_graph(nqh_ctx* ctx, int32_t on);
 
+/* launch shapes (0 = chosen from the problem): estimator samples per
+ * workgroup (1..4096; default 16), estimator waves per workgroup (4 or 8;
+ * default 8 while two workgroups fit a CU's LDS), BF16 Gram split-K chunks
+ * (1..64; default about three tiles per CU with the partial sums at most
+ * twice the input) and sampler waves per workgroup (1, 2 or 4; default 4).
+ * They change how work is spread over the device, not what is computed; the
+ * estimator's shape fixes the order of the force sum (deterministic for a
+ * given shape).  Picked per workload by nqhip.autotune and stored in
+ * nqhip/launch_shapes.json.  nqh_get_launch_shape: the four values as set.
+ * No reference counterpart. */
+nqh_status nqh_set_launch_shape(nqh_ctx* ctx, int32_t est_samples_per_block, int32_t est_waves, int32_t gram_splits,
+                                int32_t sampler_waves);
+nqh_status nqh_get_launch_shape(const nqh_ctx* ctx, int32_t* out4);
+
 /* exact sums over all 2^(2N) doubled configurations (2^N for kets), the
  * batched ExactSampler (Exact.jl:110-181) with the accumulator on top:
  * out[0] = sum p |L_loc|^2 (= ||L rho||^2 / ||rho||^2 for a Liouvillian),

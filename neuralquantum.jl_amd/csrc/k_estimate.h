@@ -745,6 +745,8 @@ __global__ NQ_EST_BOUNDS void k_estimate_pk(const nq_pk_member* M, const uint2* 
 // gradients) runs one wave per block.
 inline int est_waves(const est_args& a, size_t elt) {
     if (a.G) return 1;
+    if (a.waves == 4) return 4;
+    if (a.waves == 8 && est_lds_bytes(a, elt, 8) <= 160 * 1024) return 8;
     return est_lds_bytes(a, elt, 8) > 80 * 1024 ? 4 : 8;
 }
 

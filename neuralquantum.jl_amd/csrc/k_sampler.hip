@@ -202,17 +202,19 @@ __global__ __launch_bounds__(256) void k_metropolis_pk(const nq_pk_member* M, co
     metropolis_body<T, US, ACT>(a, (int)(b.y & 0xffff));
 }
 
-constexpr int kSampWaves = 4;   // wavefronts per workgroup
+constexpr int kSampWaves = 4;   // wavefronts per workgroup (default)
+
+int samp_waves(const samp_args& a) { return (a.waves == 1 || a.waves == 2 || a.waves == 4) ? a.waves : kSampWaves; }
 
 int64_t samp_grid(const samp_args& a) {
     const int cpw = 64 / a.lpc;
     const int64_t waves = (a.chains + cpw - 1) / cpw;
-    return (waves + kSampWaves - 1) / kSampWaves;
+    return (waves + samp_waves(a) - 1) / samp_waves(a);
 }
 
 template <typename T, int US>
 hipError_t launch_t(const samp_args& a, hipStream_t st) {
-    const int wpb = kSampWaves;
+    const int wpb = samp_waves(a);
     const int64_t grid = samp_grid(a);
     size_t lds = sizeof(T) * (size_t)a.d.np;
     if (a.d.act == NQ_ACT_LOGCOSH)
@@ -254,7 +256,7 @@ int nq_sampler_variant(const samp_args& a, bool fp64, int* threads, size_t* lds,
     if (a.d.N > 32 || a.lpc < 1 || a.lpc > 64 || (a.lpc & (a.lpc - 1)) || m > a.lpc) return -1;
     const int64_t g = samp_grid(a);
     if (g > 0xffff) return -1;
-    *threads = 64 * kSampWaves;
+    *threads = 64 * samp_waves(a);
     *lds = (fp64 ? sizeof(double) : sizeof(float)) * (size_t)a.d.np;
     *grid = (int)g;
     return (fp64 ? 1 : 0) | (a.d.act == NQ_ACT_LOGCOSH ? 2 : 0);

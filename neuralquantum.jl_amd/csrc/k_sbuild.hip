@@ -517,17 +517,24 @@ hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, in
     return hipGetLastError();
 }
 
-int nq_gram_split_plan(int nA, int nB, int64_t nsp, int64_t* kchunk) {
+// split-K chunks of the BF16 Gram: max_splits when given (a tuned shape),
+// otherwise about three tiles in flight per CU, with the partial sums (64 KB
+// per tile and chunk) kept within twice the BF16 input so that small
+// problems (many of them sharing the device) do not turn into partial-sum
+// traffic; at most one chunk per 64 samples and 64 chunks
+int nq_gram_split_plan(int nA, int nB, int64_t nsp, int max_splits, int64_t* kchunk) {
     const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
     const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
-    int s = tiles > 0 ? (3 * 256 + tiles - 1) / tiles : 1;   // about three tiles in flight per CU
-    int maxs = nsp >= 64 ? (int)(nsp / 64) : 1;
-    // the partial sums (64 KB per tile and chunk) stay within twice the BF16
-    // input, so that small problems (many of them sharing the device) do not
-    // turn into partial-sum traffic
-    const int64_t in_bytes = 2 * nsp * (int64_t)(nA + nB);
-    const int64_t cap = tiles > 0 ? 2 * in_bytes / ((int64_t)tiles * 128 * 128 * 4) : 1;
-    if (cap < maxs) maxs = cap > 1 ? (int)cap : 1;
+    int s;
+    if (max_splits > 0) {
+        s = max_splits;
+    } else {
+        s = tiles > 0 ? (3 * 256 + tiles - 1) / tiles : 1;
+        const int64_t in_bytes = 2 * nsp * (int64_t)(nA + nB);
+        const int64_t cap = tiles > 0 ? 2 * in_bytes / ((int64_t)tiles * 128 * 128 * 4) : 1;
+        if (cap < s) s = cap > 1 ? (int)cap : 1;
+    }
+    const int maxs = nsp >= 64 ? (int)(nsp / 64) : 1;
     s = s < maxs ? s : maxs;
     s = s < 64 ? s : 64;
     s = s > 1 ? s : 1;
@@ -538,18 +545,18 @@ int nq_gram_split_plan(int nA, int nB, int64_t nsp, int64_t* kchunk) {
     return (int)((nsp + kc - 1) / kc);
 }
 
-size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp) {
+size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp, int max_splits) {
     int64_t kc;
-    const int splits = nq_gram_split_plan(nA, nB, nsp, &kc);
+    const int splits = nq_gram_split_plan(nA, nB, nsp, max_splits, &kc);
     const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
     const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
     return (size_t)splits * tiles * 128 * 128;
 }
 
 hipError_t nq_launch_gram2_bf16(const __bf16* Yt, int64_t nsp, int nA, int nB, int offB, double scale, float* part,
-                                double* S, int np, int lds, hipStream_t st) {
+                                double* S, int np, int lds, int max_splits, hipStream_t st) {
     int64_t kc;
-    const int splits = nq_gram_split_plan(nA, nB, nsp, &kc);
+    const int splits = nq_gram_split_plan(nA, nB, nsp, max_splits, &kc);
     const int TA = (nA + 127) / 128, TB = (nB + 127) / 128;
     const int tiles = TA * (TA + 1) / 2 + TB * (TB + 1) / 2;
     if (tiles > 0)
@@ -577,8 +584,8 @@ hipError_t nq_launch_colsum_finish(const double* part, int chunks, int R, double
     return hipGetLastError();
 }
 
-void nq_gram_shape(int nA, int nB, int64_t nsp, int* TA, int* tiles, int* splits, int64_t* kchunk) {
-    *splits = nq_gram_split_plan(nA, nB, nsp, kchunk);
+void nq_gram_shape(int nA, int nB, int64_t nsp, int max_splits, int* TA, int* tiles, int* splits, int64_t* kchunk) {
+    *splits = nq_gram_split_plan(nA, nB, nsp, max_splits, kchunk);
     *TA = (nA + 127) / 128;
     const int TB = (nB + 127) / 128;
     *tiles = *TA * (*TA + 1) / 2 + TB * (TB + 1) / 2;

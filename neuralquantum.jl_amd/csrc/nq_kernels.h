@@ -32,6 +32,7 @@ struct est_args {
     uint2* crm;               // Ns x max_conn: their (row, column) flip masks
     int* cnum;                // Ns: changed connections per sample
     void* cwd;                // 2 Ns T: sum of the unchanged connections' mels
+    int waves;                // waves per workgroup, 4 or 8 (0: 8 while the LDS allows two workgroups per CU)
 };
 size_t est_lds_bytes(const est_args& a, size_t elt, int waves);
 hipError_t nq_launch_estimate(const est_args& a, bool fp64, int grid, hipStream_t st);
@@ -56,6 +57,7 @@ struct samp_args {
     uint64_t* samples;         // (stored [+1]) x chains, chain fastest
     unsigned long long* accepted;
     uint32_t* chain_acc;       // chains: accepted proposals per chain, added to (nullable)
+    int waves;                 // waves per workgroup, 1, 2 or 4 (0: 4)
 };
 int nq_sampler_lanes_per_chain(const ndm_dims& d);
 hipError_t nq_launch_metropolis(const samp_args& a, bool fp64, hipStream_t st);
@@ -85,10 +87,11 @@ hipError_t nq_launch_center_t(const void* Y, int y_fp64, int64_t Ns, int ldy, in
 // block B: rows [nA, nA + nB)) written into all of the (np + 1)^2 matrix S:
 // scale * A^T A at [0, nA) plus scale * B^T B at [offB, offB + nB), zeros
 // elsewhere (row and column np included); `part` holds
-// nq_gram_split_part_floats(nA, nB, nsp) floats of partial sums
-size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp);
+// nq_gram_split_part_floats(nA, nB, nsp, max_splits) floats of partial sums;
+// max_splits: split-K chunks (0: chosen from the shape)
+size_t nq_gram_split_part_floats(int nA, int nB, int64_t nsp, int max_splits);
 hipError_t nq_launch_gram2_bf16(const __bf16* Yt, int64_t nsp, int nA, int nB, int offB, double scale, float* part,
-                                double* S, int np, int lds, hipStream_t st);
+                                double* S, int np, int lds, int max_splits, hipStream_t st);
 
 // matrix-free SR (k_cg.hip): q = S v + shift(v) with S = Re(Oc^H Oc) / Ns from
 // the compact rows Y (block A = columns [0, nA) -> parameters [0, nA), block
@@ -417,5 +420,5 @@ hipError_t nq_pk_sbuild(const nq_pk_member* M, const nq_pk_launch& L, hipStream_
 hipError_t nq_pk_solve(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st);
 int nq_est_variant(const est_args& a, bool fp64, int* threads, size_t* lds);
 int nq_sampler_variant(const samp_args& a, bool fp64, int* threads, size_t* lds, int* grid);
-void nq_gram_shape(int nA, int nB, int64_t nsp, int* TA, int* tiles, int* splits, int64_t* kchunk);
+void nq_gram_shape(int nA, int nB, int64_t nsp, int max_splits, int* TA, int* tiles, int* splits, int64_t* kchunk);
 int nq_pk_gram_blocks(const nq_pk_member& m, int which);

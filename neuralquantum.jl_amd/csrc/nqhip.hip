@@ -64,6 +64,11 @@ struct nqh_ctx {
     dbuf keys_in, lp, Y, Lloc, G, Yt;
     dbuf crw, crm, cnum, cwd;          // changed connections per sample, between the two estimator launches
     int est_grid = 0;
+    // launch shapes (0: chosen from the problem; nqh_set_launch_shape)
+    int tune_spb = 0;          // estimator samples per workgroup
+    int tune_est_waves = 0;    // estimator waves per workgroup (4 or 8)
+    int tune_splits = 0;       // BF16 Gram split-K chunks
+    int tune_samp_waves = 0;   // sampler waves per workgroup (1, 2, 4)
     dbuf Fpart;
     // iteration buffers
     dbuf colpart, mu, stats, F, S, S0, delta, resid, info, flag, fpart2, winv, tmp, gpart;
@@ -253,8 +258,9 @@ nqh_status need_problem(nqh_ctx* c) {
 // workgroup, at most 1024 workgroups (every workgroup stages the parameters
 // and, for the forces, writes np partials: few enough that this stays small
 // next to the samples' work, also when many problems share the device)
-int est_grid_for(int64_t Ns) {
-    int64_t g = (Ns + 15) / 16;
+int est_grid_for(int64_t Ns, int spb = 0) {
+    if (spb <= 0) spb = 16;
+    int64_t g = (Ns + spb - 1) / spb;
     if (g > 1024) g = 1024;
     if (g < 1) g = 1;
     return (int)g;
@@ -311,7 +317,7 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
             HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         }
     }
-    c->est_grid = est_grid_for(Ns);
+    c->est_grid = est_grid_for(Ns, c->tune_spb);
     if ((s = ensure(c, c->Fpart, sizeof(double) * (size_t)c->est_grid * nsys(c)))) return s;
     if ((s = ensure(c, c->colpart, sizeof(double) * (size_t)kColChunks * ycols(c)))) return s;
     if ((s = ensure(c, c->mu, sizeof(double) * ycols(c)))) return s;
@@ -343,7 +349,8 @@ nqh_status ensure_samples(nqh_ctx* c, int64_t Ns) {
         if ((s = ensure(c, c->Yt, sizeof(uint16_t) * (size_t)ycols(c) * nsp))) return s;
         int nA, nB, offB;
         gram_blocks(c, &nA, &nB, &offB);
-        if ((s = ensure(c, c->gpart, sizeof(float) * nq_gram_split_part_floats(nA, nB, nsp)))) return s;
+        if ((s = ensure(c, c->gpart, sizeof(float) * nq_gram_split_part_floats(nA, nB, nsp, c->tune_splits))))
+            return s;
     }
     if ((s = ensure(c, c->flag, sizeof(int) * 4))) return s;
     return NQH_OK;
@@ -361,6 +368,7 @@ est_args make_est(nqh_ctx* c, const uint64_t* keys_dev, int64_t Ns) {
     a.lds_param_bytes = (int)(((size_t)c->d.np * net_elt(c) + 15) / 16 * 16);
     a.max_conn = std::max(c->max_conn, 1);
     a.tab_lds = (c->has_problem && c->tab.bytes <= kTableLdsMax) ? 1 : 0;
+    a.waves = c->tune_est_waves;
     return a;
 }
 
@@ -474,7 +482,7 @@ nqh_status run_grad_pipeline_rbm(nqh_ctx* c, const uint64_t* keys_dev, int64_t c
         __bf16* Yt = reinterpret_cast<__bf16*>(c->Yt.p);
         HIPCHK(c, nq_launch_center_t(c->Y.p, f64, Ns, R, 0, R, P<double>(c->mu), Yt, nsp, c->stream));
         HIPCHK(c, nq_launch_gram2_bf16(Yt, nsp, R, 0, 0, 1.0 / (double)Ns, P<float>(c->gpart), P<double>(c->S), n,
-                                       ldS, c->stream));
+                                       ldS, c->tune_splits, c->stream));
     } else {
         HIPCHK(c, nq_launch_fill(P<double>(c->S), (int64_t)ldS * ldS, 0.0, c->stream));
         gram_args g{};
@@ -581,7 +589,7 @@ static nqh_status build_s(nqh_ctx* c, int64_t Ns, bool f64) {
         int nA, nB, offB;
         gram_blocks(c, &nA, &nB, &offB);
         HIPCHK(c, nq_launch_gram2_bf16(Yt, nsp, nA, nB, offB, 1.0 / (double)Ns, P<float>(c->gpart), P<double>(c->S),
-                                       np, ldS, c->stream));
+                                       np, ldS, c->tune_splits, c->stream));
         phase_end(c, PH_SBUILD);
         c->last_ns = Ns;
         c->grad_ready = true;
@@ -1264,6 +1272,7 @@ static nqh_status run_sampler(nqh_ctx* c) {
     a.accepted = P<unsigned long long>(c->acc);
     a.chain_acc = P<uint32_t>(c->chain_acc);
     a.iter_dev = c->use_graph ? P<uint32_t>(c->iter_dev) : nullptr;
+    a.waves = c->tune_samp_waves;
     phase_begin(c, PH_SAMPLER);
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->acc), 4, 0u, c->stream));
     HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->chain_acc), c->chains, 0u, c->stream));
@@ -2080,6 +2089,35 @@ nqh_status nqh_set_graph(nqh_ctx* c, int32_t on) {
     return NQH_OK;
 }
 
+nqh_status nqh_set_launch_shape(nqh_ctx* c, int32_t est_samples_per_block, int32_t est_waves, int32_t gram_splits,
+                                int32_t sampler_waves) {
+    if (!c) return NQH_E_ARG;
+    if (est_samples_per_block < 0 || est_samples_per_block > 4096)
+        return fail(c, NQH_E_ARG, "estimator samples per workgroup in 0..4096");
+    if (est_waves != 0 && est_waves != 4 && est_waves != 8) return fail(c, NQH_E_ARG, "estimator waves 0, 4 or 8");
+    if (gram_splits < 0 || gram_splits > 64) return fail(c, NQH_E_ARG, "Gram split-K chunks in 0..64");
+    if (sampler_waves != 0 && sampler_waves != 1 && sampler_waves != 2 && sampler_waves != 4)
+        return fail(c, NQH_E_ARG, "sampler waves 0, 1, 2 or 4");
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    c->tune_spb = est_samples_per_block;
+    c->tune_est_waves = est_waves;
+    c->tune_splits = gram_splits;
+    c->tune_samp_waves = sampler_waves;
+    c->ns_cap = 0;   // per-sample buffers (force partials, Gram partials) resized on the next call
+    c->gen++;        // batches capture their graphs again
+    return NQH_OK;
+}
+
+nqh_status nqh_get_launch_shape(const nqh_ctx* c, int32_t* out4) {
+    if (!c || !out4) return NQH_E_ARG;
+    out4[0] = c->tune_spb;
+    out4[1] = c->tune_est_waves;
+    out4[2] = c->tune_splits;
+    out4[3] = c->tune_samp_waves;
+    return NQH_OK;
+}
+
 // enqueue one SR iteration on the context's stream without waiting for it
 static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
     if (nqh_status sm = made_maybe_retrain(c)) return sm;
@@ -2301,6 +2339,7 @@ nq_pk_member pk_describe(nqh_ctx* c, double eps, int kind, double lr) {
     a.samples = P<uint64_t>(c->samples);
     a.accepted = P<unsigned long long>(c->acc);
     a.chain_acc = P<uint32_t>(c->chain_acc);
+    a.waves = c->tune_samp_waves;
     // the estimator's two launches (run_grad_pipeline)
     est_args e = make_est(c, P<uint64_t>(c->samples), Ns);
     e.do_conn = 1;
@@ -2338,7 +2377,7 @@ nq_pk_member pk_describe(nqh_ctx* c, double eps, int kind, double lr) {
     m.nsp = (Ns + 63) / 64 * 64;
     m.Yt = reinterpret_cast<__bf16*>(c->Yt.p);
     gram_blocks(c, &m.nA, &m.nB, &m.offB);
-    nq_gram_shape(m.nA, m.nB, m.nsp, &m.TA, &m.tiles, &m.splits, &m.kchunk);
+    nq_gram_shape(m.nA, m.nB, m.nsp, c->tune_splits, &m.TA, &m.tiles, &m.splits, &m.kchunk);
     m.gpart = P<float>(c->gpart);
     m.S = P<double>(c->S);
     m.ldS = d.np + 1;

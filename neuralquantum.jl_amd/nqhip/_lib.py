@@ -94,6 +94,8 @@ SIGNATURES = {
     "nqh_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "nqh_get_phase_times": (C.c_int32, [C.c_void_p, _dp, _i64p]),
     "nqh_set_graph": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "nqh_set_launch_shape": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "nqh_get_launch_shape": (C.c_int32, [C.c_void_p, _i32p]),
     "nqh_sr_iteration_async": (C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double]),
     "nqh_wait": (C.c_int32, [C.c_void_p, _dp]),
     "nqh_set_sr_solver": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
@@ -390,6 +392,16 @@ class Context:
 
     def set_graph(self, on=True):
         self._chk(self.lib.nqh_set_graph(self.h, 1 if on else 0))
+
+    def set_launch_shape(self, est_samples_per_block=0, est_waves=0, gram_splits=0, sampler_waves=0):
+        """Launch shapes (0 = chosen from the problem); see nqh_set_launch_shape."""
+        self._chk(self.lib.nqh_set_launch_shape(self.h, int(est_samples_per_block), int(est_waves), int(gram_splits),
+                                                int(sampler_waves)))
+
+    def get_launch_shape(self):
+        out = np.zeros(4, dtype=np.int32)
+        self._chk(self.lib.nqh_get_launch_shape(self.h, _p(out, _i32p)))
+        return [int(v) for v in out]
 
     def set_profiling(self, on=True):
         self._chk(self.lib.nqh_set_profiling(self.h, 1 if on else 0))
