@@ -401,7 +401,7 @@ def kernel_rooflines(phases, form, d, pairs):
     t = per.get("factor", per["solve"]) * 1e-3
     fl = np_ ** 3 / 3
     nl, npi = N + nh + nh * N, na + 2 * na * N
-    out["solve"] = dict(kernel="k_chol_step (+ k_chol_diag32, k_split_combine)", bound="mfma",
+    out["solve"] = dict(kernel="k_chol_step (+ k_chol_diag32, k_split_copy, k_split_schur)", bound="mfma",
                         achieved=fl / t / 1e12,
                         peak=PEAK_TFLOPS["fp64"], unit="TFLOP/s", algorithmic=f"Np^3/3 = {fl:.4g} flop",
                         ms=per.get("factor", per["solve"]), phase_ms=per["solve"],

@@ -328,6 +328,16 @@ function chain_acceptance(is::HIPBatchedSampler)
     return out
 end
 
-export HIPBatchedSampler, exact_sums, chain_acceptance
+"""Launch shapes (0 = chosen from the problem): estimator samples and waves per
+workgroup, BF16 Gram split-K chunks, sampler waves per workgroup
+(`nqh_set_launch_shape`; the values `nqhip.autotune` stores in launch_shapes.json)."""
+function set_launch_shape!(is::HIPBatchedSampler; est_samples_per_block=0, est_waves=0, gram_splits=0,
+                           sampler_waves=0)
+    check(is.ctx, ccall((:nqh_set_launch_shape, libnqhip), Int32, (Ptr{Cvoid}, Int32, Int32, Int32, Int32),
+                        is.ctx, est_samples_per_block, est_waves, gram_splits, sampler_waves))
+    return is
+end
+
+export HIPBatchedSampler, exact_sums, chain_acceptance, set_launch_shape!
 
 end # module
