@@ -146,8 +146,8 @@ hipError_t nq_launch_force_part(const double* Fpart, int nb, int n, double* part
 // S[k,k] += eps (kind 0) or eps * S[k,k] (kind 1)
 hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipStream_t st);
 // in-place Cholesky of S (column-major, n x n, leading dimension lds): the
-// diagonal 32 x 32 blocks of L in the lower triangle, the panels below them
-// transposed into the upper triangle (A[j, i] = L[i, j] for i beyond j's block),
+// panels below the diagonal 32 x 32 blocks transposed into the upper triangle
+// (A[j, i] = L[i, j] for i beyond j's block), the diagonal blocks only as
 // winv[b] = L_bb^-T (32 x 32 column-major, nq_cholesky_winv_doubles(n) doubles);
 // *info = first failing column + 1 or 0.  With lds > n, row n holds a
 // right-hand side b^T on entry and L^-1 b ends in column n (read it with
