@@ -169,7 +169,10 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
 // L_kk^-T and the transposed off-diagonal panels only, and its 32 masked
 // stores cost ~0.85 us of the pivot chain's 4.8 us.
 // The block comes from A (global) or, when `src` is given, from LDS
-// (src[row][column], rows and columns of the block).
+// (src[row][column], rows and columns of the block).  The first diagonal
+// block of a column range takes this one-wavefront form (k_chol_diag32);
+// the look-ahead tile spreads the same sweep over three wavefronts
+// (factor_diag32_split below).
 constexpr int SUBP = 8;
 template <bool FROM_LDS>
 __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb, int nmax, double* w, int* info,
@@ -246,6 +249,152 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
     }
 }
 
+// ---- the look-ahead tile's diagonal factor over three wavefronts: wave 0
+// sweeps sub-panels 0 and 1 (columns 0-15), waves 2 and 3 sub-panels 2 and
+// 3; while wave 0 runs its pivots, the others apply each finished
+// sub-panel's rank-8 update to their own columns, so only the next
+// sub-panel's update stays on the pivot chain.  Same arithmetic, in the same
+// order, as factor_diag32 (bit-identical L_kk^-T); the waves hand over
+// sub-panels through a workgroup counter in LDS (release / acquire).
+
+// bounded wait for the workgroup's sub-panel counter (the producer is a wave
+// of the same workgroup, so it always arrives; the bound only keeps a fault
+// from hanging the device)
+__device__ __forceinline__ bool wait_prog(const int* prog, int v) {
+    for (int it = 0; it < (1 << 20); ++it) {
+        if (__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= v) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+// pivots of sub-panel [p0, p0 + 8) on the register columns r[c - C0]
+template <int C0>
+__device__ __forceinline__ void sweep8(double* r, int p0, int kb, int t, int& bad, double (*Lc)[64]) {
+    double lprev = 0.0;
+#pragma unroll
+    for (int j = p0; j < p0 + SUBP; ++j) {
+        const double v = rl_d(r[j - C0], j);
+        bad = (bad == 0 && j < kb && !(v > 0.0)) ? j + 1 : bad;
+        const double rp = rsqrt_nr(v);
+        if (j > p0) {
+#pragma unroll
+            for (int m = j + 1; m < p0 + SUBP; ++m) r[m - C0] -= lprev * Lc[j - 1][m];
+        }
+        const double lj = r[j - C0] * rp;
+        r[j - C0] = lj;
+        Lc[j][t] = lj;
+        if (j + 1 < p0 + SUBP) r[j + 1 - C0] -= lj * rl_d(lj, j + 1);
+        lprev = lj;
+    }
+}
+
+// rank-8 update of the register columns [cbeg, cend) from sub-panel
+// [q0, q0 + 8): the sub-panel's entries of this lane's row from registers
+// (OWN) or from the LDS copy of the columns
+template <int C0, bool OWN>
+__device__ __forceinline__ void update8(double* r, int q0, int cbeg, int cend, int t, double (*Lc)[64]) {
+    double lt[SUBP];
+#pragma unroll
+    for (int q = 0; q < SUBP; ++q) lt[q] = OWN ? r[q0 + q - C0] : Lc[q0 + q][t];
+    double2 lv[2][SUBP];
+#pragma unroll
+    for (int q = 0; q < SUBP; ++q) lv[0][q] = *reinterpret_cast<const double2*>(&Lc[q0 + q][cbeg]);
+#pragma unroll
+    for (int c = cbeg; c < cend; c += 2) {
+        const int b = ((c - cbeg) >> 1) & 1;
+        if (c + 2 < cend) {
+#pragma unroll
+            for (int q = 0; q < SUBP; ++q) lv[b ^ 1][q] = *reinterpret_cast<const double2*>(&Lc[q0 + q][c + 2]);
+        }
+        double a0 = r[c - C0], a1 = r[c + 1 - C0], e0 = 0.0, e1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < SUBP; q += 2) {
+            a0 -= lt[q] * lv[b][q].x;
+            a1 -= lt[q] * lv[b][q].y;
+            e0 -= lt[q + 1] * lv[b][q + 1].x;
+            e1 -= lt[q + 1] * lv[b][q + 1].y;
+        }
+        r[c - C0] = a0 + e0;
+        r[c + 1 - C0] = a1 + e1;
+        asm volatile("" : "+v"(r[c - C0]), "+v"(r[c + 1 - C0]));
+    }
+}
+
+// the columns [C0, C0 + NC) of the kb x kb block src (rows in lanes < 32,
+// identity rows in lanes 32-63), as factor_diag32 loads them
+template <int C0, int NC>
+__device__ __forceinline__ void load_cols(double* r, const double (*src)[CB + 1], int kb, int t) {
+    const int tr = t & (CB - 1);
+    const bool mrow = t < CB;
+#pragma unroll
+    for (int c = C0; c < C0 + NC; ++c) {
+        const double a = src[tr][c];
+        r[c - C0] = (mrow && tr < kb && c < kb && tr >= c) ? a : (tr == c ? 1.0 : 0.0);
+    }
+}
+
+// waves 0, 2 and 3 of the look-ahead workgroup (prog = 0 on entry, fbad: 2
+// ints); L_kk^-T of the block at (k, k) to w, the first failing pivot to info
+__device__ __forceinline__ void factor_diag32_split(int k, int kb, double* w, int* info, double (*Lc)[64],
+                                                    const double (*src)[CB + 1], int* prog, int* fbad) {
+    const int t = threadIdx.x & 63, tr = t & (CB - 1), wave = threadIdx.x >> 6;
+    const bool mrow = t < CB;
+    int bad = 0;
+    if (wave == 0) {
+        double r[16];
+        load_cols<0, 16>(r, src, kb, t);
+        sweep8<0>(r, 0, kb, t, bad, Lc);
+        __hip_atomic_store(prog, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_wave_barrier();
+        update8<0, true>(r, 0, 8, 16, t, Lc);
+        sweep8<0>(r, 8, kb, t, bad, Lc);
+        if (t == 0) fbad[0] = bad;
+        __hip_atomic_store(prog, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!mrow) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) w[tr + CB * c] = r[c];
+        }
+    } else if (wave == 2) {
+        double r[8];
+        load_cols<16, 8>(r, src, kb, t);
+        bool ok = wait_prog(prog, 1);
+        update8<16, false>(r, 0, 16, 24, t, Lc);
+        ok = wait_prog(prog, 2) && ok;
+        update8<16, false>(r, 8, 16, 24, t, Lc);
+        sweep8<16>(r, 16, kb, t, bad, Lc);
+        if (!ok) bad = 17;   // (never: wave 0 always publishes)
+        if (t == 0) fbad[1] = bad;
+        __hip_atomic_store(prog, 3, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!mrow) {
+#pragma unroll
+            for (int c = 16; c < 24; ++c) w[tr + CB * c] = r[c - 16];
+        }
+    } else if (wave == 3) {
+        double r[8];
+        load_cols<24, 8>(r, src, kb, t);
+        bool ok = wait_prog(prog, 1);
+        update8<24, false>(r, 0, 24, 32, t, Lc);
+        ok = wait_prog(prog, 2) && ok;
+        update8<24, false>(r, 8, 24, 32, t, Lc);
+        ok = wait_prog(prog, 3) && ok;
+        update8<24, false>(r, 16, 24, 32, t, Lc);
+        sweep8<24>(r, 24, kb, t, bad, Lc);
+        if (!ok) bad = 25;
+        if (!mrow) {
+#pragma unroll
+            for (int c = 24; c < 32; ++c) w[tr + CB * c] = r[c - 24];
+        }
+        if (t == 0) {
+            // the first failing column of the block (waves 0 and 2 published
+            // theirs before the counter this wave waited for)
+            const int b0 = fbad[0], b2 = fbad[1];
+            const int first = b0 ? b0 : (b2 ? b2 : bad);
+            if (first != 0 && *info == 0) *info = k + first;
+        }
+    }
+}
+
 // the first diagonal block of a column range [k0, ...): winv block 0;
 // blockIdx.y = instance (A and winv at the given strides)
 __device__ __forceinline__ void chol_diag32_body(double* A, int n, int lda, int k, int kb, double* winv, int* info,
@@ -272,17 +421,20 @@ __device__ __forceinline__ void step_decode(int b, int T, int TJ, int* I, int* J
 // Its critical path is load -> L_D = A_Dk W (32 x 32) -> D -= L_D L_D^T ->
 // factor D, so that part is spread over all four waves, one 16 x 16 tile
 // each, with the MFMA operands loaded from memory in their lane layout (no
-// LDS staging before the first product).  Then wave 0 factors D; waves 2
-// and 3 store the panel rows of D; wave 1 loads, transforms and stores the
-// other 32 rows (panel rows, the block below D, the lower half of the block
-// right of it), all while the pivot chain runs.
+// LDS staging before the first product).  Then waves 0, 2 and 3 factor D
+// (factor_diag32_split; waves 2 and 3 first store the panel rows of D);
+// wave 1 loads, transforms and stores the other 32 rows (panel rows, the
+// block below D, the lower half of the block right of it), all while the
+// pivot chain runs.
 // Operand layouts (f64 16x16x4 MFMA): A[i][k] from lane i + 16 k, B[k][j]
 // from lane j + 16 k; output D[lane/16 + 4v][lane%16].
 __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu, int lda, int k, int kb,
                                                const double* w, double* wnext, int* info, double (*Lr)[CB + 1],
-                                               double (*Lr2)[CB + 1], double (*Ld)[CB + 1], double (*Ls)[64]) {
+                                               double (*Lr2)[CB + 1], double (*Ld)[CB + 1], double (*Ls)[64],
+                                               int* prog) {
     const int o = k + kb, kbn = min(CB, n - o);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    if (threadIdx.x == 0) prog[0] = 0;   // published by the barriers below
     // this wave's transform tile (rows rt, columns ct of L_D) and update tile
     // (ra, ca) of D: waves 0, 1, 2 own (0, 0), (1, 0), (1, 1); wave 3's
     // update tile (0, 1) is above the diagonal
@@ -330,15 +482,17 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
     }
     __syncthreads();
     if (wave == 0) {
-        factor_diag32<true>(A, lda, o, kbn, n, wnext, info, true, Ls, Ld);
+        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1);
         return;
     }
     if (wave >= 2) {
-        // L_D out, transposed into the upper triangle
+        // L_D out, transposed into the upper triangle, then sub-panels 2 and 3
+        // of the next diagonal block
         for (int e = (int)threadIdx.x - 128; e < CB * CB; e += 128) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && o + r < nr) A[(int64_t)(k + c) + (int64_t)(o + r) * lda] = Lr[r][c];
         }
+        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1);
         return;
     }
     // wave 1: rows R = [o + 32, o + 64)
@@ -420,6 +574,7 @@ __device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu
     __shared__ double Wt[CB][CB + 1];           // L_kk^-T
     __shared__ __align__(16) double Ls[CB][64];
     __shared__ double Ld[CB][CB + 1];           // the next diagonal block (look-ahead tile)
+    __shared__ int prog[4];                     // its factor's sub-panel counter and failed pivots
     A += by * sA;
     winv += by * sW;
     const int o = k + kb;
@@ -428,7 +583,7 @@ __device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu
         // tile (0, 0): the look-ahead tile (Xi and Xj serve as its LDS)
         chol_lookahead(A, n, nr, ncu, lda, k, kb, w, winv + (size_t)((o - k0) / CB) * CB * CB, info,
                        reinterpret_cast<double (*)[CB + 1]>(&Xi[0][0]), reinterpret_cast<double (*)[CB + 1]>(&Xj[0][0]),
-                       Ld, Ls);
+                       Ld, Ls, prog);
         return;
     }
     int tI, tJ;
@@ -483,18 +638,37 @@ __device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu
     // of 16 x 16, two per wave and side (one side on diagonal tiles); f64 MFMA
     // output D[lane/16 + 4v][lane%16]; the results replace the staged panels
     // once every wave has read them
+    // (wave = row tile rt of 4, q = column tile ct of 2).  Every operand is
+    // read from LDS first and the two cases run as separate branch-free
+    // loops: a per-step test of dg between the products makes the compiler
+    // move an accumulator between AGPRs and VGPRs and wait out every product
     d4 ai[2], aj[2];
+    {
+        double xi[CB / 4], xj[CB / 4], b0[CB / 4], b1[CB / 4];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int tile = wave * 2 + q;           // 0..7
-        const int rt = tile >> 1, ct = tile & 1; // row tile (of 4), column tile (of 2)
-        ai[q] = d4{0.0, 0.0, 0.0, 0.0};
-        aj[q] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < CB / 4; ++s) {
+            const int m = 4 * s + lk;
+            xi[s] = Xi[m][wave * 16 + li];
+            xj[s] = Xj[m][wave * 16 + li];   // unused on diagonal tiles
+            b0[s] = Wt[m][li];
+            b1[s] = Wt[m][16 + li];
+        }
 #pragma unroll
-        for (int m = 0; m < CB; m += 4) {
-            const double b = Wt[m + lk][ct * 16 + li];
-            ai[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xi[m + lk][rt * 16 + li], b, ai[q], 0, 0, 0);
-            if (!dg) aj[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xj[m + lk][rt * 16 + li], b, aj[q], 0, 0, 0);
+        for (int q = 0; q < 2; ++q) ai[q] = aj[q] = d4{0.0, 0.0, 0.0, 0.0};
+        if (dg) {
+#pragma unroll
+            for (int s = 0; s < CB / 4; ++s) {
+                ai[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[s], b0[s], ai[0], 0, 0, 0);
+                ai[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[s], b1[s], ai[1], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < CB / 4; ++s) {
+                ai[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[s], b0[s], ai[0], 0, 0, 0);
+                ai[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[s], b1[s], ai[1], 0, 0, 0);
+                aj[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(xj[s], b0[s], aj[0], 0, 0, 0);
+                aj[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xj[s], b1[s], aj[1], 0, 0, 0);
+            }
         }
     }
     __syncthreads();
