@@ -249,9 +249,9 @@ __device__ __forceinline__ void factor_diag32(double* A, int lda, int k, int kb,
     }
 }
 
-// ---- the look-ahead tile's diagonal factor over three wavefronts: wave 0
-// sweeps sub-panels 0 and 1 (columns 0-15), waves 2 and 3 sub-panels 2 and
-// 3; while wave 0 runs its pivots, the others apply each finished
+// ---- the look-ahead tile's diagonal factor over three wavefronts: one
+// sweeps sub-panels 0 and 1 (columns 0-15), the others sub-panels 2 and 3;
+// while the first runs its pivots, the others apply each finished
 // sub-panel's rank-8 update to their own columns, so only the next
 // sub-panel's update stays on the pivot chain.  Same arithmetic, in the same
 // order, as factor_diag32 (bit-identical L_kk^-T); the waves hand over
@@ -334,14 +334,16 @@ __device__ __forceinline__ void load_cols(double* r, const double (*src)[CB + 1]
     }
 }
 
-// waves 0, 2 and 3 of the look-ahead workgroup (prog = 0 on entry, fbad: 2
-// ints); L_kk^-T of the block at (k, k) to w, the first failing pivot to info
+// one wavefront per part of the look-ahead workgroup: part 0 = sub-panels 0
+// and 1, part 2 = sub-panel 2, part 3 = sub-panel 3 (prog = 0 on entry,
+// fbad: 2 ints); L_kk^-T of the block at (k, k) to w, the first failing
+// pivot to info
 __device__ __forceinline__ void factor_diag32_split(int k, int kb, double* w, int* info, double (*Lc)[64],
-                                                    const double (*src)[CB + 1], int* prog, int* fbad) {
-    const int t = threadIdx.x & 63, tr = t & (CB - 1), wave = threadIdx.x >> 6;
+                                                    const double (*src)[CB + 1], int* prog, int* fbad, int part) {
+    const int t = threadIdx.x & 63, tr = t & (CB - 1);
     const bool mrow = t < CB;
     int bad = 0;
-    if (wave == 0) {
+    if (part == 0) {
         double r[16];
         load_cols<0, 16>(r, src, kb, t);
         sweep8<0>(r, 0, kb, t, bad, Lc);
@@ -355,7 +357,7 @@ __device__ __forceinline__ void factor_diag32_split(int k, int kb, double* w, in
 #pragma unroll
             for (int c = 0; c < 16; ++c) w[tr + CB * c] = r[c];
         }
-    } else if (wave == 2) {
+    } else if (part == 2) {
         double r[8];
         load_cols<16, 8>(r, src, kb, t);
         bool ok = wait_prog(prog, 1);
@@ -370,7 +372,7 @@ __device__ __forceinline__ void factor_diag32_split(int k, int kb, double* w, in
 #pragma unroll
             for (int c = 16; c < 24; ++c) w[tr + CB * c] = r[c - 16];
         }
-    } else if (wave == 3) {
+    } else if (part == 3) {
         double r[8];
         load_cols<24, 8>(r, src, kb, t);
         bool ok = wait_prog(prog, 1);
@@ -421,29 +423,37 @@ __device__ __forceinline__ void step_decode(int b, int T, int TJ, int* I, int* J
 // Its critical path is load -> L_D = A_Dk W (32 x 32) -> D -= L_D L_D^T ->
 // factor D, so that part is spread over all four waves, one 16 x 16 tile
 // each, with the MFMA operands loaded from memory in their lane layout (no
-// LDS staging before the first product).  Then waves 0, 2 and 3 factor D
-// (factor_diag32_split; waves 2 and 3 first store the panel rows of D);
-// wave 1 loads, transforms and stores the other 32 rows (panel rows, the
-// block below D, the lower half of the block right of it), all while the
-// pivot chain runs.
+// LDS staging before the first product).  Wave 3, which has no tile of D to
+// update (its quarter is above the diagonal), meanwhile issues the loads of
+// the other 32 rows R (panel rows, the block below D, the lower half of the
+// block right of it) and then transforms and stores them; waves 0, 1 and 2
+// factor D (factor_diag32_split; waves 1 and 2 first store the panel rows
+// of D, wave 0 afterwards those of R), all while the pivot chain runs.
 // Operand layouts (f64 16x16x4 MFMA): A[i][k] from lane i + 16 k, B[k][j]
-// from lane j + 16 k; output D[lane/16 + 4v][lane%16].
+// from lane j + 16 k; output D[lane/16 + 4v][lane%16].  Products whose
+// result goes to memory are formed transposed (operands swapped), so that a
+// lane holds element (lane%16, lane/16 + 4v) of the quarter and a load or
+// store instruction covers 16 consecutive rows of 4 columns.
 __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu, int lda, int k, int kb,
                                                const double* w, double* wnext, int* info, double (*Lr)[CB + 1],
                                                double (*Lr2)[CB + 1], double (*Ld)[CB + 1], double (*Ls)[64],
                                                int* prog) {
-    const int o = k + kb, kbn = min(CB, n - o);
+    const int o = k + kb, kbn = min(CB, n - o), rb = o + CB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-    if (threadIdx.x == 0) prog[0] = 0;   // published by the barriers below
+    if (threadIdx.x == 0) {
+        prog[0] = 0;   // the diagonal factor's sub-panel counter
+        prog[3] = 0;   // L_R staged in Lr2 (wave 3 -> wave 0); published by the barriers below
+    }
     // this wave's transform tile (rows rt, columns ct of L_D) and update tile
     // (ra, ca) of D: waves 0, 1, 2 own (0, 0), (1, 0), (1, 1); wave 3's
     // update tile (0, 1) is above the diagonal
     const int rt = wave >> 1, ct = wave & 1;
     const int ra = (wave == 1 || wave == 2) ? 1 : 0, ca = (wave >= 2) ? 1 : 0;
-    double xa[CB / 4], wb[CB / 4], od[4];
+    double od[4];
     {
         const int row = o + rt * 16 + li;
         const int64_t rc = min(row, nr - 1);
+        double xa[CB / 4], wb[CB / 4];
 #pragma unroll
         for (int s = 0; s < CB / 4; ++s) {
             const int m = 4 * s + lk;
@@ -452,7 +462,7 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-            od[v] = A[(int64_t)min(o + ra * 16 + lk + 4 * v, nr - 1) + (int64_t)min(o + ca * 16 + li, nr - 1) * lda];
+            od[v] = A[(int64_t)min(o + ra * 16 + li, nr - 1) + (int64_t)min(o + ca * 16 + lk + 4 * v, nr - 1) * lda];
         d4 t = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < CB / 4; ++s) {
@@ -463,15 +473,36 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
         for (int v = 0; v < 4; ++v) Lr[rt * 16 + lk + 4 * v][ct * 16 + li] = t[v];
     }
     __syncthreads();
-    if (wave < 3) {
+    double xb[2][CB / 4], wc2[2][CB / 4], q10[2][2][4], q11[2][2][4];
+    if (wave == 3) {
+        // the rows R: issued while waves 0-2 update D
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int s = 0; s < CB / 4; ++s) {
+                const int m = 4 * s + lk;
+                xb[h][s] = A[(int64_t)min(rb + h * 16 + li, nr - 1) + (int64_t)(k + min(m, kb - 1)) * lda];
+                wc2[h][s] = w[m + CB * (h * 16 + li)];
+            }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t r = min(rb + a * 16 + li, nr - 1);
+                    q10[a][b][v] = A[r + (int64_t)min(o + b * 16 + lk + 4 * v, nr - 1) * lda];
+                    q11[a][b][v] = A[r + (int64_t)min(rb + b * 16 + lk + 4 * v, nr - 1) * lda];
+                }
+    } else {
         d4 u = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < CB / 4; ++s)
-            u = __builtin_amdgcn_mfma_f64_16x16x4f64(Lr[ra * 16 + li][4 * s + lk], Lr[ca * 16 + li][4 * s + lk], u, 0,
+            u = __builtin_amdgcn_mfma_f64_16x16x4f64(Lr[ca * 16 + li][4 * s + lk], Lr[ra * 16 + li][4 * s + lk], u, 0,
                                                      0, 0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int lr = ra * 16 + lk + 4 * v, lc = ca * 16 + li;
+            const int lr = ra * 16 + li, lc = ca * 16 + lk + 4 * v;
             const double val = od[v] - u[v];
             Ld[lr][lc] = val;
             // rows past the diagonal block (the right-hand side row, or rows
@@ -482,42 +513,34 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
     }
     __syncthreads();
     if (wave == 0) {
-        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1);
+        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1, 0);
+        if (rb < nr) {
+            // then L_R's panel rows out (transposed), once wave 3 has staged
+            // them (a wait that ran out, never expected, fails the factorization)
+            if (wait_prog(prog + 3, 1)) {
+                for (int e = lane; e < CB * CB; e += 64) {
+                    const int c = e & (CB - 1), r = e >> 5;
+                    if (c < kb && rb + r < nr) A[(int64_t)(k + c) + (int64_t)(rb + r) * lda] = Lr2[r][c];
+                }
+            } else if (lane == 0 && *info == 0) {
+                *info = o + 1;
+            }
+        }
         return;
     }
-    if (wave >= 2) {
-        // L_D out, transposed into the upper triangle, then sub-panels 2 and 3
-        // of the next diagonal block
-        for (int e = (int)threadIdx.x - 128; e < CB * CB; e += 128) {
+    if (wave <= 2) {
+        // L_D out, transposed into the upper triangle, then sub-panels 3
+        // (wave 1) and 2 (wave 2) of the next diagonal block
+        for (int e = (int)threadIdx.x - 64; e < CB * CB; e += 128) {
             const int c = e & (CB - 1), r = e >> 5;
             if (c < kb && o + r < nr) A[(int64_t)(k + c) + (int64_t)(o + r) * lda] = Lr[r][c];
         }
-        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1);
+        factor_diag32_split(o, kbn, wnext, info, Ls, Ld, prog, prog + 1, wave == 1 ? 3 : 2);
         return;
     }
-    // wave 1: rows R = [o + 32, o + 64)
-    const int rb = o + CB;
+    // wave 3: rows R = [o + 32, o + 64)
     if (rb >= nr) return;
-    double xb[2][CB / 4], wc2[2][CB / 4], q10[2][2][4], q11[2][2][4];
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int s = 0; s < CB / 4; ++s) {
-            const int m = 4 * s + lk;
-            xb[h][s] = A[(int64_t)min(rb + h * 16 + li, nr - 1) + (int64_t)(k + min(m, kb - 1)) * lda];
-            wc2[h][s] = w[m + CB * (h * 16 + li)];
-        }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t r = min(rb + a * 16 + lk + 4 * v, nr - 1);
-                q10[a][b][v] = A[r + (int64_t)min(o + b * 16 + li, nr - 1) * lda];
-                q11[a][b][v] = A[r + (int64_t)min(rb + b * 16 + li, nr - 1) * lda];
-            }
-    // L_R = A_Rk W into Lr2 (this wave's own LDS), then its panel rows out
+    // L_R = A_Rk W into Lr2 (its panel rows are stored by wave 0)
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -531,13 +554,10 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
 #pragma unroll
             for (int v = 0; v < 4; ++v) Lr2[a * 16 + lk + 4 * v][b * 16 + li] = t[v];
         }
+    __hip_atomic_store(prog + 3, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_wave_barrier();
-    for (int e = lane; e < CB * CB; e += 64) {
-        const int c = e & (CB - 1), r = e >> 5;
-        if (c < kb && rb + r < nr) A[(int64_t)(k + c) + (int64_t)(rb + r) * lda] = Lr2[r][c];
-    }
     // the block below D (rows R, columns [o, o + 32)) and the lower half of
-    // the block R x R
+    // the block R x R (transposed products: element (li, lk + 4 v))
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -546,13 +566,13 @@ __device__ __forceinline__ void chol_lookahead(double* A, int n, int nr, int ncu
 #pragma unroll
             for (int s = 0; s < CB / 4; ++s) {
                 const double ar = Lr2[a * 16 + li][4 * s + lk];
-                u = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, Lr[b * 16 + li][4 * s + lk], u, 0, 0, 0);
-                if (b <= a) x = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, Lr2[b * 16 + li][4 * s + lk], x, 0, 0, 0);
+                u = __builtin_amdgcn_mfma_f64_16x16x4f64(Lr[b * 16 + li][4 * s + lk], ar, u, 0, 0, 0);
+                if (b <= a) x = __builtin_amdgcn_mfma_f64_16x16x4f64(Lr2[b * 16 + li][4 * s + lk], ar, x, 0, 0, 0);
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = rb + a * 16 + lk + 4 * v;
-                const int c0 = o + b * 16 + li, c1 = rb + b * 16 + li;
+                const int r = rb + a * 16 + li;
+                const int c0 = o + b * 16 + lk + 4 * v, c1 = rb + b * 16 + lk + 4 * v;
                 if (r < nr && c0 < ncu) A[(int64_t)r + (int64_t)c0 * lda] = q10[a][b][v] - u[v];
                 if (b <= a && r < nr && c1 < ncu && r >= c1) A[(int64_t)r + (int64_t)c1 * lda] = q11[a][b][v] - x[v];
             }
@@ -599,15 +619,19 @@ __device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu
     // every load of the kernel is issued up front, unconditionally (clamped
     // addresses, out-of-range values replaced afterwards), so the round trips
     // overlap instead of serializing one wait per guarded load
-    double old[2][2][4];   // this lane's 16 entries of the tile A_IJ
+    // this lane's 16 entries of the tile A_IJ, in the layout of the
+    // transposed product (row li, column lk + 4 v of each 16 x 16 quarter):
+    // 16 consecutive rows of 4 columns per load, 4 cache lines, where the
+    // product's own layout would touch 16
+    double old[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = min(i0 + wr * 32 + a * 16 + lk + 4 * v, nr - 1);
-                const int c = min(j0 + wc * 32 + b * 16 + li, nr - 1);
+                const int r = min(i0 + wr * 32 + a * 16 + li, nr - 1);
+                const int c = min(j0 + wc * 32 + b * 16 + lk + 4 * v, nr - 1);
                 old[a][b][v] = A[(int64_t)r + (int64_t)c * lda];
             }
     double si[64 * CB / 256], sj[64 * CB / 256], sw[CB * CB / 256];
@@ -707,16 +731,17 @@ __device__ __forceinline__ void chol_step_body(double* A, int n, int nr, int ncu
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], acc[a][b], 0, 0, 0);
     }
+    // (L_J L_I^T: the transposed quarter, element (li, lk + 4 v))
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = i0 + wr * 32 + a * 16 + lk + 4 * v;
-                const int c = j0 + wc * 32 + b * 16 + li;
+                const int r = i0 + wr * 32 + a * 16 + li;
+                const int c = j0 + wc * 32 + b * 16 + lk + 4 * v;
                 if (r < nr && c < ncu && r >= c) A[(int64_t)r + (int64_t)c * lda] = old[a][b][v] - acc[a][b][v];
             }
     if (tJ == 0) {
