@@ -722,6 +722,360 @@ __device__ __forceinline__ void estimate_body(const est_args& a, int bx, int gx)
     }
 }
 
+// ---- the forces launch of the FP32 network on the matrix cores
+// Every term of the force is an outer product over (unit, site).  A changed
+// connection c (and the unchanged configurations together, as one more with
+// W = their mel sum and no flips) adds, for each side (row configuration s,
+// column configuration e, x_c its +-1 configuration after the flips):
+//   lambda / mu weights  A_c(u) x_c(i):  A_c(u) = (1/2) g_u f'(theta_u(c)),
+//                        g_u = Re(V W_c) (lambda), -Im(V W_c) (mu), sign -
+//                        on the mu units' column side
+//   Pi weights           Re(V W_c f'(z_a(c))) (x^s_c + x^e_c)(i) / 2 (u_la)
+//                        -Im(V W_c f'(z_a(c))) (x^s_c - x^e_c)(i) / 2 (u_mu)
+// and the biases the same sums with x = 1 (hidden units, d) or with the unit
+// factor 1 (visible biases).  So the weight forces are products [units x
+// terms] [terms x sites] over all terms of a wave's samples, on
+// v_mfma_f32_16x16x4f32: four connections at a time, lane l = i + 16 k
+// taking connection k of the four, the units 16 t + i of it (operand A of
+// tile t: unit row i, term k) and site i of its configuration (operand B:
+// term k, site column i).  Each lane re-evaluates the changed angles of its
+// units for its own connection (one FMA per unit and flipped site, no
+// register indexing), the bias sums stay in per-lane registers, and the
+// products accumulate in FP32 over at most kForceFlush samples before they
+// go into the workgroup's exact accumulator (fx_add), so the result does not
+// depend on the order in which waves arrive.
+//
+// The tiles come in three sizes (class 0, 1, 2: TW = 2, 4, 8 tiles of
+// lambda / mu units, TP = 1, 2, 4 of Pi units), the smallest that holds 2 nh
+// and na.  The launch stages the parameters it needs in its own LDS layout,
+// zero padded to whole tiles (SW = 16 TW units, SP = 16 TP Pi units), so no
+// lane ever needs a bounds test on its units:
+//   HW [N + 1][SW]  row 0 the hidden biases (lambda units, then mu), row
+//                   1 + i the weights of site i
+//   PU [N][SP][2]   (u_la, u_mu) of Pi unit a, site i
+//   PD [SP]         d_la
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int kForceFlush = 8;
+
+// tile class of a network (-1: more than 64 hidden or Pi units)
+__host__ __device__ inline int force_class(const ndm_dims& d) {
+    const int k = max((2 * d.nh + 31) / 32, (d.na + 15) / 16);
+    return k <= 1 ? 0 : (k <= 2 ? 1 : (k <= 4 ? 2 : -1));
+}
+// floats of the staged parameters of the FP32 forces launch
+__host__ __device__ inline int force_stage_floats(const ndm_dims& d) {
+    const int c = force_class(d) < 0 ? 2 : force_class(d);
+    return (32 << c) * (d.N + 1) + (16 << c) * (2 * d.N + 1);
+}
+
+template <int NMAX, int CLS, int ACT>
+__device__ __forceinline__ void force_mma_body(const est_args& a, int bx, int gx) {
+    using T = float;
+    static_assert(NMAX <= 16, "the sites fill one 16-column tile");
+    constexpr int TW = 2 << CLS, TP = 1 << CLS;   // 16-unit tiles: lambda / mu units (2 nh), Pi units (na)
+    constexpr int SW = 16 * TW, SP = 16 * TP, RS = (SW + 63) / 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ndm_dims d = a.d;
+    const int N = d.N, nh = d.nh, na = d.na;
+    const int pbytes = max(a.lds_param_bytes, (4 * force_stage_floats(d) + 15) & ~15);
+    T* HW = reinterpret_cast<T*>(smem);
+    T* PU = HW + SW * (N + 1);
+    T* PD = PU + 2 * SP * N;
+    unsigned long long* sF = reinterpret_cast<unsigned long long*>(smem + pbytes);
+    int* poison = reinterpret_cast<int*>(sF + 2 * d.np);
+    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem + pbytes + fx_bytes(d.np));
+    const int wpb = blockDim.x >> 6;
+    uint2* sM = reinterpret_cast<uint2*>(sW + wpb * a.max_conn);
+    const T* gp = reinterpret_cast<const T*>(a.p);
+    for (int k = threadIdx.x; k < SW * (N + 1); k += blockDim.x) {
+        const int r = k / SW, u = k - r * SW;
+        const int src = u < nh ? (r == 0 ? d.hla + u : d.wla + u + nh * (r - 1))
+                               : (r == 0 ? d.hmu + u - nh : d.wmu + u - nh + nh * (r - 1));
+        HW[k] = u < 2 * nh ? gp[src] : T(0);
+    }
+    for (int k = threadIdx.x; k < SP * N; k += blockDim.x) {
+        const int i = k / SP, au = k - i * SP;
+        const bool ok = au < na;
+        PU[2 * k] = ok ? gp[d.ula + au + na * i] : T(0);
+        PU[2 * k + 1] = ok ? gp[d.umu + au + na * i] : T(0);
+    }
+    for (int k = threadIdx.x; k < SP; k += blockDim.x) PD[k] = k < na ? gp[d.dla + k] : T(0);
+    for (int k = threadIdx.x; k < 2 * d.np; k += blockDim.x) sF[k] = 0ull;
+    if (threadIdx.x == 0) *poison = 0;
+    __syncthreads();
+
+    const int lane0 = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t gw = (int64_t)bx * wpb + wave, nw = (int64_t)gx * wpb;
+    const uint32_t pmask = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
+    cplx<T>* wbuf = sW + wave * a.max_conn;
+    uint2* mbuf = sM + wave * a.max_conn;
+    const T half = T(0.5);
+    const int il = lane0 & 15, kg = lane0 >> 4;
+
+    f4v Fw[TW], Fr[TP], Fi[TP];
+    T hb[TW], db[TP], vbl = T(0), vbm = T(0);
+#pragma unroll
+    for (int t = 0; t < TW; ++t) { Fw[t] = f4v{0.f, 0.f, 0.f, 0.f}; hb[t] = T(0); }
+#pragma unroll
+    for (int t = 0; t < TP; ++t) { Fr[t] = Fi[t] = f4v{0.f, 0.f, 0.f, 0.f}; db[t] = T(0); }
+
+    int nacc = 0;
+    for (int64_t s = gw; s < a.Ns; s += nw) {
+        // an opaque copy of the lane index per sample keeps the compiler from
+        // hoisting (and spilling) every lane-dependent address out of the loop
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const uint64_t key = a.keys[s];
+        const uint32_t ks = (uint32_t)key & pmask;
+        const uint32_t ke = (uint32_t)(key >> N) & pmask;
+
+        // angles and f' of the sample, one unit per lane (u = lane + 64 r; Pi
+        // unit lane), then moved into the product layout (units 16 t + i);
+        // padding units have zero angles
+        T ts1[RS], te1[RS], as1[RS], ae1[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int u = lane + 64 * r;
+            T ts = HW[u], te = ts;
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                if (i < N) {
+                    const T wi = HW[SW * (1 + i) + u];
+                    ts += wi * xv<T>(ks, i);
+                    te += wi * xv<T>(ke, i);
+                }
+            }
+            ts1[r] = ts;
+            te1[r] = te;
+            as1[r] = act_df<ACT>(ts);
+            ae1[r] = act_df<ACT>(te);
+        }
+        cplx<T> pz1 = cmk(T(0), T(0));
+        if (lane < SP) {
+            T zr = PD[lane], zi = T(0);
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                if (i < N) {
+                    const T xs = xv<T>(ks, i), xe = xv<T>(ke, i);
+                    const float2 uu = *reinterpret_cast<const float2*>(PU + 2 * (SP * i + lane));
+                    zr += half * uu.x * (xs + xe);
+                    zi += half * uu.y * (xs - xe);
+                }
+            }
+            pz1 = cmk(zr, zi);
+        }
+        T ths[TW], the[TW], as[TW], ae[TW];
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            const int src = 16 * (t & 3) + il, r = t >> 2;
+            ths[t] = __shfl(ts1[r], src, 64);
+            the[t] = __shfl(te1[r], src, 64);
+            as[t] = __shfl(as1[r], src, 64);
+            ae[t] = __shfl(ae1[r], src, 64);
+        }
+        cplx<T> pz[TP];
+#pragma unroll
+        for (int t = 0; t < TP; ++t) pz[t] = cmk(__shfl(pz1.re, 16 * t + il, 64), __shfl(pz1.im, 16 * t + il, 64));
+
+        if (a.do_conn) {
+            // the connections of pass 1 from the walk launch
+            const int64_t rb = s * (int64_t)a.max_conn;
+            const int ncc = a.cnum[s];
+            const T* wd = reinterpret_cast<const T*>(a.cwd) + 2 * s;
+            const T* ll = reinterpret_cast<const T*>(a.Lloc) + 2 * s;
+            const cplx<T> Wdiag = cmk(wd[0], wd[1]);
+            const cplx<T> V = cmk((T)((double)ll[0] * a.invNs), (T)(-(double)ll[1] * a.invNs));
+            const cplx<T>* rw = reinterpret_cast<const cplx<T>*>(a.crw) + rb;
+            for (int c = lane0; c < ncc; c += 64) {
+                wbuf[c] = rw[c];
+                mbuf[c] = a.crm[rb + c];
+            }
+            __builtin_amdgcn_wave_barrier();
+
+            // four connections per round; index ncc is the unchanged configurations
+            for (int c0 = 0; c0 <= ncc; c0 += 4) {
+                const int c = c0 + kg;
+                cplx<T> W = cmk(T(0), T(0));
+                uint32_t ms = 0u, me = 0u;
+                if (c < ncc) {
+                    W = wbuf[c];
+                    const uint2 mm = mbuf[c];
+                    ms = mm.x;
+                    me = mm.y;
+                } else if (c == ncc) {
+                    W = Wdiag;
+                }
+                const cplx<T> VW = cmul(V, W);
+
+                // lambda / mu units: f' at the connection's angles on each side
+                T As[TW], Ae[TW];
+#pragma unroll
+                for (int t = 0; t < TW; ++t) { As[t] = as[t]; Ae[t] = ae[t]; }
+                if (ms) {
+                    T th[TW];
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) th[t] = ths[t];
+                    for (uint32_t m = ms; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        const T cf = cflip<T>(ks, i);
+                        const T* w = HW + SW * (1 + i) + il;
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) th[t] += cf * w[16 * t];
+                    }
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) As[t] = act_df<ACT>(th[t]);
+                }
+                if (me) {
+                    T th[TW];
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) th[t] = the[t];
+                    for (uint32_t m = me; m; m &= m - 1) {
+                        const int i = __builtin_ctz(m);
+                        const T cf = cflip<T>(ke, i);
+                        const T* w = HW + SW * (1 + i) + il;
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) th[t] += cf * w[16 * t];
+                    }
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) Ae[t] = act_df<ACT>(th[t]);
+                }
+                const T gl = half * VW.re, gm = -half * VW.im;
+                const bool sok = il < N;
+                const T Bs = sok ? xv<T>(ks ^ ms, il) : T(0), Be = sok ? xv<T>(ke ^ me, il) : T(0);
+#pragma unroll
+                for (int t = 0; t < TW; ++t) {
+                    const int u = 16 * t + il;
+                    const T gs = u < nh ? gl : (u < 2 * nh ? gm : T(0));
+                    const T ge = u < nh ? gl : -gs;
+                    const T xa = gs * As[t], ya = ge * Ae[t];
+                    hb[t] += xa + ya;
+                    Fw[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, Bs, Fw[t], 0, 0, 0);
+                    Fw[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya, Be, Fw[t], 0, 0, 0);
+                }
+
+                // Pi units: f'(z) at the connection's angle (for the unchanged
+                // configurations the sample's own)
+                const T Bp = half * (Bs + Be), Bm = half * (Bs - Be);
+                cplx<T> z[TP];
+#pragma unroll
+                for (int t = 0; t < TP; ++t) z[t] = pz[t];
+                for (uint32_t m = ms; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cc = cflip<T>(ks, i) * half;
+                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) {
+                        const float2 uu = pu[16 * t];
+                        z[t].re += cc * uu.x;
+                        z[t].im += cc * uu.y;
+                    }
+                }
+                for (uint32_t m = me; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cc = cflip<T>(ke, i) * half;
+                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) {
+                        const float2 uu = pu[16 * t];
+                        z[t].re += cc * uu.x;
+                        z[t].im -= cc * uu.y;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TP; ++t) {
+                    const cplx<T> vp = cmul(VW, cact_df<ACT>(z[t]));
+                    const bool ok = 16 * t + il < na;
+                    const T ar = ok ? vp.re : T(0), ai = ok ? -vp.im : T(0);
+                    db[t] += ar;
+                    Fr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, Bp, Fr[t], 0, 0, 0);
+                    Fi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, Bm, Fi[t], 0, 0, 0);
+                }
+                vbl += VW.re * Bp;
+                vbm -= VW.im * Bm;
+            }
+        }
+
+        // every kForceFlush samples and after the wave's last one: the terms
+        // into the exact accumulator, product entry (row 4 (lane / 16) + v,
+        // column lane % 16) of each tile and the bias sums over the four
+        // connection groups (lanes i + 16 k, k = 0..3)
+        if (++nacc == kForceFlush || s + nw >= a.Ns) {
+            nacc = 0;
+            // the entry indexes from the opaque lane copy, so that they are
+            // not hoisted out of the sample loop and spilled
+            const int il = lane & 15, kg = lane >> 4;
+            const int PB_bla = 0, PB_hla = N, PB_wla = N + nh;
+            const int PB_d = d.nlam, PB_ula = d.nlam + na, PB_umu = d.nlam + na + na * N;
+            const int PB_bmu = d.nlam + d.npi, PB_hmu = PB_bmu + N, PB_wmu = PB_hmu + nh;
+#pragma unroll
+            for (int t = 0; t < TW; ++t) {
+                if (16 * t < 2 * nh) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int u = 16 * t + 4 * kg + v;
+                        if (u < 2 * nh && il < N)
+                            fx_add(sF, (u < nh ? PB_wla + u : PB_wmu + u - nh) + nh * il, (double)Fw[t][v], poison);
+                    }
+                    Fw[t] = f4v{0.f, 0.f, 0.f, 0.f};
+                    T h = hb[t];
+                    h += __shfl_xor(h, 16, 64);
+                    h += __shfl_xor(h, 32, 64);
+                    const int u = 16 * t + il;
+                    if (kg == 0 && u < 2 * nh) fx_add(sF, u < nh ? PB_hla + u : PB_hmu + u - nh, (double)h, poison);
+                    hb[t] = T(0);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < TP; ++t) {
+                if (16 * t < na) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int au = 16 * t + 4 * kg + v;
+                        if (au < na && il < N) {
+                            fx_add(sF, PB_ula + au + na * il, (double)Fr[t][v], poison);
+                            fx_add(sF, PB_umu + au + na * il, (double)Fi[t][v], poison);
+                        }
+                    }
+                    Fr[t] = Fi[t] = f4v{0.f, 0.f, 0.f, 0.f};
+                    T h = db[t];
+                    h += __shfl_xor(h, 16, 64);
+                    h += __shfl_xor(h, 32, 64);
+                    if (kg == 0 && 16 * t + il < na) fx_add(sF, PB_d + 16 * t + il, (double)h, poison);
+                    db[t] = T(0);
+                }
+            }
+            T bl = vbl, bm = vbm;
+            bl += __shfl_xor(bl, 16, 64);
+            bl += __shfl_xor(bl, 32, 64);
+            bm += __shfl_xor(bm, 16, 64);
+            bm += __shfl_xor(bm, 32, 64);
+            if (kg == 0 && il < N) {
+                fx_add(sF, PB_bla + il, (double)bl, poison);
+                fx_add(sF, PB_bmu + il, (double)bm, poison);
+            }
+            vbl = vbm = T(0);
+        }
+    }
+
+    __syncthreads();
+    double* out = a.Fpart + (int64_t)bx * d.np;
+    const bool bad = *poison != 0;
+    for (int k = threadIdx.x; k < d.np; k += blockDim.x) out[k] = bad ? __builtin_nan("") : fx_value(sF, k);
+}
+
+// the FP32 forces launch of one tile class (and its packed form)
+template <int NMAX, int CLS, int ACT>
+__global__ __launch_bounds__(512, CLS < 2 ? 4 : 2) void k_force_mma(est_args a) {
+    force_mma_body<NMAX, CLS, ACT>(a, blockIdx.x, gridDim.x);
+}
+template <int NMAX, int CLS, int ACT>
+__global__ __launch_bounds__(512, CLS < 2 ? 4 : 2) void k_force_mma_pk(const nq_pk_member* M, const uint2* blk) {
+    const uint2 b = blk[blockIdx.x];
+    const nq_pk_member& m = M[b.x & 0xffff];
+    force_mma_body<NMAX, CLS, ACT>(m.ef, (int)(b.y & 0xffff), m.est_grid);
+}
+
 #define NQ_EST_BOUNDS __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD) ? 4 : 2)
 
 template <typename T, int NMAX, int RS, int PS, bool TAB_LDS, int KIND, int ACT>
@@ -758,9 +1112,22 @@ hipError_t launch_t(const est_args& a0, int grid, hipStream_t st) {
     if (kind == EK_FORCE) a.tab_lds = 0;                             // the records replace the walk
     const int wpb = est_waves(a, sizeof(T));
     const size_t lds = est_lds_bytes(a, sizeof(T), wpb);
-    if (kind == EK_FORCE)
-        hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_FORCE, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
-    else if (kind == EK_GRAD) {
+    if (kind == EK_FORCE) {
+        if constexpr (sizeof(T) == 4) {
+            const int cls = force_class(a.d);
+            if (cls == 0)
+                hipLaunchKernelGGL((k_force_mma<NMAX, 0, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+            else if (cls == 1)
+                hipLaunchKernelGGL((k_force_mma<NMAX, 1, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+            else if (cls == 2)
+                hipLaunchKernelGGL((k_force_mma<NMAX, 2, ACT>), dim3(grid), dim3(64 * wpb), lds, st, a);
+            else
+                return hipErrorInvalidValue;
+        } else {
+            hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, false, EK_FORCE, ACT>), dim3(grid), dim3(64 * wpb), lds, st,
+                               a);
+        }
+    } else if (kind == EK_GRAD) {
         if (a.tab_lds)
             hipLaunchKernelGGL((k_estimate<T, NMAX, RS, PS, true, EK_GRAD, ACT>), dim3(grid), dim3(64), lds, st, a);
         else
@@ -794,15 +1161,22 @@ hipError_t launch_prec(const est_args& a, int grid, hipStream_t st) {
 }
 
 // packed launches of one precision; variant bits: 0 force (else walk), 1 N > 8
-// (NMAX 16), 2 table in LDS, 3 logcosh (the walk and force kinds, one unit
-// row and one Pi row per lane only)
+// (NMAX 16), 2 table in LDS, 3 logcosh, 5 tile class 0 of the FP32 forces
+// (the walk and force kinds, one unit row and one Pi row per lane only)
 template <typename T, int NMAX, int ACT>
 hipError_t pk_launch_n(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
     const dim3 g((unsigned)L.nblk), b((unsigned)L.threads);
     const bool force = (L.variant & 1) != 0, tl = (L.variant & 4) != 0;
-    if (force)
-        hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, false, EK_FORCE, ACT>), g, b, L.lds, st, M, L.blk);
-    else if (tl)
+    if (force) {
+        if constexpr (sizeof(T) == 4) {
+            if (L.variant & 32)
+                hipLaunchKernelGGL((k_force_mma_pk<NMAX, 0, ACT>), g, b, L.lds, st, M, L.blk);
+            else
+                hipLaunchKernelGGL((k_force_mma_pk<NMAX, 1, ACT>), g, b, L.lds, st, M, L.blk);
+        } else {
+            hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, false, EK_FORCE, ACT>), g, b, L.lds, st, M, L.blk);
+        }
+    } else if (tl)
         hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, true, EK_WALK, ACT>), g, b, L.lds, st, M, L.blk);
     else
         hipLaunchKernelGGL((k_estimate_pk<T, NMAX, 1, 1, false, EK_WALK, ACT>), g, b, L.lds, st, M, L.blk);

@@ -5,8 +5,12 @@
 hipError_t nq_launch_estimate_f32(const est_args& a, int grid, hipStream_t st);
 
 size_t est_lds_bytes(const est_args& a, size_t elt, int waves) {
-    // params | exact force accumulator | waves x (W cache + flip masks) | table (16-aligned)
-    const size_t b = (size_t)a.lds_param_bytes + ((a.Fpart || a.G) ? (size_t)fx_bytes(a.d.np) : 0) +
+    // params | exact force accumulator | waves x (W cache + flip masks) | table (16-aligned);
+    // the FP32 forces launch stages its parameters in a padded layout of its own
+    size_t pb = (size_t)a.lds_param_bytes;
+    if (elt == 4 && a.rec_mode == 2 && !a.G)
+        pb = std::max(pb, ((size_t)4 * nq_est::force_stage_floats(a.d) + 15) & ~(size_t)15);
+    const size_t b = pb + ((a.Fpart || a.G) ? (size_t)fx_bytes(a.d.np) : 0) +
                      (size_t)waves * (size_t)a.max_conn * (2 * elt + 8);
     return ((b + 15) & ~(size_t)15) + (a.tab_lds ? (size_t)a.tab.bytes : 0);
 }
@@ -36,7 +40,7 @@ int nq_est_variant(const est_args& a0, bool fp64, int* threads, size_t* lds) {
     *lds = est_lds_bytes(a, elt, wpb);
     if (*lds > 160 * 1024) return -1;
     return (force ? 1 : 0) | (a.d.N > 8 ? 2 : 0) | (a.tab_lds ? 4 : 0) | (a.d.act == NQ_ACT_LOGCOSH ? 8 : 0) |
-           (fp64 ? 16 : 0);
+           (fp64 ? 16 : 0) | (force && !fp64 && nq_est::force_class(a.d) == 0 ? 32 : 0);
 }
 
 hipError_t nq_pk_estimate(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t st) {
