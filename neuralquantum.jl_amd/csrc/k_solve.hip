@@ -931,9 +931,11 @@ __device__ __forceinline__ void split_copy_body(const double* S, int lds, const 
 // chunk's two 32-column slices of U are read coalesced (64 consecutive rows
 // of a column per wave) into registers PF chunks ahead, and staged in LDS
 // for the products, so PF memory round trips are in flight while a chunk is
-// multiplied.  f64 MFMA operands: A[i][k] from lane i + 16 k, B[k][j] from
-// lane j + 16 k; output D[lane/16 + 4v][lane%16]
-constexpr int KC = 64, PF = 4;
+// multiplied.  Three chunks ahead keep the kernel within 168 VGPRs, three
+// workgroups per CU, so that 768 tiles (the 4x4 XYZ network's Pi block has
+// 595) run in one round.  f64 MFMA operands: A[i][k] from lane i + 16 k,
+// B[k][j] from lane j + 16 k; output D[lane/16 + 4v][lane%16]
+constexpr int KC = 64, PF = 3;
 __device__ __forceinline__ void split_schur_body(double* A2, int lda, int nl, int npi, int bx) {
     __shared__ double Xa[32][KC + 1], Xb[32][KC + 1];   // [tile column][chunk row]
     const int m = nl + npi, T = (npi + 1 + 31) / 32;
@@ -1149,7 +1151,7 @@ __global__ void k_split_copy(const double* S, int lds, const double* F, int nl, 
     split_copy_body(S, lds, F, nl, npi, eps, kind, A2, lda, blockIdx.x, blockIdx.y);
 }
 
-__global__ __launch_bounds__(256) void k_split_schur(double* A2, int lda, int nl, int npi) {
+__global__ __launch_bounds__(256, 3) void k_split_schur(double* A2, int lda, int nl, int npi) {
     split_schur_body(A2, lda, nl, npi, blockIdx.x);
 }
 
@@ -1330,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_chol_step_pk(const nq_pk_member* M, con
     chol_step_body(c.A, c.n, c.nr, c.ncu, c.lda, c.k, c.kb, c.winv, c.info, c.k0, c.sA, c.sW, pk_x(b), pk_y(b));
 }
 
-__global__ __launch_bounds__(256) void k_split_schur_pk(const nq_pk_member* M, const uint2* blk) {
+__global__ __launch_bounds__(256, 3) void k_split_schur_pk(const nq_pk_member* M, const uint2* blk) {
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = pk_m(M, b);
     split_schur_body(m.A2, m.nl + m.npi + 1, m.nl, m.npi, pk_x(b));
