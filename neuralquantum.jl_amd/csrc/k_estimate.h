@@ -309,128 +309,259 @@ __device__ __forceinline__ void estimate_body(const est_args& a, int bx, int gx)
             Wdiag = cmk(wave_total(Wd.re), wave_total(Wd.im));
             __builtin_amdgcn_wave_barrier();
             ncc = nc < a.max_conn ? nc : a.max_conn;
-            constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
-            // FP32 with one Pi unit per lane (na <= 32): two connections' Pi
-            // units per pass (in FP64 the second set of operands would spill)
-            constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
-            for (int c0 = 0; c0 < ncc; c0 += EB) {
-                T part[2 * EB];
+            if constexpr (WALK && sizeof(T) == 4) {
+                // FP32 walk: four changed connections per round, lane i + 16 k
+                // taking connection k of the four, its units 16 t + i (the
+                // flipped sites' FMAs per lane, no scalar bit walk) and its Pi
+                // units 16 t + i; the 16 lanes of a connection then sum its
+                // share of dlog rho (the visible biases of a flipped site on
+                // that site's lane) and one of them finishes W
+                constexpr int TW = 4 * RS, TP = 2 * PS;
+                const int il = lane0 & 15, kg = lane0 >> 4;
+                T pths[TW], pthe[TW], pfs[TW], pfe[TW];
+                int wofs[TW];
 #pragma unroll
-                for (int j = 0; j < EB; ++j) {
-                    T dre = T(0), dim = T(0);
-                    if (c0 + j < ncc) {
-                        const uint2 mm = mbuf[c0 + j];
-                        const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
-#pragma unroll
-                        for (int r = 0; r < RS; ++r) {
-                            const int u = lane + 64 * r;
-                            if (u < 2 * nh) {
-                                const bool lam = u < nh;
-                                const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
-                                T dfs = T(0), dfe = T(0);
-                                if (ms) {
-                                    T t = th_s[r];
-                                    for (uint32_t m = ms; m; m &= m - 1) {
-                                        const int i = __builtin_ctz(m);
-                                        t += cflip<T>(ks, i) * w[nh * i];
-                                    }
-                                    dfs = act_f<ACT>(t) - f_s[r];
-                                }
-                                if (me) {
-                                    T t = th_e[r];
-                                    for (uint32_t m = me; m; m &= m - 1) {
-                                        const int i = __builtin_ctz(m);
-                                        t += cflip<T>(ke, i) * w[nh * i];
-                                    }
-                                    dfe = act_f<ACT>(t) - f_e[r];
-                                }
-                                if (lam) dre += half * (dfs + dfe);
-                                else dim += half * (dfs - dfe);
-                            }
-                        }
-                        if (!PAIR_PI) {
-#pragma unroll
-                            for (int r = 0; r < PS; ++r) {
-                                const int q = lane + 64 * r;
-                                if (q < na) {
-                                    cplx<T> z = pi_z[r];
-                                    for (uint32_t m = ms; m; m &= m - 1) {
-                                        const int i = __builtin_ctz(m);
-                                        const T c = cflip<T>(ks, i) * half;
-                                        z.re += c * sp[d.ula + q + na * i];
-                                        z.im += c * sp[d.umu + q + na * i];
-                                    }
-                                    for (uint32_t m = me; m; m &= m - 1) {
-                                        const int i = __builtin_ctz(m);
-                                        const T c = cflip<T>(ke, i) * half;
-                                        z.re += c * sp[d.ula + q + na * i];
-                                        z.im -= c * sp[d.umu + q + na * i];
-                                    }
-                                    const cplx<T> f2 = cact_f<ACT>(z);
-                                    dre += f2.re - pi_f[r].re;
-                                    dim += f2.im - pi_f[r].im;
-                                }
-                            }
-                        }
-                    }
-                    part[j] = dre;
-                    part[EB + j] = dim;
-                    if (PAIR_PI && (j & 1)) {
-                        // the Pi units of connections j - 1 and j at once: lanes
-                        // [0, na) take j - 1 and lanes [na, 2 na), which hold the
-                        // same units' angles, take j (flip masks per lane)
-                        const bool hi = lane0 >= na;
-                        const int cj = c0 + (hi ? j : j - 1);
-                        if (lane0 < 2 * na && cj < ncc) {
-                            const int q = hi ? lane0 - na : lane0;
-                            const uint2 mm = mbuf[cj];
-                            cplx<T> z = pi_z[0];
-                            for (uint32_t m = mm.x; m; m &= m - 1) {
-                                const int i = __builtin_ctz(m);
-                                const T c = cflip<T>(ks, i) * half;
-                                z.re += c * sp[d.ula + q + na * i];
-                                z.im += c * sp[d.umu + q + na * i];
-                            }
-                            for (uint32_t m = mm.y; m; m &= m - 1) {
-                                const int i = __builtin_ctz(m);
-                                const T c = cflip<T>(ke, i) * half;
-                                z.re += c * sp[d.ula + q + na * i];
-                                z.im -= c * sp[d.umu + q + na * i];
-                            }
-                            const cplx<T> f2 = cact_f<ACT>(z);
-                            const T vr = f2.re - pi_f[0].re, vi = f2.im - pi_f[0].im;
-                            part[j - 1] += hi ? T(0) : vr;
-                            part[EB + j - 1] += hi ? T(0) : vi;
-                            part[j] += hi ? vr : T(0);
-                            part[EB + j] += hi ? vi : T(0);
-                        }
-                    }
+                for (int t = 0; t < TW; ++t) {
+                    const int src = 16 * (t & 3) + il, r = t >> 2, u = 16 * t + il;
+                    pths[t] = __shfl(th_s[r], src, 64);
+                    pthe[t] = __shfl(th_e[r], src, 64);
+                    pfs[t] = __shfl(f_s[r], src, 64);
+                    pfe[t] = __shfl(f_e[r], src, 64);
+                    wofs[t] = u < nh ? d.wla + u : (u < 2 * nh ? d.wmu + u - nh : d.wla);
                 }
-                // lane L < 32: sum of dre of connection c0 + L/LG; lane 32 + L: its dim
-                T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
-                plswap<true>(dre, dim);   // lanes 0-31: dre in `dre`, dim in `dim`
-                const int c = c0 + lane0 / LG;
-                if (lane0 < 32 && c < ncc) {
-                    const uint2 mm = mbuf[c];
-                    for (uint32_t m = mm.x; m; m &= m - 1) {
+                cplx<T> ppz[TP], ppf[TP];
+                int aofs[TP];
+#pragma unroll
+                for (int t = 0; t < TP; ++t) {
+                    const int src = 16 * t + il;
+                    ppz[t] = cmk(__shfl(pi_z[0].re, src, 64), __shfl(pi_z[0].im, src, 64));
+                    ppf[t] = cmk(__shfl(pi_f[0].re, src, 64), __shfl(pi_f[0].im, src, 64));
+                    aofs[t] = src < na ? src : na - 1;
+                }
+                const T bl = il < N ? sp[d.bla + il] : T(0), bm = il < N ? sp[d.bmu + il] : T(0);
+                const T cfs = half * cflip<T>(ks, il), cfe = half * cflip<T>(ke, il);
+                for (int c0 = 0; c0 < ncc; c0 += 4) {
+                    const int c = c0 + kg;
+                    uint32_t ms = 0u, me = 0u;
+                    if (c < ncc) {
+                        const uint2 mm = mbuf[c];
+                        ms = mm.x;
+                        me = mm.y;
+                    }
+                    T dre = T(0), dim = T(0);
+                    if (ms) {
+                        T th[TW];
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) th[t] = pths[t];
+                        for (uint32_t m = ms; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T cf = cflip<T>(ks, i);
+#pragma unroll
+                            for (int t = 0; t < TW; ++t) th[t] += cf * sp[wofs[t] + nh * i];
+                        }
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) {
+                            if (16 * t < 2 * nh) {
+                                const int u = 16 * t + il;
+                                const T df = half * (act_f<ACT>(th[t]) - pfs[t]);
+                                dre += u < nh ? df : T(0);
+                                dim += (u >= nh && u < 2 * nh) ? df : T(0);
+                            }
+                        }
+                    }
+                    if (me) {
+                        T th[TW];
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) th[t] = pthe[t];
+                        for (uint32_t m = me; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T cf = cflip<T>(ke, i);
+#pragma unroll
+                            for (int t = 0; t < TW; ++t) th[t] += cf * sp[wofs[t] + nh * i];
+                        }
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) {
+                            if (16 * t < 2 * nh) {
+                                const int u = 16 * t + il;
+                                const T df = half * (act_f<ACT>(th[t]) - pfe[t]);
+                                dre += u < nh ? df : T(0);
+                                dim -= (u >= nh && u < 2 * nh) ? df : T(0);
+                            }
+                        }
+                    }
+                    cplx<T> z[TP];
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) z[t] = ppz[t];
+                    for (uint32_t m = ms; m; m &= m - 1) {
                         const int i = __builtin_ctz(m);
                         const T cc = cflip<T>(ks, i) * half;
-                        dre += cc * sp[d.bla + i];
-                        dim += cc * sp[d.bmu + i];
+#pragma unroll
+                        for (int t = 0; t < TP; ++t) {
+                            z[t].re += cc * sp[d.ula + aofs[t] + na * i];
+                            z[t].im += cc * sp[d.umu + aofs[t] + na * i];
+                        }
                     }
-                    for (uint32_t m = mm.y; m; m &= m - 1) {
+                    for (uint32_t m = me; m; m &= m - 1) {
                         const int i = __builtin_ctz(m);
                         const T cc = cflip<T>(ke, i) * half;
-                        dre += cc * sp[d.bla + i];
-                        dim -= cc * sp[d.bmu + i];
+#pragma unroll
+                        for (int t = 0; t < TP; ++t) {
+                            z[t].re += cc * sp[d.ula + aofs[t] + na * i];
+                            z[t].im -= cc * sp[d.umu + aofs[t] + na * i];
+                        }
                     }
-                    T sn, cs;
-                    nq_sincos(dim, &sn, &cs);
-                    const T er = nq_exp(dre);
-                    const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
-                    if (lane0 % LG == 0) {
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) {
+                        if (16 * t < na) {
+                            const cplx<T> f2 = cact_f<ACT>(z[t]);
+                            const bool ok = 16 * t + il < na;
+                            dre += ok ? f2.re - ppf[t].re : T(0);
+                            dim += ok ? f2.im - ppf[t].im : T(0);
+                        }
+                    }
+                    if ((ms >> il) & 1u) {
+                        dre += cfs * bl;
+                        dim += cfs * bm;
+                    }
+                    if ((me >> il) & 1u) {
+                        dre += cfe * bl;
+                        dim -= cfe * bm;
+                    }
+                    dre = group_total(dre, 16);
+                    dim = group_total(dim, 16);
+                    if (c < ncc && il == 0) {
+                        T sn, cs;
+                        nq_sincos(dim, &sn, &cs);
+                        const T er = nq_exp(dre);
+                        const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
                         wbuf[c] = W;
                         Lacc = cadd(Lacc, W);
+                    }
+                }
+            } else {
+                constexpr int EB = est_batch<T>(), LG = 32 / EB;   // LG lanes hold one connection's sums
+                // FP32 with one Pi unit per lane (na <= 32): two connections' Pi
+                // units per pass (in FP64 the second set of operands would spill)
+                constexpr bool PAIR_PI = PS == 1 && sizeof(T) == 4;
+                for (int c0 = 0; c0 < ncc; c0 += EB) {
+                    T part[2 * EB];
+    #pragma unroll
+                    for (int j = 0; j < EB; ++j) {
+                        T dre = T(0), dim = T(0);
+                        if (c0 + j < ncc) {
+                            const uint2 mm = mbuf[c0 + j];
+                            const uint32_t ms = __builtin_amdgcn_readfirstlane(mm.x), me = __builtin_amdgcn_readfirstlane(mm.y);
+    #pragma unroll
+                            for (int r = 0; r < RS; ++r) {
+                                const int u = lane + 64 * r;
+                                if (u < 2 * nh) {
+                                    const bool lam = u < nh;
+                                    const T* w = lam ? sp + d.wla + u : sp + d.wmu + (u - nh);
+                                    T dfs = T(0), dfe = T(0);
+                                    if (ms) {
+                                        T t = th_s[r];
+                                        for (uint32_t m = ms; m; m &= m - 1) {
+                                            const int i = __builtin_ctz(m);
+                                            t += cflip<T>(ks, i) * w[nh * i];
+                                        }
+                                        dfs = act_f<ACT>(t) - f_s[r];
+                                    }
+                                    if (me) {
+                                        T t = th_e[r];
+                                        for (uint32_t m = me; m; m &= m - 1) {
+                                            const int i = __builtin_ctz(m);
+                                            t += cflip<T>(ke, i) * w[nh * i];
+                                        }
+                                        dfe = act_f<ACT>(t) - f_e[r];
+                                    }
+                                    if (lam) dre += half * (dfs + dfe);
+                                    else dim += half * (dfs - dfe);
+                                }
+                            }
+                            if (!PAIR_PI) {
+    #pragma unroll
+                                for (int r = 0; r < PS; ++r) {
+                                    const int q = lane + 64 * r;
+                                    if (q < na) {
+                                        cplx<T> z = pi_z[r];
+                                        for (uint32_t m = ms; m; m &= m - 1) {
+                                            const int i = __builtin_ctz(m);
+                                            const T c = cflip<T>(ks, i) * half;
+                                            z.re += c * sp[d.ula + q + na * i];
+                                            z.im += c * sp[d.umu + q + na * i];
+                                        }
+                                        for (uint32_t m = me; m; m &= m - 1) {
+                                            const int i = __builtin_ctz(m);
+                                            const T c = cflip<T>(ke, i) * half;
+                                            z.re += c * sp[d.ula + q + na * i];
+                                            z.im -= c * sp[d.umu + q + na * i];
+                                        }
+                                        const cplx<T> f2 = cact_f<ACT>(z);
+                                        dre += f2.re - pi_f[r].re;
+                                        dim += f2.im - pi_f[r].im;
+                                    }
+                                }
+                            }
+                        }
+                        part[j] = dre;
+                        part[EB + j] = dim;
+                        if (PAIR_PI && (j & 1)) {
+                            // the Pi units of connections j - 1 and j at once: lanes
+                            // [0, na) take j - 1 and lanes [na, 2 na), which hold the
+                            // same units' angles, take j (flip masks per lane)
+                            const bool hi = lane0 >= na;
+                            const int cj = c0 + (hi ? j : j - 1);
+                            if (lane0 < 2 * na && cj < ncc) {
+                                const int q = hi ? lane0 - na : lane0;
+                                const uint2 mm = mbuf[cj];
+                                cplx<T> z = pi_z[0];
+                                for (uint32_t m = mm.x; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T c = cflip<T>(ks, i) * half;
+                                    z.re += c * sp[d.ula + q + na * i];
+                                    z.im += c * sp[d.umu + q + na * i];
+                                }
+                                for (uint32_t m = mm.y; m; m &= m - 1) {
+                                    const int i = __builtin_ctz(m);
+                                    const T c = cflip<T>(ke, i) * half;
+                                    z.re += c * sp[d.ula + q + na * i];
+                                    z.im -= c * sp[d.umu + q + na * i];
+                                }
+                                const cplx<T> f2 = cact_f<ACT>(z);
+                                const T vr = f2.re - pi_f[0].re, vi = f2.im - pi_f[0].im;
+                                part[j - 1] += hi ? T(0) : vr;
+                                part[EB + j - 1] += hi ? T(0) : vi;
+                                part[j] += hi ? vr : T(0);
+                                part[EB + j] += hi ? vi : T(0);
+                            }
+                        }
+                    }
+                    // lane L < 32: sum of dre of connection c0 + L/LG; lane 32 + L: its dim
+                    T dre = wave_totals<2 * EB>(part, lane0), dim = dre;
+                    plswap<true>(dre, dim);   // lanes 0-31: dre in `dre`, dim in `dim`
+                    const int c = c0 + lane0 / LG;
+                    if (lane0 < 32 && c < ncc) {
+                        const uint2 mm = mbuf[c];
+                        for (uint32_t m = mm.x; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T cc = cflip<T>(ks, i) * half;
+                            dre += cc * sp[d.bla + i];
+                            dim += cc * sp[d.bmu + i];
+                        }
+                        for (uint32_t m = mm.y; m; m &= m - 1) {
+                            const int i = __builtin_ctz(m);
+                            const T cc = cflip<T>(ke, i) * half;
+                            dre += cc * sp[d.bla + i];
+                            dim -= cc * sp[d.bmu + i];
+                        }
+                        T sn, cs;
+                        nq_sincos(dim, &sn, &cs);
+                        const T er = nq_exp(dre);
+                        const cplx<T> W = cmul(wbuf[c], cmk(er * cs, er * sn));
+                        if (lane0 % LG == 0) {
+                            wbuf[c] = W;
+                            Lacc = cadd(Lacc, W);
+                        }
                     }
                 }
             }
