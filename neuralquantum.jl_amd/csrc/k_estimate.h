@@ -348,53 +348,19 @@ __device__ __forceinline__ void estimate_body(const est_args& a, int bx, int gx)
                         ms = mm.x;
                         me = mm.y;
                     }
-                    T dre = T(0), dim = T(0);
-                    if (ms) {
-                        T th[TW];
-#pragma unroll
-                        for (int t = 0; t < TW; ++t) th[t] = pths[t];
-                        for (uint32_t m = ms; m; m &= m - 1) {
-                            const int i = __builtin_ctz(m);
-                            const T cf = cflip<T>(ks, i);
-#pragma unroll
-                            for (int t = 0; t < TW; ++t) th[t] += cf * sp[wofs[t] + nh * i];
-                        }
-#pragma unroll
-                        for (int t = 0; t < TW; ++t) {
-                            if (16 * t < 2 * nh) {
-                                const int u = 16 * t + il;
-                                const T df = half * (act_f<ACT>(th[t]) - pfs[t]);
-                                dre += u < nh ? df : T(0);
-                                dim += (u >= nh && u < 2 * nh) ? df : T(0);
-                            }
-                        }
-                    }
-                    if (me) {
-                        T th[TW];
-#pragma unroll
-                        for (int t = 0; t < TW; ++t) th[t] = pthe[t];
-                        for (uint32_t m = me; m; m &= m - 1) {
-                            const int i = __builtin_ctz(m);
-                            const T cf = cflip<T>(ke, i);
-#pragma unroll
-                            for (int t = 0; t < TW; ++t) th[t] += cf * sp[wofs[t] + nh * i];
-                        }
-#pragma unroll
-                        for (int t = 0; t < TW; ++t) {
-                            if (16 * t < 2 * nh) {
-                                const int u = 16 * t + il;
-                                const T df = half * (act_f<ACT>(th[t]) - pfe[t]);
-                                dre += u < nh ? df : T(0);
-                                dim -= (u >= nh && u < 2 * nh) ? df : T(0);
-                            }
-                        }
-                    }
+                    // the flipped sites of each side, in one pass per side:
+                    // the lambda / mu units' angles and the Pi units' angles
+                    T ths2[TW], the2[TW];
                     cplx<T> z[TP];
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) { ths2[t] = pths[t]; the2[t] = pthe[t]; }
 #pragma unroll
                     for (int t = 0; t < TP; ++t) z[t] = ppz[t];
                     for (uint32_t m = ms; m; m &= m - 1) {
                         const int i = __builtin_ctz(m);
-                        const T cc = cflip<T>(ks, i) * half;
+                        const T cf = cflip<T>(ks, i), cc = half * cf;
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) ths2[t] += cf * sp[wofs[t] + nh * i];
 #pragma unroll
                         for (int t = 0; t < TP; ++t) {
                             z[t].re += cc * sp[d.ula + aofs[t] + na * i];
@@ -403,11 +369,36 @@ __device__ __forceinline__ void estimate_body(const est_args& a, int bx, int gx)
                     }
                     for (uint32_t m = me; m; m &= m - 1) {
                         const int i = __builtin_ctz(m);
-                        const T cc = cflip<T>(ke, i) * half;
+                        const T cf = cflip<T>(ke, i), cc = half * cf;
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) the2[t] += cf * sp[wofs[t] + nh * i];
 #pragma unroll
                         for (int t = 0; t < TP; ++t) {
                             z[t].re += cc * sp[d.ula + aofs[t] + na * i];
                             z[t].im -= cc * sp[d.umu + aofs[t] + na * i];
+                        }
+                    }
+                    T dre = T(0), dim = T(0);
+                    if (ms) {
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) {
+                            if (16 * t < 2 * nh) {
+                                const int u = 16 * t + il;
+                                const T df = half * (act_f<ACT>(ths2[t]) - pfs[t]);
+                                dre += u < nh ? df : T(0);
+                                dim += (u >= nh && u < 2 * nh) ? df : T(0);
+                            }
+                        }
+                    }
+                    if (me) {
+#pragma unroll
+                        for (int t = 0; t < TW; ++t) {
+                            if (16 * t < 2 * nh) {
+                                const int u = 16 * t + il;
+                                const T df = half * (act_f<ACT>(the2[t]) - pfe[t]);
+                                dre += u < nh ? df : T(0);
+                                dim -= (u >= nh && u < 2 * nh) ? df : T(0);
+                            }
                         }
                     }
 #pragma unroll
@@ -899,6 +890,14 @@ __host__ __device__ inline int force_stage_floats(const ndm_dims& d) {
     return (32 << c) * (d.N + 1) + (16 << c) * (2 * d.N + 1);
 }
 
+// f' of a real unit: the logistic function as 1 / (1 + e^-x) (the limits 0
+// and 1 come out of the reciprocal of inf and of 1, no select), tanh for logcosh
+template <int ACT>
+__device__ __forceinline__ float force_df(float x) {
+    if constexpr (ACT == NQ_ACT_SOFTPLUS) return nq_rcp(1.0f + nq_exp(-x));
+    else return act_df<ACT>(x);
+}
+
 template <int NMAX, int CLS, int ACT>
 __device__ __forceinline__ void force_mma_body(const est_args& a, int bx, int gx) {
     using T = float;
@@ -979,8 +978,8 @@ __device__ __forceinline__ void force_mma_body(const est_args& a, int bx, int gx
             }
             ts1[r] = ts;
             te1[r] = te;
-            as1[r] = act_df<ACT>(ts);
-            ae1[r] = act_df<ACT>(te);
+            as1[r] = force_df<ACT>(ts);
+            ae1[r] = force_df<ACT>(te);
         }
         cplx<T> pz1 = cmk(T(0), T(0));
         if (lane < SP) {
@@ -1039,37 +1038,57 @@ __device__ __forceinline__ void force_mma_body(const est_args& a, int bx, int gx
                 }
                 const cplx<T> VW = cmul(V, W);
 
-                // lambda / mu units: f' at the connection's angles on each side
-                T As[TW], Ae[TW];
+                // the flipped sites of each side, in one pass per side: the
+                // lambda / mu units' angles and the Pi units' angles
+                T ts[TW], te[TW];
+                cplx<T> z[TP];
 #pragma unroll
-                for (int t = 0; t < TW; ++t) { As[t] = as[t]; Ae[t] = ae[t]; }
-                if (ms) {
-                    T th[TW];
+                for (int t = 0; t < TW; ++t) { ts[t] = ths[t]; te[t] = the[t]; }
 #pragma unroll
-                    for (int t = 0; t < TW; ++t) th[t] = ths[t];
-                    for (uint32_t m = ms; m; m &= m - 1) {
-                        const int i = __builtin_ctz(m);
-                        const T cf = cflip<T>(ks, i);
-                        const T* w = HW + SW * (1 + i) + il;
+                for (int t = 0; t < TP; ++t) z[t] = pz[t];
+                for (uint32_t m = ms; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cf = cflip<T>(ks, i), cc = half * cf;
+                    const T* w = HW + SW * (1 + i) + il;
+                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
 #pragma unroll
-                        for (int t = 0; t < TW; ++t) th[t] += cf * w[16 * t];
+                    for (int t = 0; t < TW; ++t) ts[t] += cf * w[16 * t];
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) {
+                        const float2 uu = pu[16 * t];
+                        z[t].re += cc * uu.x;
+                        z[t].im += cc * uu.y;
                     }
+                }
+                for (uint32_t m = me; m; m &= m - 1) {
+                    const int i = __builtin_ctz(m);
+                    const T cf = cflip<T>(ke, i), cc = half * cf;
+                    const T* w = HW + SW * (1 + i) + il;
+                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
 #pragma unroll
-                    for (int t = 0; t < TW; ++t) As[t] = act_df<ACT>(th[t]);
+                    for (int t = 0; t < TW; ++t) te[t] += cf * w[16 * t];
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) {
+                        const float2 uu = pu[16 * t];
+                        z[t].re += cc * uu.x;
+                        z[t].im -= cc * uu.y;
+                    }
+                }
+                // f' at the connection's angles (the sample's own on an unflipped side)
+                T As[TW], Ae[TW];
+                if (ms) {
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) As[t] = force_df<ACT>(ts[t]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < TW; ++t) As[t] = as[t];
                 }
                 if (me) {
-                    T th[TW];
 #pragma unroll
-                    for (int t = 0; t < TW; ++t) th[t] = the[t];
-                    for (uint32_t m = me; m; m &= m - 1) {
-                        const int i = __builtin_ctz(m);
-                        const T cf = cflip<T>(ke, i);
-                        const T* w = HW + SW * (1 + i) + il;
+                    for (int t = 0; t < TW; ++t) Ae[t] = force_df<ACT>(te[t]);
+                } else {
 #pragma unroll
-                        for (int t = 0; t < TW; ++t) th[t] += cf * w[16 * t];
-                    }
-#pragma unroll
-                    for (int t = 0; t < TW; ++t) Ae[t] = act_df<ACT>(th[t]);
+                    for (int t = 0; t < TW; ++t) Ae[t] = ae[t];
                 }
                 const T gl = half * VW.re, gm = -half * VW.im;
                 const bool sok = il < N;
@@ -1088,31 +1107,6 @@ __device__ __forceinline__ void force_mma_body(const est_args& a, int bx, int gx
                 // Pi units: f'(z) at the connection's angle (for the unchanged
                 // configurations the sample's own)
                 const T Bp = half * (Bs + Be), Bm = half * (Bs - Be);
-                cplx<T> z[TP];
-#pragma unroll
-                for (int t = 0; t < TP; ++t) z[t] = pz[t];
-                for (uint32_t m = ms; m; m &= m - 1) {
-                    const int i = __builtin_ctz(m);
-                    const T cc = cflip<T>(ks, i) * half;
-                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
-#pragma unroll
-                    for (int t = 0; t < TP; ++t) {
-                        const float2 uu = pu[16 * t];
-                        z[t].re += cc * uu.x;
-                        z[t].im += cc * uu.y;
-                    }
-                }
-                for (uint32_t m = me; m; m &= m - 1) {
-                    const int i = __builtin_ctz(m);
-                    const T cc = cflip<T>(ke, i) * half;
-                    const float2* pu = reinterpret_cast<const float2*>(PU) + SP * i + il;
-#pragma unroll
-                    for (int t = 0; t < TP; ++t) {
-                        const float2 uu = pu[16 * t];
-                        z[t].re += cc * uu.x;
-                        z[t].im -= cc * uu.y;
-                    }
-                }
 #pragma unroll
                 for (int t = 0; t < TP; ++t) {
                     const cplx<T> vp = cmul(VW, cact_df<ACT>(z[t]));
