@@ -903,14 +903,37 @@ __device__ __forceinline__ int split_p(int inst, int i, int nl, int npi) {
 // lower triangle (with the shift on the diagonal) and the right-hand side row
 // of both instances (m = nl + npi, lda = m + 1; instance 1 only its mu
 // columns: its Pi columns are never read); grid (m columns, 2 instances)
-// one workgroup per column j of an instance (rows j .. m)
+// one workgroup per column j of an instance (rows j .. m).  Workgroup j = 0
+// also factors the instance's first diagonal block (the same values, taken
+// from S) into L_00^-T (winv block 0 of the instance), which the first
+// factorization step needs, in place of a launch of its own
 __device__ __forceinline__ void split_copy_body(const double* S, int lds, const double* F, int nl, int npi, double eps,
-                                                int kind, double* A2, int lda, int j, int inst) {
+                                                int kind, double* A2, int lda, int j, int inst, double* winv,
+                                                int64_t sW, int* info) {
     const int m = nl + npi;
     if (inst == 1 && j >= nl) return;
     double* A = A2 + (int64_t)inst * lda * lda;
+    // the diagonal block first, so that its factor runs beside the other workgroups' copies
+    if (j == 0) {
+        __shared__ double Db[CB][CB + 1];
+        __shared__ __align__(16) double Lc[CB][64];
+        const int kb = nl < CB ? nl : CB;
+        for (int e = threadIdx.x; e < CB * CB; e += blockDim.x) {
+            const int r = e & (CB - 1), c = e >> 5;
+            double v = 0.0;
+            if (r < kb && c < kb && r >= c) {
+                v = S[(int64_t)split_p(inst, r, nl, npi) + (int64_t)split_p(inst, c, nl, npi) * lds];
+                if (r == c) v += (kind == 0) ? eps : eps * v;
+            }
+            Db[r][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) factor_diag32<true>(A, lda, 0, kb, nl, winv + inst * sW, info, true, Lc, Db);
+    }
+    // (in workgroup 0 the waves other than the factoring one copy the column)
+    const int t0 = j == 0 ? (int)threadIdx.x - 64 : (int)threadIdx.x, nt = j == 0 ? (int)blockDim.x - 64 : blockDim.x;
     const int pj = split_p(inst, j, nl, npi);
-    for (int i = j + (int)threadIdx.x; i <= m; i += blockDim.x) {
+    for (int i = j + t0; t0 >= 0 && i <= m; i += nt) {
         double v;
         if (i == m) {
             v = F[pj];
@@ -936,8 +959,10 @@ __device__ __forceinline__ void split_copy_body(const double* S, int lds, const 
 // 595) run in one round.  f64 MFMA operands: A[i][k] from lane i + 16 k,
 // B[k][j] from lane j + 16 k; output D[lane/16 + 4v][lane%16]
 constexpr int KC = 64, PF = 3;
-__device__ __forceinline__ void split_schur_body(double* A2, int lda, int nl, int npi, int bx) {
-    __shared__ double Xa[32][KC + 1], Xb[32][KC + 1];   // [tile column][chunk row]
+__device__ __forceinline__ void split_schur_body(double* A2, int lda, int nl, int npi, int bx, double* winvP,
+                                                 int* info) {
+    __shared__ __align__(16) double Xa[32][KC + 1];   // [tile column][chunk row]
+    __shared__ __align__(16) double Xb[32][KC + 1];
     const int m = nl + npi, T = (npi + 1 + 31) / 32;
     int tI, tJ;
     step_decode(bx, T, T, &tI, &tJ);   // tJ <= tI
@@ -986,10 +1011,27 @@ __device__ __forceinline__ void split_schur_body(double* A2, int lda, int nl, in
             }
         }
     }
+    // tile (0, 0) is the Pi block's first diagonal block: its workgroup
+    // factors it into L^-T (winv block 0 of the Pi columns) from an LDS copy
+    // of the updated values, in place of a launch of its own
+    const bool diag = tI == 0 && tJ == 0;
+    double (*Db)[CB + 1] = reinterpret_cast<double (*)[CB + 1]>(&Xb[0][0]);
+    if (diag) __syncthreads();   // every wave is done with the last chunk
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = i0 + rt * 16 + lk + 4 * v, j = j0 + ct * 16 + li;
-        if (i <= m && j <= m && i >= j) A2[(int64_t)i + (int64_t)j * lda] -= acc0[v] + acc1[v];
+        if (i <= m && j <= m && i >= j) {
+            const double nv = A2[(int64_t)i + (int64_t)j * lda] - (acc0[v] + acc1[v]);
+            A2[(int64_t)i + (int64_t)j * lda] = nv;
+            if (diag) Db[i - i0][j - j0] = nv;
+        }
+    }
+    if (diag) {
+        __syncthreads();
+        const int kb = npi < CB ? npi : CB;
+        if (wave == 0)
+            factor_diag32<true>(A2, lda, nl, kb, m, winvP, info, true, reinterpret_cast<double (*)[64]>(&Xa[0][0]),
+                                Db);
     }
 }
 
@@ -1146,13 +1188,15 @@ __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, in
     back_group_body(A, lda, k, kend, n, winv, y, x, k0, sA, sW, sY, blockIdx.x, blockIdx.y);
 }
 
-__global__ void k_split_copy(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
-                             double* A2, int lda) {
-    split_copy_body(S, lds, F, nl, npi, eps, kind, A2, lda, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(256) void k_split_copy(const double* S, int lds, const double* F, int nl, int npi,
+                                                    double eps, int kind, double* A2, int lda, double* winv,
+                                                    int64_t sW, int* info) {
+    split_copy_body(S, lds, F, nl, npi, eps, kind, A2, lda, blockIdx.x, blockIdx.y, winv, sW, info);
 }
 
-__global__ __launch_bounds__(256, 3) void k_split_schur(double* A2, int lda, int nl, int npi) {
-    split_schur_body(A2, lda, nl, npi, blockIdx.x);
+__global__ __launch_bounds__(256, 3) void k_split_schur(double* A2, int lda, int nl, int npi, double* winvP,
+                                                        int* info) {
+    split_schur_body(A2, lda, nl, npi, blockIdx.x, winvP, info);
 }
 
 __global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, int nl, int m, const double* x0,
@@ -1317,13 +1361,8 @@ __global__ __launch_bounds__(256) void k_force_finish_pk(const nq_pk_member* M, 
 __global__ __launch_bounds__(256) void k_split_copy_pk(const nq_pk_member* M, const uint2* blk) {
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = pk_m(M, b);
-    split_copy_body(m.S, m.ldS, m.F, m.nl, m.npi, m.eps, m.kind, m.A2, m.nl + m.npi + 1, pk_x(b), pk_y(b));
-}
-
-__global__ __launch_bounds__(64) void k_chol_diag32_pk(const nq_pk_member* M, const uint2* blk, int phase) {
-    const uint2 b = blk[blockIdx.x];
-    const chol_desc c = pk_chol(pk_m(M, b), phase, -1);
-    chol_diag32_body(c.A, c.n, c.lda, c.k, c.kb, c.winv, c.info, c.sA, c.sW, pk_y(b));
+    split_copy_body(m.S, m.ldS, m.F, m.nl, m.npi, m.eps, m.kind, m.A2, m.nl + m.npi + 1, pk_x(b), pk_y(b), m.winv,
+                    winv_doubles(m.nl), m.info);
 }
 
 __global__ __launch_bounds__(256) void k_chol_step_pk(const nq_pk_member* M, const uint2* blk, int phase, int j) {
@@ -1335,7 +1374,7 @@ __global__ __launch_bounds__(256) void k_chol_step_pk(const nq_pk_member* M, con
 __global__ __launch_bounds__(256, 3) void k_split_schur_pk(const nq_pk_member* M, const uint2* blk) {
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = pk_m(M, b);
-    split_schur_body(m.A2, m.nl + m.npi + 1, m.nl, m.npi, pk_x(b));
+    split_schur_body(m.A2, m.nl + m.npi + 1, m.nl, m.npi, pk_x(b), m.winv + 2 * winv_doubles(m.nl), m.info);
 }
 
 // the forward-substituted right-hand sides of both instances (z = instance)
@@ -1452,10 +1491,11 @@ hipError_t nq_launch_shift(double* S, int n, int lds, double eps, int kind, hipS
 size_t nq_cholesky_winv_doubles(int n) { return (size_t)((n + CB - 1) / CB) * CB * CB; }
 
 hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda, int* info, double* winv, int ninst,
-                              int64_t sA, int64_t sW, hipStream_t st) {
+                              int64_t sA, int64_t sW, hipStream_t st, bool first_done) {
     if (k0 >= n) return hipSuccess;
-    hipLaunchKernelGGL(k_chol_diag32, dim3(1, ninst), dim3(64), 0, st, A, n, lda, k0, (n - k0 < CB ? n - k0 : CB),
-                       winv, info, sA, sW);
+    if (!first_done)
+        hipLaunchKernelGGL(k_chol_diag32, dim3(1, ninst), dim3(64), 0, st, A, n, lda, k0, (n - k0 < CB ? n - k0 : CB),
+                           winv, info, sA, sW);
     for (int k = k0; k < n; k += CB) {
         const int kb = (n - k < CB) ? n - k : CB;
         const int rest = nr - k - kb;
@@ -1472,7 +1512,7 @@ hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda,
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st) {
     // rows >= n (at most one extra row, the right-hand side) are carried along
     const int nr = (lds > n) ? n + 1 : n;
-    return nq_cholesky_range(S, 0, n, nr, nr, lds, info, winv, 1, 0, 0, st);
+    return nq_cholesky_range(S, 0, n, nr, nr, lds, info, winv, 1, 0, 0, st, false);
 }
 
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
@@ -1521,15 +1561,18 @@ hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, in
     const int m = nl + npi, lda = m + 1;
     const int64_t sA = (int64_t)lda * lda;
     const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
-    hipLaunchKernelGGL(k_split_copy, dim3(m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind, A2, lda);
+    // the copy and the Schur update also factor the first diagonal block of
+    // the range that follows them
+    hipLaunchKernelGGL(k_split_copy, dim3(m, 2), dim3(256), 0, st, S, lds, F, nl, npi, eps, kind, A2, lda, winv, sW,
+                       info);
     // both [lambda | Pi] and [mu | Pi] factored over their first nl columns
     // together, their Pi columns left as they are
-    hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, nl, lda, info, winv, 2, sA, sW, st);
+    hipError_t e = nq_cholesky_range(A2, 0, nl, m + 1, nl, lda, info, winv, 2, sA, sW, st, true);
     if (e != hipSuccess) return e;
     const int T = (npi + 1 + 31) / 32;
-    hipLaunchKernelGGL(k_split_schur, dim3(step_tiles(T, T)), dim3(256), 0, st, A2, lda, nl, npi);
+    hipLaunchKernelGGL(k_split_schur, dim3(step_tiles(T, T)), dim3(256), 0, st, A2, lda, nl, npi, winv + 2 * sW, info);
     // the Pi block of instance 0 now holds the whole matrix's Schur complement
-    return nq_cholesky_range(A2, nl, m, m + 1, m + 1, lda, info, winv + 2 * sW, 1, 0, 0, st);
+    return nq_cholesky_range(A2, nl, m, m + 1, m + 1, lda, info, winv + 2 * sW, 1, 0, 0, st, true);
 }
 
 hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,
@@ -1653,13 +1696,6 @@ hipError_t nq_pk_plan(const nq_pk_member& m, int kernel, int param, nq_pk_shape*
             o->gx = mm;
             o->gy = 2;
             break;
-        case NQ_PK_CHOL_DIAG: {
-            const chol_desc c = pk_chol(m, param, -1);
-            o->threads = 64;
-            o->gx = c.k0 < c.n ? 1 : 0;
-            o->gy = c.gy;
-            break;
-        }
         case NQ_PK_CHOL_STEP: {
             if (j >= pk_panels(m.nl, m.npi, phase)) break;
             const chol_desc c = pk_chol(m, phase, j);
@@ -1717,7 +1753,6 @@ hipError_t nq_pk_solve(const nq_pk_member* M, const nq_pk_launch& L, hipStream_t
         case NQ_PK_FORCE_FIN: hipLaunchKernelGGL(k_force_finish_pk, g, b, 0, st, M, L.blk); break;
         case NQ_PK_SFILL: hipLaunchKernelGGL(k_sfill_pk, g, b, 0, st, M, L.blk); break;
         case NQ_PK_SPLIT_COPY: hipLaunchKernelGGL(k_split_copy_pk, g, b, 0, st, M, L.blk); break;
-        case NQ_PK_CHOL_DIAG: hipLaunchKernelGGL(k_chol_diag32_pk, g, b, 0, st, M, L.blk, L.param); break;
         case NQ_PK_CHOL_STEP: hipLaunchKernelGGL(k_chol_step_pk, g, b, 0, st, M, L.blk, phase, j); break;
         case NQ_PK_SCHUR: hipLaunchKernelGGL(k_split_schur_pk, g, b, 0, st, M, L.blk); break;
         case NQ_PK_COPY_COLS: hipLaunchKernelGGL(k_copy_cols_pk, g, b, 0, st, M, L.blk); break;

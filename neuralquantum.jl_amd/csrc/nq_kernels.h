@@ -157,9 +157,10 @@ size_t nq_cholesky_winv_doubles(int n);
 // along: a partial factorization leaves their Schur complement in columns
 // [n, nr) when ncu = nr, and leaves those columns untouched when ncu = n);
 // winv block (k - k0) / 32 holds panel k; ninst instances at strides sA (A),
-// sW (winv)
+// sW (winv); first_done: winv block 0 is already there (the launch that
+// produced the range's first columns factored its diagonal block)
 hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda, int* info, double* winv, int ninst,
-                              int64_t sA, int64_t sW, hipStream_t st);
+                              int64_t sA, int64_t sW, hipStream_t st, bool first_done);
 hipError_t nq_cholesky_backsolve_range(const double* L, int k0, int n, int lds, const double* winv, double* y,
                                        double* x, int ninst, int64_t sA, int64_t sW, int64_t sY, hipStream_t st);
 // the NDM's shifted SR solve through the lambda / mu split (k_solve.hip):
@@ -373,7 +374,6 @@ enum {
     NQ_PK_SFILL,         // S = 0 before the FP64 / FP32 Gram blocks
     NQ_PK_GRAM,          // param: Gram block (0: A, 1: B)
     NQ_PK_SPLIT_COPY,
-    NQ_PK_CHOL_DIAG,     // param: phase (0: lambda / mu columns, 1: Pi columns)
     NQ_PK_CHOL_STEP,     // param: phase << 16 | panel number within the phase
     NQ_PK_SCHUR,
     NQ_PK_COPY_COLS,

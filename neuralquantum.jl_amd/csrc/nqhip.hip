@@ -2419,10 +2419,8 @@ std::vector<std::pair<int, int>> pk_sequence(const std::vector<nq_pk_member>& M)
     q.push_back({NQ_PK_GRAM, 0});
     q.push_back({NQ_PK_GRAM, 1});
     q.push_back({NQ_PK_SPLIT_COPY, 0});
-    q.push_back({NQ_PK_CHOL_DIAG, 0});
     for (int j = 0; j < pa; ++j) q.push_back({NQ_PK_CHOL_STEP, j});
     q.push_back({NQ_PK_SCHUR, 0});
-    q.push_back({NQ_PK_CHOL_DIAG, 1});
     for (int j = 0; j < pb; ++j) q.push_back({NQ_PK_CHOL_STEP, (1 << 16) | j});
     q.push_back({NQ_PK_COPY_COLS, 0});
     for (int j = 0; j < ga; ++j) q.push_back({NQ_PK_BACK, j});
