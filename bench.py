@@ -350,9 +350,10 @@ def ndm_sizes(d):
 def kernel_rooflines(phases, form, d, pairs):
     """Per-kernel roofline figures from the per-phase HIP-event times.
 
-    Each timed phase is one kernel (estimator: k_estimate; sampler:
-    k_metropolis) or a short fixed sequence (S build: centring + two Gram
-    launches; solve: shift, blocked Cholesky, solves).  Algorithmic work per
+    Each timed phase is one kernel (sampler: k_metropolis) or a short fixed
+    sequence (estimator: the walk launch k_estimate and the forces launch,
+    k_force_mma on the matrix cores for the FP32 network; S build: centring +
+    Gram launches; solve: shift, blocked Cholesky, solves).  Algorithmic work per
     launch, per unit x units:
       estimator  force contraction (8 nh N + 4 na N) flop per (sample,
                  connection) pair x M pairs; evaluation lane-slots
@@ -371,7 +372,8 @@ def kernel_rooflines(phases, form, d, pairs):
     t = per["estimator"] * 1e-3
     fl = (8 * nh * N + 4 * na * N) * pairs
     slots = pairs * (2 * nh * 1.6 + 20 * nh + 2 * na * 1.6 + 28 * na + 20)
-    out["estimator"] = dict(kernel="k_estimate", bound="mfma", achieved=fl / t / 1e12, peak=PEAK_TFLOPS[fp],
+    ek = "k_estimate (walk + forces)" if form == "fp64" else "k_estimate (walk) + k_force_mma (forces)"
+    out["estimator"] = dict(kernel=ek, bound="mfma", achieved=fl / t / 1e12, peak=PEAK_TFLOPS[fp],
                             unit="TFLOP/s", algorithmic=f"(8 nh N + 4 na N) x {pairs} pairs = {fl:.4g} flop",
                             valu_frac=slots / t / VALU_LANE_SLOTS_PER_S,
                             hbm_gbs=Ns * (2 * N / 8 + 4 * elt + R * elt) / t / 1e9, ms=per["estimator"])
@@ -401,7 +403,7 @@ def kernel_rooflines(phases, form, d, pairs):
     t = per.get("factor", per["solve"]) * 1e-3
     fl = np_ ** 3 / 3
     nl, npi = N + nh + nh * N, na + 2 * na * N
-    out["solve"] = dict(kernel="k_chol_step (+ k_chol_diag32, k_split_copy, k_split_schur)", bound="mfma",
+    out["solve"] = dict(kernel="k_chol_step (+ k_split_copy, k_split_schur)", bound="mfma",
                         achieved=fl / t / 1e12,
                         peak=PEAK_TFLOPS["fp64"], unit="TFLOP/s", algorithmic=f"Np^3/3 = {fl:.4g} flop",
                         ms=per.get("factor", per["solve"]), phase_ms=per["solve"],

@@ -24,15 +24,17 @@
 //             rec_mode 1 it writes per sample the changed connections (W_c,
 //             flip masks), their count and the unchanged connections' mel sum.
 //   EK_FORCE  reads those records and L_loc and adds the force
-//             Re(V W_c O_k(c)), V = conj(L_loc)/Ns: per unit a dense part (x_s
-//             times a per-unit scalar) plus a sparse part on the flipped sites
-//             only (x_c = x_s + Delta_c).  Each lane sums its units' entries
-//             of one sample in registers (the flipped site indexes a register
-//             vector with a wave-uniform index: VGPR index mode), then adds
-//             them once per sample into the workgroup's force accumulator in
-//             LDS, held as exact 64-bit fixed-point integers (fx_add), so the
-//             sum is the same whatever order the waves arrive in.  The
-//             per-workgroup partials are then summed in a fixed order.
+//             Re(V W_c O_k(c)), V = conj(L_loc)/Ns.  FP32 network: the
+//             products on the matrix cores (force_mma_body, k_force_mma,
+//             below).  FP64: per unit a dense part (x_s times a per-unit
+//             scalar) plus a sparse part on the flipped sites only (x_c =
+//             x_s + Delta_c); each lane sums its units' entries of one sample
+//             in registers (the flipped site indexes a register vector with a
+//             wave-uniform index: VGPR index mode), then adds them once per
+//             sample into the workgroup's force accumulator in LDS, held as
+//             exact 64-bit fixed-point integers (fx_add), so the sum is the
+//             same whatever order the waves arrive in.  The per-workgroup
+//             partials are then summed in a fixed order.
 //   EK_GRAD   the parity mode: everything for one sample per wavefront and the
 //             per-sample dL_loc(s) written out (the force pass run with V = 1
 //             and V = -i for its real and imaginary parts).

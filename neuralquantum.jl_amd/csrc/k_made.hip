@@ -22,22 +22,32 @@ constexpr int MB = 64;    // configurations per workgroup
 // C (M x N) = A (M x K) B (K x N); element (m, k) of A at A[m * sam + k * sak],
 // (k, n) of B at B[k * sbk + n * sbn]; M, N multiples of 16, K of 32.  The 16x16
 // output tiles are spread over the 4 waves; epi(m, n, value) consumes them.
+// An operand whose K index is contiguous (stride 1, rows 16-byte aligned: the
+// forward products) is read 8 values at a time (ds_read_b128); the
+// transposed operands of the backward products are gathered element by element
 template <class Epi>
 __device__ __forceinline__ void mma_lds(const __bf16* A, int sam, int sak, const __bf16* B, int sbk, int sbn,
                                         int M, int N, int K, Epi&& epi) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int tn = N / 16, tiles = (M / 16) * tn;
+    const bool av = sak == 1 && (sam & 7) == 0, bv = sbk == 1 && (sbn & 7) == 0;
     for (int t = wave; t < tiles; t += 4) {
         const int m0 = (t / tn) * 16, n0 = (t % tn) * 16;
         f4 acc = {0.f, 0.f, 0.f, 0.f};
         for (int k0 = 0; k0 < K; k0 += 32) {
             bf8 a, b;
+            if (av) {
+                a = *reinterpret_cast<const bf8*>(A + (m0 + li) * sam + k0 + lk * 8);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = k0 + lk * 8 + e;
-                a[e] = A[(m0 + li) * sam + k * sak];
-                b[e] = B[k * sbk + (n0 + li) * sbn];
+                for (int e = 0; e < 8; ++e) a[e] = A[(m0 + li) * sam + (k0 + lk * 8 + e) * sak];
+            }
+            if (bv) {
+                b = *reinterpret_cast<const bf8*>(B + (n0 + li) * sbn + k0 + lk * 8);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) b[e] = B[(k0 + lk * 8 + e) * sbk + (n0 + li) * sbn];
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }

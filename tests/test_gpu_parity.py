@@ -167,3 +167,24 @@ def test_split_solve_matches_lapack_c2_shapes(gpu_ok, ref, kind):
     d_full = ctx.precondition(0.01, kind, 1)
     assert rel_err(d_split, d_full) < 1e-9
     ctx.close()
+
+
+@pytest.mark.parametrize("kind", [_lib.SHIFT_ADD, _lib.SHIFT_DIAG])
+def test_split_solve_matches_lapack_c3_shapes(gpu_ok, kind):
+    """The benchmark's shapes: 4x4 XYZ, NDM alpha = 2 (Np = 2176: nl = 560,
+    npi = 1056, 18 + 33 panels, the look-ahead tile's three-wave diagonal
+    factor, the Schur update and both folded first-block factors), S and F
+    of 4096 x 4 samples in FP64; the split step against LAPACK on the full
+    shifted S."""
+    N, nh, na = 16, 32, 32
+    flat, _, _ = xyz(4, 4)
+    ctx = Context(0, 5)
+    ctx.set_problem(flat)
+    ctx.set_ndm(N, nh, na)
+    ctx.set_params(params(ctx.np_, seed=53, sigma=0.05))
+    keys = random_keys(N, 4096 * 4, seed=10)
+    _, S, F, _ = ctx.grad_from_samples(keys, 4096, 4)
+    Ssh = S + (0.01 * np.eye(len(F)) if kind == _lib.SHIFT_ADD else 0.01 * np.diag(np.diag(S)))
+    delta_r = scipy.linalg.cho_solve(scipy.linalg.cho_factor(Ssh, lower=True), F)
+    assert rel_err(ctx.precondition(0.01, kind, 0), delta_r) < 1e-9
+    ctx.close()
