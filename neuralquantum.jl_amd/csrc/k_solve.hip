@@ -1177,9 +1177,13 @@ __global__ __launch_bounds__(64) void k_chol_diag32(double* A, int n, int lda, i
     chol_diag32_body(A, n, lda, k, kb, winv, info, sA, sW, blockIdx.y);
 }
 
+// the instances' tiles interleaved in one grid row (block b: tile b / ninst of
+// instance b % ninst), so that every instance's look-ahead tile is among the
+// first workgroups dispatched
 __global__ __launch_bounds__(256) void k_chol_step(double* A, int n, int nr, int ncu, int lda, int k, int kb,
-                                                   double* winv, int* info, int k0, int64_t sA, int64_t sW) {
-    chol_step_body(A, n, nr, ncu, lda, k, kb, winv, info, k0, sA, sW, blockIdx.x, blockIdx.y);
+                                                   double* winv, int* info, int k0, int64_t sA, int64_t sW,
+                                                   int ninst) {
+    chol_step_body(A, n, nr, ncu, lda, k, kb, winv, info, k0, sA, sW, blockIdx.x / ninst, blockIdx.x % ninst);
 }
 
 __global__ __launch_bounds__(256) void k_back_group(const double* A, int lda, int k, int kend, int n,
@@ -1503,8 +1507,8 @@ hipError_t nq_cholesky_range(double* A, int k0, int n, int nr, int ncu, int lda,
         // row tiles of the trailing rows; column tiles of the updated columns
         // (at least one, whose tiles still transform the panel's rows)
         const int T = (rest + 63) / 64, TJ = std::max(1, (ncu - k - kb + 63) / 64);
-        hipLaunchKernelGGL(k_chol_step, dim3(step_tiles(T, TJ), ninst), dim3(256), 0, st, A, n, nr, ncu, lda, k, kb,
-                           winv, info, k0, sA, sW);
+        hipLaunchKernelGGL(k_chol_step, dim3(step_tiles(T, TJ) * ninst), dim3(256), 0, st, A, n, nr, ncu, lda, k, kb,
+                           winv, info, k0, sA, sW, ninst);
     }
     return hipGetLastError();
 }

@@ -2481,11 +2481,15 @@ nqh_status batch_build_packed(nqh_batch* b, double eps, int kind, double lr) {
                 if (done[q] || o.variant != p.L.variant || o.threads != p.L.threads) continue;
                 done[q] = 1;
                 p.L.lds = std::max(p.L.lds, o.lds);
+                // a Cholesky step's instances interleaved (both look-ahead
+                // tiles among the first workgroups dispatched)
+                const bool inter = kp.first == NQ_PK_CHOL_STEP;
                 for (int z = 0; z < o.gz; ++z)
-                    for (int y = 0; y < o.gy; ++y)
-                        for (int x = 0; x < o.gx; ++x)
-                            rec.push_back(make_uint2((uint32_t)sh[q].second | ((uint32_t)z << 16),
-                                                     (uint32_t)x | ((uint32_t)y << 16)));
+                    for (int u = 0; u < o.gx * o.gy; ++u) {
+                        const int x = inter ? u / o.gy : u % o.gx, y = inter ? u % o.gy : u / o.gx;
+                        rec.push_back(make_uint2((uint32_t)sh[q].second | ((uint32_t)z << 16),
+                                                 (uint32_t)x | ((uint32_t)y << 16)));
+                    }
             }
             p.L.nblk = (int)(rec.size() - p.off);
             launches.push_back(p);
