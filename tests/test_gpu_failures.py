@@ -83,10 +83,14 @@ def nan_table(N):
     return flat
 
 
+@pytest.mark.parametrize("prec", [(_lib.PREC_FP64, _lib.PREC_FP64), (_lib.PREC_FP32, _lib.PREC_BF16)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_nan_in_local_values_is_caught_before_the_solve(gpu_ok, graph):
+def test_nan_in_local_values_is_caught_before_the_solve(gpu_ok, graph, prec):
+    """Also with the FP32 network, whose forces go through the matrix-core
+    launch: the non-finite terms reach the exact accumulator's poison word."""
     N = 4
     ctx, p = ndm_ctx(N, 3, flat=nan_table(N))
+    ctx.set_precision(*prec)
     keys = random_keys(N, 64, seed=5)
     st, _, _, _ = ctx.grad_from_samples(keys, 8, 8)
     assert not np.isfinite(st[0])
