@@ -1200,7 +1200,7 @@ template <int NMAX, int CLS, int ACT>
 __global__ __launch_bounds__(512, CLS < 2 ? 4 : 2) void k_force_mma_pk(const nq_pk_member* M, const uint2* blk) {
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = M[b.x & 0xffff];
-    force_mma_body<NMAX, CLS, ACT>(m.ef, (int)(b.y & 0xffff), m.est_grid);
+    force_mma_body<NMAX, CLS, ACT>(m.ef, (int)(b.y & 0xffff), m.force_grid);
 }
 
 #define NQ_EST_BOUNDS __launch_bounds__(512, (sizeof(T) == 4 && RS == 1 && KIND != EK_GRAD) ? 4 : 2)
@@ -1216,7 +1216,8 @@ __global__ NQ_EST_BOUNDS void k_estimate_pk(const nq_pk_member* M, const uint2* 
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = M[b.x & 0xffff];
     const est_args a = KIND == EK_FORCE ? m.ef : m.ew;
-    estimate_body<T, NMAX, RS, PS, TAB_LDS, KIND, ACT>(a, (int)(b.y & 0xffff), m.est_grid);
+    estimate_body<T, NMAX, RS, PS, TAB_LDS, KIND, ACT>(a, (int)(b.y & 0xffff),
+                                                      KIND == EK_FORCE ? m.force_grid : m.est_grid);
 }
 #undef NQ_EST_BOUNDS
 

@@ -1352,8 +1352,8 @@ __global__ __launch_bounds__(1024) void k_stats_pk(const nq_pk_member* M, const 
 __global__ __launch_bounds__(256) void k_force_part_pk(const nq_pk_member* M, const uint2* blk) {
     const uint2 b = blk[blockIdx.x];
     const nq_pk_member& m = pk_m(M, b);
-    const int per = (m.est_grid + NQ_FORCE_GROUPS - 1) / NQ_FORCE_GROUPS;
-    force_part_body(m.Fpart, m.est_grid, per, m.np, m.fpart2, pk_x(b), pk_y(b));
+    const int per = (m.force_grid + NQ_FORCE_GROUPS - 1) / NQ_FORCE_GROUPS;
+    force_part_body(m.Fpart, m.force_grid, per, m.np, m.fpart2, pk_x(b), pk_y(b));
 }
 
 __global__ __launch_bounds__(256) void k_force_finish_pk(const nq_pk_member* M, const uint2* blk) {
@@ -1652,7 +1652,7 @@ hipError_t nq_pk_plan(const nq_pk_member& m, int kernel, int param, nq_pk_shape*
         case NQ_PK_EST_FORCE:
             o->variant = nq_est_variant(kernel == NQ_PK_EST_WALK ? m.ew : m.ef, m.fp64 != 0, &o->threads, &o->lds);
             if (o->variant < 0) return hipErrorInvalidValue;
-            o->gx = m.est_grid;
+            o->gx = kernel == NQ_PK_EST_WALK ? m.est_grid : m.force_grid;
             break;
         case NQ_PK_COLSUM:
             o->variant = m.fp64;

@@ -317,7 +317,8 @@ hipError_t nq_launch_exact_combine(const double* part, int64_t nch, double* out,
 struct nq_pk_member {
     samp_args sa;              // the sampler launch (iteration number from sa.iter_dev)
     est_args ew, ef;           // estimator: walk (rec_mode 1), forces (rec_mode 2)
-    int est_grid;              // workgroups of each estimator launch
+    int est_grid;              // workgroups of the walk launch
+    int force_grid;            // workgroups of the forces launch
     int fp64;                  // network precision of Y, L_loc
     const void* Y;             // Ns x R compact O rows
     const void* Lloc;
@@ -327,7 +328,7 @@ struct nq_pk_member {
     int col_chunks;
     double* mu;
     double* stats;
-    double* Fpart;             // est_grid x np force partials
+    double* Fpart;             // force_grid x np force partials
     double* fpart2;            // NQ_FORCE_GROUPS x np
     double* F;
     int s_mode;                // S build: NQ_GRAM_F64 / NQ_GRAM_F32 (fill + two k_gram), -1: BF16 split-K

@@ -266,6 +266,17 @@ int est_grid_for(int64_t Ns, int spb = 0) {
     return (int)g;
 }
 
+// workgroups of the forces launch: from kSplitSamples samples on, where it
+// runs beside the S build, the FP32 network's matrix-core launch takes four
+// times the walk's samples per workgroup (staging, accumulator clear and
+// partial write amortised over more samples, fewer partials to sum, and more
+// of the GPU left to the S build); otherwise, and in FP64, the walk's grid.
+// A function of the batch alone, so that every path (graph, eager, profiled,
+// packed) sums the same partials
+int force_grid_of(const nqh_ctx* c, int64_t Ns) {
+    return (fp64_net(c) || Ns < kSplitSamples) ? c->est_grid : std::max(1, (c->est_grid + 3) / 4);
+}
+
 // column-sum chunks of the NDM's O rows: 64 samples or more per chunk, at most kColChunks
 int col_chunks_for(int64_t Ns) {
     int64_t k = Ns / 64;
@@ -540,7 +551,8 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
     const bool beside = Ns >= kSplitSamples && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY;
     phase_begin(c, PH_ESTIMATE);
     HIPCHK(c, nq_launch_estimate(a1, f64, c->est_grid, c->stream));
-    if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, c->stream));
+    const int fgrid = force_grid_of(c, Ns);
+    if (!beside) HIPCHK(c, nq_launch_estimate(a2, f64, fgrid, c->stream));
     phase_end(c, PH_ESTIMATE);
     phase_begin(c, PH_POST);
     HIPCHK(c, nq_launch_colsum(c->Y.p, f64, Ns, c->d.R, P<double>(c->colpart), col_chunks_for(Ns), c->stream));
@@ -551,10 +563,10 @@ nqh_status run_grad_pipeline(nqh_ctx* c, const uint64_t* keys_dev, int64_t chain
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
         fs = c->side;
-        HIPCHK(c, nq_launch_estimate(a2, f64, c->est_grid, fs));
+        HIPCHK(c, nq_launch_estimate(a2, f64, fgrid, fs));
     }
     HIPCHK(c, nq_launch_stats(c->Lloc.p, f64, chains, Lc, P<double>(c->stats), fs));
-    HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), c->est_grid, P<double>(c->fpart2), P<double>(c->mu),
+    HIPCHK(c, nq_launch_force_finish(P<double>(c->Fpart), fgrid, P<double>(c->fpart2), P<double>(c->mu),
                                      P<double>(c->stats), c->d, P<double>(c->F), fs));
     if (beside) HIPCHK(c, hipEventRecord(c->ev_join, c->side));
     phase_end(c, PH_POST);
@@ -2357,6 +2369,7 @@ nq_pk_member pk_describe(nqh_ctx* c, double eps, int kind, double lr) {
     m.ef.tab_lds = 0;   // the records replace the walk
     m.ef.Fpart = P<double>(c->Fpart);
     m.est_grid = c->est_grid;
+    m.force_grid = force_grid_of(c, Ns);
     m.fp64 = f64 ? 1 : 0;
     m.Y = c->Y.p;
     m.Lloc = c->Lloc.p;
