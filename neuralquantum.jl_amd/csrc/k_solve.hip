@@ -30,6 +30,17 @@ __global__ void k_bump(uint32_t* x) {
     if (threadIdx.x == 0) x[0] += 1u;
 }
 
+// an iteration's outcome in one record of pinned host memory: the five loss
+// statistics, the factorization's failed-pivot word and the step's
+// finiteness flag, so that waiting for the iteration is one stream
+// synchronisation and no copies
+__global__ void k_status_out(const double* stats, const int* info, const int* flag, double* out) {
+    const int t = threadIdx.x;
+    if (t < 5) out[t] = stats[t];
+    if (t == 5) out[5] = (double)*info;
+    if (t == 6) out[6] = (double)*flag;
+}
+
 template <typename T>
 __device__ __forceinline__ double abs2_at(const T* L, int64_t s) {
     const double re = (double)L[2 * s], im = (double)L[2 * s + 1];
@@ -1460,6 +1471,11 @@ hipError_t nq_launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t st
 
 hipError_t nq_launch_bump(uint32_t* x, hipStream_t st) {
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, x);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_status_out(const double* stats, const int* info, const int* flag, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_status_out, dim3(1), dim3(64), 0, st, stats, info, flag, out);
     return hipGetLastError();
 }
 

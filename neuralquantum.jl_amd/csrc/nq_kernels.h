@@ -135,6 +135,8 @@ hipError_t nq_launch_fill(double* x, int64_t n, double v, hipStream_t st);
 hipError_t nq_launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t st);
 // *x += 1 (the iteration counter of a captured iteration)
 hipError_t nq_launch_bump(uint32_t* x, hipStream_t st);
+// statistics (5 doubles), failed-pivot word and finiteness flag into out[0..6] (pinned host memory)
+hipError_t nq_launch_status_out(const double* stats, const int* info, const int* flag, double* out, hipStream_t st);
 // stats of |L_loc|^2 over B chains x L slices (stats.jl:52-77): out[5]
 hipError_t nq_launch_stats(const void* Lloc, int fp64, int64_t B, int64_t Lc, double* out, hipStream_t st);
 // F[k] = sum_b Fpart[b][k] - C Re(Obar_k), P order; C = stats[0]; part2: NQ_FORCE_GROUPS x np scratch

@@ -30,6 +30,7 @@ struct nqh_ctx {
     // build, forked from and joined back into `stream` by the two events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    double* hstat = nullptr;   // pinned: the last iteration's statistics, pivot word, finiteness flag
     uint64_t seed = 0;
     std::string err;
 
@@ -707,6 +708,11 @@ nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
         delete c;
         return NQH_E_HIP;
     }
+    if (hipHostMalloc((void**)&c->hstat, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NQH_E_HIP;
+    }
     // the allocation-free entry point's scratch address on this device (a
     // failure here is reported by that entry point, which tries again)
     (void)scratch_setup(c);
@@ -739,6 +745,7 @@ void nqh_destroy(nqh_ctx* c) {
         (void)hipStreamDestroy(c->side);
     }
     (void)hipStreamDestroy(c->stream);
+    if (c->hstat) (void)hipHostFree(c->hstat);
     delete c;
 }
 
@@ -2017,6 +2024,7 @@ static nqh_status enqueue_iteration(nqh_ctx* c, double eps, int kind, int refine
     phase_begin(c, PH_UPDATE);
     HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
+    HIPCHK(c, nq_launch_status_out(P<double>(c->stats), P<int>(c->info), P<int>(c->flag), c->hstat, c->stream));
     return NQH_OK;
 }
 
@@ -2142,13 +2150,7 @@ static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t
         if (s0) return s0;
         return graph_iteration(c, eps, shift_kind, refine, lr);
     }
-    nqh_status s = sample_grad_async(c);
-    if (s) return s;
-    if ((s = precondition_async(c, eps, shift_kind, refine))) return s;
-    phase_begin(c, PH_UPDATE);
-    HIPCHK(c, launch_update(c, lr));
-    phase_end(c, PH_UPDATE);
-    return NQH_OK;
+    return enqueue_iteration(c, eps, shift_kind, refine, lr);
 }
 
 // wait for the enqueued iteration; statistics and solve checks
@@ -2172,13 +2174,11 @@ static nqh_status finish_status(nqh_ctx* c, const double* st, int info, int flag
 static nqh_status sr_finish(nqh_ctx* c, double* stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     phase_collect(c);
+    // the iteration's last kernel wrote its outcome into the pinned record
+    const volatile double* h = c->hstat;
     double st[5];
-    int info = 0, flag = 0;
-    HIPCHK(c, copy_sync(c->stream, st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
-    if (std::isfinite(st[0])) {
-        HIPCHK(c, copy_sync(c->stream, &info, c->info.p, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(c, copy_sync(c->stream, &flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost));
-    }
+    for (int k = 0; k < 5; ++k) st[k] = h[k];
+    const int info = (int)h[5], flag = (int)h[6];
     return finish_status(c, st, info, flag, stats);
 }
 
