@@ -2220,7 +2220,7 @@ struct nqh_batch {
     int mode = NQH_BATCH_PACKED;
     bool packed = false;          // the current graph is the packed form
     void* pk_dev = nullptr;       // its member descriptors and block records
-    double* pk_status = nullptr;  // in pk_dev: 8 doubles per member (statistics, info, flag)
+    double* pk_status = nullptr;  // pinned host memory: 8 doubles per member (statistics, info, flag)
     std::vector<double> pk_host;
     int pk_launches = 0;
 };
@@ -2247,6 +2247,9 @@ void batch_drop_graph(nqh_batch* b) {
         // ends with each member's statistics read back
         (void)hipFree(b->pk_dev);
         b->pk_dev = nullptr;
+    }
+    if (b->pk_status) {
+        (void)hipHostFree(b->pk_status);
         b->pk_status = nullptr;
     }
     b->packed = false;
@@ -2508,14 +2511,15 @@ nqh_status batch_build_packed(nqh_batch* b, double eps, int kind, double lr) {
             launches.push_back(p);
         }
     }
-    // device layout: status slots (8 doubles per member) | descriptors | block records
-    const size_t sbytes = (sizeof(double) * 8 * n + 255) / 256 * 256;
+    // device layout: descriptors | block records; the status slots (8
+    // doubles per member, written by the last launch) in pinned host memory,
+    // so that the batch iteration ends with one synchronisation and no copy
     const size_t mbytes = (sizeof(nq_pk_member) * n + 255) / 256 * 256;
-    const size_t bytes = sbytes + mbytes + sizeof(uint2) * rec.size();
+    const size_t bytes = mbytes + sizeof(uint2) * rec.size();
+    BHIPCHK(b, hipHostMalloc((void**)&b->pk_status, sizeof(double) * 8 * n, hipHostMallocDefault));
     BHIPCHK(b, hipMalloc(&b->pk_dev, bytes));
-    b->pk_status = static_cast<double*>(b->pk_dev);
     for (size_t i = 0; i < n; ++i) M[i].status = b->pk_status + 8 * i;
-    unsigned char* base = static_cast<unsigned char*>(b->pk_dev) + sbytes;
+    unsigned char* base = static_cast<unsigned char*>(b->pk_dev);
     const nq_pk_member* Md = reinterpret_cast<const nq_pk_member*>(base);
     const uint2* Rd = reinterpret_cast<const uint2*>(base + mbytes);
     BHIPCHK(b, hipMemcpyAsync(base, M.data(), sizeof(nq_pk_member) * n, hipMemcpyHostToDevice, b->stream));
@@ -2678,10 +2682,11 @@ nqh_status nqh_batch_sr_iteration(nqh_batch* b, double eps, int32_t kind, int32_
     }
     nqh_status first = NQH_OK;
     if (!eager && b->packed) {
-        // every member's statistics and status words in one read-back
+        // every member's statistics and status words, from the pinned slots
         b->pk_host.resize(8 * n);
-        BHIPCHK(b, copy_sync(b->stream, b->pk_host.data(), b->pk_status, sizeof(double) * 8 * n,
-                             hipMemcpyDeviceToHost));
+        BHIPCHK(b, hipStreamSynchronize(b->stream));
+        const volatile double* h = b->pk_status;
+        for (size_t k = 0; k < 8 * n; ++k) b->pk_host[k] = h[k];
         for (size_t i = 0; i < n; ++i) {
             const double* q = b->pk_host.data() + 8 * i;
             nqh_status s = finish_status(b->m[i], q, (int)q[5], (int)q[6], stats ? stats + 5 * i : nullptr);
