@@ -833,27 +833,29 @@ __device__ __forceinline__ void back_group_body(const double* A, int lda, int k,
             }
     if (t < BG * CB) ys[t] = (t < ng) ? yv : 0.0;
     __syncthreads();
+    // the dot products of the group solve are on the serial chain: four
+    // partial sums each (a dependent chain of CB / 4 FMAs, not CB)
     for (int p = npan - 1; p >= 0; --p) {
         if (pt == p) {
-            double s = 0.0;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int m = 0; m < CB; ++m) s += wv[m] * ys[p * CB + m];
-            xs[t] = (t < ng) ? s : 0.0;
+            for (int m = 0; m < CB; ++m) s[m & 3] = fma(wv[m], ys[p * CB + m], s[m & 3]);
+            xs[t] = (t < ng) ? (s[0] + s[1]) + (s[2] + s[3]) : 0.0;
         }
         __syncthreads();
         if (t < p * CB) {
             const int q = t / CB, r = t & (CB - 1), b = p * (p - 1) / 2 + q;
-            double s = 0.0;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int m = 0; m < CB; ++m) s += Lq[b][r][m] * xs[p * CB + m];
-            ys[t] -= s;
+            for (int m = 0; m < CB; ++m) s[m & 3] = fma(Lq[b][r][m], xs[p * CB + m], s[m & 3]);
+            ys[t] -= (s[0] + s[1]) + (s[2] + s[3]);
         }
         __syncthreads();
     }
     if (bx == 0 && t < ng) x[k + t] = xs[t];
     double acc = 0.0;
 #pragma unroll
-    for (int m = 0; m < BCW; ++m) acc += a[m] * (h * BCW + m < ng ? xs[h * BCW + m] : 0.0);
+    for (int m = 0; m < BCW; ++m) acc = fma(a[m], h * BCW + m < ng ? xs[h * BCW + m] : 0.0, acc);
 #pragma unroll
     for (int d = 1; d < LPR; d <<= 1) acc += __shfl_xor(acc, d, 64);
     if (h == 0 && i < k) y[i] -= acc;
@@ -1221,10 +1223,6 @@ __global__ __launch_bounds__(256) void k_split_upd(const double* A1, int lda, in
 
 __global__ void k_split_upd_fin(const double* part, int nl, double* y1) { split_upd_fin_body(part, nl, y1, blockIdx.x); }
 
-__global__ void k_split_gather(const double* x0, const double* x1, int nl, int npi, int np, double* delta) {
-    split_gather_body(x0, x1, nl, npi, np, delta, blockIdx.x);
-}
-
 __global__ void k_copy_col(const double* A, int n, int lda, double* y) { copy_col_body(A, n, lda, y, blockIdx.x); }
 
 __global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr, nq_step_guard g) {
@@ -1232,6 +1230,47 @@ __global__ void k_update(double* p64, float* p32, const double* delta, ndm_dims 
 }
 
 __global__ void k_check_finite(const double* x, int64_t n, int* flag) { check_finite_body(x, n, flag, blockIdx.x); }
+
+// the right-hand side columns of both split instances (blockIdx.y)
+__global__ void k_copy_col2(const double* A, int n, int lda, int64_t sA, double* y, int64_t sY) {
+    copy_col_body(A + blockIdx.y * sA, n, lda, y + blockIdx.y * sY, blockIdx.x);
+}
+
+// the split solve's tail in one workgroup, with no launch between its parts:
+// delta in P order from the two instances' solutions (split_gather_body),
+// *flag = 1 if any entry is not finite, else 0 (as the flag clear and
+// k_check_finite); then, when p64 is given, the step p -= lr delta under the
+// same guard as k_update (no failed pivot, finite step, finite loss); then,
+// when out is given, the status record [stats(5), info, flag] (as
+// k_status_out)
+__global__ __launch_bounds__(1024) void k_split_tail(const double* x0, const double* x1, int nl, int npi,
+                                                     double* delta, int* flag, double* p64, float* p32, ndm_dims d,
+                                                     double lr, const int* info, const double* stats, double* out) {
+    const int np = d.np, m = nl + npi;
+    int bad = 0;
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+        const double v = (p < m) ? x0[p] : x1[p - m];
+        delta[p] = v;
+        bad |= !isfinite(v);
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) *flag = bad;
+    if (p64 && !bad && *info == 0 && isfinite(stats[0])) {
+        for (int p = threadIdx.x; p < np; p += blockDim.x) {
+            const double v = (p < m) ? x0[p] : x1[p - m];
+            const int r = p2ref(p, d);
+            const double q = p64[r] - lr * v;
+            p64[r] = q;
+            if (p32) p32[r] = (float)q;
+        }
+    }
+    if (out) {
+        const int t = threadIdx.x;
+        if (t < 5) out[t] = stats[t];
+        if (t == 5) out[5] = (double)*info;
+        if (t == 6) out[6] = (double)bad;
+    }
+}
 
 // ---- packed forms: a member's part of the split solve, its arguments
 // derived from its descriptor the way nq_split_factor / nq_split_substitute
@@ -1596,23 +1635,29 @@ hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, in
 }
 
 hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,
-                               double* delta, hipStream_t st) {
+                               hipStream_t st) {
     const int m = nl + npi, lda = m + 1;
     const int64_t sA = (int64_t)lda * lda, sY = lda;
     const int64_t sW = (int64_t)nq_cholesky_winv_doubles(nl);
     const double* winvP = winv + 2 * sW;
     hipError_t e;
     double *y0 = y2, *y1 = y2 + sY, *x0 = x2, *x1 = x2 + sY;
-    if ((e = nq_copy_aug_col(A2, m, lda, y0, st)) != hipSuccess) return e;
-    if ((e = nq_copy_aug_col(A2 + sA, m, lda, y1, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_copy_col2, dim3(nblk(m, 256), 2), dim3(256), 0, st, A2, m, lda, sA, y0, sY);
     // x_Pi, with the lambda rows of y updated on the way; then the mu rows
     if ((e = nq_cholesky_backsolve_range(A2, nl, m, lda, winvP, y0, x0, 1, 0, 0, 0, st)) != hipSuccess) return e;
     double* part = x2 + 2 * sY;   // kUpdChunks x nl scratch after the two solutions
     hipLaunchKernelGGL(k_split_upd, dim3(nblk(nl, 64), kUpdChunks), dim3(256), 0, st, A2 + sA, lda, nl, m, x0, part);
     hipLaunchKernelGGL(k_split_upd_fin, dim3(nblk(nl, 256)), dim3(256), 0, st, part, nl, y1);
     // x_lambda and x_mu together
-    if ((e = nq_cholesky_backsolve_range(A2, 0, nl, lda, winv, y0, x0, 2, sA, sW, sY, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_split_gather, dim3(nblk(m + nl, 256)), dim3(256), 0, st, x0, x1, nl, npi, m + nl, delta);
+    return nq_cholesky_backsolve_range(A2, 0, nl, lda, winv, y0, x0, 2, sA, sW, sY, st);
+}
+
+hipError_t nq_split_tail(int nl, int npi, const double* x2, double* delta, int* flag, double* p64, float* p32,
+                         const ndm_dims& d, double lr, const int* info, const double* stats, double* out,
+                         hipStream_t st) {
+    const int lda = nl + npi + 1;
+    hipLaunchKernelGGL(k_split_tail, dim3(1), dim3(1024), 0, st, x2, x2 + lda, nl, npi, delta, flag, p64, p32, d, lr,
+                       info, stats, out);
     return hipGetLastError();
 }
 

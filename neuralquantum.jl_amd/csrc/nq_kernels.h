@@ -174,7 +174,7 @@ size_t nq_split_x_doubles(int nl, int npi);
 hipError_t nq_split_factor(const double* S, int lds, const double* F, int nl, int npi, double eps, int kind,
                            double* A2, double* winv, int* info, hipStream_t st);
 hipError_t nq_split_substitute(int nl, int npi, const double* A2, const double* winv, double* y2, double* x2,
-                               double* delta, hipStream_t st);
+                               hipStream_t st);
 hipError_t nq_cholesky(double* S, int n, int lds, int* info, double* winv, hipStream_t st);
 // x = L^-T y with the factor from nq_cholesky (y is overwritten)
 hipError_t nq_cholesky_backsolve(const double* L, int n, int lds, const double* winv, double* y, double* x,
@@ -202,6 +202,13 @@ struct nq_step_guard {
 };
 hipError_t nq_launch_update(double* p64, float* p32, const double* delta, ndm_dims d, double lr,
                             nq_step_guard g, hipStream_t st);
+// the split solve's last launch (one workgroup): delta in P order from x2
+// (nq_split_substitute's solutions), *flag = any non-finite entry (0 or 1);
+// with p64, the step p -= lr delta guarded as nq_launch_update's (info, the
+// flag, stats[0]); with out, the status record [stats(5), info, flag]
+hipError_t nq_split_tail(int nl, int npi, const double* x2, double* delta, int* flag, double* p64, float* p32,
+                         const ndm_dims& d, double lr, const int* info, const double* stats, double* out,
+                         hipStream_t st);
 // FP32 copy of the parameters
 hipError_t nq_launch_to_f32(const double* p64, float* p32, int64_t n, hipStream_t st);
 // NaN / Inf check: *flag |= 1 if any non-finite in x

@@ -1895,7 +1895,10 @@ static nqh_status lsqr_solve(nqh_ctx* c, double eps, int kind) {
     return NQH_OK;
 }
 
-static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine) {
+// lr given (a whole iteration): the split solve's last launch also applies the
+// guarded step and writes the status record, and *stepped says so
+static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refine, const double* lr = nullptr,
+                                     bool* stepped = nullptr) {
     if (!c->grad_ready) return fail(c, NQH_E_STATE, "no gradient: call nqh_sample_grad first");
     if (kind != NQH_SHIFT_ADD && kind != NQH_SHIFT_DIAG) return fail(c, NQH_E_ARG, "bad shift kind");
     if (!(eps >= 0.0) || refine < 0 || refine > 8) return fail(c, NQH_E_ARG, "eps >= 0, refine in 0..8");
@@ -1918,9 +1921,12 @@ static nqh_status precondition_async(nqh_ctx* c, double eps, int kind, int refin
                                   P<double>(c->winv2), P<int>(c->info), c->stream));
         phase_end(c, PH_FACTOR);
         HIPCHK(c, nq_split_substitute(c->d.nlam, c->d.npi, P<double>(c->A2), P<double>(c->winv2), P<double>(c->y2),
-                                      P<double>(c->x2), P<double>(c->delta), c->stream));
-        HIPCHK(c, nq_launch_fill_u32(P<uint32_t>(c->flag), 1, 0u, c->stream));
-        HIPCHK(c, nq_launch_check_finite(P<double>(c->delta), np, P<int>(c->flag), c->stream));
+                                      P<double>(c->x2), c->stream));
+        const bool step = lr != nullptr;
+        HIPCHK(c, nq_split_tail(c->d.nlam, c->d.npi, P<double>(c->x2), P<double>(c->delta), P<int>(c->flag),
+                                step ? P<double>(c->p64) : nullptr, P<float>(c->p32), c->d, step ? *lr : 0.0,
+                                P<int>(c->info), P<double>(c->stats), step ? c->hstat : nullptr, c->stream));
+        if (stepped) *stepped = step;
         phase_end(c, PH_SOLVE);
         return NQH_OK;
     }
@@ -2020,7 +2026,9 @@ static void drop_graph(nqh_ctx* c) {
 static nqh_status enqueue_iteration(nqh_ctx* c, double eps, int kind, int refine, double lr) {
     nqh_status s = sample_grad_async(c);
     if (s) return s;
-    if ((s = precondition_async(c, eps, kind, refine))) return s;
+    bool stepped = false;
+    if ((s = precondition_async(c, eps, kind, refine, &lr, &stepped))) return s;
+    if (stepped) return NQH_OK;
     phase_begin(c, PH_UPDATE);
     HIPCHK(c, launch_update(c, lr));
     phase_end(c, PH_UPDATE);
