@@ -1246,28 +1246,66 @@ __global__ void k_copy_col2(const double* A, int n, int lda, int64_t sA, double*
 __global__ __launch_bounds__(1024) void k_split_tail(const double* x0, const double* x1, int nl, int npi,
                                                      double* delta, int* flag, double* p64, float* p32, ndm_dims d,
                                                      double lr, const int* info, const double* stats, double* out) {
-    const int np = d.np, m = nl + npi;
+    // the first TPT entries of each thread stay in registers with their
+    // parameters, and the guard words are read up front, so the step costs
+    // no second round of dependent loads (the parameters are only read
+    // before the guard is known)
+    constexpr int TPT = 4;
+    const int np = d.np, m = nl + npi, t = threadIdx.x, nt = blockDim.x;
+    const bool need = p64 || out;
+    const int inf = need ? *info : 0;
+    const double loss = need ? stats[0] : 0.0;
+    double v[TPT], q[TPT];
+    int r[TPT];
     int bad = 0;
-    for (int p = threadIdx.x; p < np; p += blockDim.x) {
-        const double v = (p < m) ? x0[p] : x1[p - m];
-        delta[p] = v;
-        bad |= !isfinite(v);
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        const int p = t + u * nt;
+        v[u] = 0.0;
+        q[u] = 0.0;
+        r[u] = 0;
+        if (p < np) {
+            v[u] = (p < m) ? x0[p] : x1[p - m];
+            if (p64) {
+                r[u] = p2ref(p, d);
+                q[u] = p64[r[u]];
+            }
+        }
+    }
+    for (int p = t + TPT * nt; p < np; p += nt) {
+        const double w = (p < m) ? x0[p] : x1[p - m];
+        delta[p] = w;
+        bad |= !isfinite(w);
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        if (t + u * nt < np) {
+            delta[t + u * nt] = v[u];
+            bad |= !isfinite(v[u]);
+        }
     }
     bad = __syncthreads_or(bad);
-    if (threadIdx.x == 0) *flag = bad;
-    if (p64 && !bad && *info == 0 && isfinite(stats[0])) {
-        for (int p = threadIdx.x; p < np; p += blockDim.x) {
-            const double v = (p < m) ? x0[p] : x1[p - m];
-            const int r = p2ref(p, d);
-            const double q = p64[r] - lr * v;
-            p64[r] = q;
-            if (p32) p32[r] = (float)q;
+    if (t == 0) *flag = bad;
+    if (p64 && !bad && inf == 0 && isfinite(loss)) {
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            if (t + u * nt < np) {
+                const double w = q[u] - lr * v[u];
+                p64[r[u]] = w;
+                if (p32) p32[r[u]] = (float)w;
+            }
+        }
+        for (int p = t + TPT * nt; p < np; p += nt) {
+            const double w0 = (p < m) ? x0[p] : x1[p - m];
+            const int rr = p2ref(p, d);
+            const double w = p64[rr] - lr * w0;
+            p64[rr] = w;
+            if (p32) p32[rr] = (float)w;
         }
     }
     if (out) {
-        const int t = threadIdx.x;
         if (t < 5) out[t] = stats[t];
-        if (t == 5) out[5] = (double)*info;
+        if (t == 5) out[5] = (double)inf;
         if (t == 6) out[6] = (double)bad;
     }
 }
