@@ -34,11 +34,23 @@ __global__ void k_bump(uint32_t* x) {
 // statistics, the factorization's failed-pivot word and the step's
 // finiteness flag, so that waiting for the iteration is one stream
 // synchronisation and no copies
+// the record's last word: out[7] = 1.0 once out[0..6] are visible to the
+// host (each writer fences at system scope before the barrier, then one
+// release store), so the host may read the record as soon as it sees it
+__device__ __forceinline__ void status_done(double* out) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(out + 7), 0x3ff0000000000000ull, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_status_out(const double* stats, const int* info, const int* flag, double* out) {
     const int t = threadIdx.x;
     if (t < 5) out[t] = stats[t];
     if (t == 5) out[5] = (double)*info;
     if (t == 6) out[6] = (double)*flag;
+    status_done(out);
 }
 
 template <typename T>
@@ -1307,6 +1319,7 @@ __global__ __launch_bounds__(1024) void k_split_tail(const double* x0, const dou
         if (t < 5) out[t] = stats[t];
         if (t == 5) out[5] = (double)inf;
         if (t == 6) out[6] = (double)bad;
+        status_done(out);
     }
 }
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +31,8 @@ struct nqh_ctx {
     // build, forked from and joined back into `stream` by the two events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    double* hstat = nullptr;   // pinned: the last iteration's statistics, pivot word, finiteness flag
+    double* hstat = nullptr;   // pinned: the last iteration's statistics, pivot word, finiteness flag, done word
+    int inflight = 0;          // iterations enqueued by sr_enqueue and not yet finished
     uint64_t seed = 0;
     std::string err;
 
@@ -708,7 +710,9 @@ nqh_status nqh_create(nqh_ctx** out, int32_t device, uint64_t seed) {
         delete c;
         return NQH_E_HIP;
     }
-    if (hipHostMalloc((void**)&c->hstat, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    // coherent: the kernel's stores reach the host as they are made (the
+    // done word is polled while the last kernel may still be running)
+    if (hipHostMalloc((void**)&c->hstat, 8 * sizeof(double), hipHostMallocCoherent) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return NQH_E_HIP;
@@ -2147,7 +2151,17 @@ nqh_status nqh_get_launch_shape(const nqh_ctx* c, int32_t* out4) {
 }
 
 // enqueue one SR iteration on the context's stream without waiting for it
+static nqh_status sr_enqueue_one(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr);
+// the record's done word is cleared before the only iteration in flight is
+// enqueued (every iteration ends with the kernel that writes the record)
 static nqh_status sr_enqueue(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
+    if (c->inflight == 0) reinterpret_cast<volatile double*>(c->hstat)[7] = 0.0;
+    const nqh_status s = sr_enqueue_one(c, eps, shift_kind, refine, lr);
+    if (!s) ++c->inflight;
+    return s;
+}
+
+static nqh_status sr_enqueue_one(nqh_ctx* c, double eps, int32_t shift_kind, int32_t refine, double lr) {
     if (nqh_status sm = made_maybe_retrain(c)) return sm;
     if (c->use_graph && !c->profiling && c->solver == NQH_SOLVER_CHOLESKY) {
         nqh_status s0 = need_net(c);
@@ -2180,10 +2194,23 @@ static nqh_status finish_status(nqh_ctx* c, const double* st, int info, int flag
 }
 
 static nqh_status sr_finish(nqh_ctx* c, double* stats) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    phase_collect(c);
-    // the iteration's last kernel wrote its outcome into the pinned record
+    // the iteration's last kernel writes its outcome into the pinned record
+    // and then the done word: with one iteration in flight and no phase
+    // events to read, a short spin on that word replaces the runtime's wait;
+    // a slow or failed iteration (no done word within 200 ms) falls back to
+    // the stream synchronisation, which also reports errors
     const volatile double* h = c->hstat;
+    bool seen = false;
+    if (c->inflight == 1 && !c->profiling) {
+        const auto t0 = std::chrono::steady_clock::now();
+        while (!(seen = h[7] != 0.0)) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+            __builtin_ia32_pause();
+        }
+    }
+    c->inflight = 0;
+    if (!seen) HIPCHK(c, hipStreamSynchronize(c->stream));
+    phase_collect(c);
     double st[5];
     for (int k = 0; k < 5; ++k) st[k] = h[k];
     const int info = (int)h[5], flag = (int)h[6];
