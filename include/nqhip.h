@@ -260,8 +260,9 @@ nqh_status nqh_sr_iteration(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_
  * (returns once the work is queued), and the wait for it (statistics, solve
  * and finiteness checks as nqh_sr_iteration).  Many contexts on one device,
  * each with its own stream, then run their iterations concurrently: a batch
- * of independent problems (for example a sweep of instances of uneven size).  No
- * reference counterpart (execution strategy). */
+ * of independent problems (for example a sweep of instances of uneven size).
+ * With several iterations enqueued before one nqh_wait, the wait reports the
+ * last of them.  No reference counterpart (execution strategy). */
 nqh_status nqh_sr_iteration_async(nqh_ctx* ctx, double eps, int32_t shift_kind, int32_t refine, double lr);
 nqh_status nqh_wait(nqh_ctx* ctx, double* stats);
 
